@@ -1,0 +1,259 @@
+"""Independent string-level de Bruijn builder used to pin the C++ oracle.
+
+Everything here works on Python strings, with no 2-bit packing and no list
+ranking, so a packing, orientation or pointer-jumping mistake in the oracle
+cannot be repeated here:
+
+* k+1-mer (edge) counts come from counting k+1-mer windows in the
+  reads, not from aggregating k2 records by k-mer block;
+* unitigs come from walking unique edges between chain nodes, not from
+  list ranking.
+
+The only formula shared with the reference restatement is the palindrome
+halving of k+1-mer counts (debruijn_graph_map.hpp:1468-1507), which is a
+definition rather than a derivation; it is exercised by crafted inputs whose
+expected values are also worked out by hand in the tests.
+"""
+from collections import defaultdict
+
+COMP = {"A": "T", "C": "G", "G": "C", "T": "A"}
+IDX = {"A": 0, "C": 1, "G": 2, "T": 3}
+DNA16 = {"A": 1, "C": 2, "G": 4, "T": 8, None: 0}
+
+
+def rc(s):
+    return "".join(COMP[c] for c in reversed(s))
+
+
+def canon(s):
+    r = rc(s)
+    return s if s < r else r
+
+
+def is_pal(s):
+    return s == rc(s)
+
+
+def records(reads, k):
+    """Canonical (k-mer, left, right) instances; reads with non-ACGT bases or
+    shorter than k+2 are dropped."""
+    out = []
+    for s in reads:
+        if any(c not in IDX for c in s) or len(s) < k + 2:
+            continue
+        L = len(s)
+        for i in range(L - k + 1):
+            x = s[i:i + k]
+            a = s[i - 1] if i > 0 else None
+            b = s[i + k] if i + k < L else None
+            y = rc(x)
+            if not (x < y):
+                x, a, b = y, (COMP[b] if b else None), (COMP[a] if a else None)
+            out.append((x, a, b))
+    return out
+
+
+def kp1_counts(reads, k):
+    """Occurrences of each k+1-mer window, as written in the reads."""
+    c = defaultdict(int)
+    for s in reads:
+        if any(ch not in IDX for ch in s) or len(s) < k + 2:
+            continue
+        for i in range(len(s) - k):
+            c[s[i:i + k + 1]] += 1
+    return c
+
+
+def unfiltered(reads, k):
+    """P1: {kmer: [outA,outC,outG,outT,inA,inC,inG,inT,self]} saturating."""
+    g = {}
+    for x, a, b in records(reads, k):
+        n = g.setdefault(x, [0] * 9)
+        if b:
+            n[IDX[b]] = min(n[IDX[b]] + 1, 65535)
+        if a:
+            n[4 + IDX[a]] = min(n[4 + IDX[a]] + 1, 65535)
+        n[8] = min(n[8] + 1, 65535)
+    return g
+
+
+def filtered(reads, k, th):
+    """P2 from the k2 multiset with edge counts taken from k+1-mer windows."""
+    t0lo, t0hi, t1lo, t1hi, t2lo, t2hi = th
+    k2 = defaultdict(int)
+    for x, a, b in records(reads, k):
+        k2[(x, a, b)] += 1
+    kp1 = kp1_counts(reads, k)
+    by_k = defaultdict(list)
+    for (x, a, b), c in k2.items():
+        by_k[x].append((a, b, c))
+    g = {}
+    for x, lst in by_k.items():
+        kf = min(sum(c for _, _, c in lst), 65535)
+
+        def edge_count(e, side):
+            # e: k+1-mer on x's strand.  Palindromic k+1-mers (odd k only
+            # possible) use the reference's per-k2 halving.
+            if is_pal(e) and not is_pal(x):
+                tot = 0
+                for a, b, c in lst:
+                    ch = a if side == "in" else b
+                    if ch and ((ch + x) if side == "in" else (x + ch)) == e:
+                        tot += (c + 1) >> 1
+                return min(tot, 65535)
+            if is_pal(x):
+                # lex_less flips ties (debruijn_common.hpp:55), so every
+                # instance of an rc-palindromic k-mer is recorded from the
+                # reverse strand: its edge e is only seen as the window rc(e).
+                return min(kp1[rc(e)], 65535)
+            return min(kp1[e] + kp1[rc(e)], 65535)
+
+        node = [0] * 9
+        have = False
+        for a, b, c in lst:
+            cnt = min(c, 65535)
+            flag = 0
+            if cnt < t2lo or cnt >= t2hi:
+                flag |= 4
+            if kf < t0lo or kf >= t0hi:
+                flag |= 8
+            kin = edge_count(a + x, "in") if a else t1hi
+            if kin < t1lo or kin >= t1hi:
+                flag |= 2
+            kout = edge_count(x + b, "out") if b else t1hi
+            if kout < t1lo or kout >= t1hi:
+                flag |= 1
+            if flag >= 3:
+                continue
+            if flag == 1:
+                b = None
+            if flag == 2:
+                a = None
+            if b:
+                node[IDX[b]] = min(node[IDX[b]] + cnt, 65535)
+            if a:
+                node[4 + IDX[a]] = min(node[4 + IDX[a]] + cnt, 65535)
+            node[8] = min(node[8] + cnt, 65535)
+            have = True
+        if have:
+            g[x] = node
+    return g
+
+
+def kplus1_pal(x, node):
+    """IsKplus1Palindrome with string tests: an existing edge whose k+1-mer is
+    an rc-palindrome, or the 'paired' form where the far k-1 bases are an
+    rc-palindrome and the edge base equals the opposite end base."""
+    k = len(x)
+    if k % 2 == 0:
+        return False
+    for c in "ACGT":
+        if node[IDX[c]] > 0:
+            if is_pal(x + c) or (x[0] == c and is_pal(x[1:])):
+                return True
+        if node[4 + IDX[c]] > 0:
+            if is_pal(c + x) or (x[-1] == c and is_pal(x[:-1])):
+                return True
+    return False
+
+
+def classify(g):
+    br, ch = set(), set()
+    for x, n in g.items():
+        nin = sum(1 for i in range(4, 8) if n[i] > 0)
+        nout = sum(1 for i in range(4) if n[i] > 0)
+        pal = is_pal(x) or kplus1_pal(x, n)
+        if nin >= 2 or nout >= 2 or pal:
+            br.add(x)
+        else:
+            ch.add(x)
+    return br, ch
+
+
+def unitigs(g):
+    """Walk chains of non-branch nodes.  Returns (sorted [(rep, seq)], n_cycle_nodes).
+
+    Works on oriented k-mers: the successor of oriented k-mer u is the unique
+    out neighbour of u (in u's orientation) if u has exactly one, it is a
+    chain node, and u is its unique predecessor; otherwise u ends its chain.
+    Nodes with no edges at all are isolated and dropped; closed walks are
+    cycles and dropped."""
+    br, ch = classify(g)
+
+    def out_edges(u):
+        x = canon(u)
+        n = g[x]
+        if u == x:
+            return [u[1:] + c for c in "ACGT" if n[IDX[c]] > 0]
+        return [u[1:] + COMP[c] for c in "ACGT" if n[4 + IDX[c]] > 0]
+
+    def in_edges(u):
+        return [rc(v) for v in out_edges(rc(u))]
+
+    def succ(u):
+        o = out_edges(u)
+        if len(o) != 1:
+            return None
+        v = o[0]
+        if canon(v) not in ch:
+            return None
+        return v
+
+    seen = set()
+    result = []
+    cycle_nodes = 0
+    for x in sorted(ch):
+        if x in seen:
+            continue
+        n = g[x]
+        if sum(n[:8]) == 0:
+            seen.add(x)
+            continue  # isolated
+        # walk backwards to the start of the chain (or detect a cycle)
+        u = x
+        steps = 0
+        cyc = False
+        while True:
+            p = succ(rc(u))
+            if p is None:
+                break
+            u = rc(p)
+            steps += 1
+            if canon(u) == x:
+                cyc = True
+                break
+            if steps > 10 * len(g) + 10:
+                raise RuntimeError("walk did not terminate")
+        if cyc:
+            comp = set()
+            v = x
+            while canon(v) not in comp:
+                comp.add(canon(v))
+                v = succ(v)
+            seen |= comp
+            cycle_nodes += len(comp)
+            continue
+        path = [u]
+        while True:
+            v = succ(path[-1])
+            if v is None:
+                break
+            path.append(v)
+        for p in path:
+            seen.add(canon(p))
+        s = path[0] + "".join(p[-1] for p in path[1:])
+        rep = min(s[:len(x)], rc(s)[:len(x)])
+        seq = s if s[:len(x)] == rep else rc(s)
+        result.append((rep, seq))
+    result.sort()
+    return result, cycle_nodes
+
+
+def chain_fasta(g):
+    us, _ = unitigs(g)
+    return "\n".join(">%s\n%s" % (canon(rep), seq) for rep, seq in us)
+
+
+def branch_fasta(g):
+    br, _ = classify(g)
+    return "".join(">%s\n%s\n" % (b, b) for b in sorted(br))
