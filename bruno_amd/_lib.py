@@ -1,0 +1,119 @@
+"""ctypes binding of libbruno_hip.so (the C ABI in include/bruno_hip.h).
+
+The library is the product: there is no CPU fallback.  If it is missing or
+fails to load, every entry point raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbruno_hip.so")
+
+OK, BAD_INPUT, OOM, CAPACITY, INTERNAL, STATE, NO_DEVICE = range(7)
+
+
+class JobDesc(ctypes.Structure):
+    _fields_ = [("k", ctypes.c_int), ("thresholding", ctypes.c_int),
+                ("thresholds", ctypes.c_uint64 * 6), ("capacity", ctypes.c_uint64),
+                ("load_factor", ctypes.c_double), ("stream", ctypes.c_void_p)]
+
+
+class PhaseMs(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in
+                ("parse_count_ms", "filter_ms", "chains_ms", "list_rank_ms", "output_ms")]
+
+
+STAT_FIELDS = ("n_reads", "n_reads_dropped", "n_instances", "n_distinct", "n_k2", "n_overflow",
+               "n_nodes", "n_branches", "n_chain_nodes", "n_cycle_nodes", "n_isolated",
+               "n_unitig_nodes", "n_unitigs", "iterations", "n_conflicts", "capacity",
+               "device_bytes")
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in STAT_FIELDS]
+
+
+class SynthDesc(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_uint64), ("genome_len", ctypes.c_uint64),
+                ("n_families", ctypes.c_uint32), ("family_min_len", ctypes.c_uint32),
+                ("family_max_len", ctypes.c_uint32), ("repeat_fraction", ctypes.c_double),
+                ("family_divergence", ctypes.c_double), ("n_reads", ctypes.c_uint64),
+                ("read_len", ctypes.c_uint32), ("sub_rate", ctypes.c_double),
+                ("indel_rate", ctypes.c_double)]
+
+
+NODE_DTYPE = np.dtype([("kmer", "<u8", (2,)), ("counts", "<u2", (9,)), ("pad", "<u2", (3,))])
+CHAIN_DTYPE = np.dtype([("k", "<u8", (2,)), ("rep", "<u8", (2,)), ("dist", "<i4"), ("pad", "<i4")])
+assert NODE_DTYPE.itemsize == 40 and CHAIN_DTYPE.itemsize == 40
+
+# every entry point declared in include/bruno_hip.h
+EXPORTS = (
+    "bruno_status_string", "bruno_job_create", "bruno_job_destroy", "bruno_job_reset", "bruno_parse_count",
+    "bruno_parse_count_host", "bruno_filter", "bruno_build_chains", "bruno_list_rank",
+    "bruno_compact", "bruno_run_pipeline", "bruno_get_stats", "bruno_get_phase_ms",
+    "bruno_get_counts", "bruno_get_k2", "bruno_get_nodes", "bruno_get_branches",
+    "bruno_get_chain_nodes", "bruno_get_unitigs", "bruno_get_branch_fasta",
+    "bruno_parse_records", "bruno_synth_reads_device", "bruno_synth_reads_host",
+    "bruno_dev_alloc", "bruno_dev_free", "bruno_dev_copy", "bruno_dev_sync", "bruno_dev_mem_info",
+)
+
+_LIB = None
+
+
+class BrunoError(RuntimeError):
+    def __init__(self, code, what):
+        self.code = code
+        super().__init__("%s: %s (status %d)" % (what, status_string(code), code))
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libbruno_hip.so is not built (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, u64, i = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int
+        for name in EXPORTS:
+            getattr(L, name).restype = ctypes.c_int
+        L.bruno_status_string.restype = ctypes.c_char_p
+        L.bruno_status_string.argtypes = [i]
+        L.bruno_job_create.argtypes = [ctypes.POINTER(JobDesc), ctypes.POINTER(vp)]
+        L.bruno_job_destroy.argtypes = [vp]
+        L.bruno_job_reset.argtypes = [vp]
+        L.bruno_parse_count.argtypes = [vp, vp, vp, u64]
+        L.bruno_parse_count_host.argtypes = [vp, vp, vp, u64]
+        for n in ("bruno_filter", "bruno_build_chains", "bruno_compact"):
+            getattr(L, n).argtypes = [vp]
+        L.bruno_run_pipeline.argtypes = [vp, i]
+        L.bruno_list_rank.argtypes = [vp, i, ctypes.POINTER(ctypes.c_uint32)]
+        L.bruno_get_stats.argtypes = [vp, ctypes.POINTER(Stats)]
+        L.bruno_get_phase_ms.argtypes = [vp, ctypes.POINTER(PhaseMs)]
+        u64p = ctypes.POINTER(ctypes.c_uint64)
+        for n in ("bruno_get_counts", "bruno_get_nodes", "bruno_get_chain_nodes",
+                  "bruno_get_unitigs", "bruno_get_branch_fasta", "bruno_get_branches"):
+            getattr(L, n).argtypes = [vp, vp, u64p]
+        L.bruno_get_k2.argtypes = [vp, vp, vp, vp, u64p]
+        L.bruno_parse_records.argtypes = [i, vp, vp, u64, vp, vp, u64p, vp]
+        L.bruno_synth_reads_device.argtypes = [ctypes.POINTER(SynthDesc), vp, vp, vp]
+        L.bruno_synth_reads_host.argtypes = [ctypes.POINTER(SynthDesc), vp, vp]
+        L.bruno_dev_alloc.argtypes = [u64, ctypes.POINTER(vp)]
+        L.bruno_dev_free.argtypes = [vp]
+        L.bruno_dev_copy.argtypes = [vp, vp, u64, i]
+        L.bruno_dev_sync.argtypes = []
+        L.bruno_dev_mem_info.argtypes = [ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]
+        _LIB = L
+    return _LIB
+
+
+def status_string(code):
+    try:
+        return lib().bruno_status_string(code).decode()
+    except Exception:  # pragma: no cover - only when the library is absent
+        return "status %d" % code
+
+
+def check(code, what):
+    if code != OK:
+        raise BrunoError(code, what)

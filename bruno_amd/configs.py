@@ -1,0 +1,38 @@
+"""The benchmark workloads as generator settings.
+
+config1: 1 Mbp i.i.d. genome, 100 bp reads at 30x, 0.5% substitutions, k=31
+config2: 100 Mbp genome with 2% from 50 repeat families (500-5000 bp, 1%
+         divergence), 150 bp reads at 50x (~33.3 M reads), 1% substitutions, k=31
+config3: 20 Mbp (same repeat layer), 250 bp reads at 40x, 0.5% substitutions
+         + 0.05% 1-bp indels, k=63
+All three use the -T -L 3 thresholds.
+"""
+from .pipeline import SynthConfig, thresholds_L
+
+
+def config1(seed=1, scale=1.0):
+    G = int(1_000_000 * scale)
+    return dict(k=31, thresholds=thresholds_L(3),
+                synth=SynthConfig(seed=seed, genome_len=G, n_reads=int(round(30 * G / 100)),
+                                  read_len=100, sub_rate=0.005))
+
+
+def config2(seed=2, scale=1.0):
+    G = int(100_000_000 * scale)
+    return dict(k=31, thresholds=thresholds_L(3),
+                synth=SynthConfig(seed=seed, genome_len=G, n_reads=int(round(50 * G / 150)),
+                                  read_len=150, sub_rate=0.01, n_families=50,
+                                  family_min_len=500, family_max_len=5000,
+                                  repeat_fraction=0.02, family_divergence=0.01))
+
+
+def config3(seed=3, scale=1.0):
+    G = int(20_000_000 * scale)
+    return dict(k=63, thresholds=thresholds_L(3),
+                synth=SynthConfig(seed=seed, genome_len=G, n_reads=int(round(40 * G / 250)),
+                                  read_len=250, sub_rate=0.005, indel_rate=0.0005,
+                                  n_families=50, family_min_len=500, family_max_len=5000,
+                                  repeat_fraction=0.02, family_divergence=0.01))
+
+
+CONFIGS = {"config1": config1, "config2": config2, "config3": config3}

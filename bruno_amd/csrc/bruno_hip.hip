@@ -1,0 +1,1560 @@
+/*
+ * bruno_hip.hip -- MI355X (gfx950) implementation of the C ABI in
+ * include/bruno_hip.h: job state in HBM, phase orchestration on one HIP
+ * stream, and the graph / chain / list-ranking / output kernels.
+ */
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/bruno_hip.h"
+#include "kernels.h"
+#include "kmer.h"
+#include "synth.h"
+#include "table.h"
+
+using namespace br;
+
+namespace {
+
+/* ================================================================== */
+/* filter + classify (one thread per table slot)                        */
+/* ================================================================== */
+template <typename KT>
+__global__ __launch_bounds__(BLOCK) void k_filter(Slot<KT>* T, uint64_t cap, const Spill* sp, uint64_t spcap,
+                                                  int K, int thresholding, Thresh th, uint64_t* present,
+                                                  uint64_t present_cap, Counters* ctr) {
+  __shared__ ReserveSmem sm;
+  kops<KT> ops(K);
+  unsigned long long nb = 0, nc = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+  for (uint64_t base = (uint64_t)blockIdx.x * BLOCK; base < cap; base += stride) {
+    const uint64_t s = base + threadIdx.x;
+    bool keep = false;
+    if (s < cap) {
+      Slot<KT>* p = T + s;
+      if (SlotOps<KT>::occupied(p)) {
+        K2List L;
+        gather_k2<KT>(p, s, sp, spcap, L);
+        KT k = SlotOps<KT>::kmer(p);
+        uint16_t c[9];
+        if (thresholding) {
+          keep = p2_counts<KT>(ops, k, L, th, c);
+        } else {
+          p1_counts(L, c);
+          keep = true;
+        }
+        if (keep) {
+          bool pal = ops.is_rc_palindrome(k) || kplus1_pal<KT>(ops, k, c);
+          int ni = n_in_edges(c), no = n_out_edges(c);
+          uint8_t cls = (ni >= 2 || no >= 2 || pal) ? CLS_BRANCH : CLS_CHAIN;
+          nb += (cls == CLS_BRANCH);
+          nc += (cls == CLS_CHAIN);
+          NodeRec* nr = (NodeRec*)&p->sub[0];
+          NodeRec rec;
+          for (int q = 0; q < 9; ++q) rec.c[q] = c[q];
+          rec.cls = cls;
+          rec.pad = 0;
+          rec.idx = 0xFFFFFFFFu;
+          *nr = rec;
+          uint64_t* fw = SlotOps<KT>::flag_word(p);
+          *fw = *fw | F_PRESENT;
+        }
+      }
+    }
+    unsigned long long pos = block_reserve(keep ? 1u : 0u, &ctr->n_present, sm);
+    if (keep) {
+      if (pos < present_cap) present[pos] = s;
+      else atomicOr(&ctr->err, ERR_LIST_FULL);
+    }
+  }
+  block_add(nb, &ctr->n_branch);
+  __syncthreads();
+  block_add(nc, &ctr->n_chain);
+}
+
+/* split present slots into branch and chain lists; chain index into slot */
+template <typename KT>
+__global__ __launch_bounds__(BLOCK) void k_split(Slot<KT>* T, const uint64_t* present, uint64_t n_present,
+                                                 uint64_t* chain_slot, uint64_t* branch_slot, Counters* ctr) {
+  __shared__ ReserveSmem sm;
+  __shared__ ReserveSmem sm2;
+  const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+  for (uint64_t base = (uint64_t)blockIdx.x * BLOCK; base < n_present; base += stride) {
+    const uint64_t i = base + threadIdx.x;
+    uint8_t cls = 0;
+    uint64_t s = 0;
+    if (i < n_present) {
+      s = present[i];
+      cls = ((NodeRec*)&T[s].sub[0])->cls;
+    }
+    unsigned long long pc = block_reserve(cls == CLS_CHAIN, &ctr->lr_count, sm);
+    unsigned long long pb = block_reserve(cls == CLS_BRANCH, &ctr->lr_next, sm2);
+    if (cls == CLS_CHAIN) {
+      chain_slot[pc] = s;
+      ((NodeRec*)&T[s].sub[0])->idx = (uint32_t)pc;
+    } else if (cls == CLS_BRANCH) {
+      branch_slot[pb] = s;
+      ((NodeRec*)&T[s].sub[0])->idx = (uint32_t)pb;
+    }
+  }
+}
+
+/* ================================================================== */
+/* chain extraction                                                     */
+/* ================================================================== */
+/* Resolve neighbour k-mer n (on the pointing node's strand) to a chain
+ * pointer.  facing_if_canonical: which end of n faces the pointing node
+ * when n is itself canonical.  Non-chain targets get a virtual index
+ * holding the canonical k-mer (updates to them are dropped, as
+ * map.update(..., false, ...) drops keys absent from the chain map). */
+template <typename KT>
+__device__ uint32_t resolve(const Slot<KT>* T, uint64_t cap, const kops<KT>& ops, KT n, uint32_t facing_if_canonical,
+                            uint64_t n_chain, KT* vkmer, uint64_t vcap, Counters* ctr) {
+  KT rn = ops.rc(n);
+  bool flipped = !(n < rn);
+  KT cn = flipped ? rn : n;
+  uint32_t end = flipped ? (facing_if_canonical ^ END_OUT) : facing_if_canonical;
+  uint64_t s = SlotOps<KT>::find(T, cap, cn, true);
+  if (s != ~0ULL) {
+    const NodeRec* nr = (const NodeRec*)&T[s].sub[0];
+    if (nr->cls == CLS_CHAIN) return end | nr->idx;
+  }
+  unsigned long long v = atomicAdd(&ctr->n_virt, 1ULL);
+  if (v >= vcap) {
+    atomicOr(&ctr->err, ERR_LIST_FULL);
+    return end | IDX_MASK;
+  }
+  vkmer[v] = cn;
+  return end | (uint32_t)(n_chain + v);
+}
+
+/* to_simple_biedge (debruijn_chain_node.hpp:84-141): one neighbour per side
+ * at dist 1, else term_mask. */
+template <typename KT>
+__global__ __launch_bounds__(BLOCK) void k_chain_init(const Slot<KT>* T, uint64_t cap, int K, const uint64_t* chain_slot,
+                                                      uint64_t n_chain, KT* ckmer, LRNode* lr, KT* vkmer, uint64_t vcap,
+                                                      Counters* ctr) {
+  kops<KT> ops(K);
+  const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+  for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < n_chain; i += stride) {
+    const Slot<KT>* p = T + chain_slot[i];
+    KT k = SlotOps<KT>::kmer(p);
+    const NodeRec* nr = (const NodeRec*)&p->sub[0];
+    LRNode st;
+    st.p[0] = st.p[1] = IDX_MASK;
+    st.d[0] = st.d[1] = TERM_MASK;
+    st.mail[0] = st.mail[1] = 0ULL;
+    for (int c = 0; c < 4; ++c)
+      if (nr->c[4 + c] > 0) {
+        /* in neighbour c.k[0..K-2]: when canonical its OUT end faces k */
+        st.p[0] = resolve<KT>(T, cap, ops, ops.in_nbr(k, c), END_OUT, n_chain, vkmer, vcap, ctr);
+        st.d[0] = 1;
+      }
+    for (int c = 0; c < 4; ++c)
+      if (nr->c[c] > 0) {
+        /* out neighbour k[1..].c: when canonical its IN end faces k */
+        st.p[1] = resolve<KT>(T, cap, ops, ops.out_nbr(k, c), 0u, n_chain, vkmer, vcap, ctr);
+        st.d[1] = 1;
+      }
+    ckmer[i] = k;
+    lr[i] = st;
+  }
+}
+
+/* setup_chain_termini: each branch node marks the facing end of its chain
+ * neighbours as pointing to a branch (dist 0). */
+template <typename KT>
+__global__ __launch_bounds__(BLOCK) void k_termini(const Slot<KT>* T, uint64_t cap, int K, const uint64_t* branch_slot,
+                                                   uint64_t n_branch, LRNode* lr, KT* bkmer) {
+  kops<KT> ops(K);
+  const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+  for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < n_branch; i += stride) {
+    const Slot<KT>* p = T + branch_slot[i];
+    KT k = SlotOps<KT>::kmer(p);
+    bkmer[i] = k;
+    const NodeRec* nr = (const NodeRec*)&p->sub[0];
+    for (int side = 0; side < 2; ++side)
+      for (int c = 0; c < 4; ++c) {
+        /* out neighbours get (t, IN); in neighbours get (t, OUT); lex_less
+         * flips the orientation with the key (debruijn_chain_operations.hpp
+         * :491-497). */
+        if (!(nr->c[(side ? 4 : 0) + c] > 0)) continue;
+        KT n = side ? ops.in_nbr(k, c) : ops.out_nbr(k, c);
+        KT rn = ops.rc(n);
+        bool flipped = !(n < rn);
+        KT cn = flipped ? rn : n;
+        uint64_t s = SlotOps<KT>::find(T, cap, cn, true);
+        if (s == ~0ULL) continue;
+        const NodeRec* q = (const NodeRec*)&T[s].sub[0];
+        if (q->cls != CLS_CHAIN) continue;
+        int orient_in = (side == 0);  /* IN for out neighbours */
+        if (flipped) orient_in = !orient_in;
+        lr[q->idx].d[orient_in ? 0 : 1] = 0;
+      }
+  }
+}
+
+/* ================================================================== */
+/* list ranking: one synchronous round = send + recv                    */
+/* ================================================================== */
+__global__ __launch_bounds__(BLOCK) void k_lr_init_list(const LRNode* lr, uint64_t n_chain, uint32_t* list,
+                                                        Counters* ctr) {
+  __shared__ ReserveSmem sm;
+  const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+  for (uint64_t base = (uint64_t)blockIdx.x * BLOCK; base < n_chain; base += stride) {
+    const uint64_t i = base + threadIdx.x;
+    bool u = (i < n_chain) && is_unfinished(lr[i].d[0], lr[i].d[1]);
+    unsigned long long pos = block_reserve(u, &ctr->lr_next, sm);
+    if (u) list[pos] = (uint32_t)i;
+  }
+}
+
+__device__ __forceinline__ void lr_post(LRNode* lr, uint64_t n_chain, uint32_t target, unsigned long long msg,
+                                        Counters* ctr, unsigned long long& conflicts) {
+  uint32_t j = target & IDX_MASK;
+  if (j >= n_chain) return; /* not in the chain map: dropped */
+  int e = (target & END_OUT) ? 1 : 0;
+  /* chain_update only acts on an end that still points to a chain node and
+   * only with a distance >= the current one; the end's distance never
+   * decreases within a round, so such updates are no-ops in every
+   * application order and are not posted. */
+  const uint32_t dr = lr[j].d[e];
+  if (!points_to_chain(dr)) return;
+  if (cdist((uint32_t)(msg >> 32)) < cdist(dr)) return;
+  /* Several updates to one end in a round (list_rank sends from every
+   * non-terminal side) are applied by the reference in hash-table order.
+   * Keeping the largest (dist, ptr) word is the result of applying them in
+   * decreasing order: the first one applied is accepted, and a marked
+   * (terminal) distance or a larger plain distance then blocks or does not
+   * change the rest. */
+  unsigned long long old = atomicCAS(&lr[j].mail[e], 0ULL, msg);
+  if (old != 0ULL) {
+    ++conflicts;
+    atomicMax(&lr[j].mail[e], msg);
+  }
+}
+
+/* variant 0: send on every non-terminal side (list_rank :858-885);
+ * variant 1: only on sides pointing to a chain node (:1130-1155). */
+__global__ __launch_bounds__(BLOCK) void k_lr_send(LRNode* lr, uint64_t n_chain, const uint32_t* list, uint64_t n,
+                                                   int variant, Counters* ctr) {
+  unsigned long long conflicts = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+  for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += stride) {
+    const uint32_t t = list[i];
+    const LRNode st = lr[t];
+    const uint32_t din = st.d[0], dout = st.d[1];
+    const uint32_t dist = cdist(din) + cdist(dout);
+    bool send_in = variant ? points_to_chain(din) : !is_terminal(din);
+    bool send_out = variant ? points_to_chain(dout) : !is_terminal(dout);
+    if (send_in) {
+      /* to the in target's facing end: our out target (or ourselves, IN end) */
+      uint32_t ptr = is_terminal(dout) ? t : st.p[1];
+      uint32_t dd = points_to_chain(dout) ? dist : (dist | TERM_MASK);
+      lr_post(lr, n_chain, st.p[0], ((unsigned long long)dd << 32) | ptr, ctr, conflicts);
+    }
+    if (send_out) {
+      uint32_t ptr = is_terminal(din) ? (t | END_OUT) : st.p[0];
+      uint32_t dd = points_to_chain(din) ? dist : (dist | TERM_MASK);
+      lr_post(lr, n_chain, st.p[1], ((unsigned long long)dd << 32) | ptr, ctr, conflicts);
+    }
+  }
+  block_add(conflicts, &ctr->lr_conflict);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_lr_recv(LRNode* lr, const uint32_t* list, uint64_t n, uint32_t* next,
+                                                   uint32_t maxd, Counters* ctr) {
+  __shared__ ReserveSmem sm;
+  unsigned long long count = 0, cyc = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+  for (uint64_t base = (uint64_t)blockIdx.x * BLOCK; base < n; base += stride) {
+    const uint64_t i = base + threadIdx.x;
+    bool unf = false;
+    uint32_t t = 0;
+    if (i < n) {
+      t = list[i];
+      LRNode st = lr[t];
+      bool dirty = false;
+      for (int e = 0; e < 2; ++e) {
+        unsigned long long m = st.mail[e];
+        if (!m) continue;
+        dirty = true;
+        st.mail[e] = 0ULL;
+        uint32_t yd = (uint32_t)(m >> 32), yp = (uint32_t)m;
+        uint32_t dist = cdist(yd), distx = cdist(st.d[e]);
+        /* chain_update (debruijn_chain_operations.hpp:192-198, :245-251) */
+        if (points_to_chain(st.d[e])) {
+          if (dist > distx) st.p[e] = yp;
+          if (dist >= distx) {
+            st.d[e] = yd;
+            ++count;
+          }
+        }
+      }
+      if (dirty) lr[t] = st;
+      unf = is_unfinished(st.d[0], st.d[1]);
+      if (unf && cdist(st.d[0]) == maxd && cdist(st.d[1]) == maxd) ++cyc;
+    }
+    unsigned long long pos = block_reserve(unf, &ctr->lr_next, sm);
+    if (unf) next[pos] = t;
+  }
+  block_add(count, &ctr->lr_count);
+  __syncthreads();
+  block_add(cyc, &ctr->lr_cycle);
+}
+
+/* separate_cycles (when unfinished nodes remain) and separate_isolated. */
+__global__ __launch_bounds__(BLOCK) void k_separate(const LRNode* lr, uint64_t n_chain, int do_cycles, uint32_t maxd,
+                                                    uint8_t* alive, Counters* ctr) {
+  unsigned long long nc = 0, ni = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+  for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < n_chain; i += stride) {
+    uint32_t din = lr[i].d[0], dout = lr[i].d[1];
+    uint8_t a = 1;
+    if (do_cycles && cdist(din) == maxd && cdist(dout) == maxd) {
+      a = 0;
+      ++nc;
+    } else if (points_to_self(din) && points_to_self(dout)) {
+      a = 0;
+      ++ni;
+    }
+    alive[i] = a;
+  }
+  block_add(nc, &ctr->n_cycle);
+  __syncthreads();
+  block_add(ni, &ctr->n_isolated);
+}
+
+/* ================================================================== */
+/* output                                                               */
+/* ================================================================== */
+/* k-mer on the pointing node's strand from a pointer. */
+template <typename KT>
+__device__ __forceinline__ KT ptr_kmer(const kops<KT>& ops, uint32_t ptr, const KT* ckmer, uint64_t n_chain,
+                                       const KT* vkmer, bool in_side) {
+  uint32_t j = ptr & IDX_MASK;
+  KT c = (j < n_chain) ? ckmer[j] : vkmer[j - n_chain];
+  bool out_end = (ptr & END_OUT) != 0;
+  /* in side: the target is on our strand iff its OUT end faces us;
+   * out side: iff its IN end faces us. */
+  bool same = in_side ? out_end : !out_end;
+  return same ? c : ops.rc(c);
+}
+
+/* to_listranked_chain_node (debruijn_chain_operations.hpp:261-310) */
+template <typename KT>
+__global__ __launch_bounds__(BLOCK) void k_ranked(const LRNode* lr, const KT* ckmer, uint64_t n_chain, const KT* vkmer,
+                                                  const uint8_t* alive, int K, KT* out_k, KT* out_rep,
+                                                  uint32_t* out_dist, Counters* ctr) {
+  __shared__ ReserveSmem sm;
+  kops<KT> ops(K);
+  const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+  for (uint64_t base = (uint64_t)blockIdx.x * BLOCK; base < n_chain; base += stride) {
+    const uint64_t i = base + threadIdx.x;
+    bool a = (i < n_chain) && alive[i];
+    KT Kv = 0, rep = 0;
+    uint32_t dist = 0;
+    if (a) {
+      const LRNode st = lr[i];
+      KT x = ckmer[i];
+      KT L = is_terminal(st.d[0]) ? x : ptr_kmer<KT>(ops, st.p[0], ckmer, n_chain, vkmer, true);
+      KT R = is_terminal(st.d[1]) ? ops.rc(x) : ops.rc(ptr_kmer<KT>(ops, st.p[1], ckmer, n_chain, vkmer, false));
+      bool sense = !(R < L);
+      Kv = sense ? x : ops.rc(x);
+      rep = sense ? L : R;
+      dist = cdist(sense ? st.d[0] : st.d[1]);
+    }
+    unsigned long long pos = block_reserve(a, &ctr->n_out, sm);
+    if (a) {
+      out_k[pos] = Kv;
+      out_rep[pos] = rep;
+      out_dist[pos] = dist;
+    }
+  }
+}
+
+template <typename KT>
+__global__ void k_gather_key(const KT* src, const uint32_t* perm, uint64_t* dst, uint64_t n, int word) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    KT v = src[perm[i]];
+    dst[i] = word ? (uint64_t)((u128)v >> 64) : (uint64_t)v;
+  }
+}
+template <typename KT>
+__global__ void k_gather_kmer(const KT* src, const uint32_t* perm, KT* dst, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[perm[i]];
+}
+__global__ void k_gather_u32(const uint32_t* src, const uint32_t* perm, uint32_t* dst, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[perm[i]];
+}
+__global__ void k_iota(uint32_t* p, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = (uint32_t)i;
+}
+
+template <typename KT>
+__device__ __forceinline__ void write_kmer(char* out, KT v, int K) {
+  const char A[4] = {'A', 'C', 'G', 'T'};
+  for (int i = 0; i < K; ++i) out[i] = A[(int)((v >> (2 * (K - 1 - i))) & 3)];
+}
+
+/* bytes each sorted chain node contributes to .chain.fasta */
+__global__ void k_fasta_len(const uint32_t* dist_sorted, uint64_t n, int K, uint64_t* len) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) len[i] = dist_sorted[i] == 0 ? (uint64_t)(2 * K + 3) : 1ULL;
+}
+
+/* print_chain_as_fasta (debruijn_chain_operations.hpp:352-385): a node at
+ * dist 0 starts a record "\n>" canonical(K) "\n" rep; every other node adds
+ * the last base of K (its k-mer on the rep strand). */
+template <typename KT>
+__global__ void k_fasta_write(const uint32_t* perm, const KT* out_k, const KT* out_rep, const uint32_t* dist_sorted,
+                              const uint64_t* off, uint64_t n, int K, char* buf) {
+  kops<KT> ops(K);
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t m = perm[i];
+  KT Kv = out_k[m];
+  char* o = buf + off[i];
+  if (dist_sorted[i] == 0) {
+    o[0] = '\n';
+    o[1] = '>';
+    write_kmer<KT>(o + 2, ops.canon(Kv), K);
+    o[2 + K] = '\n';
+    write_kmer<KT>(o + 3 + K, out_rep[m], K);
+  } else {
+    const char A[4] = {'A', 'C', 'G', 'T'};
+    o[0] = A[(int)(Kv & 3)];
+  }
+}
+
+/* print_graph_node_fasta (debruijn_graph_operations.hpp:81-90) */
+template <typename KT>
+__global__ void k_branch_fasta(const KT* sorted, uint64_t n, int K, char* buf) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  char* o = buf + i * (uint64_t)(2 * K + 3);
+  o[0] = '>';
+  write_kmer<KT>(o + 1, sorted[i], K);
+  o[1 + K] = '\n';
+  write_kmer<KT>(o + 2 + K, sorted[i], K);
+  o[2 + 2 * K] = '\n';
+}
+
+/* ================================================================== */
+/* getters                                                              */
+/* ================================================================== */
+template <typename KT>
+__device__ __forceinline__ void put_kmer(bruno_node& o, KT k) {
+  o.kmer[0] = (uint64_t)k;
+  o.kmer[1] = (uint64_t)((u128)k >> 64);
+}
+
+/* P1 counters (before filter) */
+template <typename KT>
+__global__ __launch_bounds__(BLOCK) void k_get_counts(const Slot<KT>* T, uint64_t cap, const Spill* sp, uint64_t spcap,
+                                                      bruno_node* out, uint64_t out_cap, Counters* ctr) {
+  __shared__ ReserveSmem sm;
+  const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+  for (uint64_t base = (uint64_t)blockIdx.x * BLOCK; base < cap; base += stride) {
+    const uint64_t s = base + threadIdx.x;
+    bool occ = (s < cap) && SlotOps<KT>::occupied(T + s);
+    bruno_node nd;
+    if (occ) {
+      K2List L;
+      gather_k2<KT>(T + s, s, sp, spcap, L);
+      p1_counts(L, nd.counts);
+      put_kmer<KT>(nd, SlotOps<KT>::kmer(T + s));
+      nd.pad[0] = nd.pad[1] = nd.pad[2] = 0;
+    }
+    unsigned long long pos = block_reserve(occ, &ctr->n_out, sm);
+    if (occ) {
+      if (pos < out_cap) out[pos] = nd;
+      else atomicOr(&ctr->err, ERR_LIST_FULL);
+    }
+  }
+}
+
+/* k2 list (before filter): one thread per slot, up to 25 entries */
+template <typename KT>
+__global__ __launch_bounds__(BLOCK) void k_get_k2(const Slot<KT>* T, uint64_t cap, const Spill* sp, uint64_t spcap,
+                                                  uint64_t* kmers, uint8_t* biedges, uint32_t* counts, uint64_t out_cap,
+                                                  Counters* ctr) {
+  __shared__ ReserveSmem sm;
+  const int W = kt_traits<KT>::words;
+  const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+  for (uint64_t base = (uint64_t)blockIdx.x * BLOCK; base < cap; base += stride) {
+    const uint64_t s = base + threadIdx.x;
+    bool occ = (s < cap) && SlotOps<KT>::occupied(T + s);
+    K2List L;
+    L.n = 0;
+    if (occ) gather_k2<KT>(T + s, s, sp, spcap, L);
+    unsigned long long pos = block_reserve((uint32_t)L.n, &ctr->n_out, sm);
+    if (occ && kmers) {
+      KT k = SlotOps<KT>::kmer(T + s);
+      for (int j = 0; j < L.n; ++j) {
+        if (pos + j >= out_cap) {
+          atomicOr(&ctr->err, ERR_LIST_FULL);
+          break;
+        }
+        kmers[(pos + j) * W] = (uint64_t)k;
+        if (W == 2) kmers[(pos + j) * W + 1] = (uint64_t)((u128)k >> 64);
+        biedges[pos + j] = L.e[j];
+        counts[pos + j] = L.c[j];
+      }
+    }
+  }
+}
+
+template <typename KT>
+__global__ void k_get_nodes(const Slot<KT>* T, const uint64_t* present, uint64_t n, bruno_node* out) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const Slot<KT>* p = T + present[i];
+  const NodeRec* nr = (const NodeRec*)&p->sub[0];
+  bruno_node nd;
+  put_kmer<KT>(nd, SlotOps<KT>::kmer(p));
+  for (int q = 0; q < 9; ++q) nd.counts[q] = nr->c[q];
+  nd.pad[0] = nd.pad[1] = nd.pad[2] = 0;
+  out[i] = nd;
+}
+
+template <typename KT>
+__global__ void k_get_chain_nodes(const uint32_t* perm, const KT* out_k, const KT* out_rep, const uint32_t* dist_sorted,
+                                  uint64_t n, bruno_chain_node* out) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t m = perm[i];
+  bruno_chain_node c;
+  c.k[0] = (uint64_t)out_k[m];
+  c.k[1] = (uint64_t)((u128)out_k[m] >> 64);
+  c.rep[0] = (uint64_t)out_rep[m];
+  c.rep[1] = (uint64_t)((u128)out_rep[m] >> 64);
+  c.dist = (int32_t)dist_sorted[i];
+  c.pad = 0;
+  out[i] = c;
+}
+
+__global__ void k_synth(br::synth_params p, char* bases, uint64_t* offsets) {
+  uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < p.n_reads) {
+    make_read(p, r, bases + r * p.L);
+    offsets[r] = r * p.L;
+  }
+  if (r == 0) offsets[p.n_reads] = p.n_reads * p.L;
+}
+
+/* ================================================================== */
+/* host side                                                            */
+/* ================================================================== */
+inline unsigned grid_for(uint64_t n, unsigned block = BLOCK, unsigned maxg = 1u << 16) {
+  uint64_t g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > maxg) g = maxg;
+  return (unsigned)g;
+}
+
+enum Stage { ST_CREATED = 0, ST_COUNTED = 1, ST_FILTERED = 2, ST_CHAINS = 3, ST_RANKED = 4, ST_COMPACTED = 5 };
+
+}  // namespace
+
+struct bruno_job {
+  int K = 31;
+  int words = 1;
+  int thresholding = 0;
+  Thresh th{};
+  double load_factor = 0.5;
+  hipStream_t stream = nullptr;
+  int stage = ST_CREATED;
+
+  uint64_t cap = 0, spcap = 0;
+  void* table = nullptr;
+  Spill* spill = nullptr;
+  Counters* ctr = nullptr;      /* device */
+  Counters* hctr = nullptr;     /* pinned host mirror */
+  uint64_t device_bytes = 0;
+  uint64_t n_reads = 0;
+
+  /* after filter */
+  uint64_t* present = nullptr;
+  uint64_t n_present = 0, n_branch = 0, n_chain = 0, n_virt = 0;
+  uint64_t* chain_slot = nullptr;
+  uint64_t* branch_slot = nullptr;
+
+  /* chains */
+  void* ckmer = nullptr;
+  void* vkmer = nullptr;
+  void* bkmer = nullptr;
+  LRNode* lr = nullptr;
+  uint32_t* list[2] = {nullptr, nullptr};
+  uint8_t* alive = nullptr;
+  uint64_t vcap = 0;
+  uint32_t iterations = 0;
+  uint64_t n_cycle = 0, n_isolated = 0, conflicts = 0;
+
+  /* output */
+  uint64_t n_out = 0, n_unitigs = 0;
+  void* out_k = nullptr;
+  void* out_rep = nullptr;
+  uint32_t* out_dist = nullptr;
+  uint32_t* perm = nullptr;
+  uint32_t* dist_sorted = nullptr;
+  char* fasta = nullptr;
+  uint64_t fasta_len = 0;
+  void* bsorted = nullptr;
+  char* bfasta = nullptr;
+  uint64_t bfasta_len = 0;
+
+  hipEvent_t ev[12];
+  bool ev_ok = false;
+  bruno_phase_ms ms{};
+
+  std::vector<void*> allocs;
+};
+
+namespace {
+
+bruno_status dev_alloc(bruno_job* j, void** p, size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  if (hipMalloc(p, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    *p = nullptr;
+    return BRUNO_OOM;
+  }
+  j->allocs.push_back(*p);
+  j->device_bytes += bytes;
+  return BRUNO_OK;
+}
+void dev_free(bruno_job* j, void*& p, size_t bytes) {
+  if (!p) return;
+  (void)hipFree(p);
+  auto it = std::find(j->allocs.begin(), j->allocs.end(), p);
+  if (it != j->allocs.end()) j->allocs.erase(it);
+  j->device_bytes -= std::min<uint64_t>(j->device_bytes, bytes);
+  p = nullptr;
+}
+
+#define HIP_CHECK(x)                               \
+  do {                                             \
+    if ((x) != hipSuccess) {                       \
+      (void)hipGetLastError();                     \
+      return BRUNO_INTERNAL;                       \
+    }                                              \
+  } while (0)
+
+#define TRY(x)                                     \
+  do {                                             \
+    bruno_status _s = (x);                         \
+    if (_s != BRUNO_OK) return _s;                 \
+  } while (0)
+
+bruno_status sync_counters(bruno_job* j) {
+  HIP_CHECK(hipMemcpyAsync(j->hctr, j->ctr, sizeof(Counters), hipMemcpyDeviceToHost, j->stream));
+  HIP_CHECK(hipStreamSynchronize(j->stream));
+  if (j->hctr->err & (ERR_TABLE_FULL | ERR_SPILL_FULL | ERR_LIST_FULL)) return BRUNO_CAPACITY;
+  if (j->hctr->err) return BRUNO_INTERNAL;
+  return BRUNO_OK;
+}
+
+void fill64(bruno_job* j, void* p, uint64_t n_words, uint64_t v) {
+  hipLaunchKernelGGL(k_fill64, dim3(grid_for(n_words, BLOCK, 1u << 14)), dim3(BLOCK), 0, j->stream, (uint64_t*)p,
+                     n_words, v);
+}
+
+void ev_record(bruno_job* j, int i) {
+  if (j->ev_ok) (void)hipEventRecord(j->ev[i], j->stream);
+}
+double ev_ms(bruno_job* j, int a, int b) {
+  float ms = 0;
+  if (!j->ev_ok) return 0;
+  if (hipEventElapsedTime(&ms, j->ev[a], j->ev[b]) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return ms;
+}
+
+bruno_status alloc_table(bruno_job* j, uint64_t cap) {
+  j->cap = cap;
+  size_t slot = j->words == 1 ? sizeof(Slot<uint64_t>) : sizeof(Slot<u128>);
+  TRY(dev_alloc(j, &j->table, cap * slot));
+  j->spcap = std::max<uint64_t>(1024, cap / 8);
+  TRY(dev_alloc(j, (void**)&j->spill, j->spcap * sizeof(Spill)));
+  fill64(j, j->table, cap * slot / 8, EMPTY64);
+  fill64(j, j->spill, j->spcap * sizeof(Spill) / 8, 0);
+  return BRUNO_OK;
+}
+
+template <typename KT>
+bruno_status parse_count_t(bruno_job* j, const char* d_bases, const uint64_t* d_offsets, uint64_t n_reads) {
+  ParseArgs a{};
+  a.bases = (const uint8_t*)d_bases;
+  a.offsets = d_offsets;
+  a.n_reads = n_reads;
+  a.K = j->K;
+  a.table = j->table;
+  a.cap = j->cap;
+  a.spill = j->spill;
+  a.spcap = j->spcap;
+  a.ctr = j->ctr;
+  uint64_t waves = (n_reads + 3) / 4;
+  unsigned g = grid_for(waves, 1, 8192);
+  hipLaunchKernelGGL((k_parse<KT, 2>), dim3(g), dim3(BLOCK), 0, j->stream, a);
+  return BRUNO_OK;
+}
+
+template <typename KT>
+bruno_status filter_t(bruno_job* j) {
+  HIP_CHECK(hipMemsetAsync(&j->ctr->n_present, 0, 3 * sizeof(unsigned long long), j->stream));
+  /* present list: at most the distinct k-mers */
+  uint64_t n_distinct = j->hctr->n_distinct;
+  TRY(dev_alloc(j, (void**)&j->present, std::max<uint64_t>(1, n_distinct) * sizeof(uint64_t)));
+  hipLaunchKernelGGL((k_filter<KT>), dim3(grid_for(j->cap, BLOCK, 1u << 15)), dim3(BLOCK), 0, j->stream,
+                     (Slot<KT>*)j->table, j->cap, (const Spill*)j->spill, j->spcap, j->K, j->thresholding, j->th,
+                     j->present, std::max<uint64_t>(1, n_distinct), j->ctr);
+  TRY(sync_counters(j));
+  j->n_present = j->hctr->n_present;
+  j->n_branch = j->hctr->n_branch;
+  j->n_chain = j->hctr->n_chain;
+  TRY(dev_alloc(j, (void**)&j->chain_slot, std::max<uint64_t>(1, j->n_chain) * sizeof(uint64_t)));
+  TRY(dev_alloc(j, (void**)&j->branch_slot, std::max<uint64_t>(1, j->n_branch) * sizeof(uint64_t)));
+  /* lr_count / lr_next reused as list counters */
+  j->hctr->lr_count = 0;
+  j->hctr->lr_next = 0;
+  HIP_CHECK(hipMemcpyAsync(&j->ctr->lr_count, &j->hctr->lr_count, 2 * sizeof(unsigned long long),
+                           hipMemcpyHostToDevice, j->stream));
+  hipLaunchKernelGGL((k_split<KT>), dim3(grid_for(j->n_present)), dim3(BLOCK), 0, j->stream, (Slot<KT>*)j->table,
+                     j->present, j->n_present, j->chain_slot, j->branch_slot, j->ctr);
+  return sync_counters(j);
+}
+
+template <typename KT>
+bruno_status chains_t(bruno_job* j) {
+  uint64_t nc = j->n_chain;
+  TRY(dev_alloc(j, &j->ckmer, std::max<uint64_t>(1, nc) * sizeof(KT)));
+  j->vcap = 2 * nc + 16;
+  TRY(dev_alloc(j, &j->vkmer, j->vcap * sizeof(KT)));
+  TRY(dev_alloc(j, &j->bkmer, std::max<uint64_t>(1, j->n_branch) * sizeof(KT)));
+  TRY(dev_alloc(j, (void**)&j->lr, std::max<uint64_t>(1, nc) * sizeof(LRNode)));
+  hipLaunchKernelGGL((k_chain_init<KT>), dim3(grid_for(nc)), dim3(BLOCK), 0, j->stream, (const Slot<KT>*)j->table,
+                     j->cap, j->K, j->chain_slot, nc, (KT*)j->ckmer, j->lr, (KT*)j->vkmer, j->vcap, j->ctr);
+  hipLaunchKernelGGL((k_termini<KT>), dim3(grid_for(j->n_branch)), dim3(BLOCK), 0, j->stream,
+                     (const Slot<KT>*)j->table, j->cap, j->K, j->branch_slot, j->n_branch, j->lr, (KT*)j->bkmer);
+  TRY(sync_counters(j));
+  j->n_virt = j->hctr->n_virt;
+  if (nc + j->n_virt >= (1ULL << 31)) return BRUNO_CAPACITY;
+  return BRUNO_OK;
+}
+
+bruno_status list_rank_impl(bruno_job* j, int variant) {
+  uint64_t nc = j->n_chain;
+  TRY(dev_alloc(j, (void**)&j->list[0], std::max<uint64_t>(1, nc) * sizeof(uint32_t)));
+  TRY(dev_alloc(j, (void**)&j->list[1], std::max<uint64_t>(1, nc) * sizeof(uint32_t)));
+  TRY(dev_alloc(j, (void**)&j->alive, std::max<uint64_t>(1, nc)));
+  j->hctr->lr_next = 0;
+  j->hctr->lr_count = 0;
+  j->hctr->lr_cycle = 0;
+  j->hctr->lr_conflict = 0;
+  HIP_CHECK(hipMemcpyAsync(&j->ctr->lr_count, &j->hctr->lr_count, 4 * sizeof(unsigned long long),
+                           hipMemcpyHostToDevice, j->stream));
+  hipLaunchKernelGGL(k_lr_init_list, dim3(grid_for(nc)), dim3(BLOCK), 0, j->stream, j->lr, nc, j->list[0], j->ctr);
+  TRY(sync_counters(j));
+  uint64_t n = j->hctr->lr_next;
+  uint32_t iters = 0;
+  bool done = (n == 0);
+  int cur = 0;
+  uint64_t conflicts = 0;
+  while (!done) {
+    j->hctr->lr_count = 0;
+    j->hctr->lr_next = 0;
+    j->hctr->lr_cycle = 0;
+    j->hctr->lr_conflict = 0;
+    HIP_CHECK(hipMemcpyAsync(&j->ctr->lr_count, &j->hctr->lr_count, 4 * sizeof(unsigned long long),
+                             hipMemcpyHostToDevice, j->stream));
+    hipLaunchKernelGGL(k_lr_send, dim3(grid_for(n)), dim3(BLOCK), 0, j->stream, j->lr, nc, j->list[cur], n, variant,
+                       j->ctr);
+    /* after this round the iteration count is iters+1 and IsCycleNode
+     * compares against 1 << (iters+1) (debruijn_chain_filters.hpp:189-213) */
+    uint32_t maxd = 1u << (iters + 1);
+    hipLaunchKernelGGL(k_lr_recv, dim3(grid_for(n)), dim3(BLOCK), 0, j->stream, j->lr, j->list[cur], n,
+                       j->list[cur ^ 1], maxd, j->ctr);
+    TRY(sync_counters(j));
+    uint64_t count = j->hctr->lr_count;
+    n = j->hctr->lr_next;
+    uint64_t cyc = j->hctr->lr_cycle;
+    conflicts += j->hctr->lr_conflict;
+    cur ^= 1;
+    ++iters;
+    if (iters >= 30) done = true;
+    else done = (count == 0) || (cyc == n);
+  }
+  j->iterations = iters;
+  j->conflicts = conflicts;
+  /* separate_cycles only when unfinished nodes remain (:1284-1291) */
+  int do_cycles = (n > 0) ? 1 : 0;
+  uint32_t maxd = (iters < 31) ? (1u << iters) : 0u;
+  j->hctr->n_cycle = 0;
+  j->hctr->n_isolated = 0;
+  HIP_CHECK(hipMemcpyAsync(&j->ctr->n_cycle, &j->hctr->n_cycle, 2 * sizeof(unsigned long long),
+                           hipMemcpyHostToDevice, j->stream));
+  hipLaunchKernelGGL(k_separate, dim3(grid_for(nc)), dim3(BLOCK), 0, j->stream, j->lr, nc, do_cycles, maxd, j->alive,
+                     j->ctr);
+  TRY(sync_counters(j));
+  j->n_cycle = j->hctr->n_cycle;
+  j->n_isolated = j->hctr->n_isolated;
+  return BRUNO_OK;
+}
+
+template <typename KT>
+bruno_status radix_pairs(bruno_job* j, uint64_t* keys_in, uint64_t* keys_out, uint32_t* vals_in, uint32_t* vals_out,
+                         uint64_t n, int end_bit) {
+  size_t tmp = 0;
+  HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, keys_in, keys_out, vals_in, vals_out, (int)n, 0, end_bit,
+                                               j->stream));
+  void* t = nullptr;
+  TRY(dev_alloc(j, &t, tmp));
+  hipError_t e = hipcub::DeviceRadixSort::SortPairs(t, tmp, keys_in, keys_out, vals_in, vals_out, (int)n, 0, end_bit,
+                                                    j->stream);
+  HIP_CHECK(hipStreamSynchronize(j->stream));
+  dev_free(j, t, tmp);
+  HIP_CHECK(e);
+  return BRUNO_OK;
+}
+
+/* stable LSD sort of (key words) producing a permutation of [0, n) */
+template <typename KT>
+bruno_status sort_perm_kmer(bruno_job* j, const KT* keys, uint32_t* perm, uint64_t n, int K, bool have_perm) {
+  if (n == 0) return BRUNO_OK;
+  uint64_t *kin = nullptr, *kout = nullptr;
+  uint32_t* pout = nullptr;
+  TRY(dev_alloc(j, (void**)&kin, n * 8));
+  TRY(dev_alloc(j, (void**)&kout, n * 8));
+  TRY(dev_alloc(j, (void**)&pout, n * 4));
+  if (!have_perm) hipLaunchKernelGGL(k_iota, dim3(grid_for(n, BLOCK, 1u << 30)), dim3(BLOCK), 0, j->stream, perm, n);
+  int W = kt_traits<KT>::words;
+  for (int w = 0; w < W; ++w) {
+    int bits = std::min(64, 2 * K - 64 * w);
+    if (bits <= 0) break;
+    hipLaunchKernelGGL((k_gather_key<KT>), dim3(grid_for(n, BLOCK, 1u << 30)), dim3(BLOCK), 0, j->stream, keys, perm,
+                       kin, n, w);
+    TRY(radix_pairs<KT>(j, kin, kout, perm, pout, n, bits));
+    HIP_CHECK(hipMemcpyAsync(perm, pout, n * 4, hipMemcpyDeviceToDevice, j->stream));
+  }
+  void* a = kin;
+  dev_free(j, a, n * 8);
+  a = kout;
+  dev_free(j, a, n * 8);
+  a = pout;
+  dev_free(j, a, n * 4);
+  return BRUNO_OK;
+}
+
+template <typename KT>
+bruno_status compact_t(bruno_job* j) {
+  uint64_t nc = j->n_chain;
+  int K = j->K;
+  TRY(dev_alloc(j, &j->out_k, std::max<uint64_t>(1, nc) * sizeof(KT)));
+  TRY(dev_alloc(j, &j->out_rep, std::max<uint64_t>(1, nc) * sizeof(KT)));
+  TRY(dev_alloc(j, (void**)&j->out_dist, std::max<uint64_t>(1, nc) * 4));
+  TRY(dev_alloc(j, (void**)&j->perm, std::max<uint64_t>(1, nc) * 4));
+  TRY(dev_alloc(j, (void**)&j->dist_sorted, std::max<uint64_t>(1, nc) * 4));
+  j->hctr->n_out = 0;
+  HIP_CHECK(hipMemcpyAsync(&j->ctr->n_out, &j->hctr->n_out, sizeof(unsigned long long), hipMemcpyHostToDevice,
+                           j->stream));
+  hipLaunchKernelGGL((k_ranked<KT>), dim3(grid_for(nc)), dim3(BLOCK), 0, j->stream, j->lr, (const KT*)j->ckmer, nc,
+                     (const KT*)j->vkmer, j->alive, K, (KT*)j->out_k, (KT*)j->out_rep, j->out_dist, j->ctr);
+  TRY(sync_counters(j));
+  uint64_t n = j->hctr->n_out;
+  j->n_out = n;
+  /* chain_rep_less: by rep, then |dist| -> LSD: dist first, then rep words */
+  if (n > 0) {
+    uint32_t* pin = nullptr;
+    TRY(dev_alloc(j, (void**)&pin, n * 4));
+    hipLaunchKernelGGL(k_iota, dim3(grid_for(n, BLOCK, 1u << 30)), dim3(BLOCK), 0, j->stream, pin, n);
+    uint32_t* dout = nullptr;
+    TRY(dev_alloc(j, (void**)&dout, n * 4));
+    size_t tmp = 0;
+    HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, j->out_dist, dout, pin, j->perm, (int)n, 0, 32,
+                                                 j->stream));
+    void* t = nullptr;
+    TRY(dev_alloc(j, &t, tmp));
+    HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(t, tmp, j->out_dist, dout, pin, j->perm, (int)n, 0, 32, j->stream));
+    HIP_CHECK(hipStreamSynchronize(j->stream));
+    dev_free(j, t, tmp);
+    void* a = dout;
+    dev_free(j, a, n * 4);
+    a = pin;
+    dev_free(j, a, n * 4);
+    TRY(sort_perm_kmer<KT>(j, (const KT*)j->out_rep, j->perm, n, K, true));
+    hipLaunchKernelGGL(k_gather_u32, dim3(grid_for(n, BLOCK, 1u << 30)), dim3(BLOCK), 0, j->stream, j->out_dist,
+                       j->perm, j->dist_sorted, n);
+    /* .chain.fasta */
+    uint64_t *len = nullptr, *off = nullptr;
+    TRY(dev_alloc(j, (void**)&len, n * 8));
+    TRY(dev_alloc(j, (void**)&off, (n + 1) * 8));
+    hipLaunchKernelGGL(k_fasta_len, dim3(grid_for(n, BLOCK, 1u << 30)), dim3(BLOCK), 0, j->stream, j->dist_sorted, n,
+                       K, len);
+    tmp = 0;
+    HIP_CHECK(hipcub::DeviceScan::InclusiveSum(nullptr, tmp, len, off + 1, (int)n, j->stream));
+    TRY(dev_alloc(j, &t, tmp));
+    HIP_CHECK(hipMemsetAsync(off, 0, 8, j->stream));
+    HIP_CHECK(hipcub::DeviceScan::InclusiveSum(t, tmp, len, off + 1, (int)n, j->stream));
+    uint64_t total = 0;
+    HIP_CHECK(hipMemcpyAsync(&total, off + n, 8, hipMemcpyDeviceToHost, j->stream));
+    HIP_CHECK(hipStreamSynchronize(j->stream));
+    dev_free(j, t, tmp);
+    TRY(dev_alloc(j, (void**)&j->fasta, total + 1));
+    hipLaunchKernelGGL((k_fasta_write<KT>), dim3(grid_for(n, BLOCK, 1u << 30)), dim3(BLOCK), 0, j->stream, j->perm,
+                       (const KT*)j->out_k, (const KT*)j->out_rep, j->dist_sorted, off, n, K, j->fasta);
+    j->fasta_len = total;
+    a = len;
+    dev_free(j, a, n * 8);
+    a = off;
+    dev_free(j, a, (n + 1) * 8);
+  } else {
+    j->fasta_len = 0;
+    TRY(dev_alloc(j, (void**)&j->fasta, 16));
+  }
+  /* .branch.fasta: sorted branch k-mers */
+  uint64_t nb = j->n_branch;
+  TRY(dev_alloc(j, &j->bsorted, std::max<uint64_t>(1, nb) * sizeof(KT)));
+  j->bfasta_len = nb * (uint64_t)(2 * K + 3);
+  TRY(dev_alloc(j, (void**)&j->bfasta, j->bfasta_len + 1));
+  if (nb > 0) {
+    uint32_t* bp = nullptr;
+    TRY(dev_alloc(j, (void**)&bp, nb * 4));
+    TRY(sort_perm_kmer<KT>(j, (const KT*)j->bkmer, bp, nb, K, false));
+    hipLaunchKernelGGL((k_gather_kmer<KT>), dim3(grid_for(nb, BLOCK, 1u << 30)), dim3(BLOCK), 0, j->stream,
+                       (const KT*)j->bkmer, bp, (KT*)j->bsorted, nb);
+    hipLaunchKernelGGL((k_branch_fasta<KT>), dim3(grid_for(nb, BLOCK, 1u << 30)), dim3(BLOCK), 0, j->stream,
+                       (const KT*)j->bsorted, nb, K, j->bfasta);
+    void* a = bp;
+    dev_free(j, a, nb * 4);
+  }
+  HIP_CHECK(hipStreamSynchronize(j->stream));
+  /* the file drops the first character (compact_dbg_io.hpp:351-355) */
+  return BRUNO_OK;
+}
+
+}  // namespace
+
+
+/* ================================================================== */
+/* C ABI                                                                */
+/* ================================================================== */
+namespace {
+bool valid_job(const bruno_job* j) { return j != nullptr; }
+}  // namespace
+
+extern "C" {
+
+const char* bruno_status_string(bruno_status s) {
+  switch (s) {
+    case BRUNO_OK: return "ok";
+    case BRUNO_BAD_INPUT: return "bad input";
+    case BRUNO_OOM: return "device out of memory";
+    case BRUNO_CAPACITY: return "capacity exceeded (hash table, side table or list)";
+    case BRUNO_INTERNAL: return "internal device error";
+    case BRUNO_STATE: return "call out of order";
+    case BRUNO_NO_DEVICE: return "no HIP device";
+  }
+  return "unknown";
+}
+
+bruno_status bruno_job_create(const bruno_job_desc* d, bruno_job** out) {
+  if (!d || !out) return BRUNO_BAD_INPUT;
+  *out = nullptr;
+  if (d->k < 3 || d->k > 63) return BRUNO_BAD_INPUT;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    (void)hipGetLastError();
+    return BRUNO_NO_DEVICE;
+  }
+  bruno_job* j = new (std::nothrow) bruno_job();
+  if (!j) return BRUNO_OOM;
+  j->K = d->k;
+  j->words = d->k <= 31 ? 1 : 2;
+  j->thresholding = d->thresholding ? 1 : 0;
+  j->th = Thresh{d->thresholds[0], d->thresholds[1], d->thresholds[2],
+                 d->thresholds[3], d->thresholds[4], d->thresholds[5]};
+  j->load_factor = (d->load_factor > 0 && d->load_factor < 1) ? d->load_factor : 0.5;
+  j->stream = (hipStream_t)d->stream;
+  j->cap = d->capacity;
+  if (hipMalloc((void**)&j->ctr, sizeof(Counters)) != hipSuccess ||
+      hipHostMalloc((void**)&j->hctr, sizeof(Counters), hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    if (j->ctr) (void)hipFree(j->ctr);
+    delete j;
+    return BRUNO_OOM;
+  }
+  memset(j->hctr, 0, sizeof(Counters));
+  if (hipMemsetAsync(j->ctr, 0, sizeof(Counters), j->stream) != hipSuccess) {
+    (void)hipGetLastError();
+  }
+  j->ev_ok = true;
+  for (int i = 0; i < 12; ++i)
+    if (hipEventCreate(&j->ev[i]) != hipSuccess) {
+      (void)hipGetLastError();
+      j->ev_ok = false;
+    }
+  if (j->cap > 0) {
+    bruno_status s = alloc_table(j, j->cap);
+    if (s != BRUNO_OK) {
+      bruno_job_destroy(j);
+      return s;
+    }
+  }
+  *out = j;
+  return BRUNO_OK;
+}
+
+bruno_status bruno_job_destroy(bruno_job* j) {
+  if (!j) return BRUNO_BAD_INPUT;
+  (void)hipStreamSynchronize(j->stream);
+  for (void* p : j->allocs) (void)hipFree(p);
+  j->allocs.clear();
+  if (j->ctr) (void)hipFree(j->ctr);
+  if (j->hctr) (void)hipHostFree(j->hctr);
+  if (j->ev_ok)
+    for (int i = 0; i < 12; ++i) (void)hipEventDestroy(j->ev[i]);
+  delete j;
+  return BRUNO_OK;
+}
+
+bruno_status bruno_job_reset(bruno_job* j) {
+  if (!valid_job(j)) return BRUNO_BAD_INPUT;
+  HIP_CHECK(hipStreamSynchronize(j->stream));
+  void* keep_t = j->table;
+  void* keep_s = j->spill;
+  for (void* p : j->allocs)
+    if (p != keep_t && p != keep_s) (void)hipFree(p);
+  j->allocs.clear();
+  uint64_t tb = 0;
+  if (keep_t) {
+    j->allocs.push_back(keep_t);
+    j->allocs.push_back(keep_s);
+    tb = j->cap * (j->words == 1 ? sizeof(Slot<uint64_t>) : sizeof(Slot<u128>)) + j->spcap * sizeof(Spill);
+  }
+  j->device_bytes = tb;
+  j->present = j->chain_slot = j->branch_slot = nullptr;
+  j->ckmer = j->vkmer = j->bkmer = j->out_k = j->out_rep = j->bsorted = nullptr;
+  j->lr = nullptr;
+  j->list[0] = j->list[1] = nullptr;
+  j->alive = nullptr;
+  j->out_dist = j->perm = j->dist_sorted = nullptr;
+  j->fasta = j->bfasta = nullptr;
+  j->n_present = j->n_branch = j->n_chain = j->n_virt = 0;
+  j->iterations = 0;
+  j->n_cycle = j->n_isolated = j->conflicts = 0;
+  j->n_out = j->n_unitigs = j->fasta_len = j->bfasta_len = 0;
+  j->n_reads = 0;
+  j->ms = bruno_phase_ms{};
+  memset(j->hctr, 0, sizeof(Counters));
+  HIP_CHECK(hipMemsetAsync(j->ctr, 0, sizeof(Counters), j->stream));
+  if (keep_t) {
+    ev_record(j, 10);
+    size_t slot = j->words == 1 ? sizeof(Slot<uint64_t>) : sizeof(Slot<u128>);
+    fill64(j, j->table, j->cap * slot / 8, EMPTY64);
+    fill64(j, j->spill, j->spcap * sizeof(Spill) / 8, 0);
+    ev_record(j, 11);
+  }
+  HIP_CHECK(hipStreamSynchronize(j->stream));
+  j->stage = ST_CREATED;
+  return BRUNO_OK;
+}
+
+bruno_status bruno_parse_count(bruno_job* j, const char* d_bases, const uint64_t* d_offsets, uint64_t n_reads) {
+  if (!valid_job(j)) return BRUNO_BAD_INPUT;
+  if (j->stage > ST_COUNTED) return BRUNO_STATE;
+  if (n_reads == 0) {
+    j->stage = ST_COUNTED;
+    return BRUNO_OK;
+  }
+  if (!d_bases || !d_offsets) return BRUNO_BAD_INPUT;
+  if (j->cap == 0) {
+    /* size from the batch: instances <= bases; distinct <= instances */
+    uint64_t ends[2];
+    HIP_CHECK(hipMemcpyAsync(&ends[0], d_offsets, 8, hipMemcpyDeviceToHost, j->stream));
+    HIP_CHECK(hipMemcpyAsync(&ends[1], d_offsets + n_reads, 8, hipMemcpyDeviceToHost, j->stream));
+    HIP_CHECK(hipStreamSynchronize(j->stream));
+    uint64_t nb = ends[1] - ends[0];
+    uint64_t want = (uint64_t)((double)nb / j->load_factor) + 1024;
+    TRY(alloc_table(j, want));
+  }
+  ev_record(j, 0);
+  if (j->words == 1) TRY(parse_count_t<uint64_t>(j, d_bases, d_offsets, n_reads));
+  else TRY(parse_count_t<u128>(j, d_bases, d_offsets, n_reads));
+  ev_record(j, 1);
+  TRY(sync_counters(j));
+  j->ms.parse_count_ms += ev_ms(j, 0, 1);
+  j->n_reads += n_reads;
+  if (j->hctr->n_instances >= (1ULL << 32)) return BRUNO_CAPACITY; /* uint32 k2 counts */
+  j->stage = ST_COUNTED;
+  return BRUNO_OK;
+}
+
+bruno_status bruno_parse_count_host(bruno_job* j, const char* bases, const uint64_t* offsets, uint64_t n_reads) {
+  if (!valid_job(j) || (n_reads && (!bases || !offsets))) return BRUNO_BAD_INPUT;
+  if (n_reads == 0) return bruno_parse_count(j, nullptr, nullptr, 0);
+  uint64_t nb = offsets[n_reads] - offsets[0];
+  char* db = nullptr;
+  uint64_t* doff = nullptr;
+  if (hipMalloc((void**)&db, nb + 1) != hipSuccess || hipMalloc((void**)&doff, (n_reads + 1) * 8) != hipSuccess) {
+    (void)hipGetLastError();
+    if (db) (void)hipFree(db);
+    return BRUNO_OOM;
+  }
+  std::vector<uint64_t> off(offsets, offsets + n_reads + 1);
+  for (auto& o : off) o -= offsets[0];
+  bruno_status s = BRUNO_OK;
+  if (hipMemcpyAsync(db, bases + offsets[0], nb, hipMemcpyHostToDevice, j->stream) != hipSuccess ||
+      hipMemcpyAsync(doff, off.data(), (n_reads + 1) * 8, hipMemcpyHostToDevice, j->stream) != hipSuccess ||
+      hipStreamSynchronize(j->stream) != hipSuccess) {
+    (void)hipGetLastError();
+    s = BRUNO_INTERNAL;
+  }
+  if (s == BRUNO_OK) s = bruno_parse_count(j, db, doff, n_reads);
+  (void)hipStreamSynchronize(j->stream);
+  (void)hipFree(db);
+  (void)hipFree(doff);
+  return s;
+}
+
+bruno_status bruno_filter(bruno_job* j) {
+  if (!valid_job(j)) return BRUNO_BAD_INPUT;
+  if (j->stage != ST_COUNTED) return BRUNO_STATE;
+  ev_record(j, 2);
+  if (j->words == 1) TRY(filter_t<uint64_t>(j));
+  else TRY(filter_t<u128>(j));
+  ev_record(j, 3);
+  HIP_CHECK(hipStreamSynchronize(j->stream));
+  j->ms.filter_ms = ev_ms(j, 2, 3);
+  j->stage = ST_FILTERED;
+  return BRUNO_OK;
+}
+
+bruno_status bruno_build_chains(bruno_job* j) {
+  if (!valid_job(j)) return BRUNO_BAD_INPUT;
+  if (j->stage != ST_FILTERED) return BRUNO_STATE;
+  ev_record(j, 4);
+  if (j->words == 1) TRY(chains_t<uint64_t>(j));
+  else TRY(chains_t<u128>(j));
+  ev_record(j, 5);
+  HIP_CHECK(hipStreamSynchronize(j->stream));
+  j->ms.chains_ms = ev_ms(j, 4, 5);
+  j->stage = ST_CHAINS;
+  return BRUNO_OK;
+}
+
+bruno_status bruno_list_rank(bruno_job* j, int variant, uint32_t* rounds) {
+  if (!valid_job(j) || (variant != 0 && variant != 1)) return BRUNO_BAD_INPUT;
+  if (j->stage != ST_CHAINS) return BRUNO_STATE;
+  ev_record(j, 6);
+  TRY(list_rank_impl(j, variant));
+  ev_record(j, 7);
+  HIP_CHECK(hipStreamSynchronize(j->stream));
+  j->ms.list_rank_ms = ev_ms(j, 6, 7);
+  if (rounds) *rounds = j->iterations;
+  j->stage = ST_RANKED;
+  return BRUNO_OK;
+}
+
+bruno_status bruno_compact(bruno_job* j) {
+  if (!valid_job(j)) return BRUNO_BAD_INPUT;
+  if (j->stage != ST_RANKED) return BRUNO_STATE;
+  ev_record(j, 8);
+  if (j->words == 1) TRY(compact_t<uint64_t>(j));
+  else TRY(compact_t<u128>(j));
+  ev_record(j, 9);
+  HIP_CHECK(hipStreamSynchronize(j->stream));
+  j->ms.output_ms = ev_ms(j, 8, 9);
+  /* number of unitigs = number of dist-0 nodes = count of records starting
+   * a FASTA record; derived on the host from the byte length */
+  uint64_t n_body = j->n_out;
+  uint64_t per_head = 2 * (uint64_t)j->K + 3;
+  /* fasta_len = heads*per_head + (n - heads)  =>  heads = (len - n)/(per_head-1) */
+  j->n_unitigs = (j->fasta_len >= n_body && per_head > 1) ? (j->fasta_len - n_body) / (per_head - 1) : 0;
+  j->stage = ST_COMPACTED;
+  return BRUNO_OK;
+}
+
+bruno_status bruno_run_pipeline(bruno_job* j, int variant) {
+  TRY(bruno_filter(j));
+  TRY(bruno_build_chains(j));
+  TRY(bruno_list_rank(j, variant, nullptr));
+  return bruno_compact(j);
+}
+
+bruno_status bruno_get_stats(bruno_job* j, bruno_stats* o) {
+  if (!valid_job(j) || !o) return BRUNO_BAD_INPUT;
+  TRY(sync_counters(j));
+  memset(o, 0, sizeof(*o));
+  o->n_reads = j->n_reads;
+  o->n_reads_dropped = j->hctr->n_dropped;
+  o->n_instances = j->hctr->n_instances;
+  o->n_distinct = j->hctr->n_distinct;
+  o->n_k2 = j->hctr->n_k2;
+  o->n_nodes = j->n_present;
+  o->n_branches = j->n_branch;
+  o->n_chain_nodes = j->n_chain;
+  o->n_cycle_nodes = j->n_cycle;
+  o->n_isolated = j->n_isolated;
+  o->n_unitig_nodes = j->n_out;
+  o->n_unitigs = j->n_unitigs;
+  o->iterations = j->iterations;
+  o->n_conflicts = j->conflicts;
+  o->capacity = j->cap;
+  o->device_bytes = j->device_bytes;
+  return BRUNO_OK;
+}
+
+bruno_status bruno_get_phase_ms(bruno_job* j, bruno_phase_ms* o) {
+  if (!valid_job(j) || !o) return BRUNO_BAD_INPUT;
+  *o = j->ms;
+  return BRUNO_OK;
+}
+
+bruno_status bruno_get_counts(bruno_job* j, bruno_node* buf, uint64_t* n) {
+  if (!valid_job(j) || !n) return BRUNO_BAD_INPUT;
+  if (j->stage != ST_COUNTED) return BRUNO_STATE;
+  uint64_t nd = j->hctr->n_distinct;
+  if (!buf) {
+    *n = nd;
+    return BRUNO_OK;
+  }
+  if (*n < nd) return BRUNO_BAD_INPUT;
+  bruno_node* d = nullptr;
+  TRY(dev_alloc(j, (void**)&d, std::max<uint64_t>(1, nd) * sizeof(bruno_node)));
+  j->hctr->n_out = 0;
+  HIP_CHECK(hipMemcpyAsync(&j->ctr->n_out, &j->hctr->n_out, 8, hipMemcpyHostToDevice, j->stream));
+  if (j->words == 1)
+    hipLaunchKernelGGL((k_get_counts<uint64_t>), dim3(grid_for(j->cap, BLOCK, 1u << 15)), dim3(BLOCK), 0, j->stream,
+                       (const Slot<uint64_t>*)j->table, j->cap, (const Spill*)j->spill, j->spcap, d, nd, j->ctr);
+  else
+    hipLaunchKernelGGL((k_get_counts<u128>), dim3(grid_for(j->cap, BLOCK, 1u << 15)), dim3(BLOCK), 0, j->stream,
+                       (const Slot<u128>*)j->table, j->cap, (const Spill*)j->spill, j->spcap, d, nd, j->ctr);
+  HIP_CHECK(hipMemcpyAsync(buf, d, nd * sizeof(bruno_node), hipMemcpyDeviceToHost, j->stream));
+  bruno_status s = sync_counters(j);
+  void* a = d;
+  dev_free(j, a, std::max<uint64_t>(1, nd) * sizeof(bruno_node));
+  *n = nd;
+  return s;
+}
+
+bruno_status bruno_get_k2(bruno_job* j, uint64_t* kmers, uint8_t* biedges, uint32_t* counts, uint64_t* n) {
+  if (!valid_job(j) || !n) return BRUNO_BAD_INPUT;
+  if (j->stage != ST_COUNTED) return BRUNO_STATE;
+  uint64_t nk = j->hctr->n_k2;
+  if (!kmers) {
+    *n = nk;
+    return BRUNO_OK;
+  }
+  if (*n < nk || !biedges || !counts) return BRUNO_BAD_INPUT;
+  uint64_t* dk = nullptr;
+  uint8_t* de = nullptr;
+  uint32_t* dc = nullptr;
+  uint64_t m = std::max<uint64_t>(1, nk);
+  TRY(dev_alloc(j, (void**)&dk, m * 8 * j->words));
+  TRY(dev_alloc(j, (void**)&de, m));
+  TRY(dev_alloc(j, (void**)&dc, m * 4));
+  j->hctr->n_out = 0;
+  HIP_CHECK(hipMemcpyAsync(&j->ctr->n_out, &j->hctr->n_out, 8, hipMemcpyHostToDevice, j->stream));
+  if (j->words == 1)
+    hipLaunchKernelGGL((k_get_k2<uint64_t>), dim3(grid_for(j->cap, BLOCK, 1u << 15)), dim3(BLOCK), 0, j->stream,
+                       (const Slot<uint64_t>*)j->table, j->cap, (const Spill*)j->spill, j->spcap, dk, de, dc, nk, j->ctr);
+  else
+    hipLaunchKernelGGL((k_get_k2<u128>), dim3(grid_for(j->cap, BLOCK, 1u << 15)), dim3(BLOCK), 0, j->stream,
+                       (const Slot<u128>*)j->table, j->cap, (const Spill*)j->spill, j->spcap, dk, de, dc, nk, j->ctr);
+  HIP_CHECK(hipMemcpyAsync(kmers, dk, nk * 8 * j->words, hipMemcpyDeviceToHost, j->stream));
+  HIP_CHECK(hipMemcpyAsync(biedges, de, nk, hipMemcpyDeviceToHost, j->stream));
+  HIP_CHECK(hipMemcpyAsync(counts, dc, nk * 4, hipMemcpyDeviceToHost, j->stream));
+  bruno_status s = sync_counters(j);
+  if (s == BRUNO_OK && j->hctr->n_out != nk) s = BRUNO_INTERNAL;
+  void* a = dk;
+  dev_free(j, a, m * 8 * j->words);
+  a = de;
+  dev_free(j, a, m);
+  a = dc;
+  dev_free(j, a, m * 4);
+  *n = nk;
+  return s;
+}
+
+bruno_status bruno_get_nodes(bruno_job* j, bruno_node* buf, uint64_t* n) {
+  if (!valid_job(j) || !n) return BRUNO_BAD_INPUT;
+  if (j->stage < ST_FILTERED) return BRUNO_STATE;
+  uint64_t np = j->n_present;
+  if (!buf) {
+    *n = np;
+    return BRUNO_OK;
+  }
+  if (*n < np) return BRUNO_BAD_INPUT;
+  bruno_node* d = nullptr;
+  TRY(dev_alloc(j, (void**)&d, std::max<uint64_t>(1, np) * sizeof(bruno_node)));
+  if (np) {
+    if (j->words == 1)
+      hipLaunchKernelGGL((k_get_nodes<uint64_t>), dim3(grid_for(np, BLOCK, 1u << 30)), dim3(BLOCK), 0, j->stream,
+                         (const Slot<uint64_t>*)j->table, j->present, np, d);
+    else
+      hipLaunchKernelGGL((k_get_nodes<u128>), dim3(grid_for(np, BLOCK, 1u << 30)), dim3(BLOCK), 0, j->stream,
+                         (const Slot<u128>*)j->table, j->present, np, d);
+    HIP_CHECK(hipMemcpyAsync(buf, d, np * sizeof(bruno_node), hipMemcpyDeviceToHost, j->stream));
+  }
+  HIP_CHECK(hipStreamSynchronize(j->stream));
+  void* a = d;
+  dev_free(j, a, std::max<uint64_t>(1, np) * sizeof(bruno_node));
+  *n = np;
+  return BRUNO_OK;
+}
+
+bruno_status bruno_get_branches(bruno_job* j, uint64_t* kmers, uint64_t* n) {
+  if (!valid_job(j) || !n) return BRUNO_BAD_INPUT;
+  if (j->stage != ST_COMPACTED) return BRUNO_STATE;
+  uint64_t nb = j->n_branch;
+  if (!kmers) {
+    *n = nb;
+    return BRUNO_OK;
+  }
+  if (*n < nb) return BRUNO_BAD_INPUT;
+  if (nb) {
+    if (j->words == 1) {
+      HIP_CHECK(hipMemcpyAsync(kmers, j->bsorted, nb * 8, hipMemcpyDeviceToHost, j->stream));
+    } else {
+      HIP_CHECK(hipMemcpyAsync(kmers, j->bsorted, nb * 16, hipMemcpyDeviceToHost, j->stream));
+    }
+  }
+  HIP_CHECK(hipStreamSynchronize(j->stream));
+  *n = nb;
+  return BRUNO_OK;
+}
+
+bruno_status bruno_get_chain_nodes(bruno_job* j, bruno_chain_node* buf, uint64_t* n) {
+  if (!valid_job(j) || !n) return BRUNO_BAD_INPUT;
+  if (j->stage != ST_COMPACTED) return BRUNO_STATE;
+  uint64_t m = j->n_out;
+  if (!buf) {
+    *n = m;
+    return BRUNO_OK;
+  }
+  if (*n < m) return BRUNO_BAD_INPUT;
+  if (m) {
+    bruno_chain_node* d = nullptr;
+    TRY(dev_alloc(j, (void**)&d, m * sizeof(bruno_chain_node)));
+    if (j->words == 1)
+      hipLaunchKernelGGL((k_get_chain_nodes<uint64_t>), dim3(grid_for(m, BLOCK, 1u << 30)), dim3(BLOCK), 0, j->stream,
+                         j->perm, (const uint64_t*)j->out_k, (const uint64_t*)j->out_rep, j->dist_sorted, m, d);
+    else
+      hipLaunchKernelGGL((k_get_chain_nodes<u128>), dim3(grid_for(m, BLOCK, 1u << 30)), dim3(BLOCK), 0, j->stream,
+                         j->perm, (const u128*)j->out_k, (const u128*)j->out_rep, j->dist_sorted, m, d);
+    HIP_CHECK(hipMemcpyAsync(buf, d, m * sizeof(bruno_chain_node), hipMemcpyDeviceToHost, j->stream));
+    HIP_CHECK(hipStreamSynchronize(j->stream));
+    void* a = d;
+    dev_free(j, a, m * sizeof(bruno_chain_node));
+  }
+  *n = m;
+  return BRUNO_OK;
+}
+
+static bruno_status get_text(bruno_job* j, const char* dptr, uint64_t len, char* buf, uint64_t* n) {
+  if (!buf) {
+    *n = len;
+    return BRUNO_OK;
+  }
+  if (*n < len) return BRUNO_BAD_INPUT;
+  if (len) HIP_CHECK(hipMemcpyAsync(buf, dptr, len, hipMemcpyDeviceToHost, j->stream));
+  HIP_CHECK(hipStreamSynchronize(j->stream));
+  *n = len;
+  return BRUNO_OK;
+}
+
+bruno_status bruno_get_unitigs(bruno_job* j, char* buf, uint64_t* n) {
+  if (!valid_job(j) || !n) return BRUNO_BAD_INPUT;
+  if (j->stage != ST_COMPACTED) return BRUNO_STATE;
+  /* the file drops the leading newline of the first record
+   * (compact_dbg_io.hpp:351-355) */
+  uint64_t len = j->fasta_len ? j->fasta_len - 1 : 0;
+  return get_text(j, j->fasta + (j->fasta_len ? 1 : 0), len, buf, n);
+}
+
+bruno_status bruno_get_branch_fasta(bruno_job* j, char* buf, uint64_t* n) {
+  if (!valid_job(j) || !n) return BRUNO_BAD_INPUT;
+  if (j->stage != ST_COMPACTED) return BRUNO_STATE;
+  return get_text(j, j->bfasta, j->bfasta_len, buf, n);
+}
+
+bruno_status bruno_parse_records(int k, const char* d_bases, const uint64_t* d_offsets, uint64_t n_reads,
+                                 uint64_t* d_out_kmers, uint8_t* d_out_biedges, uint64_t* n_out, void* stream) {
+  if (k < 3 || k > 63 || !n_out || (n_reads && (!d_bases || !d_offsets))) return BRUNO_BAD_INPUT;
+  hipStream_t st = (hipStream_t)stream;
+  if (n_reads == 0) {
+    *n_out = 0;
+    return BRUNO_OK;
+  }
+  uint64_t *per = nullptr, *base = nullptr;
+  if (hipMalloc((void**)&per, n_reads * 8) != hipSuccess || hipMalloc((void**)&base, (n_reads + 1) * 8) != hipSuccess) {
+    (void)hipGetLastError();
+    if (per) (void)hipFree(per);
+    return BRUNO_OOM;
+  }
+  ParseArgs a{};
+  a.bases = (const uint8_t*)d_bases;
+  a.offsets = d_offsets;
+  a.n_reads = n_reads;
+  a.K = k;
+  a.per_read = per;
+  a.rec_base = base;
+  a.out_kmers = d_out_kmers;
+  a.out_biedges = d_out_biedges;
+  unsigned g = grid_for((n_reads + 3) / 4, 1, 8192);
+  bruno_status s = BRUNO_OK;
+  if (k <= 31) hipLaunchKernelGGL((k_parse<uint64_t, 0>), dim3(g), dim3(BLOCK), 0, st, a);
+  else hipLaunchKernelGGL((k_parse<u128, 0>), dim3(g), dim3(BLOCK), 0, st, a);
+  size_t tmp = 0;
+  void* t = nullptr;
+  uint64_t total = 0;
+  if (hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, per, base, (int)n_reads, st) != hipSuccess ||
+      hipMalloc(&t, tmp + 16) != hipSuccess ||
+      hipcub::DeviceScan::ExclusiveSum(t, tmp, per, base, (int)n_reads, st) != hipSuccess) {
+    (void)hipGetLastError();
+    s = BRUNO_INTERNAL;
+  }
+  uint64_t lastc = 0, lastb = 0;
+  if (s == BRUNO_OK && (hipMemcpyAsync(&lastc, per + n_reads - 1, 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                        hipMemcpyAsync(&lastb, base + n_reads - 1, 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                        hipStreamSynchronize(st) != hipSuccess)) {
+    (void)hipGetLastError();
+    s = BRUNO_INTERNAL;
+  }
+  total = lastc + lastb;
+  if (s == BRUNO_OK && d_out_kmers) {
+    if (k <= 31) hipLaunchKernelGGL((k_parse<uint64_t, 1>), dim3(g), dim3(BLOCK), 0, st, a);
+    else hipLaunchKernelGGL((k_parse<u128, 1>), dim3(g), dim3(BLOCK), 0, st, a);
+    if (hipStreamSynchronize(st) != hipSuccess) {
+      (void)hipGetLastError();
+      s = BRUNO_INTERNAL;
+    }
+  }
+  if (t) (void)hipFree(t);
+  (void)hipFree(per);
+  (void)hipFree(base);
+  *n_out = total;
+  return s;
+}
+
+bruno_status bruno_dev_alloc(uint64_t bytes, void** out) {
+  if (!out) return BRUNO_BAD_INPUT;
+  *out = nullptr;
+  if (hipMalloc(out, bytes ? bytes : 16) != hipSuccess) {
+    (void)hipGetLastError();
+    *out = nullptr;
+    return BRUNO_OOM;
+  }
+  return BRUNO_OK;
+}
+
+bruno_status bruno_dev_free(void* p) {
+  if (p && hipFree(p) != hipSuccess) {
+    (void)hipGetLastError();
+    return BRUNO_INTERNAL;
+  }
+  return BRUNO_OK;
+}
+
+bruno_status bruno_dev_copy(void* dst, const void* src, uint64_t bytes, int kind) {
+  if ((!dst || !src) && bytes) return BRUNO_BAD_INPUT;
+  hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+  if (kind < 0 || kind > 2) return BRUNO_BAD_INPUT;
+  if (bytes && hipMemcpy(dst, src, bytes, k) != hipSuccess) {
+    (void)hipGetLastError();
+    return BRUNO_INTERNAL;
+  }
+  return BRUNO_OK;
+}
+
+bruno_status bruno_dev_sync(void) {
+  if (hipDeviceSynchronize() != hipSuccess) {
+    (void)hipGetLastError();
+    return BRUNO_INTERNAL;
+  }
+  return BRUNO_OK;
+}
+
+bruno_status bruno_dev_mem_info(uint64_t* free_bytes, uint64_t* total_bytes) {
+  size_t f = 0, t = 0;
+  if (hipMemGetInfo(&f, &t) != hipSuccess) {
+    (void)hipGetLastError();
+    return BRUNO_NO_DEVICE;
+  }
+  if (free_bytes) *free_bytes = f;
+  if (total_bytes) *total_bytes = t;
+  return BRUNO_OK;
+}
+
+static bool make_params(const bruno_synth_desc* d, br::synth_params& p) {
+  if (!d || d->genome_len < 2 || d->read_len == 0 || d->read_len > d->genome_len) return false;
+  memset(&p, 0, sizeof(p));
+  p.seed = d->seed;
+  p.G = d->genome_len;
+  p.n_reads = d->n_reads;
+  p.L = d->read_len;
+  auto thr = [](double x) -> uint64_t {
+    if (x <= 0) return 0;
+    if (x >= 1) return ~0ULL;
+    return (uint64_t)(x * 18446744073709551616.0);
+  };
+  p.sub_thresh = thr(d->sub_rate);
+  p.indel_thresh = thr(d->indel_rate);
+  p.div_thresh = thr(d->family_divergence);
+  if (d->n_families > 0 && d->repeat_fraction > 0) {
+    p.n_families = d->n_families;
+    p.fmin = d->family_min_len;
+    p.fmax = std::max(d->family_max_len, d->family_min_len);
+    double mean = 0.5 * (p.fmin + p.fmax);
+    uint64_t copies = (uint64_t)(d->repeat_fraction * (double)p.G / mean + 0.5);
+    if (copies < 1) copies = 1;
+    uint64_t spacing = p.G / copies;
+    if (spacing < p.fmax) {
+      copies = p.G / p.fmax;
+      spacing = copies ? p.G / copies : 0;
+    }
+    p.n_copies = copies;
+    p.spacing = spacing;
+    if (!copies || !spacing) p.n_copies = 0;
+  }
+  return true;
+}
+
+bruno_status bruno_synth_reads_device(const bruno_synth_desc* d, char* d_bases, uint64_t* d_offsets, void* stream) {
+  br::synth_params p;
+  if (!make_params(d, p) || !d_bases || !d_offsets) return BRUNO_BAD_INPUT;
+  uint64_t g = (p.n_reads + 255) / 256;
+  if (g == 0) g = 1;
+  hipLaunchKernelGGL(k_synth, dim3((unsigned)g), dim3(256), 0, (hipStream_t)stream, p, d_bases, d_offsets);
+  if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess) {
+    (void)hipGetLastError();
+    return BRUNO_INTERNAL;
+  }
+  return BRUNO_OK;
+}
+
+bruno_status bruno_synth_reads_host(const bruno_synth_desc* d, char* bases, uint64_t* offsets) {
+  br::synth_params p;
+  if (!make_params(d, p) || !bases || !offsets) return BRUNO_BAD_INPUT;
+  for (uint64_t r = 0; r < p.n_reads; ++r) {
+    br::make_read(p, r, bases + r * p.L);
+    offsets[r] = r * p.L;
+  }
+  offsets[p.n_reads] = p.n_reads * p.L;
+  return BRUNO_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,423 @@
+/*
+ * kernels.h -- HIP kernels of the de Bruijn hot path (gfx950, wave64).
+ *
+ * Phases and the reference code each one replaces:
+ *   k_parse_count   parse reads into canonical (k-mer, biedge) records and
+ *                   count them in the HBM table (debruijn_biedge_loader.hpp
+ *                   :394-623, debruijn_common.hpp:53-58,
+ *                   debruijn_graph_map.hpp:1085-1141 / :175-197)
+ *   k_filter        local_insert_by_freqencies per k-mer block
+ *                   (debruijn_graph_map.hpp:1258-1623) + IsBranchPoint /
+ *                   IsChainNode (debruijn_graph_filters.hpp:57-220)
+ *   k_split         branch / chain lists
+ *   k_chain_init    to_simple_biedge (debruijn_chain_node.hpp:84-141) with
+ *                   neighbours resolved to chain indices
+ *   k_termini       setup_chain_termini + terminus_update
+ *                   (debruijn_chain_graph.hpp:486-596,
+ *                   debruijn_chain_operations.hpp:58-93)
+ *   k_lr_send/recv  one synchronous round of list_rank / list_rank_min_update
+ *                   (debruijn_chain_graph.hpp:793-994, :1040-1310) with
+ *                   chain_update (debruijn_chain_operations.hpp:108-256)
+ *   k_separate      separate_cycles / separate_isolated (:670-777)
+ *   k_ranked        to_listranked_chain_node (debruijn_chain_operations.hpp
+ *                   :261-310)
+ *   k_fasta_*       print_chain_as_fasta / print_graph_node_fasta bytes
+ */
+#ifndef BRUNO_KERNELS_H_
+#define BRUNO_KERNELS_H_
+
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+#include <stdint.h>
+
+#include "kmer.h"
+#include "table.h"
+
+namespace br {
+
+constexpr int BLOCK = 256;
+
+/* ------------------------------------------------------------------ */
+/* chain-node encodings (debruijn_chain_node.hpp:53-80)                 */
+/* ------------------------------------------------------------------ */
+constexpr uint32_t DIST_MASK = 0x7FFFFFFFu;
+constexpr uint32_t TERM_MASK = 0x80000000u;
+__device__ __forceinline__ uint32_t cdist(uint32_t x) { return x & DIST_MASK; }
+__device__ __forceinline__ bool is_terminal(uint32_t x) { return (x & DIST_MASK) == 0; }
+__device__ __forceinline__ bool points_to_self(uint32_t x) { return x == TERM_MASK; }
+__device__ __forceinline__ bool points_to_chain(uint32_t x) { return (x ^ TERM_MASK) > TERM_MASK; }
+__device__ __forceinline__ bool is_unfinished(uint32_t din, uint32_t dout) {
+  return points_to_chain(din) || points_to_chain(dout);
+}
+
+/* A chain-end pointer: node index (31 bits) and the end of that node that
+ * faces the pointing node (bit 31: 1 = OUT end, 0 = IN end). */
+constexpr uint32_t END_OUT = 0x80000000u;
+constexpr uint32_t IDX_MASK = 0x7FFFFFFFu;
+
+/* Per-node list-ranking state with its two mailboxes (one per end). */
+struct LRNode {
+  uint32_t p[2];  /* [0] = in pointer, [1] = out pointer                 */
+  uint32_t d[2];  /* [0] = in dist, [1] = out dist (term_mask encoding)  */
+  unsigned long long mail[2]; /* (dist << 32) | ptr, 0 = empty          */
+};
+static_assert(sizeof(LRNode) == 32, "lr node");
+
+struct Counters {
+  unsigned long long n_instances, n_dropped, n_distinct, n_k2;
+  unsigned long long n_present, n_branch, n_chain, n_virt;
+  unsigned long long lr_count, lr_next, lr_cycle, lr_conflict;
+  unsigned long long n_cycle, n_isolated, n_out, n_spill;
+  uint32_t err;
+  uint32_t pad;
+};
+
+/* ------------------------------------------------------------------ */
+/* block-wide reservation of output positions                           */
+/* ------------------------------------------------------------------ */
+typedef hipcub::BlockScan<uint32_t, BLOCK> BScan;
+struct ReserveSmem {
+  typename BScan::TempStorage scan;
+  unsigned long long base;
+};
+/* All threads of the block call this; returns this thread's first output
+ * position.  One global atomic per block. */
+__device__ __forceinline__ unsigned long long block_reserve(uint32_t my, unsigned long long* counter,
+                                                            ReserveSmem& sm) {
+  uint32_t excl, total;
+  BScan(sm.scan).ExclusiveSum(my, excl, total);
+  if (threadIdx.x == 0) sm.base = total ? atomicAdd(counter, (unsigned long long)total) : 0ULL;
+  __syncthreads();
+  unsigned long long b = sm.base;
+  __syncthreads();
+  return b + excl;
+}
+
+__device__ __forceinline__ void block_add(unsigned long long v, unsigned long long* counter) {
+  typedef hipcub::BlockReduce<unsigned long long, BLOCK> BR;
+  __shared__ typename BR::TempStorage tmp;
+  unsigned long long t = BR(tmp).Sum(v);
+  if (threadIdx.x == 0 && t) atomicAdd(counter, t);
+}
+
+/* ------------------------------------------------------------------ */
+/* fill                                                                 */
+/* ------------------------------------------------------------------ */
+__global__ void k_fill64(uint64_t* p, uint64_t n, uint64_t v) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+
+/* ------------------------------------------------------------------ */
+/* parse (+ count)                                                      */
+/* ------------------------------------------------------------------ */
+__device__ __forceinline__ int base_code(uint8_t c) {
+  switch (c) {
+    case 'A': case 'a': return 0;
+    case 'C': case 'c': return 1;
+    case 'G': case 'g': return 2;
+    case 'T': case 't': return 3;
+    default: return -1;
+  }
+}
+
+__device__ __forceinline__ uint64_t spread32(uint32_t x) {
+  uint64_t v = x;
+  v = (v | (v << 16)) & 0x0000FFFF0000FFFFULL;
+  v = (v | (v << 8)) & 0x00FF00FF00FF00FFULL;
+  v = (v | (v << 4)) & 0x0F0F0F0F0F0F0F0FULL;
+  v = (v | (v << 2)) & 0x3333333333333333ULL;
+  v = (v | (v << 1)) & 0x5555555555555555ULL;
+  return v;
+}
+
+/* Pack 64 bases (one per lane, code 0..3) into two 32-base words with the
+ * first base in the top bits, using two wavefront ballots. */
+__device__ __forceinline__ void pack64(int code, uint64_t& w0, uint64_t& w1) {
+  uint64_t m0 = __ballot(code & 1);
+  uint64_t m1 = __ballot((code >> 1) & 1);
+  w0 = spread32(__brev((uint32_t)m0)) | (spread32(__brev((uint32_t)m1)) << 1);
+  w1 = spread32(__brev((uint32_t)(m0 >> 32))) | (spread32(__brev((uint32_t)(m1 >> 32))) << 1);
+}
+
+__device__ __forceinline__ uint64_t sel4(int q, uint64_t a, uint64_t b, uint64_t c, uint64_t d) {
+  return q == 0 ? a : q == 1 ? b : q == 2 ? c : d;
+}
+/* 64 bits of the 256-bit window [a b c d] starting at base position pos
+ * (pos <= 96). */
+__device__ __forceinline__ uint64_t window64(int pos, uint64_t a, uint64_t b, uint64_t c, uint64_t d) {
+  int q = pos >> 5, r = pos & 31;
+  uint64_t hi = sel4(q, a, b, c, d), lo = sel4(q + 1, a, b, c, d);
+  return r ? ((hi << (2 * r)) | (lo >> (64 - 2 * r))) : hi;
+}
+__device__ __forceinline__ int window_base(int pos, uint64_t a, uint64_t b, uint64_t c, uint64_t d) {
+  uint64_t w = sel4(pos >> 5, a, b, c, d);
+  return (int)((w >> (62 - 2 * (pos & 31))) & 3);
+}
+
+template <typename KT> __device__ __forceinline__ KT window_kmer(int l, int K, uint64_t a, uint64_t b, uint64_t c, uint64_t d);
+template <> __device__ __forceinline__ uint64_t window_kmer<uint64_t>(int l, int K, uint64_t a, uint64_t b, uint64_t c,
+                                                                      uint64_t d) {
+  return window64(l, a, b, c, d) >> (64 - 2 * K);
+}
+template <> __device__ __forceinline__ u128 window_kmer<u128>(int l, int K, uint64_t a, uint64_t b, uint64_t c, uint64_t d) {
+  uint64_t h = window64(l, a, b, c, d), lo = window64(l + 32, a, b, c, d);
+  u128 t = ((u128)h << 64) | lo;
+  return t >> (128 - 2 * K);
+}
+
+struct ParseArgs {
+  const uint8_t* bases;
+  const uint64_t* offsets;
+  uint64_t n_reads;
+  int K;
+  /* count mode */
+  void* table;
+  uint64_t cap;
+  Spill* spill;
+  uint64_t spcap;
+  /* record mode */
+  const uint64_t* rec_base; /* per-read first record index */
+  uint64_t* out_kmers;
+  uint8_t* out_biedges;
+  uint64_t* per_read;       /* per-read record count (count pass) */
+  Counters* ctr;
+};
+
+/* MODE 0: count records per read; MODE 1: write records; MODE 2: count into
+ * the table. */
+template <typename KT, int MODE>
+__global__ __launch_bounds__(BLOCK) void k_parse(ParseArgs a) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  const int K = a.K;
+  kops<KT> ops(K);
+  unsigned long long my_inst = 0, my_drop = 0, my_new = 0, my_k2 = 0;
+  for (uint64_t r = wave; r < a.n_reads; r += nwaves) {
+    const uint64_t off = a.offsets[r];
+    const uint64_t L = a.offsets[r + 1] - off;
+    const uint8_t* s = a.bases + off;
+    /* NFilterSequencesIterator: a read with any non-ACGT base is dropped
+     * (compact_dbg_freq_construct_clean.cpp:210-213). */
+    int bad = 0;
+    for (uint64_t i = lane; i < L; i += 64) bad |= (base_code(s[i]) < 0);
+    if (__any(bad)) {
+      if (lane == 0) ++my_drop;
+      if (MODE == 0 && lane == 0) a.per_read[r] = 0;
+      continue;
+    }
+    /* parser window is k+2 (debruijn_biedge_loader.hpp:402, :464-471) */
+    if (L < (uint64_t)K + 2) {
+      if (MODE == 0 && lane == 0) a.per_read[r] = 0;
+      continue;
+    }
+    const uint64_t nk = L - K + 1;
+    if (MODE == 0) {
+      if (lane == 0) a.per_read[r] = nk;
+      continue;
+    }
+    uint64_t rbase = (MODE == 1) ? a.rec_base[r] : 0;
+    /* sliding window of 4 packed words covering bases [p0, p0 + 128) */
+    uint64_t A, B, C, D, prevB = 0;
+    {
+      int c0 = (lane < L) ? base_code(s[lane]) : 0;
+      pack64(c0, A, B);
+      int c1 = (64 + lane < L) ? base_code(s[64 + lane]) : 0;
+      pack64(c1, C, D);
+    }
+    for (uint64_t p0 = 0; p0 < nk; p0 += 64) {
+      const uint64_t i = p0 + lane;
+      if (i < nk) {
+        KT fw = window_kmer<KT>(lane, K, A, B, C, D);
+        int inb = (lane > 0) ? window_base(lane - 1, A, B, C, D) : (int)(prevB & 3);
+        int outb = window_base(lane + K, A, B, C, D);
+        uint8_t left = (i > 0) ? (uint8_t)(1u << inb) : (uint8_t)0;
+        uint8_t right = (i + K < L) ? (uint8_t)(1u << outb) : (uint8_t)0;
+        uint8_t e = (uint8_t)((left << 4) | right);
+        /* lex_less (debruijn_common.hpp:53-58): keep if x < rc(x), else
+         * rc the k-mer and the biedge. */
+        KT rv = ops.rc(fw);
+        KT km = fw;
+        if (!(fw < rv)) {
+          km = rv;
+          e = biedge_rc(e);
+        }
+        if (MODE == 1) {
+          a.out_kmers[(rbase + i) * kt_traits<KT>::words] = (uint64_t)km;
+          if (kt_traits<KT>::words == 2)
+            a.out_kmers[(rbase + i) * kt_traits<KT>::words + 1] = (uint64_t)((u128)km >> 64);
+          a.out_biedges[rbase + i] = e;
+        } else {
+          int claimed = 0;
+          uint64_t slot = SlotOps<KT>::find_or_insert((Slot<KT>*)a.table, a.cap, km, &a.ctr->err, &claimed);
+          if (slot != ~0ULL) {
+            my_new += claimed;
+            my_k2 += add_k2<KT>((Slot<KT>*)a.table, slot, e, a.spill, a.spcap, &a.ctr->err);
+          }
+        }
+        ++my_inst;
+      }
+      /* advance the window by 64 bases */
+      prevB = B;
+      A = C;
+      B = D;
+      uint64_t nxt = p0 + 128 + lane;
+      int cn = (nxt < L) ? base_code(s[nxt]) : 0;
+      pack64(cn, C, D);
+    }
+  }
+  if (MODE == 2) {
+    block_add(my_inst, &a.ctr->n_instances);
+    __syncthreads();
+    block_add(my_drop, &a.ctr->n_dropped);
+    __syncthreads();
+    block_add(my_new, &a.ctr->n_distinct);
+    __syncthreads();
+    block_add(my_k2, &a.ctr->n_k2);
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* per-slot k2 gathering                                                */
+/* ------------------------------------------------------------------ */
+/* All 25 biedge bytes possible for ACGT-only reads (nibbles 0,1,2,4,8). */
+__device__ __constant__ uint8_t BIEDGE25[25] = {
+    0x00, 0x01, 0x02, 0x04, 0x08, 0x10, 0x11, 0x12, 0x14, 0x18, 0x20, 0x21, 0x22,
+    0x24, 0x28, 0x40, 0x41, 0x42, 0x44, 0x48, 0x80, 0x81, 0x82, 0x84, 0x88};
+
+struct K2List {
+  uint8_t e[25];
+  uint32_t c[25];
+  int n;
+};
+
+template <typename KT>
+__device__ __forceinline__ void gather_k2(const Slot<KT>* p, uint64_t s, const Spill* sp, uint64_t spcap, K2List& L) {
+  L.n = 0;
+#pragma unroll
+  for (int j = 0; j < Slot<KT>::NSUB; ++j) {
+    uint64_t v = p->sub[j];
+    if (v != EMPTY64) {
+      L.e[L.n] = (uint8_t)((v >> 48) - 1);
+      L.c[L.n] = (uint32_t)(v & 0xFFFFFFFFFFFFULL);
+      ++L.n;
+    }
+  }
+  if (SlotOps<KT>::flags(p) & F_SPILL) {
+    for (int t = 0; t < 25 && L.n < 25; ++t) {
+      uint8_t e = BIEDGE25[t];
+      bool in_sub = false;
+      for (int j = 0; j < L.n; ++j) in_sub |= (L.e[j] == e);
+      if (in_sub) continue;
+      uint64_t c = spill_find(sp, spcap, (s << 9) | ((uint64_t)e + 1));
+      if (c) {
+        L.e[L.n] = e;
+        L.c[L.n] = (uint32_t)c;
+        ++L.n;
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ int nib_idx(int n) { return n == 1 ? 0 : n == 2 ? 1 : n == 4 ? 2 : n == 8 ? 3 : 8; }
+__device__ __forceinline__ uint16_t sat_add16(uint32_t a, uint32_t b) {
+  uint32_t c = a + b;
+  return (uint16_t)(c > 65535u ? 65535u : c);
+}
+/* compact_multi_biedge::update(edges, cnt) (debruijn_graph_node.hpp:200-211) */
+__device__ __forceinline__ void node_update(uint16_t* c, uint8_t e, uint32_t cnt) {
+  int o = nib_idx(e & 0xF);
+  if (o < 4) c[o] = sat_add16(c[o], cnt);
+  int i = nib_idx(e >> 4);
+  if (i < 4) c[4 + i] = sat_add16(c[4 + i], cnt);
+  c[8] = sat_add16(c[8], cnt);
+}
+
+/* P1: unthresholded counters from the k2 list. */
+__device__ __forceinline__ void p1_counts(const K2List& L, uint16_t* c) {
+  for (int t = 0; t < 9; ++t) c[t] = 0;
+  for (int j = 0; j < L.n; ++j) node_update(c, L.e[j], L.c[j] > 65535u ? 65535u : L.c[j]);
+}
+
+struct Thresh {
+  uint64_t t0lo, t0hi, t1lo, t1hi, t2lo, t2hi;
+};
+
+/* local_insert_by_freqencies for one k-mer block (debruijn_graph_map.hpp
+ * :1351-1609).  Returns true if any k2-mer is kept. */
+template <typename KT>
+__device__ bool p2_counts(const kops<KT>& ops, KT k, const K2List& L, const Thresh& t, uint16_t* c) {
+  const uint64_t cnt_max = 65535;
+  bool kpal = ops.is_rc_palindrome(k);
+  bool high = ops.km1_high(k), low = ops.km1_low(k);
+  int lastb = ops.last(k), firstb = ops.first(k);
+  uint64_t kf = 0, k1in[4] = {0, 0, 0, 0}, k1out[4] = {0, 0, 0, 0};
+  for (int j = 0; j < L.n; ++j) {
+    uint64_t cu = L.c[j];
+    kf += cu;
+    int ch = L.e[j] >> 4;
+    if (ch) {
+      int ci = nib_idx(ch);
+      /* is_k1_rc_palindrome(l, k, high): last base == comp(l) && high */
+      bool split = ((ops.K & 1) && lastb == 3 - ci && high) && !kpal;
+      k1in[ci] += split ? ((cu + 1) >> 1) : cu;
+    }
+    ch = L.e[j] & 0xF;
+    if (ch) {
+      int ci = nib_idx(ch);
+      bool split = ((ops.K & 1) && firstb == 3 - ci && low) && !kpal;
+      k1out[ci] += split ? ((cu + 1) >> 1) : cu;
+    }
+  }
+  kf = kf < cnt_max ? kf : cnt_max;
+  for (int q = 0; q < 4; ++q) {
+    k1in[q] = k1in[q] < cnt_max ? k1in[q] : cnt_max;
+    k1out[q] = k1out[q] < cnt_max ? k1out[q] : cnt_max;
+  }
+  for (int q = 0; q < 9; ++q) c[q] = 0;
+  bool have = false;
+  for (int j = 0; j < L.n; ++j) {
+    uint8_t flag = 0;
+    uint64_t cnt = L.c[j] < cnt_max ? L.c[j] : cnt_max;
+    if (cnt < t.t2lo || cnt >= t.t2hi) flag |= 4;
+    if (kf < t.t0lo || kf >= t.t0hi) flag |= 8;
+    int in = L.e[j] >> 4, out = L.e[j] & 0xF;
+    uint64_t k1 = (nib_idx(in) < 4) ? k1in[nib_idx(in)] : t.t1hi;
+    if (k1 < t.t1lo || k1 >= t.t1hi) flag |= 2;
+    k1 = (nib_idx(out) < 4) ? k1out[nib_idx(out)] : t.t1hi;
+    if (k1 < t.t1lo || k1 >= t.t1hi) flag |= 1;
+    if (flag >= 3) continue;
+    uint8_t e = L.e[j];
+    if (flag == 1) e &= 0xF0;
+    if (flag == 2) e &= 0x0F;
+    node_update(c, e, (uint32_t)cnt);
+    have = true;
+  }
+  return have;
+}
+
+/* IsKplus1Palindrome (debruijn_graph_filters.hpp:57-95) with
+ * is_k1_k1pair_rc_palindrome (kmer_traits.hpp:110-127). */
+template <typename KT>
+__device__ __forceinline__ bool kplus1_pal(const kops<KT>& ops, KT k, const uint16_t* c) {
+  if (!(ops.K & 1)) return false;
+  int firstb = ops.first(k), lastb = ops.last(k);
+  bool low = ops.km1_low(k);
+  if (low)
+    for (int q = 0; q < 4; ++q)
+      if (c[q] > 0 && (firstb == 3 - q || firstb == q)) return true;
+  bool high = ops.km1_high(k);
+  if (high)
+    for (int q = 0; q < 4; ++q)
+      if (c[4 + q] > 0 && (lastb == 3 - q || lastb == q)) return true;
+  return false;
+}
+
+__device__ __forceinline__ int n_out_edges(const uint16_t* c) { return (c[0] > 0) + (c[1] > 0) + (c[2] > 0) + (c[3] > 0); }
+__device__ __forceinline__ int n_in_edges(const uint16_t* c) { return (c[4] > 0) + (c[5] > 0) + (c[6] > 0) + (c[7] > 0); }
+
+}  // namespace br
+
+#endif

@@ -1,0 +1,262 @@
+/*
+ * table.h -- HBM open-addressing k-mer table used for counting and as the
+ * graph store.
+ *
+ * One slot per canonical k-mer.  During counting a slot holds the k-mer and
+ * NSUB k2 sub-entries (biedge code + uint count), so the joint (in, out)
+ * counts that bruno's k2 counter keeps (LocalK2CountMapType,
+ * debruijn_graph_map.hpp:160-168) sit in the same cache line as the key.
+ * A k-mer with more than NSUB distinct biedges spills the extra ones into a
+ * side table keyed by (slot << 9 | code).  After the filter the sub-entry
+ * area of a surviving slot is overwritten in place with the node record
+ * (9 saturating counters, class, chain index), so lookups by k-mer keep
+ * working on the same table.
+ *
+ * k <= 31 (KT = uint64_t): 32-byte slot {key, sub[3]}.  key = k-mer with
+ *   bit 63 = spilled, bit 62 = present after filter; EMPTY = all ones (a
+ *   canonical 31-mer is never all T, so a real key never reads as EMPTY).
+ * k <= 63 (KT = u128): 64-byte slot {meta, lo, hi, sub[5]}.  The key is two
+ *   words, so a slot is claimed by CAS on meta (EMPTY -> LOCK), the k-mer is
+ *   written, then meta is released to its flag word (bit 63 spilled, bit 62
+ *   present).
+ * Sub-entry: (code << 48) | count with code = biedge + 1 (1..256); EMPTY =
+ *   all ones.
+ */
+#ifndef BRUNO_TABLE_H_
+#define BRUNO_TABLE_H_
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kmer.h"
+
+namespace br {
+
+constexpr uint64_t EMPTY64 = ~0ULL;
+constexpr uint64_t LOCK64 = ~0ULL - 1;
+constexpr uint64_t F_SPILL = 1ULL << 63;
+constexpr uint64_t F_PRESENT = 1ULL << 62;
+constexpr uint64_t KEY_BITS = (1ULL << 62) - 1;
+constexpr uint32_t MAX_PROBE = 1u << 20;
+constexpr uint32_t MAX_SPIN = 1u << 22;
+
+/* error bits */
+constexpr uint32_t ERR_TABLE_FULL = 1;
+constexpr uint32_t ERR_SPILL_FULL = 2;
+constexpr uint32_t ERR_SPIN = 4;
+constexpr uint32_t ERR_LIST_FULL = 8;
+
+/* node record stored in a slot's sub area after the filter */
+struct NodeRec {
+  uint16_t c[9];  /* out A,C,G,T, in A,C,G,T, self */
+  uint8_t cls;    /* CLS_* */
+  uint8_t pad;
+  uint32_t idx;   /* chain / branch index */
+};
+constexpr uint8_t CLS_BRANCH = 1;
+constexpr uint8_t CLS_CHAIN = 2;
+
+struct Spill {
+  uint64_t key; /* (slot << 9) | code, 0 = empty */
+  uint64_t cnt;
+};
+
+template <typename KT> struct Slot;
+template <> struct Slot<uint64_t> {
+  uint64_t key;
+  uint64_t sub[3];
+  static constexpr int NSUB = 3;
+};
+template <> struct Slot<u128> {
+  uint64_t meta;
+  uint64_t lo, hi;
+  uint64_t sub[5];
+  static constexpr int NSUB = 5;
+};
+static_assert(sizeof(Slot<uint64_t>) == 32, "slot size");
+static_assert(sizeof(Slot<u128>) == 64, "slot size");
+static_assert(sizeof(NodeRec) == 24, "node size");
+
+__device__ __forceinline__ uint64_t ld_relaxed(const uint64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t ld_acquire(const uint64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_release(uint64_t* p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t cas64(uint64_t* p, uint64_t cmp, uint64_t val) {
+  return atomicCAS((unsigned long long*)p, (unsigned long long)cmp, (unsigned long long)val);
+}
+__device__ __forceinline__ void add64(uint64_t* p, uint64_t v) {
+  atomicAdd((unsigned long long*)p, (unsigned long long)v);
+}
+__device__ __forceinline__ void or64(uint64_t* p, uint64_t v) {
+  atomicOr((unsigned long long*)p, (unsigned long long)v);
+}
+
+__device__ __forceinline__ uint64_t home_slot(uint64_t h, uint64_t cap) { return __umul64hi(h, cap); }
+
+/* ------------------------------------------------------------------ */
+/* key access per slot type                                             */
+/* ------------------------------------------------------------------ */
+template <typename KT> struct SlotOps;
+
+template <> struct SlotOps<uint64_t> {
+  typedef Slot<uint64_t> S;
+  /* Find or claim the slot of canonical k-mer v.  Returns slot index or
+   * ~0 on a full table.  *claimed = 1 when this call inserted the key. */
+  static __device__ uint64_t find_or_insert(S* T, uint64_t cap, uint64_t v, uint32_t* err, int* claimed) {
+    uint64_t s = home_slot(kmer_hash(v), cap);
+    for (uint32_t probe = 0; probe < MAX_PROBE; ++probe) {
+      S* p = T + s;
+      uint64_t k0 = ld_relaxed(&p->key);
+      if (k0 == EMPTY64) {
+        uint64_t old = cas64(&p->key, EMPTY64, v);
+        if (old == EMPTY64) {
+          *claimed = 1;
+          return s;
+        }
+        k0 = old;
+      }
+      if ((k0 & KEY_BITS) == v) return s;
+      s = (s + 1 == cap) ? 0 : s + 1;
+    }
+    atomicOr(err, ERR_TABLE_FULL);
+    return ~0ULL;
+  }
+  /* Lookup only; ~0 when absent.  require_present: only nodes that
+   * survived the filter. */
+  static __device__ uint64_t find(const S* T, uint64_t cap, uint64_t v, bool require_present) {
+    uint64_t s = home_slot(kmer_hash(v), cap);
+    for (uint32_t probe = 0; probe < MAX_PROBE; ++probe) {
+      uint64_t k0 = T[s].key;
+      if (k0 == EMPTY64) return ~0ULL;
+      if ((k0 & KEY_BITS) == v) {
+        if (require_present && !(k0 & F_PRESENT)) return ~0ULL;
+        return s;
+      }
+      s = (s + 1 == cap) ? 0 : s + 1;
+    }
+    return ~0ULL;
+  }
+  static __device__ __forceinline__ uint64_t* flag_word(S* p) { return &p->key; }
+  static __device__ __forceinline__ bool occupied(const S* p) { return p->key != EMPTY64; }
+  static __device__ __forceinline__ uint64_t kmer(const S* p) { return p->key & KEY_BITS; }
+  static __device__ __forceinline__ uint64_t flags(const S* p) { return p->key & ~KEY_BITS; }
+};
+
+template <> struct SlotOps<u128> {
+  typedef Slot<u128> S;
+  static __device__ uint64_t find_or_insert(S* T, uint64_t cap, u128 v, uint32_t* err, int* claimed) {
+    uint64_t lo = (uint64_t)v, hi = (uint64_t)(v >> 64);
+    uint64_t s = home_slot(kmer_hash(v), cap);
+    for (uint32_t probe = 0; probe < MAX_PROBE; ++probe) {
+      S* p = T + s;
+      uint64_t m = ld_acquire(&p->meta);
+      if (m == EMPTY64) {
+        uint64_t old = cas64(&p->meta, EMPTY64, LOCK64);
+        if (old == EMPTY64) {
+          p->lo = lo;
+          p->hi = hi;
+          st_release(&p->meta, 0);
+          *claimed = 1;
+          return s;
+        }
+        m = old;
+      }
+      uint32_t spin = 0;
+      while (m == LOCK64 && spin < MAX_SPIN) {
+        m = ld_acquire(&p->meta);
+        ++spin;
+      }
+      if (m == LOCK64) {
+        atomicOr(err, ERR_SPIN);
+        return ~0ULL;
+      }
+      if (ld_relaxed(&p->lo) == lo && ld_relaxed(&p->hi) == hi) return s;
+      s = (s + 1 == cap) ? 0 : s + 1;
+    }
+    atomicOr(err, ERR_TABLE_FULL);
+    return ~0ULL;
+  }
+  static __device__ uint64_t find(const S* T, uint64_t cap, u128 v, bool require_present) {
+    uint64_t lo = (uint64_t)v, hi = (uint64_t)(v >> 64);
+    uint64_t s = home_slot(kmer_hash(v), cap);
+    for (uint32_t probe = 0; probe < MAX_PROBE; ++probe) {
+      uint64_t m = T[s].meta;
+      if (m == EMPTY64) return ~0ULL;
+      if (T[s].lo == lo && T[s].hi == hi) {
+        if (require_present && !(m & F_PRESENT)) return ~0ULL;
+        return s;
+      }
+      s = (s + 1 == cap) ? 0 : s + 1;
+    }
+    return ~0ULL;
+  }
+  static __device__ __forceinline__ uint64_t* flag_word(S* p) { return &p->meta; }
+  static __device__ __forceinline__ bool occupied(const S* p) { return p->meta != EMPTY64; }
+  static __device__ __forceinline__ u128 kmer(const S* p) { return ((u128)p->hi << 64) | p->lo; }
+  static __device__ __forceinline__ uint64_t flags(const S* p) { return p->meta; }
+};
+
+/* Add one instance of biedge e to slot s.  Returns 1 if a new k2 entry was
+ * created. */
+template <typename KT>
+__device__ int add_k2(Slot<KT>* T, uint64_t s, uint8_t e, Spill* sp, uint64_t spcap, uint32_t* err) {
+  Slot<KT>* p = T + s;
+  const uint64_t code = (uint64_t)e + 1;
+#pragma unroll
+  for (int j = 0; j < Slot<KT>::NSUB; ++j) {
+    uint64_t v = ld_relaxed(&p->sub[j]);
+    if (v == EMPTY64) {
+      uint64_t old = cas64(&p->sub[j], EMPTY64, (code << 48) | 1);
+      if (old == EMPTY64) return 1;
+      v = old;
+    }
+    if ((v >> 48) == code) {
+      add64(&p->sub[j], 1);
+      return 0;
+    }
+  }
+  /* spill */
+  uint64_t* fw = SlotOps<KT>::flag_word(p);
+  if (!(ld_relaxed(fw) & F_SPILL)) or64(fw, F_SPILL);
+  const uint64_t key = (s << 9) | code;
+  uint64_t h = home_slot(fmix64(key), spcap);
+  for (uint32_t probe = 0; probe < MAX_PROBE; ++probe) {
+    Spill* q = sp + h;
+    uint64_t k0 = ld_relaxed(&q->key);
+    if (k0 == 0) {
+      uint64_t old = cas64(&q->key, 0, key);
+      if (old == 0) {
+        add64(&q->cnt, 1);
+        return 1;
+      }
+      k0 = old;
+    }
+    if (k0 == key) {
+      add64(&q->cnt, 1);
+      return 0;
+    }
+    h = (h + 1 == spcap) ? 0 : h + 1;
+  }
+  atomicOr(err, ERR_SPILL_FULL);
+  return 0;
+}
+
+__device__ __forceinline__ uint64_t spill_find(const Spill* sp, uint64_t spcap, uint64_t key) {
+  uint64_t h = home_slot(fmix64(key), spcap);
+  for (uint32_t probe = 0; probe < MAX_PROBE; ++probe) {
+    uint64_t k0 = sp[h].key;
+    if (k0 == 0) return 0;
+    if (k0 == key) return sp[h].cnt;
+    h = (h + 1 == spcap) ? 0 : h + 1;
+  }
+  return 0;
+}
+
+}  // namespace br
+
+#endif

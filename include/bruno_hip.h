@@ -1,0 +1,224 @@
+/*
+ * bruno_hip.h -- C ABI of the MI355X de Bruijn graph pipeline.
+ *
+ * Drop-in boundary for the hot path of bruno's compact_debruijn_graph_*_freq
+ * driver (test/test/compact_dbg_freq_construct_clean.cpp:431-840).  Each entry
+ * point replaces the reference call named beside it; host façades (C++
+ * bruno_amd/host/debruijn_graph.hpp, Python bruno_amd) keep bruno's class and
+ * method names and call these.
+ *
+ * Conventions
+ *  - All entry points return a bruno_status; no exception crosses the ABI.
+ *  - One job handle is not thread-safe: calls on one handle are serialised by
+ *    the caller.  Distinct handles are independent and may use different
+ *    streams from different host threads.
+ *  - A job owns all of its device (HBM) state until bruno_job_destroy.  The
+ *    state stays resident across calls, so a serving loop can keep many jobs
+ *    on the device at once.
+ *  - K-mers cross the ABI as `words_per_kmer` uint64 words per k-mer (1 for
+ *    k <= 31, 2 for k <= 63), least significant word first.  The value is the
+ *    2-bit packed k-mer (A0 C1 G2 T3) with the first base most significant,
+ *    i.e. the value bruno's Kmer<K, DNA, uint64_t> holds, so numeric order is
+ *    lexicographic order.
+ *  - Two-call getters: pass buf == NULL to read the element (or byte) count
+ *    into *n, then call again with a buffer of that size.
+ */
+#ifndef BRUNO_HIP_H_
+#define BRUNO_HIP_H_
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  BRUNO_OK = 0,
+  BRUNO_BAD_INPUT = 1,  /* invalid argument (k, pointer, thresholds)        */
+  BRUNO_OOM = 2,        /* device allocation failed                          */
+  BRUNO_CAPACITY = 3,   /* hash table / side table full: recreate larger     */
+  BRUNO_INTERNAL = 4,   /* device error or a bounded wait ran out            */
+  BRUNO_STATE = 5,      /* call out of order for this job                    */
+  BRUNO_NO_DEVICE = 6   /* no HIP device / HIP runtime error at set-up       */
+} bruno_status;
+
+typedef struct bruno_job bruno_job;
+
+/* Job description (replaces the template parameters and TCLAP flags of the
+ * reference driver, compact_dbg_freq_construct_clean.cpp:99-258, :873-968). */
+typedef struct {
+  int k;                    /* 1..63 (k <= 31 uses one 64-bit word)                 */
+  int thresholding;         /* 0: build_index (compact_dbg_build.hpp:35-62)
+                               1: build_index_thresholded (..._clean.cpp:297-348)  */
+  uint64_t thresholds[6];   /* {k_lo,k_hi,k1_lo,k1_hi,k2_lo,k2_hi}, reference order
+                               (debruijn_graph_map.hpp:1261); -L t = {0,65536,t,65536,0,65536} */
+  uint64_t capacity;        /* hash slots (distinct k-mers / load factor);
+                               0 = derive from the first batch's k-mer count      */
+  double load_factor;       /* used when capacity == 0; 0 = default 0.5          */
+  void* stream;             /* hipStream_t for all of this job's work; NULL = 0   */
+} bruno_job_desc;
+
+/* Phase timings (ms, device events on the job's stream). */
+typedef struct {
+  double parse_count_ms;    /* parse + k2 count                                  */
+  double filter_ms;         /* local_insert_by_freqencies + node classification  */
+  double chains_ms;         /* extract_chains + setup_chain_termini              */
+  double list_rank_ms;      /* list ranking incl. cycle / isolated removal       */
+  double output_ms;         /* to_ranked_chain_nodes + sort + unitig / branch text */
+} bruno_phase_ms;
+
+typedef struct {
+  uint64_t n_reads;         /* reads fed                                         */
+  uint64_t n_reads_dropped; /* reads with a non-ACGT base                        */
+  uint64_t n_instances;     /* parsed (k-mer, biedge) records                    */
+  uint64_t n_distinct;      /* distinct canonical k-mers in the count table      */
+  uint64_t n_k2;            /* distinct canonical (k-mer, biedge) pairs          */
+  uint64_t n_overflow;      /* k2 entries in the side table                      */
+  uint64_t n_nodes;         /* graph nodes after the filter                      */
+  uint64_t n_branches;
+  uint64_t n_chain_nodes;   /* chain nodes extracted                             */
+  uint64_t n_cycle_nodes;
+  uint64_t n_isolated;
+  uint64_t n_unitig_nodes;  /* chain nodes printed as unitigs                    */
+  uint64_t n_unitigs;
+  uint64_t iterations;      /* list-ranking rounds                               */
+  uint64_t n_conflicts;     /* same-round updates to one chain end (0 for every
+                               graph built from reads; see DESIGN)               */
+  uint64_t capacity;        /* hash slots                                        */
+  uint64_t device_bytes;    /* HBM held by the job                               */
+} bruno_stats;
+
+/* Listranked chain node (debruijn_chain_node.hpp:156): K on the rep strand,
+ * rep, dist.  Words are least significant first; word [1] is 0 for k <= 31. */
+typedef struct {
+  uint64_t k[2];
+  uint64_t rep[2];
+  int32_t dist;
+  int32_t pad;
+} bruno_chain_node;
+
+/* Graph node (print_graph_node, debruijn_graph_operations.hpp:58-78):
+ * counts[0..3] out A,C,G,T, counts[4..7] in A,C,G,T, counts[8] self. */
+typedef struct {
+  uint64_t kmer[2];
+  uint16_t counts[9];
+  uint16_t pad[3];
+} bruno_node;
+
+const char* bruno_status_string(bruno_status s);
+
+/* Replaces the construction of CountDBGType idx / ChainGraphType chainmap
+ * (compact_dbg_freq_construct_clean.cpp:471-472). */
+bruno_status bruno_job_create(const bruno_job_desc* desc, bruno_job** out);
+bruno_status bruno_job_destroy(bruno_job* job);
+
+/* Returns the job to its freshly created state while keeping the hash-table
+ * allocation: the table is cleared on the job's stream and every phase
+ * buffer is released.  Replaces clearing the maps between runs
+ * (k2_counter.reset(), debruijn_graph_map.hpp:1270; chainmap.clear(),
+ * compact_dbg_freq_construct_clean.cpp:826) so a serving loop reuses HBM. */
+bruno_status bruno_job_reset(bruno_job* job);
+
+/* Replaces parse_file_data<debruijn_kmer_simple_biedge_parser> +
+ * compute_biedge_freqencies (compact_dbg_freq_construct_clean.cpp:318-328,
+ * debruijn_graph_map.hpp:1085-1141), or the unthresholded idx.insert
+ * (compact_dbg_build.hpp:45-54).  d_bases / d_offsets are DEVICE pointers:
+ * concatenated ASCII reads and n_reads+1 byte offsets.  May be called
+ * repeatedly to add batches (the incremental per-file loop, :313-330). */
+bruno_status bruno_parse_count(bruno_job* job, const char* d_bases, const uint64_t* d_offsets,
+                               uint64_t n_reads);
+/* Same from host buffers (copies over PCIe first). */
+bruno_status bruno_parse_count_host(bruno_job* job, const char* bases, const uint64_t* offsets,
+                                    uint64_t n_reads);
+
+/* Replaces local_insert_by_freqencies(k2_counter, threshes)
+ * (debruijn_graph_map.hpp:1258-1623); in the unthresholded mode it
+ * materialises the count graph (local_insert :175-197).  Also classifies
+ * nodes as branch (IsBranchPoint) or chain (IsChainNode)
+ * (debruijn_graph_filters.hpp:142-220). */
+bruno_status bruno_filter(bruno_job* job);
+
+/* Replaces chainmap.extract_chains(idx) (debruijn_chain_graph.hpp:605-667),
+ * including setup_chain_termini (:486-596). */
+bruno_status bruno_build_chains(bruno_job* job);
+
+/* Replaces list_rank() (variant 0, debruijn_chain_graph.hpp:793-994) and
+ * list_rank_min_update() (variant 1, :1040-1310), including separate_cycles
+ * and separate_isolated.  *rounds receives the iteration count. */
+bruno_status bruno_list_rank(bruno_job* job, int variant, uint32_t* rounds);
+
+/* Replaces to_ranked_chain_nodes (:1760-1773) + print_chain_string's sort
+ * and print_chain_as_fasta (compact_dbg_io.hpp:323-363), and the sort of
+ * print_branch_fasta (:254-299).  Leaves the .chain.fasta and .branch.fasta
+ * bytes in device memory for the getters. */
+bruno_status bruno_compact(bruno_job* job);
+
+/* Convenience: filter + build_chains + list_rank + compact. */
+bruno_status bruno_run_pipeline(bruno_job* job, int variant);
+
+bruno_status bruno_get_stats(bruno_job* job, bruno_stats* out);
+bruno_status bruno_get_phase_ms(bruno_job* job, bruno_phase_ms* out);
+
+/* Getters (two-call).  Counts: valid after parse_count and before filter
+ * (unthresholded node counters of the parsed k-mers; the P1 table). */
+bruno_status bruno_get_counts(bruno_job* job, bruno_node* buf, uint64_t* n);
+/* Distinct canonical k2-mers with uint32 counts (the k2 counter); valid
+ * before filter.  kmers: words_per_kmer words each. */
+bruno_status bruno_get_k2(bruno_job* job, uint64_t* kmers, uint8_t* biedges, uint32_t* counts,
+                          uint64_t* n);
+/* Filtered graph (after filter). */
+bruno_status bruno_get_nodes(bruno_job* job, bruno_node* buf, uint64_t* n);
+/* Branch k-mers, sorted (after compact). */
+bruno_status bruno_get_branches(bruno_job* job, uint64_t* kmers, uint64_t* n);
+/* Listranked chain nodes sorted by (rep, |dist|) (after compact). */
+bruno_status bruno_get_chain_nodes(bruno_job* job, bruno_chain_node* buf, uint64_t* n);
+/* .chain.fasta / .branch.fasta bytes (after compact). */
+bruno_status bruno_get_unitigs(bruno_job* job, char* buf, uint64_t* n);
+bruno_status bruno_get_branch_fasta(bruno_job* job, char* buf, uint64_t* n);
+
+/* Debug entry for the P0 parity object: parse only, write each canonical
+ * (k-mer, biedge) record in read order (device pointers; out_kmers holds
+ * words_per_kmer words per record).  *n_out receives the record count;
+ * pass out_kmers == NULL to count only. */
+bruno_status bruno_parse_records(int k, const char* d_bases, const uint64_t* d_offsets,
+                                 uint64_t n_reads, uint64_t* d_out_kmers, uint8_t* d_out_biedges,
+                                 uint64_t* n_out, void* stream);
+
+/* ---- Device memory helpers ----------------------------------------------
+ * For callers that hold no HIP runtime of their own (Python tests, the
+ * benchmark): allocate, copy and synchronise through this library's runtime.
+ * No counterpart in the reference (its buffers are host vectors). */
+bruno_status bruno_dev_alloc(uint64_t bytes, void** out);
+bruno_status bruno_dev_free(void* p);
+/* kind: 0 host->device, 1 device->host, 2 device->device */
+bruno_status bruno_dev_copy(void* dst, const void* src, uint64_t bytes, int kind);
+bruno_status bruno_dev_sync(void);
+/* free / total HBM bytes of the current device */
+bruno_status bruno_dev_mem_info(uint64_t* free_bytes, uint64_t* total_bytes);
+
+/* ---- Synthetic workload generator (the benchmark configurations) ------ */
+typedef struct {
+  uint64_t seed;
+  uint64_t genome_len;       /* bases                                          */
+  uint32_t n_families;       /* repeat families (0 = none)                     */
+  uint32_t family_min_len, family_max_len;
+  double repeat_fraction;    /* fraction of the genome covered by copies       */
+  double family_divergence;  /* substitution rate between copies               */
+  uint64_t n_reads;
+  uint32_t read_len;
+  double sub_rate;           /* per-base substitution error                    */
+  double indel_rate;         /* per-base 1-bp insertion/deletion rate          */
+} bruno_synth_desc;
+
+/* Generate reads directly in device memory (d_bases: n_reads*read_len bytes,
+ * d_offsets: n_reads+1). */
+bruno_status bruno_synth_reads_device(const bruno_synth_desc* d, char* d_bases, uint64_t* d_offsets,
+                                      void* stream);
+/* Same on the host (bit-identical to the device version). */
+bruno_status bruno_synth_reads_host(const bruno_synth_desc* d, char* bases, uint64_t* offsets);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -43,13 +43,17 @@ struct KmerOps {
   inline int base(u128 v, int p) const { return (int)((v >> (2 * (K - 1 - p))) & 3); }
   /* getCharsAtPos(pos, 1): pos 0 is the LAST base (debruijn_graph_map.hpp:442). */
   inline int chars_at_pos(u128 v, int pos) const { return (int)((v >> (2 * pos)) & 3); }
+  /* reverse complement: complement every 2-bit code (3 - c == ~c & 3) and
+   * reverse the order of the codes (swap pairs, nibbles, then bytes). */
+  static uint64_t rc64(uint64_t x) {
+    x = ~x;
+    x = ((x >> 2) & 0x3333333333333333ULL) | ((x & 0x3333333333333333ULL) << 2);
+    x = ((x >> 4) & 0x0F0F0F0F0F0F0F0FULL) | ((x & 0x0F0F0F0F0F0F0F0FULL) << 4);
+    return __builtin_bswap64(x);
+  }
   u128 rc(u128 v) const {
-    u128 r = 0;
-    for (int i = 0; i < K; ++i) {
-      r = (r << 2) | (u128)(3 - (int)(v & 3));
-      v >>= 2;
-    }
-    return r;
+    u128 r = ((u128)rc64((uint64_t)v) << 64) | (u128)rc64((uint64_t)(v >> 64));
+    return r >> (128 - 2 * K);
   }
   /* nextFromChar(c): shift in c at the end (out neighbour). */
   inline u128 next_from_char(u128 v, int c) const { return ((v << 2) | (u128)c) & mask; }
@@ -253,6 +257,7 @@ struct oracle_job {
   bool thresholding;
   uint64_t th[6];
   int nthreads;
+  uint64_t order_seed = 0;  /* 0: generation order; else shuffle updates */
 
   std::vector<std::string> reads;  // kept reads (ACGT only)
   std::vector<Rec> records;        // P0 (canonical)
@@ -621,19 +626,48 @@ struct oracle_job {
     int orient;
   };
 
-  size_t apply_updates(const std::vector<Upd>& ups) {
-    size_t count = 0;
-    for (auto u : ups) {
-      /* lex_less on (key, chain_update_md): rc key and md k-mer, negate the
-       * orientation (debruijn_chain_operations.hpp:500-505). */
-      if (canon_key(u.key)) {
-        u.k = ops.rc(u.k);
-        u.orient = -u.orient;
+  /* Applies the round's updates in generation order.  With nthreads > 1 the
+   * canonicalisation and lookups run in parallel and each thread applies, in
+   * generation order, the updates whose target falls in its index range, so
+   * the result is the same as the sequential loop. */
+  size_t apply_updates(std::vector<Upd>& ups) {
+    std::vector<long> tgt(ups.size());
+    auto prep = [&](size_t b, size_t e) {
+      for (size_t q = b; q < e; ++q) {
+        Upd& u = ups[q];
+        /* lex_less on (key, chain_update_md): rc key and md k-mer, negate the
+         * orientation (debruijn_chain_operations.hpp:500-505). */
+        if (canon_key(u.key)) {
+          u.k = ops.rc(u.k);
+          u.orient = -u.orient;
+        }
+        tgt[q] = chain.find(u.key);
       }
-      long i = chain.find(u.key);
-      if (i < 0) continue;
-      count += chain_update(chain.vals[i], u.k, u.d, u.orient);
+    };
+    int T = (nthreads > 1 && ups.size() > 100000) ? nthreads : 1;
+    if (T == 1) {
+      prep(0, ups.size());
+      size_t count = 0;
+      for (size_t q = 0; q < ups.size(); ++q)
+        if (tgt[q] >= 0) count += chain_update(chain.vals[tgt[q]], ups[q].k, ups[q].d, ups[q].orient);
+      return count;
     }
+    std::vector<std::thread> ts;
+    for (int t = 0; t < T; ++t)
+      ts.emplace_back([&, t]() { prep(ups.size() * t / T, ups.size() * (t + 1) / T); });
+    for (auto& x : ts) x.join();
+    ts.clear();
+    std::vector<size_t> cnt(T, 0);
+    long n = (long)chain.keys.size();
+    for (int t = 0; t < T; ++t)
+      ts.emplace_back([&, t]() {
+        long lo = n * t / T, hi = n * (t + 1) / T;
+        for (size_t q = 0; q < ups.size(); ++q)
+          if (tgt[q] >= lo && tgt[q] < hi) cnt[t] += chain_update(chain.vals[tgt[q]], ups[q].k, ups[q].d, ups[q].orient);
+      });
+    for (auto& x : ts) x.join();
+    size_t count = 0;
+    for (auto c : cnt) count += c;
     return count;
   }
 
@@ -677,6 +711,14 @@ struct oracle_job {
         if (send_out)
           ups.push_back(Upd{md.out, is_chain_terminal(md.din) ? km : md.in,
                             points_to_chain_node(md.din) ? dist : mark_dist_as_point_to_terminal(dist), -1});
+      }
+      if (order_seed) {
+        /* test knob: apply this round's updates in a shuffled order */
+        uint64_t x = order_seed + iters * 0x9e3779b97f4a7c15ULL;
+        for (size_t i = ups.size(); i > 1; --i) {
+          x ^= x << 13; x ^= x >> 7; x ^= x << 17;
+          std::swap(ups[i - 1], ups[x % i]);
+        }
       }
       size_t count = apply_updates(ups);
       std::stable_partition(unfinished.begin(), unfinished.end(),
@@ -822,6 +864,7 @@ int oracle_add_reads(oracle_job* j, const char* bases, const uint64_t* offsets, 
 }
 
 int oracle_run(oracle_job* j, int lr_variant) { return j->run(lr_variant); }
+void oracle_set_order_seed(oracle_job* j, uint64_t seed) { j->order_seed = seed; }
 
 uint64_t oracle_n_instances(oracle_job* j) { return j->records.size(); }
 uint64_t oracle_n_k2(oracle_job* j) { return j->k2.size(); }

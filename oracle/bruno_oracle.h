@@ -66,6 +66,11 @@ int oracle_add_reads(oracle_job* j, const char* bases, const uint64_t* offsets,
  * 0 = list_rank, 1 = list_rank_min_update), output.  Returns 0 on success. */
 int oracle_run(oracle_job* j, int lr_variant);
 
+/* Test knob: apply each list-ranking round's updates in a shuffled order
+ * (seed != 0).  The reference applies them in hash-table / rank order, which
+ * is unspecified; outputs must not depend on it. */
+void oracle_set_order_seed(oracle_job* j, uint64_t seed);
+
 /* Statistics. */
 uint64_t oracle_n_instances(oracle_job* j);  /* parsed (k-mer, biedge) records */
 uint64_t oracle_n_k2(oracle_job* j);         /* distinct canonical k2-mers     */

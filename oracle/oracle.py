@@ -62,6 +62,7 @@ def lib():
             f.restype = u64
             f.argtypes = [vp, vp, u64]
         L.oracle_phase_seconds.argtypes = [vp, vp]
+        L.oracle_set_order_seed.argtypes = [vp, u64]
         _LIB = L
     return _LIB
 
@@ -91,6 +92,9 @@ class Oracle:
                                  dtype=np.uint8)
         o = np.ascontiguousarray(offsets, dtype=np.uint64)
         lib().oracle_add_reads(self.h, b.ctypes.data_as(ctypes.c_char_p), o.ctypes.data, o.size - 1)
+
+    def set_order_seed(self, seed):
+        lib().oracle_set_order_seed(self.h, seed)
 
     def run(self, lr_variant=1):
         rc = lib().oracle_run(self.h, lr_variant)

@@ -1,0 +1,39 @@
+"""CPU-side checks of the C ABI boundary: the library loads and exports
+every entry point include/bruno_hip.h declares (no compute call)."""
+import os
+import re
+
+import bruno_amd
+from helpers import ROOT
+
+
+def declared():
+    with open(os.path.join(ROOT, "include", "bruno_hip.h")) as f:
+        txt = f.read()
+    return sorted(set(re.findall(r"\b(bruno_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = bruno_amd.lib()
+    names = declared()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(L, n), n
+    assert sorted(bruno_amd.EXPORTS) == names
+
+
+def test_status_strings_and_bad_input():
+    L = bruno_amd.lib()
+    assert L.bruno_status_string(0) == b"ok"
+    assert L.bruno_status_string(3).startswith(b"capacity")
+    # NULL descriptor is rejected before any device call
+    assert L.bruno_job_create(None, None) == 1
+
+
+def test_host_generator_is_deterministic():
+    from bruno_amd.configs import config2
+    c = config2(seed=4, scale=0.0005)["synth"]
+    a = bruno_amd.synth_reads_host(c)
+    b = bruno_amd.synth_reads_host(c)
+    assert (a[0] == b[0]).all() and (a[1] == b[1]).all()
+    assert set(bytes(a[0])) <= set(b"ACGT")
