@@ -73,6 +73,9 @@ def main():
     ap.add_argument("--variant", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="after the timed steps, copy 4 GiB device-to-device once (a plain copy kernel of "
+                         "known size, to scale FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -139,7 +142,7 @@ def main():
         },
         "latency_ms": {"p50": float(np.percentile(times, 50) * 1e3), "p99": float(np.percentile(times, 99) * 1e3)},
         "phases_ms": ph,
-        "graph": {kk: stats[kk] for kk in ("n_distinct", "n_k2", "n_nodes", "n_branches", "n_chain_nodes",
+        "graph": {kk: stats[kk] for kk in ("n_distinct", "n_k2", "n_overflow", "n_nodes", "n_branches", "n_chain_nodes",
                                            "n_unitigs", "n_cycle_nodes", "iterations", "capacity", "device_bytes")},
         "roofline": {
             "kernel": "k_parse (parse + k2 count)",
@@ -154,6 +157,15 @@ def main():
     job.close()
     bases.free()
     offs.free()
+    if args.calibrate_copy:
+        import ctypes
+        nb = 4 << 30
+        src, dst = DeviceBuffer(nb), DeviceBuffer(nb)
+        bruno_amd.lib().bruno_dev_copy(ctypes.c_void_p(dst.ptr), ctypes.c_void_p(src.ptr), nb, 2)
+        synchronize()
+        out["calibration_copy_bytes"] = nb
+        src.free()
+        dst.free()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_seconds)
     else:

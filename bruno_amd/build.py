@@ -25,7 +25,25 @@ def build(verbose=False):
     os.replace(tmp, OUT)
     if verbose:
         print("built", OUT)
+    build_driver(verbose)
     return OUT
+
+
+DRIVER_SRC = os.path.join(HERE, "host", "compact_dbg.cpp")
+DRIVER = os.path.join(HERE, "host", "compact_dbg")
+
+
+def build_driver(verbose=False):
+    """The C++ driver over the C ABI (links libbruno_hip.so, rpath $ORIGIN/..)."""
+    cmd = ["g++", "-O2", "-std=c++17", "-o", DRIVER, DRIVER_SRC, "-L" + HERE, "-lbruno_hip",
+           "-Wl,-rpath,$ORIGIN/.."]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("g++ failed building the compact_dbg driver")
+    if verbose:
+        print("built", DRIVER)
+    return DRIVER
 
 
 if __name__ == "__main__":

@@ -25,53 +25,126 @@ namespace {
 /* ================================================================== */
 /* filter + classify (one thread per table slot)                        */
 /* ================================================================== */
+/* Classify a surviving node and write its record in place. */
 template <typename KT>
-__global__ __launch_bounds__(BLOCK) void k_filter(Slot<KT>* T, uint64_t cap, const Spill* sp, uint64_t spcap,
-                                                  int K, int thresholding, Thresh th, uint64_t* present,
-                                                  uint64_t present_cap, Counters* ctr) {
+__device__ __forceinline__ uint8_t finish_node(Slot<KT>* p, const kops<KT>& ops, KT k, const uint16_t (&c)[9]) {
+  bool pal = ops.is_rc_palindrome(k) || kplus1_pal<KT>(ops, k, c);
+  int ni = n_in_edges(c), no = n_out_edges(c);
+  uint8_t cls = (ni >= 2 || no >= 2 || pal) ? CLS_BRANCH : CLS_CHAIN;
+  NodeRec rec;
+#pragma unroll
+  for (int q = 0; q < 9; ++q) rec.c[q] = c[q];
+  rec.cls = cls;
+  rec.pad = 0;
+  rec.idx = 0xFFFFFFFFu;
+  *(NodeRec*)&p->sub[0] = rec;
+  uint64_t* fw = SlotOps<KT>::flag_word(p);
+  *fw = *fw | F_PRESENT;
+  return cls;
+}
+
+constexpr int FILTER_SPT = 4; /* slots per thread per reservation */
+
+/* Main filter pass over the table: slots whose k2 entries all sit in the
+ * slot (the common case) are filtered here with register-only lists; slots
+ * with spilled entries are queued for k_filter_spilled. */
+template <typename KT>
+__global__ __launch_bounds__(BLOCK) void k_filter(Slot<KT>* T, uint64_t cap, int K, int thresholding, Thresh th,
+                                                  uint64_t* present, uint64_t present_cap, uint64_t* spilled,
+                                                  uint64_t spilled_cap, Counters* ctr) {
   __shared__ ReserveSmem sm;
   kops<KT> ops(K);
   unsigned long long nb = 0, nc = 0;
-  const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
-  for (uint64_t base = (uint64_t)blockIdx.x * BLOCK; base < cap; base += stride) {
-    const uint64_t s = base + threadIdx.x;
-    bool keep = false;
-    if (s < cap) {
+  const uint64_t tile = (uint64_t)BLOCK * FILTER_SPT;
+  const uint64_t stride = (uint64_t)gridDim.x * tile;
+  for (uint64_t base = (uint64_t)blockIdx.x * tile; base < cap; base += stride) {
+    uint32_t kept = 0, spl = 0;
+#pragma unroll
+    for (int q = 0; q < FILTER_SPT; ++q) {
+      const uint64_t s = base + (uint64_t)q * BLOCK + threadIdx.x;
+      if (s >= cap) continue;
       Slot<KT>* p = T + s;
-      if (SlotOps<KT>::occupied(p)) {
-        K2List L;
-        gather_k2<KT>(p, s, sp, spcap, L);
-        KT k = SlotOps<KT>::kmer(p);
-        uint16_t c[9];
-        if (thresholding) {
-          keep = p2_counts<KT>(ops, k, L, th, c);
-        } else {
-          p1_counts(L, c);
-          keep = true;
-        }
-        if (keep) {
-          bool pal = ops.is_rc_palindrome(k) || kplus1_pal<KT>(ops, k, c);
-          int ni = n_in_edges(c), no = n_out_edges(c);
-          uint8_t cls = (ni >= 2 || no >= 2 || pal) ? CLS_BRANCH : CLS_CHAIN;
-          nb += (cls == CLS_BRANCH);
-          nc += (cls == CLS_CHAIN);
-          NodeRec* nr = (NodeRec*)&p->sub[0];
-          NodeRec rec;
-          for (int q = 0; q < 9; ++q) rec.c[q] = c[q];
-          rec.cls = cls;
-          rec.pad = 0;
-          rec.idx = 0xFFFFFFFFu;
-          *nr = rec;
-          uint64_t* fw = SlotOps<KT>::flag_word(p);
-          *fw = *fw | F_PRESENT;
-        }
+      if (!SlotOps<KT>::occupied(p)) continue;
+      if (SlotOps<KT>::flags(p) & F_SPILL) {
+        spl |= 1u << q;
+        continue;
       }
+      K2L<Slot<KT>::NSUB> L;
+      uint64_t kf = sub_entries<KT, Slot<KT>::NSUB>(p, L);
+      /* every edge count is at most the k-mer count, so a k-mer seen fewer
+       * than k1_lo times has both edges of every k2-mer out of range
+       * (flag 3, debruijn_graph_map.hpp:1573): nothing survives */
+      if (thresholding && kf < th.t1lo) continue;
+      KT k = SlotOps<KT>::kmer(p);
+      uint16_t c[9];
+      bool keep;
+      if (thresholding) {
+        keep = p2_counts<KT, Slot<KT>::NSUB>(ops, k, L, th, c);
+      } else {
+        p1_counts(L, c);
+        keep = true;
+      }
+      if (!keep) continue;
+      uint8_t cls = finish_node<KT>(p, ops, k, c);
+      nb += (cls == CLS_BRANCH);
+      nc += (cls == CLS_CHAIN);
+      kept |= 1u << q;
     }
-    unsigned long long pos = block_reserve(keep ? 1u : 0u, &ctr->n_present, sm);
-    if (keep) {
-      if (pos < present_cap) present[pos] = s;
-      else atomicOr(&ctr->err, ERR_LIST_FULL);
+    unsigned long long pos = block_reserve((uint32_t)__popc(kept), &ctr->n_present, sm);
+#pragma unroll
+    for (int q = 0; q < FILTER_SPT; ++q)
+      if (kept & (1u << q)) {
+        if (pos < present_cap) present[pos] = base + (uint64_t)q * BLOCK + threadIdx.x;
+        else atomicOr(&ctr->err, ERR_LIST_FULL);
+        ++pos;
+      }
+    if (__syncthreads_or(spl != 0)) {
+      unsigned long long sp = block_reserve((uint32_t)__popc(spl), &ctr->n_spill, sm);
+#pragma unroll
+      for (int q = 0; q < FILTER_SPT; ++q)
+        if (spl & (1u << q)) {
+          if (sp < spilled_cap) spilled[sp] = base + (uint64_t)q * BLOCK + threadIdx.x;
+          else atomicOr(&ctr->err, ERR_LIST_FULL);
+          ++sp;
+        }
     }
+  }
+  block_add(nb, &ctr->n_branch);
+  __syncthreads();
+  block_add(nc, &ctr->n_chain);
+}
+
+/* Slots with spilled k2 entries: gather up to 25 entries from the slot and
+ * the side table, then the same filter. */
+template <typename KT>
+__global__ __launch_bounds__(BLOCK) void k_filter_spilled(Slot<KT>* T, const Spill* sp, uint64_t spcap, int K,
+                                                          int thresholding, Thresh th, const uint64_t* spilled,
+                                                          uint64_t n_spilled, uint64_t* present, uint64_t present_cap,
+                                                          Counters* ctr) {
+  kops<KT> ops(K);
+  unsigned long long nb = 0, nc = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+  for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < n_spilled; i += stride) {
+    const uint64_t s = spilled[i];
+    Slot<KT>* p = T + s;
+    K2List L;
+    gather_k2<KT>(p, s, sp, spcap, L);
+    KT k = SlotOps<KT>::kmer(p);
+    uint16_t c[9];
+    bool keep;
+    if (thresholding) {
+      keep = p2_counts<KT, 25>(ops, k, L, th, c);
+    } else {
+      p1_counts(L, c);
+      keep = true;
+    }
+    if (!keep) continue;
+    uint8_t cls = finish_node<KT>(p, ops, k, c);
+    nb += (cls == CLS_BRANCH);
+    nc += (cls == CLS_CHAIN);
+    unsigned long long pos = atomicAdd(&ctr->n_present, 1ULL);
+    if (pos < present_cap) present[pos] = s;
+    else atomicOr(&ctr->err, ERR_LIST_FULL);
   }
   block_add(nb, &ctr->n_branch);
   __syncthreads();
@@ -718,9 +791,20 @@ bruno_status filter_t(bruno_job* j) {
   /* present list: at most the distinct k-mers */
   uint64_t n_distinct = j->hctr->n_distinct;
   TRY(dev_alloc(j, (void**)&j->present, std::max<uint64_t>(1, n_distinct) * sizeof(uint64_t)));
-  hipLaunchKernelGGL((k_filter<KT>), dim3(grid_for(j->cap, BLOCK, 1u << 15)), dim3(BLOCK), 0, j->stream,
-                     (Slot<KT>*)j->table, j->cap, (const Spill*)j->spill, j->spcap, j->K, j->thresholding, j->th,
-                     j->present, std::max<uint64_t>(1, n_distinct), j->ctr);
+  /* slots with spilled entries: at most the side-table entries */
+  uint64_t n_spill_cap = std::max<uint64_t>(1, std::min<uint64_t>(j->hctr->n_k2, j->spcap));
+  uint64_t* spilled = nullptr;
+  TRY(dev_alloc(j, (void**)&spilled, n_spill_cap * sizeof(uint64_t)));
+  HIP_CHECK(hipMemsetAsync(&j->ctr->n_spill, 0, sizeof(unsigned long long), j->stream));
+  hipLaunchKernelGGL((k_filter<KT>), dim3(grid_for((j->cap + FILTER_SPT - 1) / FILTER_SPT, BLOCK, 1u << 15)),
+                     dim3(BLOCK), 0, j->stream, (Slot<KT>*)j->table, j->cap, j->K, j->thresholding, j->th,
+                     j->present, std::max<uint64_t>(1, n_distinct), spilled, n_spill_cap, j->ctr);
+  TRY(sync_counters(j));
+  uint64_t n_spilled = j->hctr->n_spill;
+  if (n_spilled)
+    hipLaunchKernelGGL((k_filter_spilled<KT>), dim3(grid_for(n_spilled)), dim3(BLOCK), 0, j->stream,
+                       (Slot<KT>*)j->table, (const Spill*)j->spill, j->spcap, j->K, j->thresholding, j->th, spilled,
+                       n_spilled, j->present, std::max<uint64_t>(1, n_distinct), j->ctr);
   TRY(sync_counters(j));
   j->n_present = j->hctr->n_present;
   j->n_branch = j->hctr->n_branch;
@@ -1203,6 +1287,7 @@ bruno_status bruno_get_stats(bruno_job* j, bruno_stats* o) {
   o->n_instances = j->hctr->n_instances;
   o->n_distinct = j->hctr->n_distinct;
   o->n_k2 = j->hctr->n_k2;
+  o->n_overflow = j->hctr->n_spill;
   o->n_nodes = j->n_present;
   o->n_branches = j->n_branch;
   o->n_chain_nodes = j->n_chain;

@@ -287,24 +287,38 @@ __device__ __constant__ uint8_t BIEDGE25[25] = {
     0x00, 0x01, 0x02, 0x04, 0x08, 0x10, 0x11, 0x12, 0x14, 0x18, 0x20, 0x21, 0x22,
     0x24, 0x28, 0x40, 0x41, 0x42, 0x44, 0x48, 0x80, 0x81, 0x82, 0x84, 0x88};
 
-struct K2List {
-  uint8_t e[25];
-  uint32_t c[25];
+/* k2 entries of one slot.  N is a compile-time bound so that every array
+ * index below is a constant after unrolling and the lists stay in registers
+ * (N = NSUB for slots without spilled entries, 25 for the rest). */
+template <int N>
+struct K2L {
+  uint8_t e[N];
+  uint32_t c[N];
   int n;
 };
+typedef K2L<25> K2List;
 
-template <typename KT>
-__device__ __forceinline__ void gather_k2(const Slot<KT>* p, uint64_t s, const Spill* sp, uint64_t spcap, K2List& L) {
+template <typename KT, int N>
+__device__ __forceinline__ uint64_t sub_entries(const Slot<KT>* p, K2L<N>& L) {
+  static_assert(N >= Slot<KT>::NSUB, "list too small");
   L.n = 0;
+  uint64_t kf = 0;
 #pragma unroll
   for (int j = 0; j < Slot<KT>::NSUB; ++j) {
     uint64_t v = p->sub[j];
-    if (v != EMPTY64) {
-      L.e[L.n] = (uint8_t)((v >> 48) - 1);
-      L.c[L.n] = (uint32_t)(v & 0xFFFFFFFFFFFFULL);
-      ++L.n;
-    }
+    bool ok = v != EMPTY64;
+    /* sub-entries fill in order, so the valid ones are a prefix */
+    L.e[j] = ok ? (uint8_t)((v >> 48) - 1) : 0;
+    L.c[j] = ok ? (uint32_t)(v & 0xFFFFFFFFFFFFULL) : 0;
+    L.n += ok;
+    kf += ok ? (v & 0xFFFFFFFFFFFFULL) : 0;
   }
+  return kf;
+}
+
+template <typename KT>
+__device__ __forceinline__ void gather_k2(const Slot<KT>* p, uint64_t s, const Spill* sp, uint64_t spcap, K2List& L) {
+  sub_entries<KT, 25>(p, L);
   if (SlotOps<KT>::flags(p) & F_SPILL) {
     for (int t = 0; t < 25 && L.n < 25; ++t) {
       uint8_t e = BIEDGE25[t];
@@ -326,19 +340,36 @@ __device__ __forceinline__ uint16_t sat_add16(uint32_t a, uint32_t b) {
   uint32_t c = a + b;
   return (uint16_t)(c > 65535u ? 65535u : c);
 }
-/* compact_multi_biedge::update(edges, cnt) (debruijn_graph_node.hpp:200-211) */
-__device__ __forceinline__ void node_update(uint16_t* c, uint8_t e, uint32_t cnt) {
-  int o = nib_idx(e & 0xF);
-  if (o < 4) c[o] = sat_add16(c[o], cnt);
-  int i = nib_idx(e >> 4);
-  if (i < 4) c[4 + i] = sat_add16(c[4 + i], cnt);
+/* compact_multi_biedge::update(edges, cnt) (debruijn_graph_node.hpp:200-211),
+ * written with constant indices only */
+__device__ __forceinline__ void node_update(uint16_t (&c)[9], uint8_t e, uint32_t cnt) {
+  int o = nib_idx(e & 0xF), i = nib_idx(e >> 4);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if (q == o) c[q] = sat_add16(c[q], cnt);
+    if (q == i) c[4 + q] = sat_add16(c[4 + q], cnt);
+  }
   c[8] = sat_add16(c[8], cnt);
+}
+__device__ __forceinline__ void add_sel4(uint64_t (&a)[4], int idx, uint64_t v) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q) a[q] += (q == idx) ? v : 0;
+}
+__device__ __forceinline__ uint64_t get_sel4(const uint64_t (&a)[4], int idx, uint64_t dflt) {
+  uint64_t r = dflt;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) r = (q == idx) ? a[q] : r;
+  return r;
 }
 
 /* P1: unthresholded counters from the k2 list. */
-__device__ __forceinline__ void p1_counts(const K2List& L, uint16_t* c) {
+template <int N>
+__device__ __forceinline__ void p1_counts(const K2L<N>& L, uint16_t (&c)[9]) {
+#pragma unroll
   for (int t = 0; t < 9; ++t) c[t] = 0;
-  for (int j = 0; j < L.n; ++j) node_update(c, L.e[j], L.c[j] > 65535u ? 65535u : L.c[j]);
+#pragma unroll
+  for (int j = 0; j < N; ++j)
+    if (j < L.n) node_update(c, L.e[j], L.c[j] > 65535u ? 65535u : L.c[j]);
 }
 
 struct Thresh {
@@ -347,14 +378,16 @@ struct Thresh {
 
 /* local_insert_by_freqencies for one k-mer block (debruijn_graph_map.hpp
  * :1351-1609).  Returns true if any k2-mer is kept. */
-template <typename KT>
-__device__ bool p2_counts(const kops<KT>& ops, KT k, const K2List& L, const Thresh& t, uint16_t* c) {
+template <typename KT, int N>
+__device__ bool p2_counts(const kops<KT>& ops, KT k, const K2L<N>& L, const Thresh& t, uint16_t (&c)[9]) {
   const uint64_t cnt_max = 65535;
   bool kpal = ops.is_rc_palindrome(k);
   bool high = ops.km1_high(k), low = ops.km1_low(k);
   int lastb = ops.last(k), firstb = ops.first(k);
   uint64_t kf = 0, k1in[4] = {0, 0, 0, 0}, k1out[4] = {0, 0, 0, 0};
-  for (int j = 0; j < L.n; ++j) {
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    if (j >= L.n) break;
     uint64_t cu = L.c[j];
     kf += cu;
     int ch = L.e[j] >> 4;
@@ -362,31 +395,35 @@ __device__ bool p2_counts(const kops<KT>& ops, KT k, const K2List& L, const Thre
       int ci = nib_idx(ch);
       /* is_k1_rc_palindrome(l, k, high): last base == comp(l) && high */
       bool split = ((ops.K & 1) && lastb == 3 - ci && high) && !kpal;
-      k1in[ci] += split ? ((cu + 1) >> 1) : cu;
+      add_sel4(k1in, ci, split ? ((cu + 1) >> 1) : cu);
     }
     ch = L.e[j] & 0xF;
     if (ch) {
       int ci = nib_idx(ch);
       bool split = ((ops.K & 1) && firstb == 3 - ci && low) && !kpal;
-      k1out[ci] += split ? ((cu + 1) >> 1) : cu;
+      add_sel4(k1out, ci, split ? ((cu + 1) >> 1) : cu);
     }
   }
   kf = kf < cnt_max ? kf : cnt_max;
+#pragma unroll
   for (int q = 0; q < 4; ++q) {
     k1in[q] = k1in[q] < cnt_max ? k1in[q] : cnt_max;
     k1out[q] = k1out[q] < cnt_max ? k1out[q] : cnt_max;
   }
+#pragma unroll
   for (int q = 0; q < 9; ++q) c[q] = 0;
   bool have = false;
-  for (int j = 0; j < L.n; ++j) {
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    if (j >= L.n) break;
     uint8_t flag = 0;
     uint64_t cnt = L.c[j] < cnt_max ? L.c[j] : cnt_max;
     if (cnt < t.t2lo || cnt >= t.t2hi) flag |= 4;
     if (kf < t.t0lo || kf >= t.t0hi) flag |= 8;
     int in = L.e[j] >> 4, out = L.e[j] & 0xF;
-    uint64_t k1 = (nib_idx(in) < 4) ? k1in[nib_idx(in)] : t.t1hi;
+    uint64_t k1 = get_sel4(k1in, nib_idx(in), t.t1hi);
     if (k1 < t.t1lo || k1 >= t.t1hi) flag |= 2;
-    k1 = (nib_idx(out) < 4) ? k1out[nib_idx(out)] : t.t1hi;
+    k1 = get_sel4(k1out, nib_idx(out), t.t1hi);
     if (k1 < t.t1lo || k1 >= t.t1hi) flag |= 1;
     if (flag >= 3) continue;
     uint8_t e = L.e[j];

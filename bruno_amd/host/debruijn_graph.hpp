@@ -1,0 +1,202 @@
+/*
+ * debruijn_graph.hpp -- C++ host façade over the C ABI (include/bruno_hip.h).
+ *
+ * Keeps the class and method names of bruno's graph map, graph and chain graph
+ * (src/debruijn/debruijn_graph_map.hpp, debruijn_graph.hpp,
+ * debruijn_chain_graph.hpp) so that a driver written against bruno reads the
+ * same, while every method body is one call into libbruno_hip.so.  Errors
+ * surface as bruno_amd::error (the reference throws std::logic_error).
+ */
+#ifndef BRUNO_AMD_DEBRUIJN_GRAPH_HPP_
+#define BRUNO_AMD_DEBRUIJN_GRAPH_HPP_
+
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/bruno_hip.h"
+
+namespace bruno_amd {
+
+struct error : std::runtime_error {
+  bruno_status status;
+  error(bruno_status s, const std::string& what)
+      : std::runtime_error(what + ": " + bruno_status_string(s)), status(s) {}
+};
+
+inline void check(bruno_status s, const char* what) {
+  if (s != BRUNO_OK) throw error(s, what);
+}
+
+/* Reads in host memory: concatenated bases and n+1 offsets. */
+struct read_batch {
+  std::string bases;
+  std::vector<uint64_t> offsets{0};
+  void add(const std::string& s) {
+    bases += s;
+    offsets.push_back(bases.size());
+  }
+  uint64_t size() const { return offsets.size() - 1; }
+};
+
+/* A k-mer value as the ABI returns it (words least significant first). */
+struct kmer_value {
+  uint64_t w[2] = {0, 0};
+  bool operator<(const kmer_value& o) const { return w[1] != o.w[1] ? w[1] < o.w[1] : w[0] < o.w[0]; }
+  bool operator==(const kmer_value& o) const { return w[0] == o.w[0] && w[1] == o.w[1]; }
+  std::string to_string(int k) const {
+    static const char A[4] = {'A', 'C', 'G', 'T'};
+    std::string s(k, 'A');
+    for (int i = 0; i < k; ++i) {
+      int sh = 2 * (k - 1 - i);
+      uint64_t code = sh >= 64 ? (w[1] >> (sh - 64)) : (w[0] >> sh);
+      s[i] = A[code & 3];
+    }
+    return s;
+  }
+};
+
+/* Shared device job (one per graph + chain graph pair). */
+class job {
+ public:
+  job(int k, const std::vector<size_t>* threshes, uint64_t capacity = 0, void* stream = nullptr) : k_(k) {
+    bruno_job_desc d{};
+    d.k = k;
+    d.thresholding = threshes ? 1 : 0;
+    for (int i = 0; i < 6; ++i) d.thresholds[i] = threshes ? (*threshes)[i] : 65536;
+    d.capacity = capacity;
+    d.load_factor = 0.5;
+    d.stream = stream;
+    bruno_job* j = nullptr;
+    check(bruno_job_create(&d, &j), "bruno_job_create");
+    h_.reset(j, [](bruno_job* p) { bruno_job_destroy(p); });
+  }
+  bruno_job* get() const { return h_.get(); }
+  int k() const { return k_; }
+
+ private:
+  std::shared_ptr<bruno_job> h_;
+  int k_;
+};
+
+/* debruijn_graph_map (debruijn_graph_map.hpp:52-1626) */
+class debruijn_graph_map {
+ public:
+  explicit debruijn_graph_map(job& j) : j_(j) {}
+  /* compute_biedge_freqencies (:1085-1141): parse + count one batch */
+  size_t compute_biedge_freqencies(const read_batch& reads) {
+    check(bruno_parse_count_host(j_.get(), reads.bases.data(), reads.offsets.data(), reads.size()),
+          "compute_biedge_freqencies");
+    return k2_size();
+  }
+  /* insert (:871-921), the unthresholded build_index path */
+  size_t insert(const read_batch& reads) { return compute_biedge_freqencies(reads); }
+  /* local_insert_by_freqencies (:1258-1623); thresholds come from the job */
+  size_t local_insert_by_freqencies() {
+    check(bruno_filter(j_.get()), "local_insert_by_freqencies");
+    return local_size();
+  }
+  size_t k2_size() const { return stats().n_k2; }
+  size_t local_size() const { return stats().n_nodes; }
+  bruno_stats stats() const {
+    bruno_stats s;
+    check(bruno_get_stats(j_.get(), &s), "bruno_get_stats");
+    return s;
+  }
+
+ private:
+  job& j_;
+};
+
+/* debruijn_graph (debruijn_graph.hpp:46-295) */
+class debruijn_graph {
+ public:
+  explicit debruijn_graph(job& j) : j_(j), map_(j) {}
+  debruijn_graph_map& get_map() { return map_; }
+  size_t local_size() const { return map_.local_size(); }
+  size_t size() const { return local_size(); }
+  /* nodes with their 9 counters (print_graph_node order) */
+  std::vector<bruno_node> nodes() const {
+    uint64_t n = 0;
+    check(bruno_get_nodes(j_.get(), nullptr, &n), "bruno_get_nodes");
+    std::vector<bruno_node> v(n);
+    if (n) check(bruno_get_nodes(j_.get(), v.data(), &n), "bruno_get_nodes");
+    return v;
+  }
+  job& get_job() { return j_; }
+
+ private:
+  job& j_;
+  debruijn_graph_map map_;
+};
+
+/* listranked_chain_node (debruijn_chain_node.hpp:156) */
+struct listranked_chain_node {
+  kmer_value kmer, rep;
+  int dist;
+};
+
+/* debruijn_chain_graph (debruijn_chain_graph.hpp:126-2284) */
+class debruijn_chain_graph {
+ public:
+  explicit debruijn_chain_graph(job& j) : j_(j) {}
+  /* extract_chains (:605-667) incl. setup_chain_termini (:486-596) */
+  void extract_chains(debruijn_graph&) { check(bruno_build_chains(j_.get()), "extract_chains"); }
+  /* list_rank_min_update (:1040-1310) */
+  size_t list_rank_min_update() { return rank(1); }
+  /* list_rank (:793-994) */
+  size_t list_rank() { return rank(0); }
+  /* to_ranked_chain_nodes (:1760-1773), sorted by chain_rep_less */
+  std::vector<listranked_chain_node> to_ranked_chain_nodes() {
+    compact();
+    uint64_t n = 0;
+    check(bruno_get_chain_nodes(j_.get(), nullptr, &n), "to_ranked_chain_nodes");
+    std::vector<bruno_chain_node> v(n);
+    if (n) check(bruno_get_chain_nodes(j_.get(), v.data(), &n), "to_ranked_chain_nodes");
+    std::vector<listranked_chain_node> out(n);
+    for (uint64_t i = 0; i < n; ++i) {
+      out[i].kmer.w[0] = v[i].k[0];
+      out[i].kmer.w[1] = v[i].k[1];
+      out[i].rep.w[0] = v[i].rep[0];
+      out[i].rep.w[1] = v[i].rep[1];
+      out[i].dist = v[i].dist;
+    }
+    return out;
+  }
+  /* the .chain.fasta bytes of print_chain_string (compact_dbg_io.hpp:323-363) */
+  std::string chain_fasta() {
+    compact();
+    return text(bruno_get_unitigs);
+  }
+  /* the .branch.fasta bytes of print_branch_fasta (compact_dbg_io.hpp:254-299) */
+  std::string branch_fasta() {
+    compact();
+    return text(bruno_get_branch_fasta);
+  }
+
+ private:
+  size_t rank(int variant) {
+    uint32_t r = 0;
+    check(bruno_list_rank(j_.get(), variant, &r), "list_rank");
+    return r;
+  }
+  void compact() {
+    if (!compacted_) check(bruno_compact(j_.get()), "bruno_compact");
+    compacted_ = true;
+  }
+  std::string text(bruno_status (*fn)(bruno_job*, char*, uint64_t*)) {
+    uint64_t n = 0;
+    check(fn(j_.get(), nullptr, &n), "getter");
+    std::string s(n, '\0');
+    if (n) check(fn(j_.get(), &s[0], &n), "getter");
+    return s;
+  }
+  job& j_;
+  bool compacted_ = false;
+};
+
+}  // namespace bruno_amd
+
+#endif

@@ -74,7 +74,7 @@ typedef struct {
   uint64_t n_instances;     /* parsed (k-mer, biedge) records                    */
   uint64_t n_distinct;      /* distinct canonical k-mers in the count table      */
   uint64_t n_k2;            /* distinct canonical (k-mer, biedge) pairs          */
-  uint64_t n_overflow;      /* k2 entries in the side table                      */
+  uint64_t n_overflow;      /* table slots with k2 entries in the side table     */
   uint64_t n_nodes;         /* graph nodes after the filter                      */
   uint64_t n_branches;
   uint64_t n_chain_nodes;   /* chain nodes extracted                             */
