@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -380,6 +381,251 @@ __global__ __launch_bounds__(BLOCK) void k_lr_recv(LRNode* lr, const uint32_t* l
   block_add(count, &ctr->lr_count);
   __syncthreads();
   block_add(cyc, &ctr->lr_cycle);
+}
+
+/* ================================================================== */
+/* list ranking by ruler contraction (well-formed chain graphs)         */
+/* ================================================================== */
+/* On a chain graph where every end pointing at a chain node is pointed back
+ * at, the chain nodes form disjoint paths and cycles, and the end state of
+ * bruno's pointer jumping is fixed: every non-terminal end points at the
+ * chain end it leads to, at its exact distance, marked as pointing to a
+ * terminal; cycle nodes end with both distances 2^T.  The number of rounds
+ * T is the largest ceil(log2 L) over paths of L >= 2 nodes, at least 1 when
+ * a cycle exists.  This computes that state without the rounds: rulers
+ * (chain ends plus sampled nodes) walk to the next ruler, heads walk the
+ * ruler graph, then every node reads its two chain ends off its rulers. */
+constexpr uint32_t NONE32 = 0xFFFFFFFFu;
+constexpr uint32_t RULER_SAMPLE = 64;
+constexpr uint32_t WALK_MAX = 1u << 16;
+constexpr uint32_t ERR_NOT_WELL_FORMED = 16;
+constexpr uint32_t ERR_WALK = 32;
+
+__device__ __forceinline__ uint32_t end_bit(int e) { return e ? END_OUT : 0u; }
+
+__global__ __launch_bounds__(BLOCK) void k_lr_check(const LRNode* lr, uint64_t n_chain, uint8_t* ruler, Counters* ctr) {
+  const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+  unsigned long long bad = 0;
+  for (uint64_t t = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; t < n_chain; t += stride) {
+    const LRNode st = lr[t];
+    bool head = false;
+    for (int E = 0; E < 2; ++E) {
+      uint32_t d = st.d[E];
+      if (is_terminal(d)) {
+        head = true;
+        continue;
+      }
+      /* initial state: a non-terminal end has dist 1 and a resolved pointer */
+      uint32_t j = st.p[E] & IDX_MASK;
+      int e = (st.p[E] & END_OUT) ? 1 : 0;
+      if (d != 1u || j >= n_chain || (j == t && e == E)) {
+        ++bad;
+        continue;
+      }
+      const LRNode o = lr[j];
+      if (o.d[e] != 1u || o.p[e] != ((uint32_t)t | end_bit(E))) ++bad;
+    }
+    bool sampled = (fmix64(t * 0x9e3779b97f4a7c15ULL + 0x1234567ULL) % RULER_SAMPLE) == 0;
+    ruler[t] = (head || sampled) ? 1 : 0;
+  }
+  block_add(bad, &ctr->lr_conflict);
+}
+
+struct WalkRec {
+  uint32_t who;   /* ruler (node walk) or chain end (ruler walk) */
+  uint32_t edist; /* bit 31: end, low 31 bits: distance */
+};
+
+__global__ __launch_bounds__(BLOCK) void k_lr_list_rulers(const uint8_t* ruler, uint64_t n_chain, uint32_t* list,
+                                                          Counters* ctr) {
+  __shared__ ReserveSmem sm;
+  const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+  for (uint64_t base = (uint64_t)blockIdx.x * BLOCK; base < n_chain; base += stride) {
+    const uint64_t t = base + threadIdx.x;
+    bool r = t < n_chain && ruler[t];
+    unsigned long long pos = block_reserve(r, &ctr->lr_next, sm);
+    if (r) list[pos] = (uint32_t)t;
+  }
+}
+
+/* From every ruler, along each non-terminal end, to the next ruler.  Each
+ * visited node records, at the end it was entered through, the ruler and
+ * the ruler's end the walk left from, and the distance. */
+__global__ __launch_bounds__(BLOCK) void k_lr_walk_nodes(const LRNode* lr, const uint8_t* ruler, const uint32_t* rlist,
+                                                         uint64_t n_rulers, WalkRec* rec, WalkRec* link, Counters* ctr) {
+  const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+  for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < 2 * n_rulers; i += stride) {
+    const uint32_t r = rlist[i >> 1];
+    const int E = (int)(i & 1);
+    if (is_terminal(lr[r].d[E])) continue;
+    uint32_t cur = r;
+    int leave = E;
+    uint32_t dist = 0;
+    for (;;) {
+      uint32_t ptr = lr[cur].p[leave];
+      uint32_t j = ptr & IDX_MASK;
+      int e = (ptr & END_OUT) ? 1 : 0;
+      ++dist;
+      if (ruler[j]) {
+        link[2 * (uint64_t)r + E] = WalkRec{j, end_bit(e) | dist};
+        break;
+      }
+      rec[2 * (uint64_t)j + e] = WalkRec{r, end_bit(E) | dist};
+      if (dist >= WALK_MAX) {
+        atomicOr(&ctr->err, ERR_WALK);
+        break;
+      }
+      cur = j;
+      leave = 1 - e;
+    }
+  }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_lr_list_heads(const LRNode* lr, const uint32_t* rlist, uint64_t n_rulers,
+                                                         uint32_t* heads, Counters* ctr) {
+  __shared__ ReserveSmem sm;
+  const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+  for (uint64_t base = (uint64_t)blockIdx.x * BLOCK; base < 2 * n_rulers; base += stride) {
+    const uint64_t i = base + threadIdx.x;
+    bool h = false;
+    uint32_t r = 0;
+    int E = (int)(i & 1);
+    if (i < 2 * n_rulers) {
+      r = rlist[i >> 1];
+      /* a head end: this end leads into the chain, the other is terminal */
+      h = !is_terminal(lr[r].d[E]) && is_terminal(lr[r].d[1 - E]);
+    }
+    unsigned long long pos = block_reserve(h, &ctr->lr_next, sm);
+    if (h) heads[pos] = r | end_bit(E);
+  }
+}
+
+/* From every head along the ruler graph: each ruler learns, for the end
+ * facing the head, which chain end lies that way and how far. */
+__global__ __launch_bounds__(BLOCK) void k_lr_walk_rulers(const LRNode* lr, const WalkRec* link, const uint32_t* heads,
+                                                          uint64_t n_heads, uint64_t n_rulers, WalkRec* info,
+                                                          Counters* ctr) {
+  const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+  for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < n_heads; i += stride) {
+    const uint32_t h = heads[i] & IDX_MASK;
+    const int E = (heads[i] & END_OUT) ? 1 : 0;
+    uint32_t cur = h;
+    int leave = E;
+    uint32_t D = 0;
+    uint64_t steps = 0;
+    for (;;) {
+      WalkRec l = link[2 * (uint64_t)cur + leave];
+      if (l.who == NONE32) {
+        atomicOr(&ctr->err, ERR_WALK);
+        break;
+      }
+      uint32_t r = l.who;
+      int e = (l.edist & END_OUT) ? 1 : 0;
+      D += l.edist & DIST_MASK;
+      /* from r through e lies head h, entered through its end E */
+      info[2 * (uint64_t)r + e] = WalkRec{h, end_bit(E) | D};
+      if (is_terminal(lr[r].d[1 - e])) {
+        info[2 * (uint64_t)h + E] = WalkRec{r, end_bit(e) | D};
+        break;
+      }
+      if (++steps > n_rulers) {
+        atomicOr(&ctr->err, ERR_WALK);
+        break;
+      }
+      cur = r;
+      leave = 1 - e;
+    }
+  }
+}
+
+__device__ __forceinline__ uint32_t ceil_log2(uint32_t L) { return L <= 1 ? 0 : 32 - __clz(L - 1); }
+
+/* Final state of every chain node; cycle nodes get both distances 2^T
+ * (written in a second pass once T is known). */
+__global__ __launch_bounds__(BLOCK) void k_lr_finish(LRNode* lr, uint64_t n_chain, const uint8_t* ruler,
+                                                     const WalkRec* rec, const WalkRec* info, uint8_t* cyc,
+                                                     Counters* ctr) {
+  const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+  unsigned long long ncyc = 0, maxr = 0;
+  for (uint64_t t = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; t < n_chain; t += stride) {
+    LRNode st = lr[t];
+    bool is_cycle = false;
+    uint32_t np[2], nd[2];
+    for (int E = 0; E < 2; ++E) {
+      np[E] = st.p[E];
+      nd[E] = st.d[E];
+      if (is_terminal(st.d[E])) continue;
+      uint32_t Z, f, D;
+      if (ruler[t]) {
+        WalkRec w = info[2 * t + E];
+        if (w.who == NONE32) {
+          is_cycle = true;
+          break;
+        }
+        Z = w.who;
+        f = w.edist & END_OUT;
+        D = w.edist & DIST_MASK;
+      } else {
+        WalkRec w = rec[2 * t + E];
+        if (w.who == NONE32) {
+          is_cycle = true;
+          break;
+        }
+        uint32_t r = w.who;
+        int Er = (w.edist & END_OUT) ? 1 : 0;
+        uint32_t d = w.edist & DIST_MASK;
+        /* continue past ruler r, leaving through its other end */
+        if (is_terminal(lr[r].d[1 - Er])) {
+          Z = r;
+          f = end_bit(Er);
+          D = d;
+        } else {
+          WalkRec v = info[2 * (uint64_t)r + (1 - Er)];
+          if (v.who == NONE32) {
+            is_cycle = true;
+            break;
+          }
+          Z = v.who;
+          f = v.edist & END_OUT;
+          D = d + (v.edist & DIST_MASK);
+        }
+      }
+      np[E] = Z | f;
+      nd[E] = D | TERM_MASK;
+    }
+    if (is_cycle) {
+      cyc[t] = 1;
+      ++ncyc;
+      continue;
+    }
+    cyc[t] = 0;
+    /* path length from a head's point of view: L = 1 + dist to the far end */
+    if (is_terminal(st.d[0]) != is_terminal(st.d[1])) {
+      uint32_t L = 1 + cdist(is_terminal(st.d[0]) ? nd[1] : nd[0]);
+      uint32_t R = ceil_log2(L);
+      if (R > maxr) maxr = R;
+    }
+    st.p[0] = np[0];
+    st.p[1] = np[1];
+    st.d[0] = nd[0];
+    st.d[1] = nd[1];
+    lr[t] = st;
+  }
+  block_add(ncyc, &ctr->lr_cycle);
+  typedef hipcub::BlockReduce<unsigned long long, BLOCK> BR;
+  __shared__ typename BR::TempStorage tmp;
+  __syncthreads();
+  unsigned long long m = BR(tmp).Reduce(maxr, hipcub::Max());
+  if (threadIdx.x == 0 && m) atomicMax(&ctr->lr_count, m);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_lr_cycles(LRNode* lr, uint64_t n_chain, const uint8_t* cyc, uint32_t maxd) {
+  const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+  for (uint64_t t = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; t < n_chain; t += stride)
+    if (cyc[t]) {
+      lr[t].d[0] = maxd;
+      lr[t].d[1] = maxd;
+    }
 }
 
 /* separate_cycles (when unfinished nodes remain) and separate_isolated. */
@@ -839,7 +1085,110 @@ bruno_status chains_t(bruno_job* j) {
   return BRUNO_OK;
 }
 
-bruno_status list_rank_impl(bruno_job* j, int variant) {
+/* Returns BRUNO_OK with *used = true when the contraction path produced the
+ * final state; *used = false means the graph is not well-formed (or a walk
+ * ran out) and the caller runs the round-by-round emulation instead. */
+bruno_status list_rank_fast(bruno_job* j, bool* used) {
+  *used = false;
+  uint64_t nc = j->n_chain;
+  if (nc == 0) return BRUNO_OK;
+  uint8_t* ruler = nullptr;
+  uint32_t* rlist = nullptr;
+  uint32_t* heads = nullptr;
+  WalkRec *rec = nullptr, *link = nullptr, *info = nullptr;
+  TRY(dev_alloc(j, (void**)&ruler, nc));
+  j->hctr->lr_conflict = 0;
+  j->hctr->lr_next = 0;
+  j->hctr->lr_count = 0;
+  j->hctr->lr_cycle = 0;
+  HIP_CHECK(hipMemcpyAsync(&j->ctr->lr_count, &j->hctr->lr_count, 4 * sizeof(unsigned long long),
+                           hipMemcpyHostToDevice, j->stream));
+  hipLaunchKernelGGL(k_lr_check, dim3(grid_for(nc)), dim3(BLOCK), 0, j->stream, j->lr, nc, ruler, j->ctr);
+  TRY(sync_counters(j));
+  bool well_formed = j->hctr->lr_conflict == 0;
+  if (!well_formed) {
+    void* a = ruler;
+    dev_free(j, a, nc);
+    return BRUNO_OK;
+  }
+  TRY(dev_alloc(j, (void**)&rlist, nc * 4));
+  hipLaunchKernelGGL(k_lr_list_rulers, dim3(grid_for(nc)), dim3(BLOCK), 0, j->stream, ruler, nc, rlist, j->ctr);
+  TRY(sync_counters(j));
+  uint64_t nr = j->hctr->lr_next;
+  TRY(dev_alloc(j, (void**)&rec, nc * 2 * sizeof(WalkRec)));
+  TRY(dev_alloc(j, (void**)&link, nc * 2 * sizeof(WalkRec)));
+  TRY(dev_alloc(j, (void**)&info, nc * 2 * sizeof(WalkRec)));
+  fill64(j, rec, nc * 2, ~0ULL);
+  fill64(j, link, nc * 2, ~0ULL);
+  fill64(j, info, nc * 2, ~0ULL);
+  if (nr)
+    hipLaunchKernelGGL(k_lr_walk_nodes, dim3(grid_for(2 * nr, BLOCK, 1u << 20)), dim3(BLOCK), 0, j->stream, j->lr,
+                       ruler, rlist, nr, rec, link, j->ctr);
+  TRY(dev_alloc(j, (void**)&heads, std::max<uint64_t>(1, 2 * nr) * 4));
+  j->hctr->lr_next = 0;
+  HIP_CHECK(hipMemcpyAsync(&j->ctr->lr_next, &j->hctr->lr_next, sizeof(unsigned long long), hipMemcpyHostToDevice,
+                           j->stream));
+  if (nr)
+    hipLaunchKernelGGL(k_lr_list_heads, dim3(grid_for(2 * nr, BLOCK, 1u << 20)), dim3(BLOCK), 0, j->stream, j->lr,
+                       rlist, nr, heads, j->ctr);
+  /* a walk that ran past its bound leaves the chain state untouched: clear
+   * the flag and let the round-by-round path run */
+  auto walk_failed = [&](bruno_status st, bool* fell_back) -> bruno_status {
+    *fell_back = false;
+    if (j->hctr->err == ERR_WALK) {
+      j->hctr->err = 0;
+      HIP_CHECK(hipMemsetAsync(&j->ctr->err, 0, 4, j->stream));
+      HIP_CHECK(hipStreamSynchronize(j->stream));
+      *fell_back = true;
+      return BRUNO_OK;
+    }
+    return st;
+  };
+  bool fell = false;
+  bruno_status s = sync_counters(j);
+  if (s != BRUNO_OK) {
+    s = walk_failed(s, &fell);
+    return s;
+  }
+  uint64_t nh = j->hctr->lr_next;
+  if (nh)
+    hipLaunchKernelGGL(k_lr_walk_rulers, dim3(grid_for(nh)), dim3(BLOCK), 0, j->stream, j->lr, link, heads, nh, nr,
+                       info, j->ctr);
+  s = sync_counters(j);
+  if (s != BRUNO_OK) {
+    s = walk_failed(s, &fell);
+    return s;
+  }
+  /* cycle flags reuse the ruler buffer after the walks */
+  uint8_t* cyc = nullptr;
+  TRY(dev_alloc(j, (void**)&cyc, nc));
+  hipLaunchKernelGGL(k_lr_finish, dim3(grid_for(nc)), dim3(BLOCK), 0, j->stream, j->lr, nc, ruler, rec, info, cyc,
+                     j->ctr);
+  TRY(sync_counters(j));
+  uint64_t maxr = j->hctr->lr_count;
+  uint64_t ncyc = j->hctr->lr_cycle;
+  uint32_t T = (uint32_t)maxr;
+  if (ncyc > 0 && T < 1) T = 1;
+  if (T >= 30) return BRUNO_INTERNAL; /* the reference stops at 30 rounds; not reachable below 2^29 nodes */
+  if (ncyc) hipLaunchKernelGGL(k_lr_cycles, dim3(grid_for(nc)), dim3(BLOCK), 0, j->stream, j->lr, nc, cyc, 1u << T);
+  j->iterations = T;
+  j->conflicts = 0;
+  /* separate_cycles only when unfinished nodes remain, i.e. cycles */
+  j->hctr->n_cycle = 0;
+  j->hctr->n_isolated = 0;
+  HIP_CHECK(hipMemcpyAsync(&j->ctr->n_cycle, &j->hctr->n_cycle, 2 * sizeof(unsigned long long),
+                           hipMemcpyHostToDevice, j->stream));
+  TRY(dev_alloc(j, (void**)&j->alive, nc));
+  hipLaunchKernelGGL(k_separate, dim3(grid_for(nc)), dim3(BLOCK), 0, j->stream, j->lr, nc, ncyc ? 1 : 0, 1u << T,
+                     j->alive, j->ctr);
+  TRY(sync_counters(j));
+  j->n_cycle = j->hctr->n_cycle;
+  j->n_isolated = j->hctr->n_isolated;
+  *used = true;
+  return BRUNO_OK;
+}
+
+bruno_status list_rank_rounds(bruno_job* j, int variant) {
   uint64_t nc = j->n_chain;
   TRY(dev_alloc(j, (void**)&j->list[0], std::max<uint64_t>(1, nc) * sizeof(uint32_t)));
   TRY(dev_alloc(j, (void**)&j->list[1], std::max<uint64_t>(1, nc) * sizeof(uint32_t)));
@@ -1243,7 +1592,10 @@ bruno_status bruno_list_rank(bruno_job* j, int variant, uint32_t* rounds) {
   if (!valid_job(j) || (variant != 0 && variant != 1)) return BRUNO_BAD_INPUT;
   if (j->stage != ST_CHAINS) return BRUNO_STATE;
   ev_record(j, 6);
-  TRY(list_rank_impl(j, variant));
+  bool used = false;
+  const char* force = getenv("BRUNO_LR_ROUNDS");
+  if (!(force && force[0] == '1')) TRY(list_rank_fast(j, &used));
+  if (!used) TRY(list_rank_rounds(j, variant));
   ev_record(j, 7);
   HIP_CHECK(hipStreamSynchronize(j->stream));
   j->ms.list_rank_ms = ev_ms(j, 6, 7);

@@ -63,6 +63,35 @@ def test_random_small(case, tl, variant):
     compare_all(k, None if tl is None else thresholds_L(tl), b, off, variant=variant)
 
 
+@pytest.mark.parametrize("case", CASES[:8])
+@pytest.mark.parametrize("variant", [0, 1])
+def test_random_small_round_emulation(case, variant, monkeypatch):
+    """The round-by-round list ranking (used when a chain graph is not
+    well-formed) on the same inputs: same outputs."""
+    monkeypatch.setenv("BRUNO_LR_ROUNDS", "1")
+    seed, G, n, L, err, k = case
+    _, reads = random_reads(seed, G, n, L, err)
+    b, off = pack(reads)
+    for tl in (None, 2):
+        compare_all(k, None if tl is None else thresholds_L(tl), b, off, variant=variant)
+
+
+def test_long_chains_and_cycles():
+    """Single long paths (ceil(log2 L) rounds) and circular genomes (cycles
+    that are removed), alone and mixed."""
+    rng = np.random.default_rng(77)
+    reads = []
+    for L in (3, 4, 5, 17, 1000, 4097, 20001):
+        reads.append("".join(rng.choice(list("ACGT"), L + 30)))
+    g = "".join(rng.choice(list("ACGT"), 3000))
+    circ = g + g[:200]
+    cyc_reads = [circ[i:i + 150] for i in range(0, 3000, 20)]
+    for batch in (reads, cyc_reads, reads + cyc_reads):
+        b, off = pack(batch)
+        for variant in (0, 1):
+            compare_all(31, None, b, off, variant=variant)
+
+
 def test_edge_cases():
     k = 31
     long_read = "".join(np.random.default_rng(3).choice(list("ACGT"), 20000))
