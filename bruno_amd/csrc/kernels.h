@@ -103,10 +103,21 @@ __device__ __forceinline__ void block_add(unsigned long long v, unsigned long lo
 /* ------------------------------------------------------------------ */
 /* fill                                                                 */
 /* ------------------------------------------------------------------ */
+/* n 64-bit words of value v; 16-byte stores, 4 per thread per iteration */
 __global__ void k_fill64(uint64_t* p, uint64_t n, uint64_t v) {
+  const uint64_t n2 = n / 2;
+  ulonglong2* q = (ulonglong2*)p;
+  const ulonglong2 w = make_ulonglong2(v, v);
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (; i < n; i += stride) p[i] = v;
+  for (; i + 3 * stride < n2; i += 4 * stride) {
+    q[i] = w;
+    q[i + stride] = w;
+    q[i + 2 * stride] = w;
+    q[i + 3 * stride] = w;
+  }
+  for (; i < n2; i += stride) q[i] = w;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && (n & 1)) p[n - 1] = v;
 }
 
 /* ------------------------------------------------------------------ */

@@ -17,7 +17,7 @@
  *   canonical 31-mer is never all T, so a real key never reads as EMPTY).
  * k <= 63 (KT = u128): 64-byte slot {meta, lo, hi, sub[5]}.  The key is two
  *   words, so a slot is claimed by CAS on meta (EMPTY -> LOCK), the k-mer is
- *   written, then meta is released to its flag word (bit 63 spilled, bit 62
+ *   written, then meta is published as its flag word (bit 63 spilled, bit 62
  *   present).
  * Sub-entry: (code << 48) | count with code = biedge + 1 (1..256); EMPTY =
  *   all ones.
@@ -80,12 +80,7 @@ static_assert(sizeof(NodeRec) == 24, "node size");
 __device__ __forceinline__ uint64_t ld_relaxed(const uint64_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ uint64_t ld_acquire(const uint64_t* p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_release(uint64_t* p, uint64_t v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-}
+
 __device__ __forceinline__ uint64_t cas64(uint64_t* p, uint64_t cmp, uint64_t val) {
   return atomicCAS((unsigned long long*)p, (unsigned long long)cmp, (unsigned long long)val);
 }
@@ -103,10 +98,19 @@ __device__ __forceinline__ uint64_t home_slot(uint64_t h, uint64_t cap) { return
 /* ------------------------------------------------------------------ */
 template <typename KT> struct SlotOps;
 
+/* Counting-time access rules.  Every word a counting thread reads is read
+ * with an untorn, device-coherent relaxed load, and only when it is needed
+ * (the key first, then sub-entries in order).  A key, a k-mer word or a
+ * biedge code is written once and never changes, and every transition out
+ * of EMPTY is made by a CAS that returns the current value.  No acquire /
+ * release fence is used: on a multi-XCD device an agent-scope fence writes
+ * back or invalidates the L2 caches. */
+template <typename KT> struct SlotOps;
+
 template <> struct SlotOps<uint64_t> {
   typedef Slot<uint64_t> S;
-  /* Find or claim the slot of canonical k-mer v.  Returns slot index or
-   * ~0 on a full table.  *claimed = 1 when this call inserted the key. */
+  /* Find or claim the slot of canonical k-mer v.  Returns the slot index or
+   * ~0 on a full table; *claimed = 1 when this call inserted the key. */
   static __device__ uint64_t find_or_insert(S* T, uint64_t cap, uint64_t v, uint32_t* err, int* claimed) {
     uint64_t s = home_slot(kmer_hash(v), cap);
     for (uint32_t probe = 0; probe < MAX_PROBE; ++probe) {
@@ -126,8 +130,8 @@ template <> struct SlotOps<uint64_t> {
     atomicOr(err, ERR_TABLE_FULL);
     return ~0ULL;
   }
-  /* Lookup only; ~0 when absent.  require_present: only nodes that
-   * survived the filter. */
+  /* Lookup only (separate kernels after counting); ~0 when absent.
+   * require_present: only nodes that survived the filter. */
   static __device__ uint64_t find(const S* T, uint64_t cap, uint64_t v, bool require_present) {
     uint64_t s = home_slot(kmer_hash(v), cap);
     for (uint32_t probe = 0; probe < MAX_PROBE; ++probe) {
@@ -147,6 +151,11 @@ template <> struct SlotOps<uint64_t> {
   static __device__ __forceinline__ uint64_t flags(const S* p) { return p->key & ~KEY_BITS; }
 };
 
+/* Two-word keys: meta is claimed by CAS (EMPTY -> LOCK), lo and hi are
+ * written with atomic exchanges, then meta is published with a third
+ * exchange whose value depends on both returned words, so it cannot be
+ * performed before them.  Readers spin (bounded) while meta is LOCK, then
+ * read lo and hi. */
 template <> struct SlotOps<u128> {
   typedef Slot<u128> S;
   static __device__ uint64_t find_or_insert(S* T, uint64_t cap, u128 v, uint32_t* err, int* claimed) {
@@ -154,13 +163,17 @@ template <> struct SlotOps<u128> {
     uint64_t s = home_slot(kmer_hash(v), cap);
     for (uint32_t probe = 0; probe < MAX_PROBE; ++probe) {
       S* p = T + s;
-      uint64_t m = ld_acquire(&p->meta);
+      uint64_t m = ld_relaxed(&p->meta);
       if (m == EMPTY64) {
         uint64_t old = cas64(&p->meta, EMPTY64, LOCK64);
         if (old == EMPTY64) {
-          p->lo = lo;
-          p->hi = hi;
-          st_release(&p->meta, 0);
+          uint64_t a = atomicExch((unsigned long long*)&p->lo, (unsigned long long)lo);
+          uint64_t b = atomicExch((unsigned long long*)&p->hi, (unsigned long long)hi);
+          /* An EMPTY slot holds all-ones words, so pub is 0; any other old
+           * value would only set the spilled flag, which makes the filter
+           * consult the side table and changes no result. */
+          uint64_t pub = ((a & b) ^ EMPTY64) & F_SPILL;
+          atomicExch((unsigned long long*)&p->meta, (unsigned long long)pub);
           *claimed = 1;
           return s;
         }
@@ -168,7 +181,7 @@ template <> struct SlotOps<u128> {
       }
       uint32_t spin = 0;
       while (m == LOCK64 && spin < MAX_SPIN) {
-        m = ld_acquire(&p->meta);
+        m = ld_relaxed(&p->meta);
         ++spin;
       }
       if (m == LOCK64) {
@@ -220,7 +233,7 @@ __device__ int add_k2(Slot<KT>* T, uint64_t s, uint8_t e, Spill* sp, uint64_t sp
       return 0;
     }
   }
-  /* spill */
+  /* spill: mark the slot, then count in the side table */
   uint64_t* fw = SlotOps<KT>::flag_word(p);
   if (!(ld_relaxed(fw) & F_SPILL)) or64(fw, F_SPILL);
   const uint64_t key = (s << 9) | code;

@@ -223,3 +223,24 @@ def test_config1_full():
     b, off = bruno_amd.synth_reads_host(cfg["synth"])
     g, o = compare_all(cfg["k"], cfg["thresholds"], b, off, nthreads=8, stages=False)
     assert g["stats"]["n_instances"] == 300000 * 70
+
+
+def test_spilled_k2_entries_under_contention():
+    """Many k-mers with more distinct biedges than fit in their slot, hit
+    concurrently: every spilled (k-mer, biedge) pair must be counted.  A read
+    set with one centre k-mer flanked by all 16 (in, out) base pairs,
+    repeated, plus the reverse strand."""
+    rng = np.random.default_rng(5)
+    core = "".join(rng.choice(list("ACGT"), 31))
+    reads = []
+    for a in "ACGT":
+        for b in "ACGT":
+            r = "".join(rng.choice(list("ACGT"), 10)) + a + core + b + "".join(rng.choice(list("ACGT"), 10))
+            reads += [r] * 37
+            reads += [r.translate(str.maketrans("ACGT", "TGCA"))[::-1]] * 11
+    order = rng.permutation(len(reads))
+    reads = [reads[i] for i in order]
+    b, off = pack(reads)
+    for k in (31, 63):
+        for th in (None, thresholds_L(2)):
+            compare_all(k, th, b, off)
