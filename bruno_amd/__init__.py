@@ -7,4 +7,4 @@ the reference driver's call sequence (compact_dbg_freq_construct_clean.cpp
 :431-840) for tests, the benchmark and scripting.
 """
 from ._lib import (BrunoError, lib, NODE_DTYPE, CHAIN_DTYPE, EXPORTS, LIB_PATH)  # noqa: F401
-from .pipeline import (GraphJob, SynthConfig, synth_reads_host, thresholds_L, DEFAULT_THRESH)  # noqa: F401
+from .pipeline import (GraphBatch, GraphJob, SynthConfig, synth_reads_host, thresholds_L, DEFAULT_THRESH)  # noqa: F401

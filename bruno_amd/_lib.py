@@ -57,6 +57,9 @@ EXPORTS = (
     "bruno_get_chain_nodes", "bruno_get_unitigs", "bruno_get_branch_fasta",
     "bruno_parse_records", "bruno_synth_reads_device", "bruno_synth_reads_host",
     "bruno_dev_alloc", "bruno_dev_free", "bruno_dev_copy", "bruno_dev_sync", "bruno_dev_mem_info",
+    "bruno_batch_create", "bruno_batch_parse_count", "bruno_batch_get_stats", "bruno_batch_get_counts",
+    "bruno_batch_get_k2", "bruno_batch_get_nodes", "bruno_batch_get_branches", "bruno_batch_get_chain_nodes",
+    "bruno_batch_get_unitigs", "bruno_batch_get_branch_fasta",
 )
 
 _LIB = None
@@ -98,6 +101,14 @@ def lib():
         L.bruno_parse_records.argtypes = [i, vp, vp, u64, vp, vp, u64p, vp]
         L.bruno_synth_reads_device.argtypes = [ctypes.POINTER(SynthDesc), vp, vp, vp]
         L.bruno_synth_reads_host.argtypes = [ctypes.POINTER(SynthDesc), vp, vp]
+        u32 = ctypes.c_uint32
+        L.bruno_batch_create.argtypes = [ctypes.POINTER(JobDesc), u32, vp, ctypes.POINTER(vp)]
+        L.bruno_batch_parse_count.argtypes = [vp, vp, vp, u64, vp]
+        L.bruno_batch_get_stats.argtypes = [vp, u32, ctypes.POINTER(Stats)]
+        for n in ("bruno_batch_get_counts", "bruno_batch_get_nodes", "bruno_batch_get_chain_nodes",
+                  "bruno_batch_get_unitigs", "bruno_batch_get_branch_fasta", "bruno_batch_get_branches"):
+            getattr(L, n).argtypes = [vp, u32, vp, u64p]
+        L.bruno_batch_get_k2.argtypes = [vp, u32, vp, vp, vp, u64p]
         L.bruno_dev_alloc.argtypes = [u64, ctypes.POINTER(vp)]
         L.bruno_dev_free.argtypes = [vp]
         L.bruno_dev_copy.argtypes = [vp, vp, u64, i]

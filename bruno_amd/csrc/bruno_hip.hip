@@ -152,10 +152,13 @@ __global__ __launch_bounds__(BLOCK) void k_filter_spilled(Slot<KT>* T, const Spi
   block_add(nc, &ctr->n_chain);
 }
 
-/* split present slots into branch and chain lists; chain index into slot */
+/* split present slots into branch and chain lists; chain index into slot.
+ * In a batch, each node's job is the table segment holding its slot. */
 template <typename KT>
 __global__ __launch_bounds__(BLOCK) void k_split(Slot<KT>* T, const uint64_t* present, uint64_t n_present,
-                                                 uint64_t* chain_slot, uint64_t* branch_slot, Counters* ctr) {
+                                                 uint64_t* chain_slot, uint64_t* branch_slot, const uint64_t* seg,
+                                                 uint32_t n_jobs, uint32_t* chain_job, uint32_t* branch_job,
+                                                 Counters* ctr) {
   __shared__ ReserveSmem sm;
   __shared__ ReserveSmem sm2;
   const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
@@ -169,11 +172,14 @@ __global__ __launch_bounds__(BLOCK) void k_split(Slot<KT>* T, const uint64_t* pr
     }
     unsigned long long pc = block_reserve(cls == CLS_CHAIN, &ctr->lr_count, sm);
     unsigned long long pb = block_reserve(cls == CLS_BRANCH, &ctr->lr_next, sm2);
+    const uint32_t job = (cls && n_jobs > 1) ? job_of(seg, n_jobs, s) : 0u;
     if (cls == CLS_CHAIN) {
       chain_slot[pc] = s;
+      if (n_jobs > 1) chain_job[pc] = job;
       ((NodeRec*)&T[s].sub[0])->idx = (uint32_t)pc;
     } else if (cls == CLS_BRANCH) {
       branch_slot[pb] = s;
+      if (n_jobs > 1) branch_job[pb] = job;
       ((NodeRec*)&T[s].sub[0])->idx = (uint32_t)pb;
     }
   }
@@ -188,15 +194,15 @@ __global__ __launch_bounds__(BLOCK) void k_split(Slot<KT>* T, const uint64_t* pr
  * holding the canonical k-mer (updates to them are dropped, as
  * map.update(..., false, ...) drops keys absent from the chain map). */
 template <typename KT>
-__device__ uint32_t resolve(const Slot<KT>* T, uint64_t cap, const kops<KT>& ops, KT n, uint32_t facing_if_canonical,
-                            uint64_t n_chain, KT* vkmer, uint64_t vcap, Counters* ctr) {
+__device__ uint32_t resolve(const Slot<KT>* T, uint64_t tbase, uint64_t tcap, const kops<KT>& ops, KT n,
+                            uint32_t facing_if_canonical, uint64_t n_chain, KT* vkmer, uint64_t vcap, Counters* ctr) {
   KT rn = ops.rc(n);
   bool flipped = !(n < rn);
   KT cn = flipped ? rn : n;
   uint32_t end = flipped ? (facing_if_canonical ^ END_OUT) : facing_if_canonical;
-  uint64_t s = SlotOps<KT>::find(T, cap, cn, true);
+  uint64_t s = SlotOps<KT>::find(T + tbase, tcap, cn, true);
   if (s != ~0ULL) {
-    const NodeRec* nr = (const NodeRec*)&T[s].sub[0];
+    const NodeRec* nr = (const NodeRec*)&T[tbase + s].sub[0];
     if (nr->cls == CLS_CHAIN) return end | nr->idx;
   }
   unsigned long long v = atomicAdd(&ctr->n_virt, 1ULL);
@@ -212,12 +218,19 @@ __device__ uint32_t resolve(const Slot<KT>* T, uint64_t cap, const kops<KT>& ops
  * at dist 1, else term_mask. */
 template <typename KT>
 __global__ __launch_bounds__(BLOCK) void k_chain_init(const Slot<KT>* T, uint64_t cap, int K, const uint64_t* chain_slot,
+                                                      const uint32_t* chain_job, const uint64_t* seg, uint32_t n_jobs,
                                                       uint64_t n_chain, KT* ckmer, LRNode* lr, KT* vkmer, uint64_t vcap,
                                                       Counters* ctr) {
   kops<KT> ops(K);
   const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
   for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < n_chain; i += stride) {
     const Slot<KT>* p = T + chain_slot[i];
+    uint64_t tbase = 0, tcap = cap;
+    if (n_jobs > 1) {
+      const uint32_t job = chain_job[i];
+      tbase = seg[job];
+      tcap = seg[job + 1] - tbase;
+    }
     KT k = SlotOps<KT>::kmer(p);
     const NodeRec* nr = (const NodeRec*)&p->sub[0];
     LRNode st;
@@ -227,13 +240,13 @@ __global__ __launch_bounds__(BLOCK) void k_chain_init(const Slot<KT>* T, uint64_
     for (int c = 0; c < 4; ++c)
       if (nr->c[4 + c] > 0) {
         /* in neighbour c.k[0..K-2]: when canonical its OUT end faces k */
-        st.p[0] = resolve<KT>(T, cap, ops, ops.in_nbr(k, c), END_OUT, n_chain, vkmer, vcap, ctr);
+        st.p[0] = resolve<KT>(T, tbase, tcap, ops, ops.in_nbr(k, c), END_OUT, n_chain, vkmer, vcap, ctr);
         st.d[0] = 1;
       }
     for (int c = 0; c < 4; ++c)
       if (nr->c[c] > 0) {
         /* out neighbour k[1..].c: when canonical its IN end faces k */
-        st.p[1] = resolve<KT>(T, cap, ops, ops.out_nbr(k, c), 0u, n_chain, vkmer, vcap, ctr);
+        st.p[1] = resolve<KT>(T, tbase, tcap, ops, ops.out_nbr(k, c), 0u, n_chain, vkmer, vcap, ctr);
         st.d[1] = 1;
       }
     ckmer[i] = k;
@@ -245,11 +258,18 @@ __global__ __launch_bounds__(BLOCK) void k_chain_init(const Slot<KT>* T, uint64_
  * neighbours as pointing to a branch (dist 0). */
 template <typename KT>
 __global__ __launch_bounds__(BLOCK) void k_termini(const Slot<KT>* T, uint64_t cap, int K, const uint64_t* branch_slot,
+                                                   const uint32_t* branch_job, const uint64_t* seg, uint32_t n_jobs,
                                                    uint64_t n_branch, LRNode* lr, KT* bkmer) {
   kops<KT> ops(K);
   const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
   for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < n_branch; i += stride) {
     const Slot<KT>* p = T + branch_slot[i];
+    uint64_t tbase = 0, tcap = cap;
+    if (n_jobs > 1) {
+      const uint32_t job = branch_job[i];
+      tbase = seg[job];
+      tcap = seg[job + 1] - tbase;
+    }
     KT k = SlotOps<KT>::kmer(p);
     bkmer[i] = k;
     const NodeRec* nr = (const NodeRec*)&p->sub[0];
@@ -263,9 +283,9 @@ __global__ __launch_bounds__(BLOCK) void k_termini(const Slot<KT>* T, uint64_t c
         KT rn = ops.rc(n);
         bool flipped = !(n < rn);
         KT cn = flipped ? rn : n;
-        uint64_t s = SlotOps<KT>::find(T, cap, cn, true);
+        uint64_t s = SlotOps<KT>::find(T + tbase, tcap, cn, true);
         if (s == ~0ULL) continue;
-        const NodeRec* q = (const NodeRec*)&T[s].sub[0];
+        const NodeRec* q = (const NodeRec*)&T[tbase + s].sub[0];
         if (q->cls != CLS_CHAIN) continue;
         int orient_in = (side == 0);  /* IN for out neighbours */
         if (flipped) orient_in = !orient_in;
@@ -278,12 +298,13 @@ __global__ __launch_bounds__(BLOCK) void k_termini(const Slot<KT>* T, uint64_t c
 /* list ranking: one synchronous round = send + recv                    */
 /* ================================================================== */
 __global__ __launch_bounds__(BLOCK) void k_lr_init_list(const LRNode* lr, uint64_t n_chain, uint32_t* list,
-                                                        Counters* ctr) {
+                                                        const uint32_t* cjob, unsigned long long* jctr, Counters* ctr) {
   __shared__ ReserveSmem sm;
   const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
   for (uint64_t base = (uint64_t)blockIdx.x * BLOCK; base < n_chain; base += stride) {
     const uint64_t i = base + threadIdx.x;
     bool u = (i < n_chain) && is_unfinished(lr[i].d[0], lr[i].d[1]);
+    if (u && jctr) atomicAdd(&jctr[3 * (uint64_t)cjob[i] + 1], 1ULL);
     unsigned long long pos = block_reserve(u, &ctr->lr_next, sm);
     if (u) list[pos] = (uint32_t)i;
   }
@@ -317,11 +338,14 @@ __device__ __forceinline__ void lr_post(LRNode* lr, uint64_t n_chain, uint32_t t
 /* variant 0: send on every non-terminal side (list_rank :858-885);
  * variant 1: only on sides pointing to a chain node (:1130-1155). */
 __global__ __launch_bounds__(BLOCK) void k_lr_send(LRNode* lr, uint64_t n_chain, const uint32_t* list, uint64_t n,
-                                                   int variant, Counters* ctr) {
+                                                   int variant, const uint32_t* cjob, const uint8_t* active,
+                                                   Counters* ctr) {
   unsigned long long conflicts = 0;
   const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
   for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += stride) {
     const uint32_t t = list[i];
+    /* a finished job of a batch takes no further rounds */
+    if (active && !active[cjob[t]]) continue;
     const LRNode st = lr[t];
     const uint32_t din = st.d[0], dout = st.d[1];
     const uint32_t dist = cdist(din) + cdist(dout);
@@ -342,8 +366,11 @@ __global__ __launch_bounds__(BLOCK) void k_lr_send(LRNode* lr, uint64_t n_chain,
   block_add(conflicts, &ctr->lr_conflict);
 }
 
+/* Per-job counters of a batch round (jctr[3 * job + {0, 1, 2}]: updates,
+ * unfinished nodes, cycle-like nodes); a single job uses ctr. */
 __global__ __launch_bounds__(BLOCK) void k_lr_recv(LRNode* lr, const uint32_t* list, uint64_t n, uint32_t* next,
-                                                   uint32_t maxd, Counters* ctr) {
+                                                   uint32_t maxd, const uint32_t* cjob, const uint8_t* active,
+                                                   unsigned long long* jctr, Counters* ctr) {
   __shared__ ReserveSmem sm;
   unsigned long long count = 0, cyc = 0;
   const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
@@ -351,8 +378,9 @@ __global__ __launch_bounds__(BLOCK) void k_lr_recv(LRNode* lr, const uint32_t* l
     const uint64_t i = base + threadIdx.x;
     bool unf = false;
     uint32_t t = 0;
-    if (i < n) {
+    if (i < n && !(active && !active[cjob[list[i]]])) {
       t = list[i];
+      unsigned long long my_count = 0;
       LRNode st = lr[t];
       bool dirty = false;
       for (int e = 0; e < 2; ++e) {
@@ -367,13 +395,21 @@ __global__ __launch_bounds__(BLOCK) void k_lr_recv(LRNode* lr, const uint32_t* l
           if (dist > distx) st.p[e] = yp;
           if (dist >= distx) {
             st.d[e] = yd;
-            ++count;
+            ++my_count;
           }
         }
       }
       if (dirty) lr[t] = st;
       unf = is_unfinished(st.d[0], st.d[1]);
-      if (unf && cdist(st.d[0]) == maxd && cdist(st.d[1]) == maxd) ++cyc;
+      const bool my_cyc = unf && cdist(st.d[0]) == maxd && cdist(st.d[1]) == maxd;
+      if (jctr) {
+        unsigned long long* jc = jctr + 3 * (uint64_t)cjob[t];
+        if (my_count) atomicAdd(jc, my_count);
+        if (unf) atomicAdd(jc + 1, 1ULL);
+        if (my_cyc) atomicAdd(jc + 2, 1ULL);
+      }
+      count += my_count;
+      cyc += my_cyc;
     }
     unsigned long long pos = block_reserve(unf, &ctr->lr_next, sm);
     if (unf) next[pos] = t;
@@ -542,9 +578,12 @@ __device__ __forceinline__ uint32_t ceil_log2(uint32_t L) { return L <= 1 ? 0 : 
 
 /* Final state of every chain node; cycle nodes get both distances 2^T
  * (written in a second pass once T is known). */
+/* jT[job]: largest ceil(log2 L) over the job's paths; jcyc[job] = 1 when
+ * the job has a cycle. */
 __global__ __launch_bounds__(BLOCK) void k_lr_finish(LRNode* lr, uint64_t n_chain, const uint8_t* ruler,
                                                      const WalkRec* rec, const WalkRec* info, uint8_t* cyc,
-                                                     Counters* ctr) {
+                                                     const uint32_t* cjob, uint32_t n_jobs, uint32_t* jT,
+                                                     uint32_t* jcyc, Counters* ctr) {
   const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
   unsigned long long ncyc = 0, maxr = 0;
   for (uint64_t t = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; t < n_chain; t += stride) {
@@ -593,9 +632,11 @@ __global__ __launch_bounds__(BLOCK) void k_lr_finish(LRNode* lr, uint64_t n_chai
       np[E] = Z | f;
       nd[E] = D | TERM_MASK;
     }
+    const uint32_t job = n_jobs > 1 ? cjob[t] : 0u;
     if (is_cycle) {
       cyc[t] = 1;
       ++ncyc;
+      if (!jcyc[job]) jcyc[job] = 1;
       continue;
     }
     cyc[t] = 0;
@@ -604,6 +645,7 @@ __global__ __launch_bounds__(BLOCK) void k_lr_finish(LRNode* lr, uint64_t n_chai
       uint32_t L = 1 + cdist(is_terminal(st.d[0]) ? nd[1] : nd[0]);
       uint32_t R = ceil_log2(L);
       if (R > maxr) maxr = R;
+      if (n_jobs > 1 && R > jT[job]) atomicMax(&jT[job], R);
     }
     st.p[0] = np[0];
     st.p[1] = np[1];
@@ -616,38 +658,51 @@ __global__ __launch_bounds__(BLOCK) void k_lr_finish(LRNode* lr, uint64_t n_chai
   __shared__ typename BR::TempStorage tmp;
   __syncthreads();
   unsigned long long m = BR(tmp).Reduce(maxr, hipcub::Max());
-  if (threadIdx.x == 0 && m) atomicMax(&ctr->lr_count, m);
+  if (n_jobs == 1 && threadIdx.x == 0 && m) atomicMax(&jT[0], (uint32_t)m);
 }
 
-__global__ __launch_bounds__(BLOCK) void k_lr_cycles(LRNode* lr, uint64_t n_chain, const uint8_t* cyc, uint32_t maxd) {
+/* cycle nodes end with both distances 2^T of their job: jcm[job] */
+__global__ __launch_bounds__(BLOCK) void k_lr_cycles(LRNode* lr, uint64_t n_chain, const uint8_t* cyc,
+                                                     const uint32_t* cjob, uint32_t n_jobs, const uint32_t* jcm) {
   const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
   for (uint64_t t = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; t < n_chain; t += stride)
     if (cyc[t]) {
+      const uint32_t maxd = jcm[n_jobs > 1 ? cjob[t] : 0u];
       lr[t].d[0] = maxd;
       lr[t].d[1] = maxd;
     }
 }
 
-/* separate_cycles (when unfinished nodes remain) and separate_isolated. */
-__global__ __launch_bounds__(BLOCK) void k_separate(const LRNode* lr, uint64_t n_chain, int do_cycles, uint32_t maxd,
-                                                    uint8_t* alive, Counters* ctr) {
+/* separate_cycles (when unfinished nodes remain) and separate_isolated.
+ * jcm[job] = 2^T of the job when its cycles are separated, else 0 (2^T >= 2
+ * whenever cycles are separated).  jsep[2 * job + {0, 1}] counts cycle and
+ * isolated nodes per job. */
+__global__ __launch_bounds__(BLOCK) void k_separate(const LRNode* lr, uint64_t n_chain, const uint32_t* cjob,
+                                                    uint32_t n_jobs, const uint32_t* jcm, uint8_t* alive,
+                                                    unsigned long long* jsep) {
   unsigned long long nc = 0, ni = 0;
   const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
   for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < n_chain; i += stride) {
     uint32_t din = lr[i].d[0], dout = lr[i].d[1];
+    const uint32_t job = n_jobs > 1 ? cjob[i] : 0u;
+    const uint32_t maxd = jcm[job];
     uint8_t a = 1;
-    if (do_cycles && cdist(din) == maxd && cdist(dout) == maxd) {
+    if (maxd && cdist(din) == maxd && cdist(dout) == maxd) {
       a = 0;
-      ++nc;
+      if (n_jobs > 1) atomicAdd(&jsep[2 * (uint64_t)job], 1ULL);
+      else ++nc;
     } else if (points_to_self(din) && points_to_self(dout)) {
       a = 0;
-      ++ni;
+      if (n_jobs > 1) atomicAdd(&jsep[2 * (uint64_t)job + 1], 1ULL);
+      else ++ni;
     }
     alive[i] = a;
   }
-  block_add(nc, &ctr->n_cycle);
-  __syncthreads();
-  block_add(ni, &ctr->n_isolated);
+  if (n_jobs == 1) {
+    block_add(nc, &jsep[0]);
+    __syncthreads();
+    block_add(ni, &jsep[1]);
+  }
 }
 
 /* ================================================================== */
@@ -669,8 +724,8 @@ __device__ __forceinline__ KT ptr_kmer(const kops<KT>& ops, uint32_t ptr, const 
 /* to_listranked_chain_node (debruijn_chain_operations.hpp:261-310) */
 template <typename KT>
 __global__ __launch_bounds__(BLOCK) void k_ranked(const LRNode* lr, const KT* ckmer, uint64_t n_chain, const KT* vkmer,
-                                                  const uint8_t* alive, int K, KT* out_k, KT* out_rep,
-                                                  uint32_t* out_dist, Counters* ctr) {
+                                                  const uint8_t* alive, int K, const uint32_t* cjob, KT* out_k,
+                                                  KT* out_rep, uint32_t* out_dist, uint32_t* out_job, Counters* ctr) {
   __shared__ ReserveSmem sm;
   kops<KT> ops(K);
   const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
@@ -694,6 +749,7 @@ __global__ __launch_bounds__(BLOCK) void k_ranked(const LRNode* lr, const KT* ck
       out_k[pos] = Kv;
       out_rep[pos] = rep;
       out_dist[pos] = dist;
+      if (out_job) out_job[pos] = cjob[i];
     }
   }
 }
@@ -714,6 +770,19 @@ __global__ void k_gather_kmer(const KT* src, const uint32_t* perm, KT* dst, uint
 __global__ void k_gather_u32(const uint32_t* src, const uint32_t* perm, uint32_t* dst, uint64_t n) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) dst[i] = src[perm[i]];
+}
+/* start[q] = first position of job q in a job-sorted array (start[n_jobs]
+ * = n): position i starts every job in (job[i-1], job[i]]. */
+__global__ void k_seg_bounds(const uint32_t* job_sorted, uint64_t n, uint32_t n_jobs, uint64_t* start) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > n) return;
+  const int64_t prev = i ? (int64_t)job_sorted[i - 1] : -1;
+  const int64_t cur = i < n ? (int64_t)job_sorted[i] : (int64_t)n_jobs;
+  for (int64_t q = prev + 1; q <= cur; ++q) start[q] = i;
+}
+__global__ void k_take64(const uint64_t* src, const uint64_t* idx, uint64_t n, uint64_t* dst) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[idx[i]];
 }
 __global__ void k_iota(uint32_t* p, uint64_t n) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -778,17 +847,17 @@ __device__ __forceinline__ void put_kmer(bruno_node& o, KT k) {
   o.kmer[1] = (uint64_t)((u128)k >> 64);
 }
 
-/* P1 counters (before filter) */
+/* P1 counters (before filter) of slots [s0, s1); out == nullptr counts only */
 template <typename KT>
-__global__ __launch_bounds__(BLOCK) void k_get_counts(const Slot<KT>* T, uint64_t cap, const Spill* sp, uint64_t spcap,
-                                                      bruno_node* out, uint64_t out_cap, Counters* ctr) {
+__global__ __launch_bounds__(BLOCK) void k_get_counts(const Slot<KT>* T, uint64_t s0, uint64_t s1, const Spill* sp,
+                                                      uint64_t spcap, bruno_node* out, uint64_t out_cap, Counters* ctr) {
   __shared__ ReserveSmem sm;
   const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
-  for (uint64_t base = (uint64_t)blockIdx.x * BLOCK; base < cap; base += stride) {
+  for (uint64_t base = s0 + (uint64_t)blockIdx.x * BLOCK; base < s1; base += stride) {
     const uint64_t s = base + threadIdx.x;
-    bool occ = (s < cap) && SlotOps<KT>::occupied(T + s);
+    bool occ = (s < s1) && SlotOps<KT>::occupied(T + s);
     bruno_node nd;
-    if (occ) {
+    if (occ && out) {
       K2List L;
       gather_k2<KT>(T + s, s, sp, spcap, L);
       p1_counts(L, nd.counts);
@@ -796,24 +865,25 @@ __global__ __launch_bounds__(BLOCK) void k_get_counts(const Slot<KT>* T, uint64_
       nd.pad[0] = nd.pad[1] = nd.pad[2] = 0;
     }
     unsigned long long pos = block_reserve(occ, &ctr->n_out, sm);
-    if (occ) {
+    if (occ && out) {
       if (pos < out_cap) out[pos] = nd;
       else atomicOr(&ctr->err, ERR_LIST_FULL);
     }
   }
 }
 
-/* k2 list (before filter): one thread per slot, up to 25 entries */
+/* k2 list (before filter) of slots [s0, s1): one thread per slot, up to 25
+ * entries; kmers == nullptr counts only */
 template <typename KT>
-__global__ __launch_bounds__(BLOCK) void k_get_k2(const Slot<KT>* T, uint64_t cap, const Spill* sp, uint64_t spcap,
-                                                  uint64_t* kmers, uint8_t* biedges, uint32_t* counts, uint64_t out_cap,
-                                                  Counters* ctr) {
+__global__ __launch_bounds__(BLOCK) void k_get_k2(const Slot<KT>* T, uint64_t s0, uint64_t s1, const Spill* sp,
+                                                  uint64_t spcap, uint64_t* kmers, uint8_t* biedges, uint32_t* counts,
+                                                  uint64_t out_cap, Counters* ctr) {
   __shared__ ReserveSmem sm;
   const int W = kt_traits<KT>::words;
   const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
-  for (uint64_t base = (uint64_t)blockIdx.x * BLOCK; base < cap; base += stride) {
+  for (uint64_t base = s0 + (uint64_t)blockIdx.x * BLOCK; base < s1; base += stride) {
     const uint64_t s = base + threadIdx.x;
-    bool occ = (s < cap) && SlotOps<KT>::occupied(T + s);
+    bool occ = (s < s1) && SlotOps<KT>::occupied(T + s);
     K2List L;
     L.n = 0;
     if (occ) gather_k2<KT>(T + s, s, sp, spcap, L);
@@ -830,6 +900,38 @@ __global__ __launch_bounds__(BLOCK) void k_get_k2(const Slot<KT>* T, uint64_t ca
         biedges[pos + j] = L.e[j];
         counts[pos + j] = L.c[j];
       }
+    }
+  }
+}
+
+/* present nodes whose slot lies in [s0, s1) (one job of a batch) */
+template <typename KT>
+__global__ __launch_bounds__(BLOCK) void k_get_nodes_range(const Slot<KT>* T, const uint64_t* present, uint64_t n,
+                                                           uint64_t s0, uint64_t s1, bruno_node* out,
+                                                           uint64_t out_cap, Counters* ctr) {
+  __shared__ ReserveSmem sm;
+  const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+  for (uint64_t base = (uint64_t)blockIdx.x * BLOCK; base < n; base += stride) {
+    const uint64_t i = base + threadIdx.x;
+    uint64_t s = 0;
+    bool in = false;
+    if (i < n) {
+      s = present[i];
+      in = s >= s0 && s < s1;
+    }
+    unsigned long long pos = block_reserve(in, &ctr->n_out, sm);
+    if (in && out) {
+      if (pos >= out_cap) {
+        atomicOr(&ctr->err, ERR_LIST_FULL);
+        continue;
+      }
+      const Slot<KT>* p = T + s;
+      const NodeRec* nr = (const NodeRec*)&p->sub[0];
+      bruno_node nd;
+      put_kmer<KT>(nd, SlotOps<KT>::kmer(p));
+      for (int q = 0; q < 9; ++q) nd.counts[q] = nr->c[q];
+      nd.pad[0] = nd.pad[1] = nd.pad[2] = 0;
+      out[pos] = nd;
     }
   }
 }
@@ -902,6 +1004,20 @@ struct bruno_job {
   Counters* hctr = nullptr;     /* pinned host mirror */
   uint64_t device_bytes = 0;
   uint64_t n_reads = 0;
+
+  /* A batch of n_jobs independent jobs shares the table: job i owns slots
+   * [seg[i], seg[i+1]).  A plain job is the one-segment case. */
+  uint32_t n_jobs = 1;
+  std::vector<uint64_t> seg;
+  uint64_t* d_seg = nullptr;
+  unsigned long long* d_jstat = nullptr; /* [0, n): instances, [n, 2n): dropped reads */
+  std::vector<uint64_t> job_reads;
+  uint32_t* chain_job = nullptr;
+  uint32_t* branch_job = nullptr;
+  uint32_t* out_job = nullptr;
+  std::vector<uint32_t> job_iters;
+  std::vector<uint64_t> job_cycle, job_isolated;
+  std::vector<uint64_t> chain_start, fasta_off, branch_start; /* n_jobs + 1 each, after compact */
 
   /* after filter */
   uint64_t* present = nullptr;
@@ -1002,20 +1118,45 @@ double ev_ms(bruno_job* j, int a, int b) {
   return ms;
 }
 
-bruno_status alloc_table(bruno_job* j, uint64_t cap) {
+size_t slot_bytes(const bruno_job* j) { return j->words == 1 ? sizeof(Slot<uint64_t>) : sizeof(Slot<u128>); }
+
+/* One table for all jobs: job i gets caps[i] slots from seg[i]. */
+bruno_status alloc_table(bruno_job* j, const uint64_t* caps) {
+  const uint32_t nj = j->n_jobs;
+  j->seg.assign(nj + 1, 0);
+  for (uint32_t i = 0; i < nj; ++i) j->seg[i + 1] = j->seg[i] + caps[i];
+  const uint64_t cap = j->seg[nj];
   j->cap = cap;
-  size_t slot = j->words == 1 ? sizeof(Slot<uint64_t>) : sizeof(Slot<u128>);
-  TRY(dev_alloc(j, &j->table, cap * slot));
+  TRY(dev_alloc(j, &j->table, cap * slot_bytes(j)));
   j->spcap = std::max<uint64_t>(1024, cap / 8);
   TRY(dev_alloc(j, (void**)&j->spill, j->spcap * sizeof(Spill)));
-  fill64(j, j->table, cap * slot / 8, EMPTY64);
+  TRY(dev_alloc(j, (void**)&j->d_seg, (nj + 1) * sizeof(uint64_t)));
+  TRY(dev_alloc(j, (void**)&j->d_jstat, 2 * nj * sizeof(unsigned long long)));
+  HIP_CHECK(hipMemcpyAsync(j->d_seg, j->seg.data(), (nj + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, j->stream));
+  fill64(j, j->table, cap * slot_bytes(j) / 8, EMPTY64);
   fill64(j, j->spill, j->spcap * sizeof(Spill) / 8, 0);
+  fill64(j, j->d_jstat, 2 * nj, 0);
   return BRUNO_OK;
 }
 
+bool persistent(const bruno_job* j, const void* p) {
+  return p == j->table || p == j->spill || p == j->d_seg || p == j->d_jstat;
+}
+uint64_t persistent_bytes(const bruno_job* j) {
+  if (!j->table) return 0;
+  return j->cap * slot_bytes(j) + j->spcap * sizeof(Spill) + (j->n_jobs + 1) * sizeof(uint64_t) +
+         2 * j->n_jobs * sizeof(unsigned long long);
+}
+
 template <typename KT>
-bruno_status parse_count_t(bruno_job* j, const char* d_bases, const uint64_t* d_offsets, uint64_t n_reads) {
+bruno_status parse_count_t(bruno_job* j, const char* d_bases, const uint64_t* d_offsets, uint64_t n_reads,
+                           const uint64_t* d_first) {
   ParseArgs a{};
+  a.n_jobs = j->n_jobs;
+  a.job_first = d_first;
+  a.seg = j->d_seg;
+  a.job_inst = j->d_jstat;
+  a.job_drop = j->d_jstat + j->n_jobs;
   a.bases = (const uint8_t*)d_bases;
   a.offsets = d_offsets;
   a.n_reads = n_reads;
@@ -1062,8 +1203,13 @@ bruno_status filter_t(bruno_job* j) {
   j->hctr->lr_next = 0;
   HIP_CHECK(hipMemcpyAsync(&j->ctr->lr_count, &j->hctr->lr_count, 2 * sizeof(unsigned long long),
                            hipMemcpyHostToDevice, j->stream));
+  if (j->n_jobs > 1) {
+    TRY(dev_alloc(j, (void**)&j->chain_job, std::max<uint64_t>(1, j->n_chain) * sizeof(uint32_t)));
+    TRY(dev_alloc(j, (void**)&j->branch_job, std::max<uint64_t>(1, j->n_branch) * sizeof(uint32_t)));
+  }
   hipLaunchKernelGGL((k_split<KT>), dim3(grid_for(j->n_present)), dim3(BLOCK), 0, j->stream, (Slot<KT>*)j->table,
-                     j->present, j->n_present, j->chain_slot, j->branch_slot, j->ctr);
+                     j->present, j->n_present, j->chain_slot, j->branch_slot, j->d_seg, j->n_jobs, j->chain_job,
+                     j->branch_job, j->ctr);
   return sync_counters(j);
 }
 
@@ -1076,12 +1222,42 @@ bruno_status chains_t(bruno_job* j) {
   TRY(dev_alloc(j, &j->bkmer, std::max<uint64_t>(1, j->n_branch) * sizeof(KT)));
   TRY(dev_alloc(j, (void**)&j->lr, std::max<uint64_t>(1, nc) * sizeof(LRNode)));
   hipLaunchKernelGGL((k_chain_init<KT>), dim3(grid_for(nc)), dim3(BLOCK), 0, j->stream, (const Slot<KT>*)j->table,
-                     j->cap, j->K, j->chain_slot, nc, (KT*)j->ckmer, j->lr, (KT*)j->vkmer, j->vcap, j->ctr);
+                     j->cap, j->K, j->chain_slot, j->chain_job, j->d_seg, j->n_jobs, nc, (KT*)j->ckmer, j->lr,
+                     (KT*)j->vkmer, j->vcap, j->ctr);
   hipLaunchKernelGGL((k_termini<KT>), dim3(grid_for(j->n_branch)), dim3(BLOCK), 0, j->stream,
-                     (const Slot<KT>*)j->table, j->cap, j->K, j->branch_slot, j->n_branch, j->lr, (KT*)j->bkmer);
+                     (const Slot<KT>*)j->table, j->cap, j->K, j->branch_slot, j->branch_job, j->d_seg, j->n_jobs,
+                     j->n_branch, j->lr, (KT*)j->bkmer);
   TRY(sync_counters(j));
   j->n_virt = j->hctr->n_virt;
   if (nc + j->n_virt >= (1ULL << 31)) return BRUNO_CAPACITY;
+  return BRUNO_OK;
+}
+
+/* separate_cycles / separate_isolated with per-job cycle distances jcm
+ * (device, 0 = no cycle separation for that job); fills the per-job and
+ * total cycle / isolated counts. */
+bruno_status separate(bruno_job* j, const uint32_t* jcm) {
+  const uint32_t nj = j->n_jobs;
+  const uint64_t nc = j->n_chain;
+  unsigned long long* jsep = nullptr;
+  TRY(dev_alloc(j, (void**)&jsep, 2 * nj * sizeof(unsigned long long)));
+  HIP_CHECK(hipMemsetAsync(jsep, 0, 2 * nj * sizeof(unsigned long long), j->stream));
+  hipLaunchKernelGGL(k_separate, dim3(grid_for(nc)), dim3(BLOCK), 0, j->stream, j->lr, nc, j->chain_job, nj, jcm,
+                     j->alive, jsep);
+  std::vector<unsigned long long> h(2 * nj);
+  HIP_CHECK(hipMemcpyAsync(h.data(), jsep, 2 * nj * sizeof(unsigned long long), hipMemcpyDeviceToHost, j->stream));
+  TRY(sync_counters(j));
+  j->job_cycle.assign(nj, 0);
+  j->job_isolated.assign(nj, 0);
+  j->n_cycle = j->n_isolated = 0;
+  for (uint32_t q = 0; q < nj; ++q) {
+    j->job_cycle[q] = h[2 * q];
+    j->job_isolated[q] = h[2 * q + 1];
+    j->n_cycle += h[2 * q];
+    j->n_isolated += h[2 * q + 1];
+  }
+  void* a = jsep;
+  dev_free(j, a, 2 * nj * sizeof(unsigned long long));
   return BRUNO_OK;
 }
 
@@ -1159,92 +1335,140 @@ bruno_status list_rank_fast(bruno_job* j, bool* used) {
     s = walk_failed(s, &fell);
     return s;
   }
-  /* cycle flags reuse the ruler buffer after the walks */
+  const uint32_t nj = j->n_jobs;
   uint8_t* cyc = nullptr;
+  uint32_t* jbuf = nullptr; /* jT[nj], jcyc[nj], jcm[nj] */
   TRY(dev_alloc(j, (void**)&cyc, nc));
+  TRY(dev_alloc(j, (void**)&jbuf, 3 * nj * sizeof(uint32_t)));
+  HIP_CHECK(hipMemsetAsync(jbuf, 0, 2 * nj * sizeof(uint32_t), j->stream));
   hipLaunchKernelGGL(k_lr_finish, dim3(grid_for(nc)), dim3(BLOCK), 0, j->stream, j->lr, nc, ruler, rec, info, cyc,
-                     j->ctr);
+                     j->chain_job, nj, jbuf, jbuf + nj, j->ctr);
+  std::vector<uint32_t> hj(3 * nj, 0);
+  HIP_CHECK(hipMemcpyAsync(hj.data(), jbuf, 2 * nj * sizeof(uint32_t), hipMemcpyDeviceToHost, j->stream));
   TRY(sync_counters(j));
-  uint64_t maxr = j->hctr->lr_count;
-  uint64_t ncyc = j->hctr->lr_cycle;
-  uint32_t T = (uint32_t)maxr;
-  if (ncyc > 0 && T < 1) T = 1;
-  if (T >= 30) return BRUNO_INTERNAL; /* the reference stops at 30 rounds; not reachable below 2^29 nodes */
-  if (ncyc) hipLaunchKernelGGL(k_lr_cycles, dim3(grid_for(nc)), dim3(BLOCK), 0, j->stream, j->lr, nc, cyc, 1u << T);
-  j->iterations = T;
+  const uint64_t ncyc = j->hctr->lr_cycle;
+  /* per job: T = largest ceil(log2 L), at least 1 with a cycle; cycle nodes
+   * end at distance 2^T and are separated */
+  j->job_iters.assign(nj, 0);
+  uint32_t Tmax = 0;
+  for (uint32_t q = 0; q < nj; ++q) {
+    uint32_t T = hj[q];
+    if (hj[nj + q] && T < 1) T = 1;
+    if (T >= 30) return BRUNO_INTERNAL; /* the reference stops at 30 rounds; not reachable below 2^29 nodes */
+    j->job_iters[q] = T;
+    hj[2 * nj + q] = hj[nj + q] ? (1u << T) : 0u;
+    Tmax = std::max(Tmax, T);
+  }
+  uint32_t* jcm = jbuf + 2 * nj;
+  HIP_CHECK(hipMemcpyAsync(jcm, hj.data() + 2 * nj, nj * sizeof(uint32_t), hipMemcpyHostToDevice, j->stream));
+  if (ncyc)
+    hipLaunchKernelGGL(k_lr_cycles, dim3(grid_for(nc)), dim3(BLOCK), 0, j->stream, j->lr, nc, cyc, j->chain_job, nj,
+                       jcm);
+  j->iterations = Tmax;
   j->conflicts = 0;
-  /* separate_cycles only when unfinished nodes remain, i.e. cycles */
-  j->hctr->n_cycle = 0;
-  j->hctr->n_isolated = 0;
-  HIP_CHECK(hipMemcpyAsync(&j->ctr->n_cycle, &j->hctr->n_cycle, 2 * sizeof(unsigned long long),
-                           hipMemcpyHostToDevice, j->stream));
   TRY(dev_alloc(j, (void**)&j->alive, nc));
-  hipLaunchKernelGGL(k_separate, dim3(grid_for(nc)), dim3(BLOCK), 0, j->stream, j->lr, nc, ncyc ? 1 : 0, 1u << T,
-                     j->alive, j->ctr);
-  TRY(sync_counters(j));
-  j->n_cycle = j->hctr->n_cycle;
-  j->n_isolated = j->hctr->n_isolated;
+  TRY(separate(j, jcm));
   *used = true;
   return BRUNO_OK;
 }
 
 bruno_status list_rank_rounds(bruno_job* j, int variant) {
-  uint64_t nc = j->n_chain;
+  const uint64_t nc = j->n_chain;
+  const uint32_t nj = j->n_jobs;
+  const bool batch = nj > 1;
   TRY(dev_alloc(j, (void**)&j->list[0], std::max<uint64_t>(1, nc) * sizeof(uint32_t)));
   TRY(dev_alloc(j, (void**)&j->list[1], std::max<uint64_t>(1, nc) * sizeof(uint32_t)));
   TRY(dev_alloc(j, (void**)&j->alive, std::max<uint64_t>(1, nc)));
+  /* batch: per-job round counters and active flags */
+  unsigned long long* jctr = nullptr;
+  uint8_t* active = nullptr;
+  uint32_t* jcm = nullptr;
+  std::vector<unsigned long long> hc(3 * nj, 0);
+  std::vector<uint8_t> act(nj, 0);
+  TRY(dev_alloc(j, (void**)&jcm, nj * sizeof(uint32_t)));
+  if (batch) {
+    TRY(dev_alloc(j, (void**)&jctr, 3 * nj * sizeof(unsigned long long)));
+    TRY(dev_alloc(j, (void**)&active, nj));
+    HIP_CHECK(hipMemsetAsync(jctr, 0, 3 * nj * sizeof(unsigned long long), j->stream));
+  }
   j->hctr->lr_next = 0;
   j->hctr->lr_count = 0;
   j->hctr->lr_cycle = 0;
   j->hctr->lr_conflict = 0;
   HIP_CHECK(hipMemcpyAsync(&j->ctr->lr_count, &j->hctr->lr_count, 4 * sizeof(unsigned long long),
                            hipMemcpyHostToDevice, j->stream));
-  hipLaunchKernelGGL(k_lr_init_list, dim3(grid_for(nc)), dim3(BLOCK), 0, j->stream, j->lr, nc, j->list[0], j->ctr);
+  hipLaunchKernelGGL(k_lr_init_list, dim3(grid_for(nc)), dim3(BLOCK), 0, j->stream, j->lr, nc, j->list[0],
+                     j->chain_job, jctr, j->ctr);
+  if (batch) HIP_CHECK(hipMemcpyAsync(hc.data(), jctr, 3 * nj * 8, hipMemcpyDeviceToHost, j->stream));
   TRY(sync_counters(j));
   uint64_t n = j->hctr->lr_next;
+  /* unfinished nodes per job; a job without any takes no round */
+  std::vector<uint64_t> unf(nj, 0);
+  if (batch) {
+    for (uint32_t q = 0; q < nj; ++q) unf[q] = hc[3 * q + 1];
+  } else {
+    unf[0] = n;
+  }
+  uint32_t n_active = 0;
+  for (uint32_t q = 0; q < nj; ++q) {
+    act[q] = unf[q] > 0;
+    n_active += act[q];
+  }
+  if (batch) HIP_CHECK(hipMemcpyAsync(active, act.data(), nj, hipMemcpyHostToDevice, j->stream));
+  j->job_iters.assign(nj, 0);
   uint32_t iters = 0;
-  bool done = (n == 0);
   int cur = 0;
   uint64_t conflicts = 0;
-  while (!done) {
+  while (n_active > 0) {
     j->hctr->lr_count = 0;
     j->hctr->lr_next = 0;
     j->hctr->lr_cycle = 0;
     j->hctr->lr_conflict = 0;
     HIP_CHECK(hipMemcpyAsync(&j->ctr->lr_count, &j->hctr->lr_count, 4 * sizeof(unsigned long long),
                              hipMemcpyHostToDevice, j->stream));
+    if (batch) HIP_CHECK(hipMemsetAsync(jctr, 0, 3 * nj * sizeof(unsigned long long), j->stream));
     hipLaunchKernelGGL(k_lr_send, dim3(grid_for(n)), dim3(BLOCK), 0, j->stream, j->lr, nc, j->list[cur], n, variant,
-                       j->ctr);
+                       j->chain_job, active, j->ctr);
     /* after this round the iteration count is iters+1 and IsCycleNode
-     * compares against 1 << (iters+1) (debruijn_chain_filters.hpp:189-213) */
+     * compares against 1 << (iters+1) (debruijn_chain_filters.hpp:189-213);
+     * every active job has taken every round so far */
     uint32_t maxd = 1u << (iters + 1);
     hipLaunchKernelGGL(k_lr_recv, dim3(grid_for(n)), dim3(BLOCK), 0, j->stream, j->lr, j->list[cur], n,
-                       j->list[cur ^ 1], maxd, j->ctr);
+                       j->list[cur ^ 1], maxd, j->chain_job, active, jctr, j->ctr);
+    if (batch) HIP_CHECK(hipMemcpyAsync(hc.data(), jctr, 3 * nj * 8, hipMemcpyDeviceToHost, j->stream));
     TRY(sync_counters(j));
-    uint64_t count = j->hctr->lr_count;
     n = j->hctr->lr_next;
-    uint64_t cyc = j->hctr->lr_cycle;
     conflicts += j->hctr->lr_conflict;
     cur ^= 1;
     ++iters;
-    if (iters >= 30) done = true;
-    else done = (count == 0) || (cyc == n);
+    if (!batch) {
+      hc[0] = j->hctr->lr_count;
+      hc[1] = n;
+      hc[2] = j->hctr->lr_cycle;
+    }
+    bool changed = false;
+    for (uint32_t q = 0; q < nj; ++q) {
+      if (!act[q]) continue;
+      ++j->job_iters[q];
+      const uint64_t count = hc[3 * q], u = hc[3 * q + 1], cyc = hc[3 * q + 2];
+      unf[q] = u;
+      /* stop when nothing changed or only cycle-like nodes remain, or at
+       * 30 rounds (debruijn_chain_graph.hpp:1238-1277) */
+      if (j->job_iters[q] >= 30 || count == 0 || cyc == u) {
+        act[q] = 0;
+        --n_active;
+        changed = true;
+      }
+    }
+    if (batch && changed) HIP_CHECK(hipMemcpyAsync(active, act.data(), nj, hipMemcpyHostToDevice, j->stream));
   }
   j->iterations = iters;
   j->conflicts = conflicts;
   /* separate_cycles only when unfinished nodes remain (:1284-1291) */
-  int do_cycles = (n > 0) ? 1 : 0;
-  uint32_t maxd = (iters < 31) ? (1u << iters) : 0u;
-  j->hctr->n_cycle = 0;
-  j->hctr->n_isolated = 0;
-  HIP_CHECK(hipMemcpyAsync(&j->ctr->n_cycle, &j->hctr->n_cycle, 2 * sizeof(unsigned long long),
-                           hipMemcpyHostToDevice, j->stream));
-  hipLaunchKernelGGL(k_separate, dim3(grid_for(nc)), dim3(BLOCK), 0, j->stream, j->lr, nc, do_cycles, maxd, j->alive,
-                     j->ctr);
-  TRY(sync_counters(j));
-  j->n_cycle = j->hctr->n_cycle;
-  j->n_isolated = j->hctr->n_isolated;
-  return BRUNO_OK;
+  std::vector<uint32_t> hm(nj, 0);
+  for (uint32_t q = 0; q < nj; ++q) hm[q] = unf[q] > 0 ? (1u << j->job_iters[q]) : 0u;
+  HIP_CHECK(hipMemcpyAsync(jcm, hm.data(), nj * sizeof(uint32_t), hipMemcpyHostToDevice, j->stream));
+  return separate(j, jcm);
 }
 
 template <typename KT>
@@ -1291,6 +1515,63 @@ bruno_status sort_perm_kmer(bruno_job* j, const KT* keys, uint32_t* perm, uint64
   return BRUNO_OK;
 }
 
+/* Stable pass over a 32-bit key: perm is reordered by keys[perm[i]]'s low
+ * `bits` bits, keeping the current order within equal keys. */
+bruno_status sort_perm_u32(bruno_job* j, const uint32_t* keys, uint32_t* perm, uint64_t n, int bits) {
+  if (n == 0 || bits <= 0) return BRUNO_OK;
+  uint32_t *kin = nullptr, *kout = nullptr, *pout = nullptr;
+  TRY(dev_alloc(j, (void**)&kin, n * 4));
+  TRY(dev_alloc(j, (void**)&kout, n * 4));
+  TRY(dev_alloc(j, (void**)&pout, n * 4));
+  hipLaunchKernelGGL(k_gather_u32, dim3(grid_for(n, BLOCK, 1u << 30)), dim3(BLOCK), 0, j->stream, keys, perm, kin, n);
+  size_t tmp = 0;
+  HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, kin, kout, perm, pout, (int)n, 0, bits, j->stream));
+  void* t = nullptr;
+  TRY(dev_alloc(j, &t, tmp));
+  HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(t, tmp, kin, kout, perm, pout, (int)n, 0, bits, j->stream));
+  HIP_CHECK(hipMemcpyAsync(perm, pout, n * 4, hipMemcpyDeviceToDevice, j->stream));
+  HIP_CHECK(hipStreamSynchronize(j->stream));
+  dev_free(j, t, tmp);
+  void* a = kin;
+  dev_free(j, a, n * 4);
+  a = kout;
+  dev_free(j, a, n * 4);
+  a = pout;
+  dev_free(j, a, n * 4);
+  return BRUNO_OK;
+}
+
+int job_bits(uint32_t n_jobs) {
+  int b = 0;
+  while (b < 32 && (1ull << b) < n_jobs) ++b;
+  return b;
+}
+
+/* Per-job start positions in a job-sorted order: out[0..n_jobs] on the
+ * host.  job_of_item == nullptr (one job) gives {0, n}. */
+bruno_status job_bounds(bruno_job* j, const uint32_t* job_of_item, const uint32_t* perm, uint64_t n,
+                        std::vector<uint64_t>& out) {
+  const uint32_t nj = j->n_jobs;
+  out.assign(nj + 1, 0);
+  out[nj] = n;
+  if (nj == 1) return BRUNO_OK;
+  uint32_t* js = nullptr;
+  uint64_t* st = nullptr;
+  TRY(dev_alloc(j, (void**)&js, std::max<uint64_t>(1, n) * 4));
+  TRY(dev_alloc(j, (void**)&st, (nj + 1) * 8));
+  if (n)
+    hipLaunchKernelGGL(k_gather_u32, dim3(grid_for(n, BLOCK, 1u << 30)), dim3(BLOCK), 0, j->stream, job_of_item, perm,
+                       js, n);
+  hipLaunchKernelGGL(k_seg_bounds, dim3(grid_for(n + 1, BLOCK, 1u << 30)), dim3(BLOCK), 0, j->stream, js, n, nj, st);
+  HIP_CHECK(hipMemcpyAsync(out.data(), st, (nj + 1) * 8, hipMemcpyDeviceToHost, j->stream));
+  HIP_CHECK(hipStreamSynchronize(j->stream));
+  void* a = js;
+  dev_free(j, a, std::max<uint64_t>(1, n) * 4);
+  a = st;
+  dev_free(j, a, (nj + 1) * 8);
+  return BRUNO_OK;
+}
+
 template <typename KT>
 bruno_status compact_t(bruno_job* j) {
   uint64_t nc = j->n_chain;
@@ -1300,15 +1581,20 @@ bruno_status compact_t(bruno_job* j) {
   TRY(dev_alloc(j, (void**)&j->out_dist, std::max<uint64_t>(1, nc) * 4));
   TRY(dev_alloc(j, (void**)&j->perm, std::max<uint64_t>(1, nc) * 4));
   TRY(dev_alloc(j, (void**)&j->dist_sorted, std::max<uint64_t>(1, nc) * 4));
+  const uint32_t nj = j->n_jobs;
+  if (nj > 1) TRY(dev_alloc(j, (void**)&j->out_job, std::max<uint64_t>(1, nc) * 4));
   j->hctr->n_out = 0;
   HIP_CHECK(hipMemcpyAsync(&j->ctr->n_out, &j->hctr->n_out, sizeof(unsigned long long), hipMemcpyHostToDevice,
                            j->stream));
   hipLaunchKernelGGL((k_ranked<KT>), dim3(grid_for(nc)), dim3(BLOCK), 0, j->stream, j->lr, (const KT*)j->ckmer, nc,
-                     (const KT*)j->vkmer, j->alive, K, (KT*)j->out_k, (KT*)j->out_rep, j->out_dist, j->ctr);
+                     (const KT*)j->vkmer, j->alive, K, j->chain_job, (KT*)j->out_k, (KT*)j->out_rep, j->out_dist,
+                     j->out_job, j->ctr);
   TRY(sync_counters(j));
   uint64_t n = j->hctr->n_out;
   j->n_out = n;
-  /* chain_rep_less: by rep, then |dist| -> LSD: dist first, then rep words */
+  /* chain_rep_less: by rep, then |dist| -> LSD: dist first, then rep words;
+   * a batch adds a last pass on the job, so each job's nodes are contiguous */
+  j->fasta_off.assign(nj + 1, 0);
   if (n > 0) {
     uint32_t* pin = nullptr;
     TRY(dev_alloc(j, (void**)&pin, n * 4));
@@ -1328,6 +1614,8 @@ bruno_status compact_t(bruno_job* j) {
     a = pin;
     dev_free(j, a, n * 4);
     TRY(sort_perm_kmer<KT>(j, (const KT*)j->out_rep, j->perm, n, K, true));
+    if (nj > 1) TRY(sort_perm_u32(j, j->out_job, j->perm, n, job_bits(nj)));
+    TRY(job_bounds(j, j->out_job, j->perm, n, j->chain_start));
     hipLaunchKernelGGL(k_gather_u32, dim3(grid_for(n, BLOCK, 1u << 30)), dim3(BLOCK), 0, j->stream, j->out_dist,
                        j->perm, j->dist_sorted, n);
     /* .chain.fasta */
@@ -1349,16 +1637,33 @@ bruno_status compact_t(bruno_job* j) {
     hipLaunchKernelGGL((k_fasta_write<KT>), dim3(grid_for(n, BLOCK, 1u << 30)), dim3(BLOCK), 0, j->stream, j->perm,
                        (const KT*)j->out_k, (const KT*)j->out_rep, j->dist_sorted, off, n, K, j->fasta);
     j->fasta_len = total;
+    /* byte range of each job's records */
+    j->fasta_off[nj] = total;
+    if (nj > 1) {
+      uint64_t *cs = nullptr, *fo = nullptr;
+      TRY(dev_alloc(j, (void**)&cs, (nj + 1) * 8));
+      TRY(dev_alloc(j, (void**)&fo, (nj + 1) * 8));
+      HIP_CHECK(hipMemcpyAsync(cs, j->chain_start.data(), (nj + 1) * 8, hipMemcpyHostToDevice, j->stream));
+      hipLaunchKernelGGL(k_take64, dim3(grid_for(nj + 1)), dim3(BLOCK), 0, j->stream, off, cs, nj + 1, fo);
+      HIP_CHECK(hipMemcpyAsync(j->fasta_off.data(), fo, (nj + 1) * 8, hipMemcpyDeviceToHost, j->stream));
+      HIP_CHECK(hipStreamSynchronize(j->stream));
+      a = cs;
+      dev_free(j, a, (nj + 1) * 8);
+      a = fo;
+      dev_free(j, a, (nj + 1) * 8);
+    }
     a = len;
     dev_free(j, a, n * 8);
     a = off;
     dev_free(j, a, (n + 1) * 8);
   } else {
     j->fasta_len = 0;
+    j->chain_start.assign(nj + 1, 0);
     TRY(dev_alloc(j, (void**)&j->fasta, 16));
   }
   /* .branch.fasta: sorted branch k-mers */
   uint64_t nb = j->n_branch;
+  j->branch_start.assign(nj + 1, 0);
   TRY(dev_alloc(j, &j->bsorted, std::max<uint64_t>(1, nb) * sizeof(KT)));
   j->bfasta_len = nb * (uint64_t)(2 * K + 3);
   TRY(dev_alloc(j, (void**)&j->bfasta, j->bfasta_len + 1));
@@ -1366,6 +1671,8 @@ bruno_status compact_t(bruno_job* j) {
     uint32_t* bp = nullptr;
     TRY(dev_alloc(j, (void**)&bp, nb * 4));
     TRY(sort_perm_kmer<KT>(j, (const KT*)j->bkmer, bp, nb, K, false));
+    if (nj > 1) TRY(sort_perm_u32(j, j->branch_job, bp, nb, job_bits(nj)));
+    TRY(job_bounds(j, j->branch_job, bp, nb, j->branch_start));
     hipLaunchKernelGGL((k_gather_kmer<KT>), dim3(grid_for(nb, BLOCK, 1u << 30)), dim3(BLOCK), 0, j->stream,
                        (const KT*)j->bkmer, bp, (KT*)j->bsorted, nb);
     hipLaunchKernelGGL((k_branch_fasta<KT>), dim3(grid_for(nb, BLOCK, 1u << 30)), dim3(BLOCK), 0, j->stream,
@@ -1403,10 +1710,10 @@ const char* bruno_status_string(bruno_status s) {
   return "unknown";
 }
 
-bruno_status bruno_job_create(const bruno_job_desc* d, bruno_job** out) {
+static bruno_status create_common(const bruno_job_desc* d, uint32_t n_jobs, const uint64_t* caps, bruno_job** out) {
   if (!d || !out) return BRUNO_BAD_INPUT;
   *out = nullptr;
-  if (d->k < 3 || d->k > 63) return BRUNO_BAD_INPUT;
+  if (d->k < 3 || d->k > 63 || n_jobs == 0) return BRUNO_BAD_INPUT;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
     (void)hipGetLastError();
@@ -1421,7 +1728,8 @@ bruno_status bruno_job_create(const bruno_job_desc* d, bruno_job** out) {
                  d->thresholds[3], d->thresholds[4], d->thresholds[5]};
   j->load_factor = (d->load_factor > 0 && d->load_factor < 1) ? d->load_factor : 0.5;
   j->stream = (hipStream_t)d->stream;
-  j->cap = d->capacity;
+  j->n_jobs = n_jobs;
+  j->job_reads.assign(n_jobs, 0);
   if (hipMalloc((void**)&j->ctr, sizeof(Counters)) != hipSuccess ||
       hipHostMalloc((void**)&j->hctr, sizeof(Counters), hipHostMallocDefault) != hipSuccess) {
     (void)hipGetLastError();
@@ -1439,8 +1747,12 @@ bruno_status bruno_job_create(const bruno_job_desc* d, bruno_job** out) {
       (void)hipGetLastError();
       j->ev_ok = false;
     }
-  if (j->cap > 0) {
-    bruno_status s = alloc_table(j, j->cap);
+  if (caps) {
+    bruno_status s = alloc_table(j, caps);
+    if (s == BRUNO_OK && hipStreamSynchronize(j->stream) != hipSuccess) {
+      (void)hipGetLastError();
+      s = BRUNO_INTERNAL;
+    }
     if (s != BRUNO_OK) {
       bruno_job_destroy(j);
       return s;
@@ -1448,6 +1760,20 @@ bruno_status bruno_job_create(const bruno_job_desc* d, bruno_job** out) {
   }
   *out = j;
   return BRUNO_OK;
+}
+
+bruno_status bruno_job_create(const bruno_job_desc* d, bruno_job** out) {
+  if (!d) return BRUNO_BAD_INPUT;
+  const uint64_t cap = d->capacity;
+  return create_common(d, 1, cap ? &cap : nullptr, out);
+}
+
+bruno_status bruno_batch_create(const bruno_job_desc* d, uint32_t n_jobs, const uint64_t* capacities,
+                                bruno_job** out) {
+  if (!capacities || n_jobs == 0 || n_jobs > BRUNO_MAX_BATCH_JOBS) return BRUNO_BAD_INPUT;
+  for (uint32_t i = 0; i < n_jobs; ++i)
+    if (capacities[i] == 0) return BRUNO_BAD_INPUT;
+  return create_common(d, n_jobs, capacities, out);
 }
 
 bruno_status bruno_job_destroy(bruno_job* j) {
@@ -1466,38 +1792,41 @@ bruno_status bruno_job_destroy(bruno_job* j) {
 bruno_status bruno_job_reset(bruno_job* j) {
   if (!valid_job(j)) return BRUNO_BAD_INPUT;
   HIP_CHECK(hipStreamSynchronize(j->stream));
-  void* keep_t = j->table;
-  void* keep_s = j->spill;
-  for (void* p : j->allocs)
-    if (p != keep_t && p != keep_s) (void)hipFree(p);
-  j->allocs.clear();
-  uint64_t tb = 0;
-  if (keep_t) {
-    j->allocs.push_back(keep_t);
-    j->allocs.push_back(keep_s);
-    tb = j->cap * (j->words == 1 ? sizeof(Slot<uint64_t>) : sizeof(Slot<u128>)) + j->spcap * sizeof(Spill);
+  std::vector<void*> keep;
+  for (void* p : j->allocs) {
+    if (persistent(j, p)) keep.push_back(p);
+    else (void)hipFree(p);
   }
-  j->device_bytes = tb;
+  j->allocs = keep;
+  j->device_bytes = persistent_bytes(j);
   j->present = j->chain_slot = j->branch_slot = nullptr;
   j->ckmer = j->vkmer = j->bkmer = j->out_k = j->out_rep = j->bsorted = nullptr;
   j->lr = nullptr;
   j->list[0] = j->list[1] = nullptr;
   j->alive = nullptr;
   j->out_dist = j->perm = j->dist_sorted = nullptr;
+  j->chain_job = j->branch_job = j->out_job = nullptr;
   j->fasta = j->bfasta = nullptr;
   j->n_present = j->n_branch = j->n_chain = j->n_virt = 0;
   j->iterations = 0;
   j->n_cycle = j->n_isolated = j->conflicts = 0;
   j->n_out = j->n_unitigs = j->fasta_len = j->bfasta_len = 0;
   j->n_reads = 0;
+  j->job_reads.assign(j->n_jobs, 0);
+  j->job_iters.clear();
+  j->job_cycle.clear();
+  j->job_isolated.clear();
+  j->chain_start.clear();
+  j->fasta_off.clear();
+  j->branch_start.clear();
   j->ms = bruno_phase_ms{};
   memset(j->hctr, 0, sizeof(Counters));
   HIP_CHECK(hipMemsetAsync(j->ctr, 0, sizeof(Counters), j->stream));
-  if (keep_t) {
+  if (j->table) {
     ev_record(j, 10);
-    size_t slot = j->words == 1 ? sizeof(Slot<uint64_t>) : sizeof(Slot<u128>);
-    fill64(j, j->table, j->cap * slot / 8, EMPTY64);
+    fill64(j, j->table, j->cap * slot_bytes(j) / 8, EMPTY64);
     fill64(j, j->spill, j->spcap * sizeof(Spill) / 8, 0);
+    fill64(j, j->d_jstat, 2 * j->n_jobs, 0);
     ev_record(j, 11);
   }
   HIP_CHECK(hipStreamSynchronize(j->stream));
@@ -1505,8 +1834,32 @@ bruno_status bruno_job_reset(bruno_job* j) {
   return BRUNO_OK;
 }
 
+static bruno_status parse_count_common(bruno_job* j, const char* d_bases, const uint64_t* d_offsets, uint64_t n_reads,
+                                       const uint64_t* d_first) {
+  ev_record(j, 0);
+  if (j->words == 1) TRY(parse_count_t<uint64_t>(j, d_bases, d_offsets, n_reads, d_first));
+  else TRY(parse_count_t<u128>(j, d_bases, d_offsets, n_reads, d_first));
+  ev_record(j, 1);
+  TRY(sync_counters(j));
+  j->ms.parse_count_ms += ev_ms(j, 0, 1);
+  j->n_reads += n_reads;
+  /* k2 counts are uint32 in the reference (debruijn_graph_map.hpp:1124):
+   * a job must stay below 2^32 instances */
+  if (j->n_jobs == 1) {
+    if (j->hctr->n_instances >= (1ULL << 32)) return BRUNO_CAPACITY;
+  } else {
+    std::vector<unsigned long long> js(j->n_jobs);
+    HIP_CHECK(hipMemcpy(js.data(), j->d_jstat, j->n_jobs * 8, hipMemcpyDeviceToHost));
+    for (auto v : js)
+      if (v >= (1ULL << 32)) return BRUNO_CAPACITY;
+  }
+  j->stage = ST_COUNTED;
+  return BRUNO_OK;
+}
+
 bruno_status bruno_parse_count(bruno_job* j, const char* d_bases, const uint64_t* d_offsets, uint64_t n_reads) {
   if (!valid_job(j)) return BRUNO_BAD_INPUT;
+  if (j->n_jobs != 1) return BRUNO_BAD_INPUT; /* a batch names each read's job */
   if (j->stage > ST_COUNTED) return BRUNO_STATE;
   if (n_reads == 0) {
     j->stage = ST_COUNTED;
@@ -1521,18 +1874,39 @@ bruno_status bruno_parse_count(bruno_job* j, const char* d_bases, const uint64_t
     HIP_CHECK(hipStreamSynchronize(j->stream));
     uint64_t nb = ends[1] - ends[0];
     uint64_t want = (uint64_t)((double)nb / j->load_factor) + 1024;
-    TRY(alloc_table(j, want));
+    TRY(alloc_table(j, &want));
   }
-  ev_record(j, 0);
-  if (j->words == 1) TRY(parse_count_t<uint64_t>(j, d_bases, d_offsets, n_reads));
-  else TRY(parse_count_t<u128>(j, d_bases, d_offsets, n_reads));
-  ev_record(j, 1);
-  TRY(sync_counters(j));
-  j->ms.parse_count_ms += ev_ms(j, 0, 1);
-  j->n_reads += n_reads;
-  if (j->hctr->n_instances >= (1ULL << 32)) return BRUNO_CAPACITY; /* uint32 k2 counts */
-  j->stage = ST_COUNTED;
-  return BRUNO_OK;
+  j->job_reads[0] += n_reads;
+  return parse_count_common(j, d_bases, d_offsets, n_reads, nullptr);
+}
+
+bruno_status bruno_batch_parse_count(bruno_job* j, const char* d_bases, const uint64_t* d_offsets, uint64_t n_reads,
+                                     const uint64_t* job_first_read) {
+  if (!valid_job(j) || !job_first_read) return BRUNO_BAD_INPUT;
+  if (j->stage > ST_COUNTED) return BRUNO_STATE;
+  const uint32_t nj = j->n_jobs;
+  if (job_first_read[0] != 0 || job_first_read[nj] != n_reads) return BRUNO_BAD_INPUT;
+  for (uint32_t i = 0; i < nj; ++i)
+    if (job_first_read[i + 1] < job_first_read[i]) return BRUNO_BAD_INPUT;
+  if (n_reads == 0) {
+    j->stage = ST_COUNTED;
+    return BRUNO_OK;
+  }
+  if (!d_bases || !d_offsets || j->cap == 0) return BRUNO_BAD_INPUT;
+  uint64_t* d_first = nullptr;
+  if (nj > 1) {
+    TRY(dev_alloc(j, (void**)&d_first, (nj + 1) * 8));
+    HIP_CHECK(hipMemcpyAsync(d_first, job_first_read, (nj + 1) * 8, hipMemcpyHostToDevice, j->stream));
+  }
+  bruno_status s = parse_count_common(j, d_bases, d_offsets, n_reads, d_first);
+  if (d_first) {
+    (void)hipStreamSynchronize(j->stream);
+    void* a = d_first;
+    dev_free(j, a, (nj + 1) * 8);
+  }
+  if (s == BRUNO_OK)
+    for (uint32_t i = 0; i < nj; ++i) j->job_reads[i] += job_first_read[i + 1] - job_first_read[i];
+  return s;
 }
 
 bruno_status bruno_parse_count_host(bruno_job* j, const char* bases, const uint64_t* offsets, uint64_t n_reads) {
@@ -1660,124 +2034,249 @@ bruno_status bruno_get_phase_ms(bruno_job* j, bruno_phase_ms* o) {
   return BRUNO_OK;
 }
 
-bruno_status bruno_get_counts(bruno_job* j, bruno_node* buf, uint64_t* n) {
-  if (!valid_job(j) || !n) return BRUNO_BAD_INPUT;
+/* ---- per-job views ------------------------------------------------- */
+static bool valid_idx(const bruno_job* j, uint32_t idx) { return valid_job(j) && idx < j->n_jobs; }
+
+static uint64_t unitigs_of(const bruno_job* j, uint64_t bytes, uint64_t nodes) {
+  /* bytes = heads * (2K + 3) + (nodes - heads) */
+  const uint64_t per_head = 2 * (uint64_t)j->K + 3;
+  return bytes >= nodes ? (bytes - nodes) / (per_head - 1) : 0;
+}
+
+bruno_status bruno_batch_get_stats(bruno_job* j, uint32_t idx, bruno_stats* o) {
+  if (!valid_idx(j, idx) || !o) return BRUNO_BAD_INPUT;
+  if (j->n_jobs == 1) return bruno_get_stats(j, o);
+  TRY(sync_counters(j));
+  memset(o, 0, sizeof(*o));
+  const uint32_t nj = j->n_jobs;
+  std::vector<unsigned long long> js(2 * nj);
+  HIP_CHECK(hipMemcpyAsync(js.data(), j->d_jstat, 2 * nj * 8, hipMemcpyDeviceToHost, j->stream));
+  HIP_CHECK(hipStreamSynchronize(j->stream));
+  o->n_reads = j->job_reads[idx];
+  o->n_instances = js[idx];
+  o->n_reads_dropped = js[nj + idx];
+  o->capacity = j->seg[idx + 1] - j->seg[idx];
+  if (j->stage >= ST_RANKED) {
+    o->iterations = j->job_iters[idx];
+    o->n_cycle_nodes = j->job_cycle[idx];
+    o->n_isolated = j->job_isolated[idx];
+  }
+  if (j->stage >= ST_COMPACTED) {
+    o->n_branches = j->branch_start[idx + 1] - j->branch_start[idx];
+    o->n_unitig_nodes = j->chain_start[idx + 1] - j->chain_start[idx];
+    o->n_chain_nodes = o->n_unitig_nodes + o->n_cycle_nodes + o->n_isolated;
+    o->n_nodes = o->n_branches + o->n_chain_nodes;
+    o->n_unitigs = unitigs_of(j, j->fasta_off[idx + 1] - j->fasta_off[idx], o->n_unitig_nodes);
+  }
+  return BRUNO_OK;
+}
+
+bruno_status bruno_batch_get_counts(bruno_job* j, uint32_t idx, bruno_node* buf, uint64_t* n) {
+  if (!valid_idx(j, idx) || !n) return BRUNO_BAD_INPUT;
   if (j->stage != ST_COUNTED) return BRUNO_STATE;
-  uint64_t nd = j->hctr->n_distinct;
+  const uint64_t s0 = j->seg.empty() ? 0 : j->seg[idx], s1 = j->seg.empty() ? 0 : j->seg[idx + 1];
+  const bool one = j->n_jobs == 1;
   if (!buf) {
-    *n = nd;
+    if (one) {
+      *n = j->hctr->n_distinct;
+      return BRUNO_OK;
+    }
+    j->hctr->n_out = 0;
+    HIP_CHECK(hipMemcpyAsync(&j->ctr->n_out, &j->hctr->n_out, 8, hipMemcpyHostToDevice, j->stream));
+    if (j->words == 1)
+      hipLaunchKernelGGL((k_get_counts<uint64_t>), dim3(grid_for(s1 - s0, BLOCK, 1u << 15)), dim3(BLOCK), 0, j->stream,
+                         (const Slot<uint64_t>*)j->table, s0, s1, (const Spill*)j->spill, j->spcap, nullptr, 0,
+                         j->ctr);
+    else
+      hipLaunchKernelGGL((k_get_counts<u128>), dim3(grid_for(s1 - s0, BLOCK, 1u << 15)), dim3(BLOCK), 0, j->stream,
+                         (const Slot<u128>*)j->table, s0, s1, (const Spill*)j->spill, j->spcap, nullptr, 0, j->ctr);
+    TRY(sync_counters(j));
+    *n = j->hctr->n_out;
     return BRUNO_OK;
   }
-  if (*n < nd) return BRUNO_BAD_INPUT;
+  const uint64_t cap_out = *n;
   bruno_node* d = nullptr;
-  TRY(dev_alloc(j, (void**)&d, std::max<uint64_t>(1, nd) * sizeof(bruno_node)));
+  TRY(dev_alloc(j, (void**)&d, std::max<uint64_t>(1, cap_out) * sizeof(bruno_node)));
   j->hctr->n_out = 0;
   HIP_CHECK(hipMemcpyAsync(&j->ctr->n_out, &j->hctr->n_out, 8, hipMemcpyHostToDevice, j->stream));
   if (j->words == 1)
-    hipLaunchKernelGGL((k_get_counts<uint64_t>), dim3(grid_for(j->cap, BLOCK, 1u << 15)), dim3(BLOCK), 0, j->stream,
-                       (const Slot<uint64_t>*)j->table, j->cap, (const Spill*)j->spill, j->spcap, d, nd, j->ctr);
+    hipLaunchKernelGGL((k_get_counts<uint64_t>), dim3(grid_for(s1 - s0, BLOCK, 1u << 15)), dim3(BLOCK), 0, j->stream,
+                       (const Slot<uint64_t>*)j->table, s0, s1, (const Spill*)j->spill, j->spcap, d, cap_out, j->ctr);
   else
-    hipLaunchKernelGGL((k_get_counts<u128>), dim3(grid_for(j->cap, BLOCK, 1u << 15)), dim3(BLOCK), 0, j->stream,
-                       (const Slot<u128>*)j->table, j->cap, (const Spill*)j->spill, j->spcap, d, nd, j->ctr);
-  HIP_CHECK(hipMemcpyAsync(buf, d, nd * sizeof(bruno_node), hipMemcpyDeviceToHost, j->stream));
+    hipLaunchKernelGGL((k_get_counts<u128>), dim3(grid_for(s1 - s0, BLOCK, 1u << 15)), dim3(BLOCK), 0, j->stream,
+                       (const Slot<u128>*)j->table, s0, s1, (const Spill*)j->spill, j->spcap, d, cap_out, j->ctr);
   bruno_status s = sync_counters(j);
+  const uint64_t got = j->hctr->n_out;
+  if (s == BRUNO_CAPACITY) s = BRUNO_BAD_INPUT; /* buffer smaller than the count */
+  if (s == BRUNO_OK && got) {
+    if (hipMemcpy(buf, d, got * sizeof(bruno_node), hipMemcpyDeviceToHost) != hipSuccess) {
+      (void)hipGetLastError();
+      s = BRUNO_INTERNAL;
+    }
+  }
+  if (s == BRUNO_BAD_INPUT) {
+    /* clear the list-full flag so the job stays usable */
+    j->hctr->err = 0;
+    (void)hipMemsetAsync(&j->ctr->err, 0, 4, j->stream);
+    (void)hipStreamSynchronize(j->stream);
+  }
   void* a = d;
-  dev_free(j, a, std::max<uint64_t>(1, nd) * sizeof(bruno_node));
-  *n = nd;
+  dev_free(j, a, std::max<uint64_t>(1, cap_out) * sizeof(bruno_node));
+  *n = got;
   return s;
 }
 
-bruno_status bruno_get_k2(bruno_job* j, uint64_t* kmers, uint8_t* biedges, uint32_t* counts, uint64_t* n) {
-  if (!valid_job(j) || !n) return BRUNO_BAD_INPUT;
+bruno_status bruno_batch_get_k2(bruno_job* j, uint32_t idx, uint64_t* kmers, uint8_t* biedges, uint32_t* counts,
+                                uint64_t* n) {
+  if (!valid_idx(j, idx) || !n) return BRUNO_BAD_INPUT;
   if (j->stage != ST_COUNTED) return BRUNO_STATE;
-  uint64_t nk = j->hctr->n_k2;
+  const uint64_t s0 = j->seg.empty() ? 0 : j->seg[idx], s1 = j->seg.empty() ? 0 : j->seg[idx + 1];
+  const bool one = j->n_jobs == 1;
   if (!kmers) {
-    *n = nk;
+    if (one) {
+      *n = j->hctr->n_k2;
+      return BRUNO_OK;
+    }
+    j->hctr->n_out = 0;
+    HIP_CHECK(hipMemcpyAsync(&j->ctr->n_out, &j->hctr->n_out, 8, hipMemcpyHostToDevice, j->stream));
+    if (j->words == 1)
+      hipLaunchKernelGGL((k_get_k2<uint64_t>), dim3(grid_for(s1 - s0, BLOCK, 1u << 15)), dim3(BLOCK), 0, j->stream,
+                         (const Slot<uint64_t>*)j->table, s0, s1, (const Spill*)j->spill, j->spcap, nullptr, nullptr,
+                         nullptr, 0, j->ctr);
+    else
+      hipLaunchKernelGGL((k_get_k2<u128>), dim3(grid_for(s1 - s0, BLOCK, 1u << 15)), dim3(BLOCK), 0, j->stream,
+                         (const Slot<u128>*)j->table, s0, s1, (const Spill*)j->spill, j->spcap, nullptr, nullptr,
+                         nullptr, 0, j->ctr);
+    TRY(sync_counters(j));
+    *n = j->hctr->n_out;
     return BRUNO_OK;
   }
-  if (*n < nk || !biedges || !counts) return BRUNO_BAD_INPUT;
+  if (!biedges || !counts) return BRUNO_BAD_INPUT;
+  const uint64_t m = std::max<uint64_t>(1, *n);
   uint64_t* dk = nullptr;
   uint8_t* de = nullptr;
   uint32_t* dc = nullptr;
-  uint64_t m = std::max<uint64_t>(1, nk);
   TRY(dev_alloc(j, (void**)&dk, m * 8 * j->words));
   TRY(dev_alloc(j, (void**)&de, m));
   TRY(dev_alloc(j, (void**)&dc, m * 4));
   j->hctr->n_out = 0;
   HIP_CHECK(hipMemcpyAsync(&j->ctr->n_out, &j->hctr->n_out, 8, hipMemcpyHostToDevice, j->stream));
   if (j->words == 1)
-    hipLaunchKernelGGL((k_get_k2<uint64_t>), dim3(grid_for(j->cap, BLOCK, 1u << 15)), dim3(BLOCK), 0, j->stream,
-                       (const Slot<uint64_t>*)j->table, j->cap, (const Spill*)j->spill, j->spcap, dk, de, dc, nk, j->ctr);
+    hipLaunchKernelGGL((k_get_k2<uint64_t>), dim3(grid_for(s1 - s0, BLOCK, 1u << 15)), dim3(BLOCK), 0, j->stream,
+                       (const Slot<uint64_t>*)j->table, s0, s1, (const Spill*)j->spill, j->spcap, dk, de, dc, *n,
+                       j->ctr);
   else
-    hipLaunchKernelGGL((k_get_k2<u128>), dim3(grid_for(j->cap, BLOCK, 1u << 15)), dim3(BLOCK), 0, j->stream,
-                       (const Slot<u128>*)j->table, j->cap, (const Spill*)j->spill, j->spcap, dk, de, dc, nk, j->ctr);
-  HIP_CHECK(hipMemcpyAsync(kmers, dk, nk * 8 * j->words, hipMemcpyDeviceToHost, j->stream));
-  HIP_CHECK(hipMemcpyAsync(biedges, de, nk, hipMemcpyDeviceToHost, j->stream));
-  HIP_CHECK(hipMemcpyAsync(counts, dc, nk * 4, hipMemcpyDeviceToHost, j->stream));
+    hipLaunchKernelGGL((k_get_k2<u128>), dim3(grid_for(s1 - s0, BLOCK, 1u << 15)), dim3(BLOCK), 0, j->stream,
+                       (const Slot<u128>*)j->table, s0, s1, (const Spill*)j->spill, j->spcap, dk, de, dc, *n, j->ctr);
   bruno_status s = sync_counters(j);
-  if (s == BRUNO_OK && j->hctr->n_out != nk) s = BRUNO_INTERNAL;
+  const uint64_t got = j->hctr->n_out;
+  if (s == BRUNO_CAPACITY) s = BRUNO_BAD_INPUT;
+  if (s == BRUNO_OK && got) {
+    if (hipMemcpy(kmers, dk, got * 8 * j->words, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(biedges, de, got, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(counts, dc, got * 4, hipMemcpyDeviceToHost) != hipSuccess) {
+      (void)hipGetLastError();
+      s = BRUNO_INTERNAL;
+    }
+  }
+  if (s == BRUNO_BAD_INPUT) {
+    j->hctr->err = 0;
+    (void)hipMemsetAsync(&j->ctr->err, 0, 4, j->stream);
+    (void)hipStreamSynchronize(j->stream);
+  }
   void* a = dk;
   dev_free(j, a, m * 8 * j->words);
   a = de;
   dev_free(j, a, m);
   a = dc;
   dev_free(j, a, m * 4);
-  *n = nk;
+  *n = got;
   return s;
 }
 
-bruno_status bruno_get_nodes(bruno_job* j, bruno_node* buf, uint64_t* n) {
-  if (!valid_job(j) || !n) return BRUNO_BAD_INPUT;
+bruno_status bruno_batch_get_nodes(bruno_job* j, uint32_t idx, bruno_node* buf, uint64_t* n) {
+  if (!valid_idx(j, idx) || !n) return BRUNO_BAD_INPUT;
   if (j->stage < ST_FILTERED) return BRUNO_STATE;
-  uint64_t np = j->n_present;
-  if (!buf) {
+  const uint64_t np = j->n_present;
+  if (j->n_jobs == 1) {
+    if (!buf) {
+      *n = np;
+      return BRUNO_OK;
+    }
+    if (*n < np) return BRUNO_BAD_INPUT;
+    bruno_node* d = nullptr;
+    TRY(dev_alloc(j, (void**)&d, std::max<uint64_t>(1, np) * sizeof(bruno_node)));
+    if (np) {
+      if (j->words == 1)
+        hipLaunchKernelGGL((k_get_nodes<uint64_t>), dim3(grid_for(np, BLOCK, 1u << 30)), dim3(BLOCK), 0, j->stream,
+                           (const Slot<uint64_t>*)j->table, j->present, np, d);
+      else
+        hipLaunchKernelGGL((k_get_nodes<u128>), dim3(grid_for(np, BLOCK, 1u << 30)), dim3(BLOCK), 0, j->stream,
+                           (const Slot<u128>*)j->table, j->present, np, d);
+      HIP_CHECK(hipMemcpyAsync(buf, d, np * sizeof(bruno_node), hipMemcpyDeviceToHost, j->stream));
+    }
+    HIP_CHECK(hipStreamSynchronize(j->stream));
+    void* a = d;
+    dev_free(j, a, std::max<uint64_t>(1, np) * sizeof(bruno_node));
     *n = np;
     return BRUNO_OK;
   }
-  if (*n < np) return BRUNO_BAD_INPUT;
+  const uint64_t s0 = j->seg[idx], s1 = j->seg[idx + 1];
+  const uint64_t cap_out = buf ? *n : 0;
   bruno_node* d = nullptr;
-  TRY(dev_alloc(j, (void**)&d, std::max<uint64_t>(1, np) * sizeof(bruno_node)));
+  if (buf) TRY(dev_alloc(j, (void**)&d, std::max<uint64_t>(1, cap_out) * sizeof(bruno_node)));
+  j->hctr->n_out = 0;
+  HIP_CHECK(hipMemcpyAsync(&j->ctr->n_out, &j->hctr->n_out, 8, hipMemcpyHostToDevice, j->stream));
   if (np) {
     if (j->words == 1)
-      hipLaunchKernelGGL((k_get_nodes<uint64_t>), dim3(grid_for(np, BLOCK, 1u << 30)), dim3(BLOCK), 0, j->stream,
-                         (const Slot<uint64_t>*)j->table, j->present, np, d);
+      hipLaunchKernelGGL((k_get_nodes_range<uint64_t>), dim3(grid_for(np)), dim3(BLOCK), 0, j->stream,
+                         (const Slot<uint64_t>*)j->table, j->present, np, s0, s1, d, cap_out, j->ctr);
     else
-      hipLaunchKernelGGL((k_get_nodes<u128>), dim3(grid_for(np, BLOCK, 1u << 30)), dim3(BLOCK), 0, j->stream,
-                         (const Slot<u128>*)j->table, j->present, np, d);
-    HIP_CHECK(hipMemcpyAsync(buf, d, np * sizeof(bruno_node), hipMemcpyDeviceToHost, j->stream));
+      hipLaunchKernelGGL((k_get_nodes_range<u128>), dim3(grid_for(np)), dim3(BLOCK), 0, j->stream,
+                         (const Slot<u128>*)j->table, j->present, np, s0, s1, d, cap_out, j->ctr);
   }
-  HIP_CHECK(hipStreamSynchronize(j->stream));
-  void* a = d;
-  dev_free(j, a, std::max<uint64_t>(1, np) * sizeof(bruno_node));
-  *n = np;
-  return BRUNO_OK;
+  bruno_status s = sync_counters(j);
+  const uint64_t got = j->hctr->n_out;
+  if (s == BRUNO_CAPACITY) s = BRUNO_BAD_INPUT;
+  if (s == BRUNO_OK && buf && got) {
+    if (hipMemcpy(buf, d, got * sizeof(bruno_node), hipMemcpyDeviceToHost) != hipSuccess) {
+      (void)hipGetLastError();
+      s = BRUNO_INTERNAL;
+    }
+  }
+  if (s == BRUNO_BAD_INPUT) {
+    j->hctr->err = 0;
+    (void)hipMemsetAsync(&j->ctr->err, 0, 4, j->stream);
+    (void)hipStreamSynchronize(j->stream);
+  }
+  if (d) {
+    void* a = d;
+    dev_free(j, a, std::max<uint64_t>(1, cap_out) * sizeof(bruno_node));
+  }
+  *n = got;
+  return s;
 }
 
-bruno_status bruno_get_branches(bruno_job* j, uint64_t* kmers, uint64_t* n) {
-  if (!valid_job(j) || !n) return BRUNO_BAD_INPUT;
+bruno_status bruno_batch_get_branches(bruno_job* j, uint32_t idx, uint64_t* kmers, uint64_t* n) {
+  if (!valid_idx(j, idx) || !n) return BRUNO_BAD_INPUT;
   if (j->stage != ST_COMPACTED) return BRUNO_STATE;
-  uint64_t nb = j->n_branch;
+  const uint64_t b0 = j->branch_start[idx], nb = j->branch_start[idx + 1] - b0;
   if (!kmers) {
     *n = nb;
     return BRUNO_OK;
   }
   if (*n < nb) return BRUNO_BAD_INPUT;
-  if (nb) {
-    if (j->words == 1) {
-      HIP_CHECK(hipMemcpyAsync(kmers, j->bsorted, nb * 8, hipMemcpyDeviceToHost, j->stream));
-    } else {
-      HIP_CHECK(hipMemcpyAsync(kmers, j->bsorted, nb * 16, hipMemcpyDeviceToHost, j->stream));
-    }
-  }
+  const uint64_t w = 8 * (uint64_t)j->words;
+  if (nb) HIP_CHECK(hipMemcpyAsync(kmers, (const char*)j->bsorted + b0 * w, nb * w, hipMemcpyDeviceToHost, j->stream));
   HIP_CHECK(hipStreamSynchronize(j->stream));
   *n = nb;
   return BRUNO_OK;
 }
 
-bruno_status bruno_get_chain_nodes(bruno_job* j, bruno_chain_node* buf, uint64_t* n) {
-  if (!valid_job(j) || !n) return BRUNO_BAD_INPUT;
+bruno_status bruno_batch_get_chain_nodes(bruno_job* j, uint32_t idx, bruno_chain_node* buf, uint64_t* n) {
+  if (!valid_idx(j, idx) || !n) return BRUNO_BAD_INPUT;
   if (j->stage != ST_COMPACTED) return BRUNO_STATE;
-  uint64_t m = j->n_out;
+  const uint64_t c0 = j->chain_start[idx], m = j->chain_start[idx + 1] - c0;
   if (!buf) {
     *n = m;
     return BRUNO_OK;
@@ -1788,10 +2287,11 @@ bruno_status bruno_get_chain_nodes(bruno_job* j, bruno_chain_node* buf, uint64_t
     TRY(dev_alloc(j, (void**)&d, m * sizeof(bruno_chain_node)));
     if (j->words == 1)
       hipLaunchKernelGGL((k_get_chain_nodes<uint64_t>), dim3(grid_for(m, BLOCK, 1u << 30)), dim3(BLOCK), 0, j->stream,
-                         j->perm, (const uint64_t*)j->out_k, (const uint64_t*)j->out_rep, j->dist_sorted, m, d);
+                         j->perm + c0, (const uint64_t*)j->out_k, (const uint64_t*)j->out_rep, j->dist_sorted + c0, m,
+                         d);
     else
       hipLaunchKernelGGL((k_get_chain_nodes<u128>), dim3(grid_for(m, BLOCK, 1u << 30)), dim3(BLOCK), 0, j->stream,
-                         j->perm, (const u128*)j->out_k, (const u128*)j->out_rep, j->dist_sorted, m, d);
+                         j->perm + c0, (const u128*)j->out_k, (const u128*)j->out_rep, j->dist_sorted + c0, m, d);
     HIP_CHECK(hipMemcpyAsync(buf, d, m * sizeof(bruno_chain_node), hipMemcpyDeviceToHost, j->stream));
     HIP_CHECK(hipStreamSynchronize(j->stream));
     void* a = d;
@@ -1813,19 +2313,55 @@ static bruno_status get_text(bruno_job* j, const char* dptr, uint64_t len, char*
   return BRUNO_OK;
 }
 
-bruno_status bruno_get_unitigs(bruno_job* j, char* buf, uint64_t* n) {
-  if (!valid_job(j) || !n) return BRUNO_BAD_INPUT;
+bruno_status bruno_batch_get_unitigs(bruno_job* j, uint32_t idx, char* buf, uint64_t* n) {
+  if (!valid_idx(j, idx) || !n) return BRUNO_BAD_INPUT;
   if (j->stage != ST_COMPACTED) return BRUNO_STATE;
-  /* the file drops the leading newline of the first record
+  /* each job's file drops the leading newline of its first record
    * (compact_dbg_io.hpp:351-355) */
-  uint64_t len = j->fasta_len ? j->fasta_len - 1 : 0;
-  return get_text(j, j->fasta + (j->fasta_len ? 1 : 0), len, buf, n);
+  const uint64_t b0 = j->fasta_off[idx], len = j->fasta_off[idx + 1] - b0;
+  return get_text(j, j->fasta + b0 + (len ? 1 : 0), len ? len - 1 : 0, buf, n);
 }
 
-bruno_status bruno_get_branch_fasta(bruno_job* j, char* buf, uint64_t* n) {
-  if (!valid_job(j) || !n) return BRUNO_BAD_INPUT;
+bruno_status bruno_batch_get_branch_fasta(bruno_job* j, uint32_t idx, char* buf, uint64_t* n) {
+  if (!valid_idx(j, idx) || !n) return BRUNO_BAD_INPUT;
   if (j->stage != ST_COMPACTED) return BRUNO_STATE;
-  return get_text(j, j->bfasta, j->bfasta_len, buf, n);
+  const uint64_t rec = 2 * (uint64_t)j->K + 3;
+  const uint64_t b0 = j->branch_start[idx] * rec, len = j->branch_start[idx + 1] * rec - b0;
+  return get_text(j, j->bfasta + b0, len, buf, n);
+}
+
+/* whole-job getters: the one-job case of the per-job views */
+#define ONE_JOB(j) \
+  if (!valid_job(j)) return BRUNO_BAD_INPUT; \
+  if ((j)->n_jobs != 1) return BRUNO_BAD_INPUT
+
+bruno_status bruno_get_counts(bruno_job* j, bruno_node* buf, uint64_t* n) {
+  ONE_JOB(j);
+  return bruno_batch_get_counts(j, 0, buf, n);
+}
+bruno_status bruno_get_k2(bruno_job* j, uint64_t* kmers, uint8_t* biedges, uint32_t* counts, uint64_t* n) {
+  ONE_JOB(j);
+  return bruno_batch_get_k2(j, 0, kmers, biedges, counts, n);
+}
+bruno_status bruno_get_nodes(bruno_job* j, bruno_node* buf, uint64_t* n) {
+  ONE_JOB(j);
+  return bruno_batch_get_nodes(j, 0, buf, n);
+}
+bruno_status bruno_get_branches(bruno_job* j, uint64_t* kmers, uint64_t* n) {
+  ONE_JOB(j);
+  return bruno_batch_get_branches(j, 0, kmers, n);
+}
+bruno_status bruno_get_chain_nodes(bruno_job* j, bruno_chain_node* buf, uint64_t* n) {
+  ONE_JOB(j);
+  return bruno_batch_get_chain_nodes(j, 0, buf, n);
+}
+bruno_status bruno_get_unitigs(bruno_job* j, char* buf, uint64_t* n) {
+  ONE_JOB(j);
+  return bruno_batch_get_unitigs(j, 0, buf, n);
+}
+bruno_status bruno_get_branch_fasta(bruno_job* j, char* buf, uint64_t* n) {
+  ONE_JOB(j);
+  return bruno_batch_get_branch_fasta(j, 0, buf, n);
 }
 
 bruno_status bruno_parse_records(int k, const char* d_bases, const uint64_t* d_offsets, uint64_t n_reads,

@@ -178,6 +178,18 @@ template <> __device__ __forceinline__ u128 window_kmer<u128>(int l, int K, uint
   return t >> (128 - 2 * K);
 }
 
+/* Index j of the segment holding position x: the largest j < n with
+ * first[j] <= x (first[0] <= x < first[n]; empty segments are skipped). */
+__device__ __forceinline__ uint32_t job_of(const uint64_t* first, uint32_t n, uint64_t x) {
+  uint32_t lo = 0, hi = n; /* first[lo] <= x < first[hi] */
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (first[mid] <= x) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
 struct ParseArgs {
   const uint8_t* bases;
   const uint64_t* offsets;
@@ -188,6 +200,13 @@ struct ParseArgs {
   uint64_t cap;
   Spill* spill;
   uint64_t spcap;
+  /* batches: job j owns reads [job_first[j], job_first[j+1]) of this call
+   * and table slots [seg[j], seg[j+1]); n_jobs == 1 uses the whole table */
+  uint32_t n_jobs;
+  const uint64_t* job_first;
+  const uint64_t* seg;
+  unsigned long long* job_inst; /* per-job k-mer instances (n_jobs > 1) */
+  unsigned long long* job_drop; /* per-job dropped reads (n_jobs > 1) */
   /* record mode */
   const uint64_t* rec_base; /* per-read first record index */
   uint64_t* out_kmers;
@@ -217,6 +236,7 @@ __global__ __launch_bounds__(BLOCK) void k_parse(ParseArgs a) {
     if (__any(bad)) {
       if (lane == 0) ++my_drop;
       if (MODE == 0 && lane == 0) a.per_read[r] = 0;
+      if (MODE == 2 && a.n_jobs > 1 && lane == 0) atomicAdd(&a.job_drop[job_of(a.job_first, a.n_jobs, r)], 1ULL);
       continue;
     }
     /* parser window is k+2 (debruijn_biedge_loader.hpp:402, :464-471) */
@@ -230,6 +250,13 @@ __global__ __launch_bounds__(BLOCK) void k_parse(ParseArgs a) {
       continue;
     }
     uint64_t rbase = (MODE == 1) ? a.rec_base[r] : 0;
+    uint64_t tbase = 0, tcap = a.cap;
+    if (MODE == 2 && a.n_jobs > 1) {
+      const uint32_t j = job_of(a.job_first, a.n_jobs, r);
+      tbase = a.seg[j];
+      tcap = a.seg[j + 1] - tbase;
+      if (lane == 0) atomicAdd(&a.job_inst[j], (unsigned long long)nk);
+    }
     /* sliding window of 4 packed words covering bases [p0, p0 + 128) */
     uint64_t A, B, C, D, prevB = 0;
     {
@@ -262,10 +289,10 @@ __global__ __launch_bounds__(BLOCK) void k_parse(ParseArgs a) {
           a.out_biedges[rbase + i] = e;
         } else {
           int claimed = 0;
-          uint64_t slot = SlotOps<KT>::find_or_insert((Slot<KT>*)a.table, a.cap, km, &a.ctr->err, &claimed);
+          uint64_t slot = SlotOps<KT>::find_or_insert((Slot<KT>*)a.table + tbase, tcap, km, &a.ctr->err, &claimed);
           if (slot != ~0ULL) {
             my_new += claimed;
-            my_k2 += add_k2<KT>((Slot<KT>*)a.table, slot, e, a.spill, a.spcap, &a.ctr->err);
+            my_k2 += add_k2<KT>((Slot<KT>*)a.table, tbase + slot, e, a.spill, a.spcap, &a.ctr->err);
           }
         }
         ++my_inst;

@@ -113,7 +113,8 @@ template <> struct SlotOps<uint64_t> {
    * ~0 on a full table; *claimed = 1 when this call inserted the key. */
   static __device__ uint64_t find_or_insert(S* T, uint64_t cap, uint64_t v, uint32_t* err, int* claimed) {
     uint64_t s = home_slot(kmer_hash(v), cap);
-    for (uint32_t probe = 0; probe < MAX_PROBE; ++probe) {
+    const uint64_t lim = cap < MAX_PROBE ? cap : MAX_PROBE;
+    for (uint64_t probe = 0; probe < lim; ++probe) {
       S* p = T + s;
       uint64_t k0 = ld_relaxed(&p->key);
       if (k0 == EMPTY64) {
@@ -134,7 +135,8 @@ template <> struct SlotOps<uint64_t> {
    * require_present: only nodes that survived the filter. */
   static __device__ uint64_t find(const S* T, uint64_t cap, uint64_t v, bool require_present) {
     uint64_t s = home_slot(kmer_hash(v), cap);
-    for (uint32_t probe = 0; probe < MAX_PROBE; ++probe) {
+    const uint64_t lim = cap < MAX_PROBE ? cap : MAX_PROBE;
+    for (uint64_t probe = 0; probe < lim; ++probe) {
       uint64_t k0 = T[s].key;
       if (k0 == EMPTY64) return ~0ULL;
       if ((k0 & KEY_BITS) == v) {
@@ -161,7 +163,8 @@ template <> struct SlotOps<u128> {
   static __device__ uint64_t find_or_insert(S* T, uint64_t cap, u128 v, uint32_t* err, int* claimed) {
     uint64_t lo = (uint64_t)v, hi = (uint64_t)(v >> 64);
     uint64_t s = home_slot(kmer_hash(v), cap);
-    for (uint32_t probe = 0; probe < MAX_PROBE; ++probe) {
+    const uint64_t lim = cap < MAX_PROBE ? cap : MAX_PROBE;
+    for (uint64_t probe = 0; probe < lim; ++probe) {
       S* p = T + s;
       uint64_t m = ld_relaxed(&p->meta);
       if (m == EMPTY64) {
@@ -197,7 +200,8 @@ template <> struct SlotOps<u128> {
   static __device__ uint64_t find(const S* T, uint64_t cap, u128 v, bool require_present) {
     uint64_t lo = (uint64_t)v, hi = (uint64_t)(v >> 64);
     uint64_t s = home_slot(kmer_hash(v), cap);
-    for (uint32_t probe = 0; probe < MAX_PROBE; ++probe) {
+    const uint64_t lim = cap < MAX_PROBE ? cap : MAX_PROBE;
+    for (uint64_t probe = 0; probe < lim; ++probe) {
       uint64_t m = T[s].meta;
       if (m == EMPTY64) return ~0ULL;
       if (T[s].lo == lo && T[s].hi == hi) {
@@ -238,7 +242,8 @@ __device__ int add_k2(Slot<KT>* T, uint64_t s, uint8_t e, Spill* sp, uint64_t sp
   if (!(ld_relaxed(fw) & F_SPILL)) or64(fw, F_SPILL);
   const uint64_t key = (s << 9) | code;
   uint64_t h = home_slot(fmix64(key), spcap);
-  for (uint32_t probe = 0; probe < MAX_PROBE; ++probe) {
+  const uint64_t lim = spcap < MAX_PROBE ? spcap : MAX_PROBE;
+  for (uint64_t probe = 0; probe < lim; ++probe) {
     Spill* q = sp + h;
     uint64_t k0 = ld_relaxed(&q->key);
     if (k0 == 0) {
@@ -261,7 +266,8 @@ __device__ int add_k2(Slot<KT>* T, uint64_t s, uint8_t e, Spill* sp, uint64_t sp
 
 __device__ __forceinline__ uint64_t spill_find(const Spill* sp, uint64_t spcap, uint64_t key) {
   uint64_t h = home_slot(fmix64(key), spcap);
-  for (uint32_t probe = 0; probe < MAX_PROBE; ++probe) {
+  const uint64_t lim = spcap < MAX_PROBE ? spcap : MAX_PROBE;
+  for (uint64_t probe = 0; probe < lim; ++probe) {
     uint64_t k0 = sp[h].key;
     if (k0 == 0) return 0;
     if (k0 == key) return sp[h].cnt;

@@ -29,7 +29,60 @@ def _ptr(a):
     return ctypes.c_void_p(a.ctypes.data)
 
 
-class GraphJob:
+class _Views:
+    """Getters of one job's results; `_call(name, *args)` binds them to a
+    whole job or to one job of a batch."""
+    words = 1
+
+    def _two_call(self, name, dtype=None, text=False):
+        n = ctypes.c_uint64(0)
+        self._call(name, None, ctypes.byref(n))
+        if text:
+            buf = ctypes.create_string_buffer(max(1, n.value))
+            self._call(name, buf, ctypes.byref(n))
+            return buf.raw[:n.value]
+        out = np.zeros(n.value, dtype)
+        if n.value:
+            self._call(name, _ptr(out), ctypes.byref(n))
+        return out[:n.value]
+
+    def counts(self):
+        """P1 table (before filter)."""
+        return self._two_call("counts", NODE_DTYPE)
+
+    def nodes(self):
+        """Filtered graph (after filter)."""
+        return self._two_call("nodes", NODE_DTYPE)
+
+    def chain_nodes(self):
+        return self._two_call("chain_nodes", CHAIN_DTYPE)
+
+    def unitigs(self):
+        return self._two_call("unitigs", text=True)
+
+    def branch_fasta(self):
+        return self._two_call("branch_fasta", text=True)
+
+    def branches(self):
+        n = ctypes.c_uint64(0)
+        self._call("branches", None, ctypes.byref(n))
+        out = np.zeros(n.value * self.words, np.uint64)
+        if n.value:
+            self._call("branches", _ptr(out), ctypes.byref(n))
+        return out.reshape(-1, self.words)
+
+    def k2(self):
+        n = ctypes.c_uint64(0)
+        self._call("k2", None, None, None, ctypes.byref(n))
+        km = np.zeros(n.value * self.words, np.uint64)
+        e = np.zeros(n.value, np.uint8)
+        c = np.zeros(n.value, np.uint32)
+        if n.value:
+            self._call("k2", _ptr(km), _ptr(e), _ptr(c), ctypes.byref(n))
+        return km.reshape(-1, self.words), e, c
+
+
+class GraphJob(_Views):
     def __init__(self, k, thresholds=None, capacity=0, load_factor=0.0, stream=None):
         d = _lib.JobDesc()
         d.k = int(k)
@@ -93,53 +146,9 @@ class GraphJob:
         check(lib().bruno_run_pipeline(self.h, int(variant)), "bruno_run_pipeline")
 
     # ---- getters -----------------------------------------------------
-    def _two_call(self, fn, dtype=None, text=False):
-        n = ctypes.c_uint64(0)
-        check(fn(self.h, None, ctypes.byref(n)), fn.__name__)
-        if text:
-            buf = ctypes.create_string_buffer(max(1, n.value))
-            check(fn(self.h, buf, ctypes.byref(n)), fn.__name__)
-            return buf.raw[:n.value]
-        out = np.zeros(n.value, dtype)
-        if n.value:
-            check(fn(self.h, _ptr(out), ctypes.byref(n)), fn.__name__)
-        return out
-
-    def counts(self):
-        """P1 table (before filter)."""
-        return self._two_call(lib().bruno_get_counts, NODE_DTYPE)
-
-    def nodes(self):
-        """Filtered graph (after filter)."""
-        return self._two_call(lib().bruno_get_nodes, NODE_DTYPE)
-
-    def chain_nodes(self):
-        return self._two_call(lib().bruno_get_chain_nodes, CHAIN_DTYPE)
-
-    def unitigs(self):
-        return self._two_call(lib().bruno_get_unitigs, text=True)
-
-    def branch_fasta(self):
-        return self._two_call(lib().bruno_get_branch_fasta, text=True)
-
-    def branches(self):
-        n = ctypes.c_uint64(0)
-        check(lib().bruno_get_branches(self.h, None, ctypes.byref(n)), "bruno_get_branches")
-        out = np.zeros(n.value * self.words, np.uint64)
-        if n.value:
-            check(lib().bruno_get_branches(self.h, _ptr(out), ctypes.byref(n)), "bruno_get_branches")
-        return out.reshape(-1, self.words)
-
-    def k2(self):
-        n = ctypes.c_uint64(0)
-        L = lib()
-        check(L.bruno_get_k2(self.h, None, None, None, ctypes.byref(n)), "bruno_get_k2")
-        km = np.zeros(n.value * self.words, np.uint64)
-        e = np.zeros(n.value, np.uint8)
-        c = np.zeros(n.value, np.uint32)
-        if n.value:
-            check(L.bruno_get_k2(self.h, _ptr(km), _ptr(e), _ptr(c), ctypes.byref(n)), "bruno_get_k2")
-        return km.reshape(-1, self.words), e, c
+    def _call(self, name, *args):
+        fn = getattr(lib(), "bruno_get_" + name)
+        return check(fn(self.h, *args), "bruno_get_" + name)
 
     def stats(self):
         s = _lib.Stats()
@@ -150,6 +159,74 @@ class GraphJob:
         p = _lib.PhaseMs()
         check(lib().bruno_get_phase_ms(self.h, ctypes.byref(p)), "bruno_get_phase_ms")
         return {f: float(getattr(p, f)) for f, _ in _lib.PhaseMs._fields_}
+
+
+class BatchJobView(_Views):
+    """Results of job `idx` of a GraphBatch."""
+
+    def __init__(self, batch, idx):
+        self.batch = batch
+        self.idx = int(idx)
+        self.k = batch.k
+        self.words = batch.words
+
+    def _call(self, name, *args):
+        fn = getattr(lib(), "bruno_batch_get_" + name)
+        return check(fn(self.batch.h, self.idx, *args), "bruno_batch_get_" + name)
+
+    def stats(self):
+        s = _lib.Stats()
+        check(lib().bruno_batch_get_stats(self.batch.h, self.idx, ctypes.byref(s)), "bruno_batch_get_stats")
+        return {f: int(getattr(s, f)) for f in _lib.STAT_FIELDS}
+
+
+class GraphBatch(GraphJob):
+    """Many independent jobs (same k and thresholds) through one set of
+    launches; job i owns capacities[i] hash slots.  Results per job via
+    job(i), bit-identical to running each job alone."""
+
+    def __init__(self, k, thresholds, capacities, stream=None):
+        d = _lib.JobDesc()
+        d.k = int(k)
+        d.thresholding = 0 if thresholds is None else 1
+        th = thresholds if thresholds is not None else [DEFAULT_THRESH] * 6
+        for i in range(6):
+            d.thresholds[i] = int(th[i])
+        d.stream = stream
+        caps = np.ascontiguousarray(capacities, np.uint64)
+        h = ctypes.c_void_p()
+        check(lib().bruno_batch_create(ctypes.byref(d), int(caps.size), _ptr(caps), ctypes.byref(h)),
+              "bruno_batch_create")
+        self.h = h
+        self.k = int(k)
+        self.words = 1 if k <= 31 else 2
+        self.n_jobs = int(caps.size)
+
+    def parse_count_device(self, bases_ptr, offsets_ptr, n_reads, job_first_read):
+        f = np.ascontiguousarray(job_first_read, np.uint64)
+        if f.size != self.n_jobs + 1:
+            raise ValueError("job_first_read needs n_jobs + 1 entries")
+        check(lib().bruno_batch_parse_count(self.h, ctypes.c_void_p(bases_ptr), ctypes.c_void_p(offsets_ptr),
+                                            int(n_reads), _ptr(f)), "bruno_batch_parse_count")
+
+    def parse_count(self, bases, offsets, job_first_read):
+        """Host arrays: copied to device memory, then parsed as one batch."""
+        from .device import DeviceBuffer
+        b = np.frombuffer(bases, np.uint8) if isinstance(bases, (bytes, bytearray)) else np.ascontiguousarray(bases, np.uint8)
+        o = np.ascontiguousarray(offsets, np.uint64)
+        n = o.size - 1
+        db = DeviceBuffer.from_numpy(b[int(o[0]):int(o[-1])])
+        do = DeviceBuffer.from_numpy(o - o[0])
+        try:
+            self.parse_count_device(db.ptr, do.ptr, n, job_first_read)
+        finally:
+            db.free()
+            do.free()
+
+    def job(self, idx):
+        if not 0 <= idx < self.n_jobs:
+            raise IndexError(idx)
+        return BatchJobView(self, idx)
 
 
 @dataclass

@@ -177,6 +177,46 @@ bruno_status bruno_get_chain_nodes(bruno_job* job, bruno_chain_node* buf, uint64
 bruno_status bruno_get_unitigs(bruno_job* job, char* buf, uint64_t* n);
 bruno_status bruno_get_branch_fasta(bruno_job* job, char* buf, uint64_t* n);
 
+/* ---- Batches of independent jobs ---------------------------------------
+ * A batch runs n_jobs independent assembly jobs (same k and thresholds,
+ * different read sets of any size and read length) through every phase
+ * with one set of launches: the jobs share one HBM table in which job i
+ * owns capacities[i] slots, so k-mers of different jobs never meet, and
+ * each job's results are bit-identical to running it alone (including its
+ * own list-ranking round count).  The phase calls are the ones above
+ * (bruno_filter, bruno_build_chains, bruno_list_rank, bruno_compact,
+ * bruno_run_pipeline, bruno_job_reset, bruno_job_destroy); the whole-job
+ * getters return BRUNO_BAD_INPUT on a batch of more than one job, and the
+ * per-job getters below take the job index.  The reference runs one
+ * assembly per MPI job (compact_dbg_freq_construct_clean.cpp:431-840); it
+ * has no batched counterpart. */
+#define BRUNO_MAX_BATCH_JOBS 65536u
+
+bruno_status bruno_batch_create(const bruno_job_desc* desc, uint32_t n_jobs, const uint64_t* capacities,
+                                bruno_job** out);
+/* d_bases / d_offsets as for bruno_parse_count; job_first_read is a HOST
+ * array of n_jobs + 1 read indices: job i owns reads
+ * [job_first_read[i], job_first_read[i+1]) of this call (first entry 0,
+ * last n_reads, non-decreasing).  May be called repeatedly. */
+bruno_status bruno_batch_parse_count(bruno_job* job, const char* d_bases, const uint64_t* d_offsets,
+                                     uint64_t n_reads, const uint64_t* job_first_read);
+/* Per-job statistics: n_reads, n_instances, n_reads_dropped, capacity, and
+ * after the phases that produce them iterations, n_cycle_nodes, n_isolated,
+ * n_branches, n_chain_nodes, n_nodes, n_unitig_nodes, n_unitigs.  The other
+ * fields are batch-wide only (bruno_get_stats) and read 0 here. */
+bruno_status bruno_batch_get_stats(bruno_job* job, uint32_t idx, bruno_stats* out);
+/* Per-job forms of the getters above, same contents and order.  For
+ * counts, k2 and nodes a NULL buffer makes the call count job idx's entries
+ * with a pass over its table segment. */
+bruno_status bruno_batch_get_counts(bruno_job* job, uint32_t idx, bruno_node* buf, uint64_t* n);
+bruno_status bruno_batch_get_k2(bruno_job* job, uint32_t idx, uint64_t* kmers, uint8_t* biedges, uint32_t* counts,
+                                uint64_t* n);
+bruno_status bruno_batch_get_nodes(bruno_job* job, uint32_t idx, bruno_node* buf, uint64_t* n);
+bruno_status bruno_batch_get_branches(bruno_job* job, uint32_t idx, uint64_t* kmers, uint64_t* n);
+bruno_status bruno_batch_get_chain_nodes(bruno_job* job, uint32_t idx, bruno_chain_node* buf, uint64_t* n);
+bruno_status bruno_batch_get_unitigs(bruno_job* job, uint32_t idx, char* buf, uint64_t* n);
+bruno_status bruno_batch_get_branch_fasta(bruno_job* job, uint32_t idx, char* buf, uint64_t* n);
+
 /* Debug entry for the P0 parity object: parse only, write each canonical
  * (k-mer, biedge) record in read order (device pointers; out_kmers holds
  * words_per_kmer words per record).  *n_out receives the record count;

@@ -51,6 +51,12 @@ def oracle_node_table(arr):
 def compare_all(k, th, bases, offs, variant=1, nthreads=1, stages=True, capacity=0):
     g = run_gpu(k, th, bases, offs, variant, capacity=capacity, stages=stages)
     o = run_oracle(k, th, bases, offs, variant, nthreads)
+    compare_results(k, g, o, variant, stages)
+    return g, o
+
+
+def compare_results(k, g, o, variant=1, stages=True):
+    """Every parity object of one job's GPU results `g` against the oracle."""
     st = g["stats"]
     if variant == 1:
         # list_rank_min_update only sends to ends that still point to a chain
@@ -89,4 +95,3 @@ def compare_all(k, th, bases, offs, variant=1, nthreads=1, stages=True, capacity
     assert g["rounds"] == o.stat("iterations")
     assert st["n_cycle_nodes"] == o.stat("n_cycle_nodes")
     assert st["n_isolated"] == o.stat("n_isolated")
-    return g, o

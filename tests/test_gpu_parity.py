@@ -244,3 +244,15 @@ def test_spilled_k2_entries_under_contention():
     for k in (31, 63):
         for th in (None, thresholds_L(2)):
             compare_all(k, th, b, off)
+
+
+@pytest.mark.parametrize("case", CASES[:6])
+@pytest.mark.parametrize("variant", [0, 1])
+def test_kmer_only_threshold(case, variant):
+    """-L 3 0 0 (k-mer threshold only) keeps edges into removed k-mers, so
+    chain ends can point at nodes that are not in the graph: the list
+    ranking falls back to round-by-round emulation."""
+    seed, G, n, L, err, k = case
+    _, reads = random_reads(seed, G, n, L, err)
+    b, off = pack(reads)
+    compare_all(k, [3, 65536, 0, 65536, 0, 65536], b, off, variant=variant)
