@@ -6,6 +6,11 @@ compact to the unitig / branch FASTA bytes in device memory.  The default
 workload is 100 Mbp with 2% repeats (50 families, 1% divergence), 33.3 M
 150 bp reads at 50x with 1% substitutions, k=31, -T -L 3.
 
+--config config4 runs the batched workload instead: 256 independent jobs of
+uneven size and read length, packed by estimated HBM footprint into launch
+groups, each group one batch (one set of launches for all its jobs), timed
+against the same jobs run one job per launch.
+
 Prints one JSON line with the metric, a roofline object for the count kernel
 (the dominant kernel) and a CPU baseline: the oracle (the C++ restatement of
 bruno's algorithm, hash-partitioned over host threads) timed on a bounded
@@ -21,6 +26,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+POOL_FRACTION = 0.9
 
 
 def count_bytes(n_bases, n_inst):
@@ -29,16 +35,11 @@ def count_bytes(n_bases, n_inst):
     return n_bases * 1 + n_inst * 2 * 64
 
 
-def capacity_for(synth, k):
+def capacity_for(synth, k, load_factor=0.5):
     """Hash slots for a synthetic workload: genome k-mers plus the novel
-    k-mers each substitution / indel creates, at load factor ~0.5."""
-    L = synth.read_len
-    per_read_kmers = max(0, L - k + 1)
-    err = synth.sub_rate + 2 * synth.indel_rate
-    cover = min(k, L) * per_read_kmers / max(1, L)
-    novel = synth.n_reads * L * err * cover
-    distinct = synth.genome_len + novel
-    return int(distinct / 0.5) + 4096
+    k-mers each substitution / indel creates, at the given load factor."""
+    from bruno_amd.planner import capacity
+    return capacity(synth, k, load_factor)
 
 
 def cpu_baseline(cfg, seconds_target=15.0, threads=None):
@@ -73,6 +74,11 @@ def main():
     ap.add_argument("--variant", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--load-factor", type=float, default=0.5)
+    ap.add_argument("--budget-gb", type=float, default=0.0,
+                    help="HBM budget per launch group for config4 (0 = 80%% of free HBM)")
+    ap.add_argument("--n-jobs", type=int, default=256, help="config4: number of jobs")
+    ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the timed steps, copy 4 GiB device-to-device once (a plain copy kernel of "
                          "known size, to scale FETCH_SIZE / WRITE_SIZE counters)")
@@ -85,9 +91,15 @@ def main():
     import numpy as np
     import bruno_amd
     from bruno_amd.configs import CONFIGS
-    from bruno_amd.device import DeviceBuffer, synchronize
+    from bruno_amd.device import DeviceBuffer, mem_info, pool_reserve, synchronize
     from bruno_amd.pipeline import synth_reads_device
 
+    # one pool chunk for everything the run allocates: no HIP allocator
+    # calls inside the timed steps
+    free0, _ = mem_info()
+    pool_reserve(int(POOL_FRACTION * free0))
+    if args.config == "config4":
+        return run_batched(args, free0)
     cfg = CONFIGS[args.config](scale=args.scale)
     s = cfg["synth"]
     k = cfg["k"]
@@ -96,7 +108,7 @@ def main():
     synth_reads_device(s, bases.ptr, offs.ptr)
     synchronize()
 
-    cap = capacity_for(s, k)
+    cap = capacity_for(s, k, args.load_factor)
     job = bruno_amd.GraphJob(k, cfg["thresholds"], capacity=cap)
     times, phases = [], []
     stats = None
@@ -139,11 +151,13 @@ def main():
                            s.n_families, 100 * s.repeat_fraction, k),
             "k": k, "n_reads": s.n_reads, "read_len": s.read_len, "genome_bp": s.genome_len,
             "k_mer_instances": n_inst, "list_rank": "min_update" if args.variant else "list_rank",
+            "load_factor": args.load_factor,
         },
         "latency_ms": {"p50": float(np.percentile(times, 50) * 1e3), "p99": float(np.percentile(times, 99) * 1e3)},
         "phases_ms": ph,
         "graph": {kk: stats[kk] for kk in ("n_distinct", "n_k2", "n_overflow", "n_nodes", "n_branches", "n_chain_nodes",
-                                           "n_unitigs", "n_cycle_nodes", "iterations", "capacity", "device_bytes")},
+                                           "n_unitigs", "n_cycle_nodes", "iterations", "capacity", "device_bytes",
+                                           "peak_device_bytes")},
         "roofline": {
             "kernel": "k_parse (parse + k2 count)",
             "bound": "hbm",
@@ -171,6 +185,176 @@ def main():
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))
+
+
+def run_batched(args, free):
+    """The config4 workload: batches vs one job per launch."""
+    import numpy as np
+    import bruno_amd
+    from bruno_amd.configs import config4
+    from bruno_amd.device import DeviceBuffer, synchronize
+    from bruno_amd.pipeline import synth_batch_device, synth_reads_device
+    from bruno_amd.planner import capacity, plan_groups
+
+    cfg = config4(scale=args.scale, n_jobs=args.n_jobs)
+    k, th, jobs = cfg["k"], cfg["thresholds"], cfg["jobs"]
+    lf = args.load_factor
+    budget = args.budget_gb * 1e9 if args.budget_gb > 0 else 0.80 * free
+    groups = plan_groups(jobs, k, budget, lf)
+    log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
+    log("config4: %d jobs, %d launch groups, budget %.1f GB" % (len(jobs), len(groups), budget / 1e9))
+
+    def run_group(idx):
+        cfgs = [jobs[i] for i in idx]
+        nb, nr = sum(c.n_bases for c in cfgs), sum(c.n_reads for c in cfgs)
+        bases, offs = DeviceBuffer(nb), DeviceBuffer((nr + 1) * 8)
+        first = synth_batch_device(cfgs, bases.ptr, offs.ptr)
+        synchronize()
+        t0 = time.perf_counter()
+        bt = bruno_amd.GraphBatch(k, th, [capacity(c, k, lf) for c in cfgs])
+        bt.parse_count_device(bases.ptr, offs.ptr, nr, first)
+        bt.run(args.variant)
+        synchronize()
+        dt = time.perf_counter() - t0
+        per_job = [bt.job(q).stats() for q in range(len(cfgs))]
+        st, ph = bt.stats(), bt.phase_ms()
+        bt.close()
+        bases.free()
+        offs.free()
+        return dt, st, ph, per_job, nb
+
+    def run_alone(i):
+        c = jobs[i]
+        bases, offs = DeviceBuffer(c.n_bases), DeviceBuffer((c.n_reads + 1) * 8)
+        synth_reads_device(c, bases.ptr, offs.ptr)
+        synchronize()
+        t0 = time.perf_counter()
+        job = bruno_amd.GraphJob(k, th, capacity=capacity(c, k, lf))
+        job.parse_count_device(bases.ptr, offs.ptr, c.n_reads)
+        job.run(args.variant)
+        synchronize()
+        dt = time.perf_counter() - t0
+        st = job.stats()
+        ph = job.phase_ms()
+        job.close()
+        bases.free()
+        offs.free()
+        return dt, st, ph
+
+    # warm-up: the first two jobs, batched and alone
+    for _ in range(args.warmup):
+        run_group([0, 1])
+        run_alone(0)
+
+    passes = []
+    for step in range(max(1, args.steps)):
+        clock, lat, inst, pc_ms, pc_bytes, peak = 0.0, [], 0, 0.0, 0, 0
+        results = {}
+        for gi, idx in enumerate(groups):
+            dt, st, ph, per_job, nb = run_group(idx)
+            clock += dt
+            inst += st["n_instances"]
+            pc_ms += ph["parse_count_ms"]
+            pc_bytes += count_bytes(nb, st["n_instances"])
+            peak = max(peak, st["peak_device_bytes"])
+            for q, i in enumerate(idx):
+                lat.append(clock)
+                results[i] = (per_job[q]["n_unitigs"], per_job[q]["n_unitig_nodes"], per_job[q]["iterations"])
+            log("group %d: %d jobs, %.0f ms, %.3g k-mers, peak %.1f GB, phases %s"
+                % (gi, len(idx), dt * 1e3, st["n_instances"], st["peak_device_bytes"] / 1e9,
+                   json.dumps({a: round(b, 1) for a, b in ph.items()})))
+        passes.append((clock, lat, inst, pc_ms, pc_bytes, peak, results))
+    clock, lat, inst, pc_ms, pc_bytes, peak, results = min(passes, key=lambda p: p[0])
+
+    clock1, lat1, inst1, same = 0.0, [], 0, True
+    ph1 = {}
+    for i in range(len(jobs)):
+        dt, st, ph = run_alone(i)
+        clock1 += dt
+        inst1 += st["n_instances"]
+        lat1.append(clock1)
+        same &= results[i] == (st["n_unitigs"], st["n_unitig_nodes"], st["iterations"])
+        for a, b in ph.items():
+            ph1[a] = ph1.get(a, 0.0) + b
+        if args.verbose:
+            log("job %d: G %d L %d reads %d: %.1f ms %s" % (i, jobs[i].genome_len, jobs[i].read_len, jobs[i].n_reads,
+                                                          dt * 1e3, json.dumps({a: round(b, 2) for a, b in ph.items()})))
+    log("one job per launch: %.0f ms, phases %s" % (clock1 * 1e3, json.dumps({a: round(b, 1) for a, b in ph1.items()})))
+
+    achieved = pc_bytes / (pc_ms * 1e-3) / 1e9
+    G = sum(c.genome_len for c in jobs)
+    out = {
+        "metric": "k-mers/s (parse+count+filter+compact)",
+        "value": inst / clock,
+        "unit": "k-mers/s",
+        "n_gpus": 1,
+        "steps": max(1, args.steps),
+        "warmup": args.warmup,
+        "ms_per_step": clock * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic (seeded generator, reads generated in HBM)",
+        "config": {
+            "workload": "config4: %d independent jobs, genomes log-uniform 50 kbp-20 Mbp (total %.0f Mbp), "
+                        "reads {100,150,250} bp, 20-60x, 1%% substitutions, k=31, -T -L 3, batched"
+                        % (len(jobs), G / 1e6),
+            "n_jobs": len(jobs), "launch_groups": len(groups), "k_mer_instances": inst,
+            "bases": sum(c.n_bases for c in jobs), "load_factor": lf, "budget_gb": budget / 1e9,
+        },
+        "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3), "p99": float(np.percentile(lat, 99) * 1e3),
+                       "note": "all jobs submitted at t=0; a job completes with its launch group"},
+        "one_job_per_launch": {"value": inst1 / clock1, "ms_total": clock1 * 1e3,
+                               "latency_ms": {"p50": float(np.percentile(lat1, 50) * 1e3),
+                                              "p99": float(np.percentile(lat1, 99) * 1e3)}},
+        "speedup_vs_one_job_per_launch": clock1 / clock,
+        "per_job_results_equal": bool(same and inst1 == inst),
+        "peak_device_gb": peak / 1e9,
+        "roofline": {
+            "kernel": "k_parse (parse + k2 count), all groups",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+        },
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline_jobs(k, th, jobs, args.cpu_seconds)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+def cpu_baseline_jobs(k, th, jobs, seconds_target=15.0, threads=None):
+    """The oracle on the first jobs of a job list, each job alone, until the
+    sample's k-mer budget is used (the last job sampled by its first reads)."""
+    import numpy as np
+    import bruno_amd
+    from bruno_amd.pipeline import SynthConfig
+    from oracle.oracle import Oracle  # the checker, timed as the baseline
+    threads = threads or min(16, os.cpu_count() or 1)
+    budget = int(240_000_000 * seconds_target / 15.0)
+    inst, dt, used = 0, 0.0, []
+    for c in jobs:
+        if inst >= budget:
+            break
+        per = max(1, c.read_len - k + 1)
+        n = max(1, min(c.n_reads, (budget - inst) // per + 1))
+        sample = SynthConfig(**{**c.__dict__, "n_reads": n})
+        b, o = bruno_amd.synth_reads_host(sample)
+        t0 = time.time()
+        orc = Oracle(k, np.array(th, np.uint64), threads)
+        orc.add_reads(b, o)
+        orc.run(1)
+        dt += time.time() - t0
+        inst += orc.stat("n_instances")
+        used.append("%d/%d" % (n, c.n_reads))
+    return {"value": inst / dt, "unit": "k-mers/s", "cores": threads, "kind": "port",
+            "sample": "jobs 0..%d of the same workload, each alone (reads used per job: %s), "
+                      "%d k-mer instances, all phases, %.1f s" % (len(used) - 1, ", ".join(used), inst, dt)}
 
 
 if __name__ == "__main__":

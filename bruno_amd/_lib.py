@@ -28,7 +28,7 @@ class PhaseMs(ctypes.Structure):
 STAT_FIELDS = ("n_reads", "n_reads_dropped", "n_instances", "n_distinct", "n_k2", "n_overflow",
                "n_nodes", "n_branches", "n_chain_nodes", "n_cycle_nodes", "n_isolated",
                "n_unitig_nodes", "n_unitigs", "iterations", "n_conflicts", "capacity",
-               "device_bytes")
+               "device_bytes", "peak_device_bytes")
 
 
 class Stats(ctypes.Structure):
@@ -59,7 +59,8 @@ EXPORTS = (
     "bruno_dev_alloc", "bruno_dev_free", "bruno_dev_copy", "bruno_dev_sync", "bruno_dev_mem_info",
     "bruno_batch_create", "bruno_batch_parse_count", "bruno_batch_get_stats", "bruno_batch_get_counts",
     "bruno_batch_get_k2", "bruno_batch_get_nodes", "bruno_batch_get_branches", "bruno_batch_get_chain_nodes",
-    "bruno_batch_get_unitigs", "bruno_batch_get_branch_fasta",
+    "bruno_batch_get_unitigs", "bruno_batch_get_branch_fasta", "bruno_synth_batch_device",
+    "bruno_synth_batch_host", "bruno_pool_reserve", "bruno_pool_trim", "bruno_pool_stats",
 )
 
 _LIB = None
@@ -109,6 +110,11 @@ def lib():
                   "bruno_batch_get_unitigs", "bruno_batch_get_branch_fasta", "bruno_batch_get_branches"):
             getattr(L, n).argtypes = [vp, u32, vp, u64p]
         L.bruno_batch_get_k2.argtypes = [vp, u32, vp, vp, vp, u64p]
+        L.bruno_synth_batch_device.argtypes = [vp, u32, vp, vp, vp]
+        L.bruno_synth_batch_host.argtypes = [vp, u32, vp, vp]
+        L.bruno_pool_reserve.argtypes = [u64]
+        L.bruno_pool_trim.argtypes = []
+        L.bruno_pool_stats.argtypes = [u64p, u64p, u64p]
         L.bruno_dev_alloc.argtypes = [u64, ctypes.POINTER(vp)]
         L.bruno_dev_free.argtypes = [vp]
         L.bruno_dev_copy.argtypes = [vp, vp, u64, i]

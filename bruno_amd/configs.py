@@ -5,8 +5,15 @@ config2: 100 Mbp genome with 2% from 50 repeat families (500-5000 bp, 1%
          divergence), 150 bp reads at 50x (~33.3 M reads), 1% substitutions, k=31
 config3: 20 Mbp (same repeat layer), 250 bp reads at 40x, 0.5% substitutions
          + 0.05% 1-bp indels, k=63
-All three use the -T -L 3 thresholds.
+config4: 256 independent jobs: random genomes log-uniform in 50 kbp - 20 Mbp,
+         read length uniform in {100, 150, 250}, coverage uniform in 20-60x,
+         1% substitutions, k=31 (run as batches)
+All use the -T -L 3 thresholds.
 """
+import math
+
+import numpy as np
+
 from .pipeline import SynthConfig, thresholds_L
 
 
@@ -35,4 +42,22 @@ def config3(seed=3, scale=1.0):
                                   repeat_fraction=0.02, family_divergence=0.01))
 
 
-CONFIGS = {"config1": config1, "config2": config2, "config3": config3}
+def config4_jobs(seed=4, n_jobs=256, scale=1.0, k=31):
+    """The job list of the batched workload (seeded, fixed)."""
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for i in range(n_jobs):
+        G = int(round(math.exp(rng.uniform(math.log(50_000), math.log(20_000_000))) * scale))
+        G = max(G, 1000)
+        L = int(rng.choice([100, 150, 250]))
+        cov = float(rng.uniform(20, 60))
+        jobs.append(SynthConfig(seed=seed * 100003 + i, genome_len=G, n_reads=max(1, int(round(cov * G / L))),
+                                read_len=L, sub_rate=0.01))
+    return jobs
+
+
+def config4(seed=4, scale=1.0, n_jobs=256):
+    return dict(k=31, thresholds=thresholds_L(3), jobs=config4_jobs(seed, n_jobs, scale))
+
+
+CONFIGS = {"config1": config1, "config2": config2, "config3": config3, "config4": config4}

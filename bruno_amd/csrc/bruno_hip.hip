@@ -16,6 +16,7 @@
 #include "../../include/bruno_hip.h"
 #include "kernels.h"
 #include "kmer.h"
+#include "pool.h"
 #include "synth.h"
 #include "table.h"
 
@@ -965,13 +966,14 @@ __global__ void k_get_chain_nodes(const uint32_t* perm, const KT* out_k, const K
   out[i] = c;
 }
 
-__global__ void k_synth(br::synth_params p, char* bases, uint64_t* offsets) {
+/* reads of one synthetic job at bases, offsets[r] = obase + r * L */
+__global__ void k_synth(br::synth_params p, char* bases, uint64_t* offsets, uint64_t obase) {
   uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r < p.n_reads) {
     make_read(p, r, bases + r * p.L);
-    offsets[r] = r * p.L;
+    offsets[r] = obase + r * p.L;
   }
-  if (r == 0) offsets[p.n_reads] = p.n_reads * p.L;
+  if (r == 0) offsets[p.n_reads] = obase + p.n_reads * p.L;
 }
 
 /* ================================================================== */
@@ -1003,6 +1005,7 @@ struct bruno_job {
   Counters* ctr = nullptr;      /* device */
   Counters* hctr = nullptr;     /* pinned host mirror */
   uint64_t device_bytes = 0;
+  uint64_t peak_bytes = 0;
   uint64_t n_reads = 0;
 
   /* A batch of n_jobs independent jobs shares the table: job i owns slots
@@ -1058,20 +1061,27 @@ struct bruno_job {
 
 namespace {
 
+/* All device memory comes from the per-device pool. */
+hipError_t pool_alloc(void** p, uint64_t bytes) { return DevicePool::current().alloc(p, bytes); }
+void pool_free(void* p) { DevicePool::current().release(p); }
+
 bruno_status dev_alloc(bruno_job* j, void** p, size_t bytes) {
   if (bytes == 0) bytes = 16;
-  if (hipMalloc(p, bytes) != hipSuccess) {
-    (void)hipGetLastError();
+  if (pool_alloc(p, bytes) != hipSuccess) {
     *p = nullptr;
     return BRUNO_OOM;
   }
   j->allocs.push_back(*p);
   j->device_bytes += bytes;
+  j->peak_bytes = std::max(j->peak_bytes, j->device_bytes);
   return BRUNO_OK;
 }
+/* The block goes back to the pool once the job's stream has finished with
+ * it (another stream may be handed the block next). */
 void dev_free(bruno_job* j, void*& p, size_t bytes) {
   if (!p) return;
-  (void)hipFree(p);
+  (void)hipStreamSynchronize(j->stream);
+  pool_free(p);
   auto it = std::find(j->allocs.begin(), j->allocs.end(), p);
   if (it != j->allocs.end()) j->allocs.erase(it);
   j->device_bytes -= std::min<uint64_t>(j->device_bytes, bytes);
@@ -1730,10 +1740,10 @@ static bruno_status create_common(const bruno_job_desc* d, uint32_t n_jobs, cons
   j->stream = (hipStream_t)d->stream;
   j->n_jobs = n_jobs;
   j->job_reads.assign(n_jobs, 0);
-  if (hipMalloc((void**)&j->ctr, sizeof(Counters)) != hipSuccess ||
+  if (pool_alloc((void**)&j->ctr, sizeof(Counters)) != hipSuccess ||
       hipHostMalloc((void**)&j->hctr, sizeof(Counters), hipHostMallocDefault) != hipSuccess) {
     (void)hipGetLastError();
-    if (j->ctr) (void)hipFree(j->ctr);
+    pool_free(j->ctr);
     delete j;
     return BRUNO_OOM;
   }
@@ -1779,9 +1789,9 @@ bruno_status bruno_batch_create(const bruno_job_desc* d, uint32_t n_jobs, const 
 bruno_status bruno_job_destroy(bruno_job* j) {
   if (!j) return BRUNO_BAD_INPUT;
   (void)hipStreamSynchronize(j->stream);
-  for (void* p : j->allocs) (void)hipFree(p);
+  for (void* p : j->allocs) pool_free(p);
   j->allocs.clear();
-  if (j->ctr) (void)hipFree(j->ctr);
+  pool_free(j->ctr);
   if (j->hctr) (void)hipHostFree(j->hctr);
   if (j->ev_ok)
     for (int i = 0; i < 12; ++i) (void)hipEventDestroy(j->ev[i]);
@@ -1795,10 +1805,11 @@ bruno_status bruno_job_reset(bruno_job* j) {
   std::vector<void*> keep;
   for (void* p : j->allocs) {
     if (persistent(j, p)) keep.push_back(p);
-    else (void)hipFree(p);
+    else pool_free(p);
   }
   j->allocs = keep;
   j->device_bytes = persistent_bytes(j);
+  j->peak_bytes = j->device_bytes;
   j->present = j->chain_slot = j->branch_slot = nullptr;
   j->ckmer = j->vkmer = j->bkmer = j->out_k = j->out_rep = j->bsorted = nullptr;
   j->lr = nullptr;
@@ -1915,9 +1926,8 @@ bruno_status bruno_parse_count_host(bruno_job* j, const char* bases, const uint6
   uint64_t nb = offsets[n_reads] - offsets[0];
   char* db = nullptr;
   uint64_t* doff = nullptr;
-  if (hipMalloc((void**)&db, nb + 1) != hipSuccess || hipMalloc((void**)&doff, (n_reads + 1) * 8) != hipSuccess) {
-    (void)hipGetLastError();
-    if (db) (void)hipFree(db);
+  if (pool_alloc((void**)&db, nb + 1) != hipSuccess || pool_alloc((void**)&doff, (n_reads + 1) * 8) != hipSuccess) {
+    pool_free(db);
     return BRUNO_OOM;
   }
   std::vector<uint64_t> off(offsets, offsets + n_reads + 1);
@@ -1931,8 +1941,8 @@ bruno_status bruno_parse_count_host(bruno_job* j, const char* bases, const uint6
   }
   if (s == BRUNO_OK) s = bruno_parse_count(j, db, doff, n_reads);
   (void)hipStreamSynchronize(j->stream);
-  (void)hipFree(db);
-  (void)hipFree(doff);
+  pool_free(db);
+  pool_free(doff);
   return s;
 }
 
@@ -2025,6 +2035,7 @@ bruno_status bruno_get_stats(bruno_job* j, bruno_stats* o) {
   o->n_conflicts = j->conflicts;
   o->capacity = j->cap;
   o->device_bytes = j->device_bytes;
+  o->peak_device_bytes = j->peak_bytes;
   return BRUNO_OK;
 }
 
@@ -2373,9 +2384,8 @@ bruno_status bruno_parse_records(int k, const char* d_bases, const uint64_t* d_o
     return BRUNO_OK;
   }
   uint64_t *per = nullptr, *base = nullptr;
-  if (hipMalloc((void**)&per, n_reads * 8) != hipSuccess || hipMalloc((void**)&base, (n_reads + 1) * 8) != hipSuccess) {
-    (void)hipGetLastError();
-    if (per) (void)hipFree(per);
+  if (pool_alloc((void**)&per, n_reads * 8) != hipSuccess || pool_alloc((void**)&base, (n_reads + 1) * 8) != hipSuccess) {
+    pool_free(per);
     return BRUNO_OOM;
   }
   ParseArgs a{};
@@ -2395,7 +2405,7 @@ bruno_status bruno_parse_records(int k, const char* d_bases, const uint64_t* d_o
   void* t = nullptr;
   uint64_t total = 0;
   if (hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, per, base, (int)n_reads, st) != hipSuccess ||
-      hipMalloc(&t, tmp + 16) != hipSuccess ||
+      pool_alloc(&t, tmp + 16) != hipSuccess ||
       hipcub::DeviceScan::ExclusiveSum(t, tmp, per, base, (int)n_reads, st) != hipSuccess) {
     (void)hipGetLastError();
     s = BRUNO_INTERNAL;
@@ -2416,9 +2426,10 @@ bruno_status bruno_parse_records(int k, const char* d_bases, const uint64_t* d_o
       s = BRUNO_INTERNAL;
     }
   }
-  if (t) (void)hipFree(t);
-  (void)hipFree(per);
-  (void)hipFree(base);
+  (void)hipStreamSynchronize(st);
+  pool_free(t);
+  pool_free(per);
+  pool_free(base);
   *n_out = total;
   return s;
 }
@@ -2426,8 +2437,7 @@ bruno_status bruno_parse_records(int k, const char* d_bases, const uint64_t* d_o
 bruno_status bruno_dev_alloc(uint64_t bytes, void** out) {
   if (!out) return BRUNO_BAD_INPUT;
   *out = nullptr;
-  if (hipMalloc(out, bytes ? bytes : 16) != hipSuccess) {
-    (void)hipGetLastError();
+  if (pool_alloc(out, bytes ? bytes : 16) != hipSuccess) {
     *out = nullptr;
     return BRUNO_OOM;
   }
@@ -2435,10 +2445,32 @@ bruno_status bruno_dev_alloc(uint64_t bytes, void** out) {
 }
 
 bruno_status bruno_dev_free(void* p) {
-  if (p && hipFree(p) != hipSuccess) {
+  if (!p) return BRUNO_OK;
+  /* the caller may have queued work on any stream that uses the block */
+  if (hipDeviceSynchronize() != hipSuccess) {
     (void)hipGetLastError();
     return BRUNO_INTERNAL;
   }
+  pool_free(p);
+  return BRUNO_OK;
+}
+
+bruno_status bruno_pool_reserve(uint64_t bytes) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+    (void)hipGetLastError();
+    return BRUNO_NO_DEVICE;
+  }
+  return DevicePool::current().reserve(bytes) == hipSuccess ? BRUNO_OK : BRUNO_OOM;
+}
+
+bruno_status bruno_pool_trim(void) {
+  DevicePool::current().trim();
+  return BRUNO_OK;
+}
+
+bruno_status bruno_pool_stats(uint64_t* reserved, uint64_t* in_use, uint64_t* largest_free) {
+  DevicePool::current().stats(reserved, in_use, largest_free);
   return BRUNO_OK;
 }
 
@@ -2511,7 +2543,33 @@ bruno_status bruno_synth_reads_device(const bruno_synth_desc* d, char* d_bases, 
   if (!make_params(d, p) || !d_bases || !d_offsets) return BRUNO_BAD_INPUT;
   uint64_t g = (p.n_reads + 255) / 256;
   if (g == 0) g = 1;
-  hipLaunchKernelGGL(k_synth, dim3((unsigned)g), dim3(256), 0, (hipStream_t)stream, p, d_bases, d_offsets);
+  hipLaunchKernelGGL(k_synth, dim3((unsigned)g), dim3(256), 0, (hipStream_t)stream, p, d_bases, d_offsets, 0ULL);
+  if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess) {
+    (void)hipGetLastError();
+    return BRUNO_INTERNAL;
+  }
+  return BRUNO_OK;
+}
+
+bruno_status bruno_synth_batch_device(const bruno_synth_desc* descs, uint32_t n_jobs, char* d_bases,
+                                      uint64_t* d_offsets, void* stream) {
+  if (!descs || !d_bases || !d_offsets) return BRUNO_BAD_INPUT;
+  std::vector<br::synth_params> ps(n_jobs);
+  for (uint32_t i = 0; i < n_jobs; ++i)
+    if (!make_params(&descs[i], ps[i])) return BRUNO_BAD_INPUT;
+  uint64_t rsum = 0, bsum = 0;
+  for (uint32_t i = 0; i < n_jobs; ++i) {
+    const br::synth_params& p = ps[i];
+    uint64_t g = std::max<uint64_t>(1, (p.n_reads + 255) / 256);
+    hipLaunchKernelGGL(k_synth, dim3((unsigned)g), dim3(256), 0, (hipStream_t)stream, p, d_bases + bsum,
+                       d_offsets + rsum, bsum);
+    rsum += p.n_reads;
+    bsum += p.n_reads * p.L;
+  }
+  if (n_jobs == 0 && hipMemsetAsync(d_offsets, 0, 8, (hipStream_t)stream) != hipSuccess) {
+    (void)hipGetLastError();
+    return BRUNO_INTERNAL;
+  }
   if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess) {
     (void)hipGetLastError();
     return BRUNO_INTERNAL;
@@ -2520,13 +2578,26 @@ bruno_status bruno_synth_reads_device(const bruno_synth_desc* d, char* d_bases, 
 }
 
 bruno_status bruno_synth_reads_host(const bruno_synth_desc* d, char* bases, uint64_t* offsets) {
-  br::synth_params p;
-  if (!make_params(d, p) || !bases || !offsets) return BRUNO_BAD_INPUT;
-  for (uint64_t r = 0; r < p.n_reads; ++r) {
-    br::make_read(p, r, bases + r * p.L);
-    offsets[r] = r * p.L;
+  return bruno_synth_batch_host(d, 1, bases, offsets);
+}
+
+bruno_status bruno_synth_batch_host(const bruno_synth_desc* descs, uint32_t n_jobs, char* bases, uint64_t* offsets) {
+  if (!descs || !bases || !offsets) return BRUNO_BAD_INPUT;
+  std::vector<br::synth_params> ps(n_jobs);
+  for (uint32_t i = 0; i < n_jobs; ++i)
+    if (!make_params(&descs[i], ps[i])) return BRUNO_BAD_INPUT;
+  uint64_t rsum = 0, bsum = 0;
+  offsets[0] = 0;
+  for (uint32_t i = 0; i < n_jobs; ++i) {
+    const br::synth_params& p = ps[i];
+    for (uint64_t r = 0; r < p.n_reads; ++r) {
+      br::make_read(p, r, bases + bsum + r * p.L);
+      offsets[rsum + r] = bsum + r * p.L;
+    }
+    rsum += p.n_reads;
+    bsum += p.n_reads * p.L;
+    offsets[rsum] = bsum;
   }
-  offsets[p.n_reads] = p.n_reads * p.L;
   return BRUNO_OK;
 }
 

@@ -225,7 +225,22 @@ __global__ __launch_bounds__(BLOCK) void k_parse(ParseArgs a) {
   const int K = a.K;
   kops<KT> ops(K);
   unsigned long long my_inst = 0, my_drop = 0, my_new = 0, my_k2 = 0;
+  /* batch: the job of the wave's current read, its read range [j_lo, j_hi)
+   * and table segment, and the wave's instance count for it (flushed when
+   * the job changes, so the per-job counters see few atomics) */
+  uint32_t cj = 0;
+  uint64_t j_lo = 0, j_hi = 0, tbase = 0, tcap = a.cap;
+  unsigned long long j_inst = 0;
   for (uint64_t r = wave; r < a.n_reads; r += nwaves) {
+    if (MODE == 2 && a.n_jobs > 1 && !(r >= j_lo && r < j_hi)) {
+      if (lane == 0 && j_inst) atomicAdd(&a.job_inst[cj], j_inst);
+      j_inst = 0;
+      cj = job_of(a.job_first, a.n_jobs, r);
+      j_lo = a.job_first[cj];
+      j_hi = a.job_first[cj + 1];
+      tbase = a.seg[cj];
+      tcap = a.seg[cj + 1] - tbase;
+    }
     const uint64_t off = a.offsets[r];
     const uint64_t L = a.offsets[r + 1] - off;
     const uint8_t* s = a.bases + off;
@@ -236,7 +251,7 @@ __global__ __launch_bounds__(BLOCK) void k_parse(ParseArgs a) {
     if (__any(bad)) {
       if (lane == 0) ++my_drop;
       if (MODE == 0 && lane == 0) a.per_read[r] = 0;
-      if (MODE == 2 && a.n_jobs > 1 && lane == 0) atomicAdd(&a.job_drop[job_of(a.job_first, a.n_jobs, r)], 1ULL);
+      if (MODE == 2 && a.n_jobs > 1 && lane == 0) atomicAdd(&a.job_drop[cj], 1ULL);
       continue;
     }
     /* parser window is k+2 (debruijn_biedge_loader.hpp:402, :464-471) */
@@ -250,13 +265,7 @@ __global__ __launch_bounds__(BLOCK) void k_parse(ParseArgs a) {
       continue;
     }
     uint64_t rbase = (MODE == 1) ? a.rec_base[r] : 0;
-    uint64_t tbase = 0, tcap = a.cap;
-    if (MODE == 2 && a.n_jobs > 1) {
-      const uint32_t j = job_of(a.job_first, a.n_jobs, r);
-      tbase = a.seg[j];
-      tcap = a.seg[j + 1] - tbase;
-      if (lane == 0) atomicAdd(&a.job_inst[j], (unsigned long long)nk);
-    }
+    if (MODE == 2 && a.n_jobs > 1) j_inst += nk;
     /* sliding window of 4 packed words covering bases [p0, p0 + 128) */
     uint64_t A, B, C, D, prevB = 0;
     {
@@ -306,6 +315,7 @@ __global__ __launch_bounds__(BLOCK) void k_parse(ParseArgs a) {
       pack64(cn, C, D);
     }
   }
+  if (MODE == 2 && a.n_jobs > 1 && lane == 0 && j_inst) atomicAdd(&a.job_inst[cj], j_inst);
   if (MODE == 2) {
     block_add(my_inst, &a.ctr->n_instances);
     __syncthreads();

@@ -50,3 +50,19 @@ def mem_info():
     f, t = ctypes.c_uint64(), ctypes.c_uint64()
     check(lib().bruno_dev_mem_info(ctypes.byref(f), ctypes.byref(t)), "bruno_dev_mem_info")
     return f.value, t.value
+
+
+def pool_reserve(nbytes):
+    """Reserve one pool chunk of nbytes so later allocations make no HIP
+    allocator calls."""
+    check(lib().bruno_pool_reserve(int(nbytes)), "bruno_pool_reserve")
+
+
+def pool_trim():
+    check(lib().bruno_pool_trim(), "bruno_pool_trim")
+
+
+def pool_stats():
+    r, u, f = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+    check(lib().bruno_pool_stats(ctypes.byref(r), ctypes.byref(u), ctypes.byref(f)), "bruno_pool_stats")
+    return {"reserved": r.value, "in_use": u.value, "largest_free": f.value}

@@ -270,3 +270,33 @@ def synth_reads_device(cfg, bases_ptr, offsets_ptr, stream=None):
     d = cfg.desc()
     check(lib().bruno_synth_reads_device(ctypes.byref(d), ctypes.c_void_p(bases_ptr),
                                          ctypes.c_void_p(offsets_ptr), stream), "bruno_synth_reads_device")
+
+
+def _descs(cfgs):
+    arr = (_lib.SynthDesc * max(1, len(cfgs)))()
+    for i, c in enumerate(cfgs):
+        arr[i] = c.desc()
+    return arr
+
+
+def synth_batch_host(cfgs):
+    """Reads of several synthetic workloads back to back (one batch), with
+    global offsets and the job_first_read array of the batch."""
+    nb = sum(c.n_bases for c in cfgs)
+    nr = sum(c.n_reads for c in cfgs)
+    b = np.empty(max(1, nb), np.uint8)
+    o = np.empty(nr + 1, np.uint64)
+    check(lib().bruno_synth_batch_host(_descs(cfgs), len(cfgs), _ptr(b), _ptr(o)), "bruno_synth_batch_host")
+    return b[:nb], o, job_first_read(cfgs)
+
+
+def synth_batch_device(cfgs, bases_ptr, offsets_ptr, stream=None):
+    check(lib().bruno_synth_batch_device(_descs(cfgs), len(cfgs), ctypes.c_void_p(bases_ptr),
+                                         ctypes.c_void_p(offsets_ptr), stream), "bruno_synth_batch_device")
+    return job_first_read(cfgs)
+
+
+def job_first_read(cfgs):
+    f = np.zeros(len(cfgs) + 1, np.uint64)
+    f[1:] = np.cumsum([c.n_reads for c in cfgs])
+    return f

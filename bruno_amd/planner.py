@@ -1,0 +1,55 @@
+"""Sizing of jobs and batches for synthetic workloads.
+
+A job's hash table needs one slot per distinct canonical k-mer at the chosen
+load factor.  For reads sampled from a genome of G bases with substitution
+rate e (and 1-bp indel rate d), the distinct k-mers are about the genome's G
+plus the novel k-mers each error creates: an error at a base is covered by
+min(k, L) * (L - k + 1) / L of the read's k-mers on average, and almost every
+such k-mer is new.  Peak device memory of a job is its table and side table
+plus the reads plus the per-node lists of the later phases.
+"""
+SLOT_BYTES = {1: 32, 2: 64}
+
+
+def words(k):
+    return 1 if k <= 31 else 2
+
+
+def est_distinct(s, k):
+    L = s.read_len
+    per_read = max(0, L - k + 1)
+    err = s.sub_rate + 2 * s.indel_rate
+    cover = min(k, L) * per_read / max(1, L)
+    return s.genome_len + s.n_reads * L * err * cover
+
+
+def capacity(s, k, load_factor=0.5):
+    return int(est_distinct(s, k) / load_factor) + 4096
+
+
+def footprint(s, k, load_factor=0.5, cap=None):
+    """Estimated peak HBM bytes of one job: table + side table (1/8 of the
+    slots, 16 B each) + present list (8 B per distinct k-mer) + reads +
+    ~300 B per genome k-mer for the chain, list-ranking and output arrays."""
+    cap = capacity(s, k, load_factor) if cap is None else cap
+    table = cap * SLOT_BYTES[words(k)] + (cap // 8) * 16
+    present = 8 * est_distinct(s, k)
+    reads = s.n_bases + 8 * (s.n_reads + 1)
+    chains = 300 * s.genome_len * words(k)
+    return int(table + present + reads + chains)
+
+
+def plan_groups(cfgs, k, budget_bytes, load_factor=0.5):
+    """Consecutive jobs packed into launch groups whose estimated footprint
+    stays within the budget (a job larger than the budget runs alone)."""
+    groups, cur, used = [], [], 0
+    for i, s in enumerate(cfgs):
+        f = footprint(s, k, load_factor)
+        if cur and used + f > budget_bytes:
+            groups.append(cur)
+            cur, used = [], 0
+        cur.append(i)
+        used += f
+    if cur:
+        groups.append(cur)
+    return groups

@@ -83,10 +83,12 @@ typedef struct {
   uint64_t n_unitig_nodes;  /* chain nodes printed as unitigs                    */
   uint64_t n_unitigs;
   uint64_t iterations;      /* list-ranking rounds                               */
-  uint64_t n_conflicts;     /* same-round updates to one chain end (0 for every
-                               graph built from reads; see DESIGN)               */
+  uint64_t n_conflicts;     /* same-round updates to one chain end in the
+                               round-by-round list ranking (outputs do not depend
+                               on their order)                                   */
   uint64_t capacity;        /* hash slots                                        */
-  uint64_t device_bytes;    /* HBM held by the job                               */
+  uint64_t device_bytes;    /* HBM held by the job now                           */
+  uint64_t peak_device_bytes; /* most HBM held at once since create / reset      */
 } bruno_stats;
 
 /* Listranked chain node (debruijn_chain_node.hpp:156): K on the rep strand,
@@ -234,8 +236,20 @@ bruno_status bruno_dev_free(void* p);
 /* kind: 0 host->device, 1 device->host, 2 device->device */
 bruno_status bruno_dev_copy(void* dst, const void* src, uint64_t bytes, int kind);
 bruno_status bruno_dev_sync(void);
-/* free / total HBM bytes of the current device */
+/* free / total HBM bytes of the current device (memory held by the pool
+ * below counts as used) */
 bruno_status bruno_dev_mem_info(uint64_t* free_bytes, uint64_t* total_bytes);
+
+/* ---- Device memory pool --------------------------------------------------
+ * Every device allocation of the library (job tables, phase buffers,
+ * bruno_dev_alloc) is a block of a per-device pool, so a steady-state run
+ * makes no hipMalloc / hipFree calls (each costs tens of ms per GB).  The
+ * pool grows by chunks on demand; a server or benchmark reserves one large
+ * chunk up front.  Memory returns to HIP only through bruno_pool_trim (fully
+ * free chunks).  No counterpart in the reference (host containers). */
+bruno_status bruno_pool_reserve(uint64_t bytes);
+bruno_status bruno_pool_trim(void);
+bruno_status bruno_pool_stats(uint64_t* reserved, uint64_t* in_use, uint64_t* largest_free);
 
 /* ---- Synthetic workload generator (the benchmark configurations) ------ */
 typedef struct {
@@ -257,6 +271,11 @@ bruno_status bruno_synth_reads_device(const bruno_synth_desc* d, char* d_bases, 
                                       void* stream);
 /* Same on the host (bit-identical to the device version). */
 bruno_status bruno_synth_reads_host(const bruno_synth_desc* d, char* bases, uint64_t* offsets);
+/* n_jobs workloads back to back (a batch's reads): job i's reads follow job
+ * i-1's, offsets are global (n_reads_total + 1 entries). */
+bruno_status bruno_synth_batch_device(const bruno_synth_desc* descs, uint32_t n_jobs, char* d_bases,
+                                      uint64_t* d_offsets, void* stream);
+bruno_status bruno_synth_batch_host(const bruno_synth_desc* descs, uint32_t n_jobs, char* bases, uint64_t* offsets);
 
 #ifdef __cplusplus
 }

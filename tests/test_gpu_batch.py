@@ -193,3 +193,46 @@ def test_whole_job_getters_refuse_a_batch():
     with pytest.raises(bruno_amd.BrunoError):
         bruno_amd.GraphJob.parse_count_device(bt, 0, 0, 0)
     bt.close()
+
+
+def test_synth_batch_device_equals_host():
+    from bruno_amd.configs import config4_jobs
+    from bruno_amd.device import DeviceBuffer, synchronize
+    from bruno_amd.pipeline import synth_batch_device, synth_batch_host
+    jobs = config4_jobs(seed=4, n_jobs=6, scale=0.003)
+    b, o, first = synth_batch_host(jobs)
+    db, do = DeviceBuffer(max(1, b.size)), DeviceBuffer(o.size * 8)
+    f2 = synth_batch_device(jobs, db.ptr, do.ptr)
+    synchronize()
+    assert np.array_equal(f2, first)
+    assert np.array_equal(do.to_numpy(np.uint64, o.size), o)
+    assert np.array_equal(db.to_numpy(np.uint8, b.size), b)
+    db.free()
+    do.free()
+
+
+def test_device_pool_reuses_and_coalesces():
+    """Blocks freed in any order coalesce back into one free block of the
+    chunk; a reserved chunk serves later allocations without growing."""
+    from bruno_amd.device import DeviceBuffer, pool_reserve, pool_stats
+    pool_reserve(1 << 30)
+    before = pool_stats()
+    bufs = [DeviceBuffer(s) for s in (100, 1 << 20, 12345, 1 << 26, 7, 3 << 20)]
+    mid = pool_stats()
+    assert mid["reserved"] == before["reserved"]
+    assert mid["in_use"] >= before["in_use"] + sum(b.nbytes for b in bufs)
+    for i in (3, 0, 5, 1, 4, 2):
+        bufs[i].free()
+    after = pool_stats()
+    assert after["in_use"] == before["in_use"]
+    assert after["largest_free"] == before["largest_free"]
+    # a job's table and phase buffers come from the pool as well
+    k, th = 31, thresholds_L(2)
+    jobs = mixed_jobs(3)[:3]
+    b, off, first = pack_jobs(jobs)
+    bt = bruno_amd.GraphBatch(k, th, [capacity(j, k) for j in jobs])
+    bt.parse_count(b, off, first)
+    bt.run(1)
+    assert pool_stats()["in_use"] > after["in_use"]
+    bt.close()
+    assert pool_stats()["in_use"] == after["in_use"]

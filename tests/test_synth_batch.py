@@ -1,0 +1,29 @@
+"""The batch generator lays jobs back to back: the same bytes as each job's
+own generator, with global offsets (host generator; no GPU)."""
+import numpy as np
+
+import bruno_amd
+from bruno_amd.configs import config4_jobs
+from bruno_amd.pipeline import synth_batch_host
+from bruno_amd.planner import plan_groups
+
+
+def test_batch_host_equals_jobs_back_to_back():
+    jobs = config4_jobs(seed=4, n_jobs=5, scale=0.002)
+    b, o, first = synth_batch_host(jobs)
+    assert first[0] == 0 and first[-1] == sum(j.n_reads for j in jobs)
+    for i, j in enumerate(jobs):
+        bj, oj = bruno_amd.synth_reads_host(j)
+        r0, r1 = int(first[i]), int(first[i + 1])
+        assert np.array_equal(o[r0:r1 + 1] - o[r0], oj)
+        assert bytes(b[int(o[r0]):int(o[r1])]) == bytes(bj)
+
+
+def test_plan_groups_respects_budget_and_order():
+    jobs = config4_jobs(seed=4, n_jobs=64)
+    from bruno_amd.planner import footprint
+    budget = 40e9
+    groups = plan_groups(jobs, 31, budget)
+    assert [i for g in groups for i in g] == list(range(64))
+    for g in groups:
+        assert len(g) == 1 or sum(footprint(jobs[i], 31) for i in g) <= budget
