@@ -11,6 +11,11 @@ uneven size and read length, packed by estimated HBM footprint into launch
 groups, each group one batch (one set of launches for all its jobs), timed
 against the same jobs run one job per launch.
 
+--config config5 serves jobs drawn from the config4 distribution (k = 31 or
+63, 80/20) arriving as a Poisson process: the request loop admits them by
+estimated HBM footprint and batches whatever is waiting on two streams; the
+same arrivals are replayed against a one-job-at-a-time server.
+
 Prints one JSON line with the metric, a roofline object for the count kernel
 (the dominant kernel) and a CPU baseline: the oracle (the C++ restatement of
 bruno's algorithm, hash-partitioned over host threads) timed on a bounded
@@ -79,6 +84,8 @@ def main():
                     help="HBM budget per launch group for config4 (0 = 80%% of free HBM)")
     ap.add_argument("--n-jobs", type=int, default=256, help="config4: number of jobs")
     ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--rate", type=float, default=40.0, help="config5: arrivals per second")
+    ap.add_argument("--workers", type=int, default=2, help="config5: batches in flight")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the timed steps, copy 4 GiB device-to-device once (a plain copy kernel of "
                          "known size, to scale FETCH_SIZE / WRITE_SIZE counters)")
@@ -100,6 +107,8 @@ def main():
     pool_reserve(int(POOL_FRACTION * free0))
     if args.config == "config4":
         return run_batched(args, free0)
+    if args.config == "config5":
+        return run_serving(args, free0)
     cfg = CONFIGS[args.config](scale=args.scale)
     s = cfg["synth"]
     k = cfg["k"]
@@ -325,6 +334,91 @@ def run_batched(args, free):
         out["cpu_baseline"] = cpu_baseline_jobs(k, th, jobs, args.cpu_seconds)
     else:
         out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+def serving_jobs(seed, n_jobs, rate, scale=1.0):
+    """config5: config4-distribution jobs, k = 31 or 63 (80/20), Poisson
+    arrival times at `rate` per second."""
+    import numpy as np
+    from bruno_amd.configs import config4_jobs
+    cfgs = config4_jobs(seed=seed, n_jobs=n_jobs, scale=scale)
+    rng = np.random.default_rng(seed + 1)
+    ks = [31 if x < 0.8 else 63 for x in rng.random(n_jobs)]
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_jobs))
+    return cfgs, ks, arrivals
+
+
+def serve(cfgs, ks, arrivals, th, lf, workers, max_batch, budget):
+    from bruno_amd.planner import capacity
+    from bruno_amd.server import Server
+    srv = Server(hbm_budget=int(budget), n_workers=workers, max_batch_jobs=max_batch, reserve_pool=False)
+    t0 = time.perf_counter()
+    tickets = []
+    for c, k, a in zip(cfgs, ks, arrivals):
+        d = t0 + a - time.perf_counter()
+        if d > 0:
+            time.sleep(d)
+        tickets.append(srv.submit_synth(k, th, c, capacity=capacity(c, k, lf)))
+    infos = [srv.wait(t) for t in tickets]
+    st = srv.stats()
+    for t in tickets:
+        srv.release(t)
+    srv.close()
+    return infos, st
+
+
+def run_serving(args, free):
+    import numpy as np
+    from bruno_amd.pipeline import thresholds_L
+    th = thresholds_L(3)
+    cfgs, ks, arrivals = serving_jobs(5, args.n_jobs, args.rate, args.scale)
+    budget = 0.85 * free
+    log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
+    runs = {}
+    for name, workers, max_batch in (("continuous_batching", args.workers, 64), ("one_job_at_a_time", 1, 1)):
+        infos, st = serve(cfgs, ks, arrivals, th, args.load_factor, workers, max_batch, budget)
+        bad = [i for i in infos if i["status"] != 0]
+        lat = np.array([i["end_s"] - i["submit_s"] for i in infos])
+        inst = sum(i["n_instances"] for i in infos)
+        span = max(i["end_s"] for i in infos) - min(i["submit_s"] for i in infos)
+        runs[name] = {"value": inst / span, "span_s": span, "k_mer_instances": inst, "failed": len(bad),
+                      "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
+                                     "p99": float(np.percentile(lat, 99) * 1e3),
+                                     "mean": float(lat.mean() * 1e3)},
+                      "batches": st["batches"], "max_batch_jobs": st["max_batch_jobs"], "retries": st["retries"],
+                      "peak_reserved_gb": st["peak_reserved"] / 1e9,
+                      "unitigs": [i["n_unitigs"] for i in infos]}
+        log("%s: %.3g k-mers/s over %.2f s, p50 %.1f ms p99 %.1f ms, %d batches (max %d jobs), %d retries"
+            % (name, inst / span, span, runs[name]["latency_ms"]["p50"], runs[name]["latency_ms"]["p99"],
+               st["batches"], st["max_batch_jobs"], st["retries"]))
+    cb, seq = runs["continuous_batching"], runs["one_job_at_a_time"]
+    same = cb.pop("unitigs") == seq.pop("unitigs")
+    out = {
+        "metric": "k-mers/s (parse+count+filter+compact)",
+        "value": cb["value"],
+        "unit": "k-mers/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 0,
+        "ms_per_step": cb["span_s"] * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic (seeded generator, reads generated in HBM per request)",
+        "config": {
+            "workload": "config5: %d jobs from the config4 distribution, k 31/63 (80/20), Poisson arrivals at "
+                        "%.0f/s, -T -L 3, served with admission by HBM footprint" % (len(cfgs), args.rate),
+            "n_jobs": len(cfgs), "rate_per_s": args.rate, "k63_jobs": int(sum(k == 63 for k in ks)),
+            "workers": args.workers, "budget_gb": budget / 1e9, "load_factor": args.load_factor,
+        },
+        "latency_ms": cb["latency_ms"],
+        "continuous_batching": cb,
+        "one_job_at_a_time": seq,
+        "per_job_results_equal": bool(same),
+    }
+    out["cpu_baseline"] = None
     print(json.dumps(out))
 
 

@@ -44,6 +44,36 @@ class SynthDesc(ctypes.Structure):
                 ("indel_rate", ctypes.c_double)]
 
 
+class ServerDesc(ctypes.Structure):
+    _fields_ = [("hbm_budget", ctypes.c_uint64), ("n_workers", ctypes.c_uint32),
+                ("max_batch_jobs", ctypes.c_uint32), ("variant", ctypes.c_int), ("reserve_pool", ctypes.c_int)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("job", JobDesc), ("bases", ctypes.c_void_p), ("offsets", ctypes.c_void_p),
+                ("n_reads", ctypes.c_uint64), ("synth", ctypes.POINTER(SynthDesc))]
+
+
+RESULT_FIELDS = [("status", ctypes.c_int32), ("batch_jobs", ctypes.c_uint32), ("batch_id", ctypes.c_uint64),
+                 ("n_reads", ctypes.c_uint64), ("n_instances", ctypes.c_uint64), ("n_unitigs", ctypes.c_uint64),
+                 ("n_unitig_nodes", ctypes.c_uint64), ("n_branches", ctypes.c_uint64),
+                 ("unitig_bytes", ctypes.c_uint64), ("branch_bytes", ctypes.c_uint64),
+                 ("iterations", ctypes.c_uint32), ("attempts", ctypes.c_uint32),
+                 ("submit_s", ctypes.c_double), ("start_s", ctypes.c_double), ("end_s", ctypes.c_double)]
+
+
+class ResultInfo(ctypes.Structure):
+    _fields_ = RESULT_FIELDS
+
+
+SERVER_STAT_FIELDS = ("submitted", "jobs_done", "jobs_failed", "batches", "retries", "max_batch_jobs",
+                      "queued", "reserved_now", "peak_reserved", "hbm_budget")
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in SERVER_STAT_FIELDS]
+
+
 NODE_DTYPE = np.dtype([("kmer", "<u8", (2,)), ("counts", "<u2", (9,)), ("pad", "<u2", (3,))])
 CHAIN_DTYPE = np.dtype([("k", "<u8", (2,)), ("rep", "<u8", (2,)), ("dist", "<i4"), ("pad", "<i4")])
 assert NODE_DTYPE.itemsize == 40 and CHAIN_DTYPE.itemsize == 40
@@ -61,6 +91,9 @@ EXPORTS = (
     "bruno_batch_get_k2", "bruno_batch_get_nodes", "bruno_batch_get_branches", "bruno_batch_get_chain_nodes",
     "bruno_batch_get_unitigs", "bruno_batch_get_branch_fasta", "bruno_synth_batch_device",
     "bruno_synth_batch_host", "bruno_pool_reserve", "bruno_pool_trim", "bruno_pool_stats",
+    "bruno_server_create", "bruno_server_submit", "bruno_server_poll", "bruno_server_wait",
+    "bruno_server_get_unitigs", "bruno_server_get_branch_fasta", "bruno_server_release",
+    "bruno_server_get_stats", "bruno_server_destroy",
 )
 
 _LIB = None
@@ -115,6 +148,15 @@ def lib():
         L.bruno_pool_reserve.argtypes = [u64]
         L.bruno_pool_trim.argtypes = []
         L.bruno_pool_stats.argtypes = [u64p, u64p, u64p]
+        L.bruno_server_create.argtypes = [ctypes.POINTER(ServerDesc), ctypes.POINTER(vp)]
+        L.bruno_server_submit.argtypes = [vp, ctypes.POINTER(Request), u64p]
+        L.bruno_server_poll.argtypes = [vp, u64, ctypes.POINTER(ctypes.c_int)]
+        L.bruno_server_wait.argtypes = [vp, u64, ctypes.POINTER(ResultInfo)]
+        L.bruno_server_get_unitigs.argtypes = [vp, u64, vp, u64p]
+        L.bruno_server_get_branch_fasta.argtypes = [vp, u64, vp, u64p]
+        L.bruno_server_release.argtypes = [vp, u64]
+        L.bruno_server_get_stats.argtypes = [vp, ctypes.POINTER(ServerStats)]
+        L.bruno_server_destroy.argtypes = [vp]
         L.bruno_dev_alloc.argtypes = [u64, ctypes.POINTER(vp)]
         L.bruno_dev_free.argtypes = [vp]
         L.bruno_dev_copy.argtypes = [vp, vp, u64, i]

@@ -4,7 +4,7 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = os.path.join(HERE, "csrc", "bruno_hip.hip")
+SRCS = [os.path.join(HERE, "csrc", "bruno_hip.hip"), os.path.join(HERE, "csrc", "server.hip")]
 OUT = os.path.join(HERE, "libbruno_hip.so")
 
 HIPCC_FLAGS = [
@@ -17,7 +17,7 @@ HIPCC_FLAGS = [
 def build(verbose=False):
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     tmp = OUT + ".tmp"
-    cmd = [hipcc] + HIPCC_FLAGS + ["-o", tmp, SRC]
+    cmd = [hipcc] + HIPCC_FLAGS + ["-o", tmp] + SRCS
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)

@@ -1860,7 +1860,8 @@ static bruno_status parse_count_common(bruno_job* j, const char* d_bases, const 
     if (j->hctr->n_instances >= (1ULL << 32)) return BRUNO_CAPACITY;
   } else {
     std::vector<unsigned long long> js(j->n_jobs);
-    HIP_CHECK(hipMemcpy(js.data(), j->d_jstat, j->n_jobs * 8, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpyAsync(js.data(), j->d_jstat, j->n_jobs * 8, hipMemcpyDeviceToHost, j->stream));
+    HIP_CHECK(hipStreamSynchronize(j->stream));
     for (auto v : js)
       if (v >= (1ULL << 32)) return BRUNO_CAPACITY;
   }
@@ -2046,6 +2047,14 @@ bruno_status bruno_get_phase_ms(bruno_job* j, bruno_phase_ms* o) {
 }
 
 /* ---- per-job views ------------------------------------------------- */
+/* device -> host on the job's stream (never the null stream, which would
+ * wait for every other job's work) */
+static hipError_t copy_out(bruno_job* j, void* dst, const void* src, uint64_t bytes) {
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, j->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(j->stream);
+  if (e != hipSuccess) (void)hipGetLastError();
+  return e;
+}
 static bool valid_idx(const bruno_job* j, uint32_t idx) { return valid_job(j) && idx < j->n_jobs; }
 
 static uint64_t unitigs_of(const bruno_job* j, uint64_t bytes, uint64_t nodes) {
@@ -2120,7 +2129,7 @@ bruno_status bruno_batch_get_counts(bruno_job* j, uint32_t idx, bruno_node* buf,
   const uint64_t got = j->hctr->n_out;
   if (s == BRUNO_CAPACITY) s = BRUNO_BAD_INPUT; /* buffer smaller than the count */
   if (s == BRUNO_OK && got) {
-    if (hipMemcpy(buf, d, got * sizeof(bruno_node), hipMemcpyDeviceToHost) != hipSuccess) {
+    if (copy_out(j, buf, d, got * sizeof(bruno_node)) != hipSuccess) {
       (void)hipGetLastError();
       s = BRUNO_INTERNAL;
     }
@@ -2183,9 +2192,8 @@ bruno_status bruno_batch_get_k2(bruno_job* j, uint32_t idx, uint64_t* kmers, uin
   const uint64_t got = j->hctr->n_out;
   if (s == BRUNO_CAPACITY) s = BRUNO_BAD_INPUT;
   if (s == BRUNO_OK && got) {
-    if (hipMemcpy(kmers, dk, got * 8 * j->words, hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(biedges, de, got, hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(counts, dc, got * 4, hipMemcpyDeviceToHost) != hipSuccess) {
+    if (copy_out(j, kmers, dk, got * 8 * j->words) != hipSuccess || copy_out(j, biedges, de, got) != hipSuccess ||
+        copy_out(j, counts, dc, got * 4) != hipSuccess) {
       (void)hipGetLastError();
       s = BRUNO_INTERNAL;
     }
@@ -2250,7 +2258,7 @@ bruno_status bruno_batch_get_nodes(bruno_job* j, uint32_t idx, bruno_node* buf, 
   const uint64_t got = j->hctr->n_out;
   if (s == BRUNO_CAPACITY) s = BRUNO_BAD_INPUT;
   if (s == BRUNO_OK && buf && got) {
-    if (hipMemcpy(buf, d, got * sizeof(bruno_node), hipMemcpyDeviceToHost) != hipSuccess) {
+    if (copy_out(j, buf, d, got * sizeof(bruno_node)) != hipSuccess) {
       (void)hipGetLastError();
       s = BRUNO_INTERNAL;
     }

@@ -277,6 +277,67 @@ bruno_status bruno_synth_batch_device(const bruno_synth_desc* descs, uint32_t n_
                                       uint64_t* d_offsets, void* stream);
 bruno_status bruno_synth_batch_host(const bruno_synth_desc* descs, uint32_t n_jobs, char* bases, uint64_t* offsets);
 
+/* ---- Request loop (serving) ---------------------------------------------
+ * A server admits independent assembly jobs as they arrive and runs them
+ * in batches on its worker streams: the oldest waiting request and every
+ * later waiting one with the same k and thresholds whose estimated HBM
+ * footprint fits the free budget form the next batch, so requests that
+ * arrive while others are on the device join the next batch and batches
+ * overlap.  A job whose table proves too small runs again on its own with
+ * four times the slots, and then with a table that cannot fill (two slots
+ * per base).  Each job's result equals what
+ * bruno_run_pipeline gives for it alone.  No counterpart in the reference
+ * (one assembly per MPI job); this serves its per-job outputs
+ * (.chain.fasta / .branch.fasta, compact_dbg_io.hpp:254-363). */
+typedef struct bruno_server bruno_server;
+
+typedef struct {
+  uint64_t hbm_budget;      /* HBM the server's batches may hold at once; 0 = 85% of free HBM */
+  uint32_t n_workers;       /* batches in flight, one stream each; 0 = 2                      */
+  uint32_t max_batch_jobs;  /* most jobs per batch; 0 = 64                                    */
+  int variant;              /* list ranking: 1 = list_rank_min_update, 0 = list_rank          */
+  int reserve_pool;         /* nonzero: reserve the budget in the memory pool at create       */
+} bruno_server_desc;
+
+typedef struct {
+  bruno_job_desc job;       /* k, thresholds; capacity = hash slots (0: one slot per base at
+                               load 1/2); stream ignored                                      */
+  const char* bases;        /* host reads, copied by bruno_server_submit ...                  */
+  const uint64_t* offsets;  /* ... n_reads + 1 byte offsets                                    */
+  uint64_t n_reads;
+  const bruno_synth_desc* synth; /* or, instead of reads: generate this workload on the device */
+} bruno_request;
+
+typedef struct {
+  int32_t status;           /* bruno_status of the job                                        */
+  uint32_t batch_jobs;      /* jobs in the batch that produced the result                     */
+  uint64_t batch_id;
+  uint64_t n_reads, n_instances, n_unitigs, n_unitig_nodes, n_branches;
+  uint64_t unitig_bytes, branch_bytes;
+  uint32_t iterations;      /* list-ranking rounds of this job                                */
+  uint32_t attempts;
+  double submit_s, start_s, end_s; /* seconds since bruno_server_create                       */
+} bruno_result_info;
+
+typedef struct {
+  uint64_t submitted, jobs_done, jobs_failed, batches, retries, max_batch_jobs;
+  uint64_t queued, reserved_now, peak_reserved, hbm_budget;
+} bruno_server_stats;
+
+bruno_status bruno_server_create(const bruno_server_desc* desc, bruno_server** out);
+bruno_status bruno_server_submit(bruno_server* s, const bruno_request* req, uint64_t* ticket);
+bruno_status bruno_server_poll(bruno_server* s, uint64_t ticket, int* done);
+/* blocks until the job has finished (successfully or not) */
+bruno_status bruno_server_wait(bruno_server* s, uint64_t ticket, bruno_result_info* info);
+/* two-call getters of a finished job's .chain.fasta / .branch.fasta bytes */
+bruno_status bruno_server_get_unitigs(bruno_server* s, uint64_t ticket, char* buf, uint64_t* n);
+bruno_status bruno_server_get_branch_fasta(bruno_server* s, uint64_t ticket, char* buf, uint64_t* n);
+/* forget a finished job's result */
+bruno_status bruno_server_release(bruno_server* s, uint64_t ticket);
+bruno_status bruno_server_get_stats(bruno_server* s, bruno_server_stats* out);
+/* finishes every submitted job, then stops the workers */
+bruno_status bruno_server_destroy(bruno_server* s);
+
 #ifdef __cplusplus
 }
 #endif
