@@ -27,93 +27,203 @@ namespace {
 /* ================================================================== */
 /* filter + classify (one thread per table slot)                        */
 /* ================================================================== */
-/* Classify a surviving node and write its record in place. */
+/* Classify a surviving node (IsBranchPoint / IsChainNode,
+ * debruijn_graph_filters.hpp:142-220). */
 template <typename KT>
-__device__ __forceinline__ uint8_t finish_node(Slot<KT>* p, const kops<KT>& ops, KT k, const uint16_t (&c)[9]) {
+__device__ __forceinline__ NodeRec make_node(const kops<KT>& ops, KT k, const uint16_t (&c)[9]) {
   bool pal = ops.is_rc_palindrome(k) || kplus1_pal<KT>(ops, k, c);
   int ni = n_in_edges(c), no = n_out_edges(c);
-  uint8_t cls = (ni >= 2 || no >= 2 || pal) ? CLS_BRANCH : CLS_CHAIN;
   NodeRec rec;
 #pragma unroll
   for (int q = 0; q < 9; ++q) rec.c[q] = c[q];
-  rec.cls = cls;
+  rec.cls = (ni >= 2 || no >= 2 || pal) ? CLS_BRANCH : CLS_CHAIN;
   rec.pad = 0;
   rec.idx = 0xFFFFFFFFu;
+  return rec;
+}
+
+/* Node lists written by the filter: chain and branch slots (the index in
+ * its list goes into the node record) and, in a batch, each node's job. */
+struct NodeLists {
+  uint64_t* chain_slot;
+  uint64_t* branch_slot;
+  uint32_t* chain_job;
+  uint32_t* branch_job;
+  uint64_t cap; /* capacity of each list */
+  const uint64_t* seg;
+  uint32_t n_jobs;
+};
+
+/* Write the record in place (sub area of the slot) with its list index,
+ * mark the slot present and append it to its list at position pos. */
+template <typename KT>
+__device__ __forceinline__ void store_node(Slot<KT>* T, uint64_t s, NodeRec rec, unsigned long long pos,
+                                           const NodeLists& nl, Counters* ctr) {
+  if (pos >= nl.cap || pos > 0x7FFFFFFFULL) {
+    atomicOr(&ctr->err, ERR_LIST_FULL);
+    return;
+  }
+  rec.idx = (uint32_t)pos;
+  Slot<KT>* p = T + s;
   *(NodeRec*)&p->sub[0] = rec;
   uint64_t* fw = SlotOps<KT>::flag_word(p);
   *fw = *fw | F_PRESENT;
-  return cls;
+  const bool chain = rec.cls == CLS_CHAIN;
+  (chain ? nl.chain_slot : nl.branch_slot)[pos] = s;
+  if (nl.n_jobs > 1) (chain ? nl.chain_job : nl.branch_job)[pos] = job_of(nl.seg, nl.n_jobs, s);
 }
 
 constexpr int FILTER_SPT = 4; /* slots per thread per reservation */
 
-/* Main filter pass over the table: slots whose k2 entries all sit in the
- * slot (the common case) are filtered here with register-only lists; slots
- * with spilled entries are queued for k_filter_spilled. */
+/* Flag word and k-mer count (sum of the in-slot k2 counts) of one slot,
+ * read with 16-byte streaming loads. */
+typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void scan_slot(const Slot<uint64_t>* p, uint64_t& fw, uint64_t& kf) {
+  const u64x2* q = (const u64x2*)p;
+  const u64x2 a = __builtin_nontemporal_load(q), b = __builtin_nontemporal_load(q + 1);
+  fw = a.x;
+  const uint64_t v[3] = {a.y, b.x, b.y};
+  kf = 0;
+#pragma unroll
+  for (int j = 0; j < 3; ++j) kf += v[j] != EMPTY64 ? (v[j] & 0xFFFFFFFFFFFFULL) : 0;
+}
+__device__ __forceinline__ void scan_slot(const Slot<u128>* p, uint64_t& fw, uint64_t& kf) {
+  const u64x2* q = (const u64x2*)p;
+  const u64x2 a = __builtin_nontemporal_load(q), b = __builtin_nontemporal_load(q + 1);
+  const u64x2 c = __builtin_nontemporal_load(q + 2), d = __builtin_nontemporal_load(q + 3);
+  fw = a.x;
+  const uint64_t v[5] = {b.y, c.x, c.y, d.x, d.y};
+  kf = 0;
+#pragma unroll
+  for (int j = 0; j < 5; ++j) kf += v[j] != EMPTY64 ? (v[j] & 0xFFFFFFFFFFFFULL) : 0;
+}
+
+/* Output indices staged in LDS and appended to a global list with one
+ * atomic per flush (the filter scan keeps a few percent of the slots, and a
+ * global atomic per tile would serialise on the list counter). */
+template <int CAP>
+struct LdsList {
+  uint64_t v[CAP];
+  uint32_t n;
+  unsigned long long base;
+};
+template <int CAP>
+__device__ __forceinline__ void lds_flush(LdsList<CAP>& L, uint64_t* out, uint64_t out_cap, unsigned long long* counter,
+                                          uint32_t* err) {
+  __syncthreads();
+  const uint32_t n = L.n;
+  if (n == 0) return; /* uniform: every thread read L.n after the barrier */
+  if (threadIdx.x == 0) L.base = atomicAdd(counter, (unsigned long long)n);
+  __syncthreads();
+  const unsigned long long b = L.base;
+  for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
+    if (b + i < out_cap) out[b + i] = L.v[i];
+    else atomicOr(err, ERR_LIST_FULL);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) L.n = 0;
+  __syncthreads();
+}
+
+constexpr int SCAN_LDS = 2048; /* staged candidates per block */
+constexpr int SPILL_LDS = 1536;
+
+/* Pass 1 of the filter: stream the table once and list the slots that can
+ * survive.  Every edge count is at most the k-mer count, so a k-mer seen
+ * fewer than k1_lo times has both edges of every k2-mer out of range (flag
+ * 3, debruijn_graph_map.hpp:1573) and nothing of it survives; in the
+ * unthresholded mode every k-mer is a node.  Slots with spilled k2 entries
+ * go to their own list (their count needs the side table). */
 template <typename KT>
-__global__ __launch_bounds__(BLOCK) void k_filter(Slot<KT>* T, uint64_t cap, int K, int thresholding, Thresh th,
-                                                  uint64_t* present, uint64_t present_cap, uint64_t* spilled,
-                                                  uint64_t spilled_cap, Counters* ctr) {
-  __shared__ ReserveSmem sm;
-  kops<KT> ops(K);
-  unsigned long long nb = 0, nc = 0;
-  const uint64_t tile = (uint64_t)BLOCK * FILTER_SPT;
-  const uint64_t stride = (uint64_t)gridDim.x * tile;
-  for (uint64_t base = (uint64_t)blockIdx.x * tile; base < cap; base += stride) {
-    uint32_t kept = 0, spl = 0;
+__global__ __launch_bounds__(BLOCK) void k_filter_scan(const Slot<KT>* T, uint64_t cap, int thresholding,
+                                                       uint64_t t1lo, uint64_t* cand, uint64_t cand_cap,
+                                                       uint64_t* spilled, uint64_t spilled_cap, Counters* ctr) {
+  constexpr uint32_t TILE = BLOCK * FILTER_SPT;
+  static_assert(SCAN_LDS > (int)TILE && SPILL_LDS > (int)TILE, "a tile must fit the staging lists");
+  __shared__ LdsList<SCAN_LDS> Lc;
+  __shared__ LdsList<SPILL_LDS> Ls;
+  if (threadIdx.x == 0) {
+    Lc.n = 0;
+    Ls.n = 0;
+  }
+  __syncthreads();
+  const uint64_t stride = (uint64_t)gridDim.x * TILE;
+  for (uint64_t base = (uint64_t)blockIdx.x * TILE; base < cap; base += stride) {
+    uint64_t fw[FILTER_SPT], kf[FILTER_SPT];
 #pragma unroll
     for (int q = 0; q < FILTER_SPT; ++q) {
       const uint64_t s = base + (uint64_t)q * BLOCK + threadIdx.x;
-      if (s >= cap) continue;
-      Slot<KT>* p = T + s;
-      if (!SlotOps<KT>::occupied(p)) continue;
-      if (SlotOps<KT>::flags(p) & F_SPILL) {
-        spl |= 1u << q;
-        continue;
-      }
-      K2L<Slot<KT>::NSUB> L;
-      uint64_t kf = sub_entries<KT, Slot<KT>::NSUB>(p, L);
-      /* every edge count is at most the k-mer count, so a k-mer seen fewer
-       * than k1_lo times has both edges of every k2-mer out of range
-       * (flag 3, debruijn_graph_map.hpp:1573): nothing survives */
-      if (thresholding && kf < th.t1lo) continue;
-      KT k = SlotOps<KT>::kmer(p);
-      uint16_t c[9];
-      bool keep;
-      if (thresholding) {
-        keep = p2_counts<KT, Slot<KT>::NSUB>(ops, k, L, th, c);
-      } else {
-        p1_counts(L, c);
-        keep = true;
-      }
-      if (!keep) continue;
-      uint8_t cls = finish_node<KT>(p, ops, k, c);
-      nb += (cls == CLS_BRANCH);
-      nc += (cls == CLS_CHAIN);
-      kept |= 1u << q;
+      fw[q] = EMPTY64;
+      kf[q] = 0;
+      if (s < cap) scan_slot(T + s, fw[q], kf[q]);
     }
-    unsigned long long pos = block_reserve((uint32_t)__popc(kept), &ctr->n_present, sm);
+    uint32_t keep = 0, spl = 0;
 #pragma unroll
-    for (int q = 0; q < FILTER_SPT; ++q)
-      if (kept & (1u << q)) {
-        if (pos < present_cap) present[pos] = base + (uint64_t)q * BLOCK + threadIdx.x;
-        else atomicOr(&ctr->err, ERR_LIST_FULL);
-        ++pos;
-      }
-    if (__syncthreads_or(spl != 0)) {
-      unsigned long long sp = block_reserve((uint32_t)__popc(spl), &ctr->n_spill, sm);
+    for (int q = 0; q < FILTER_SPT; ++q) {
+      if (fw[q] == EMPTY64) continue;
+      if (fw[q] & F_SPILL) spl |= 1u << q;
+      else if (!thresholding || kf[q] >= t1lo) keep |= 1u << q;
+    }
+    if (keep) {
+      uint32_t p = atomicAdd(&Lc.n, (uint32_t)__popc(keep));
 #pragma unroll
       for (int q = 0; q < FILTER_SPT; ++q)
-        if (spl & (1u << q)) {
-          if (sp < spilled_cap) spilled[sp] = base + (uint64_t)q * BLOCK + threadIdx.x;
-          else atomicOr(&ctr->err, ERR_LIST_FULL);
-          ++sp;
-        }
+        if (keep & (1u << q)) Lc.v[p++] = base + (uint64_t)q * BLOCK + threadIdx.x;
     }
+    if (spl) {
+      uint32_t p = atomicAdd(&Ls.n, (uint32_t)__popc(spl));
+#pragma unroll
+      for (int q = 0; q < FILTER_SPT; ++q)
+        if (spl & (1u << q)) Ls.v[p++] = base + (uint64_t)q * BLOCK + threadIdx.x;
+    }
+    /* every thread reads both counts between two barriers, so all take the
+     * same branch; then keep room for one more tile in each list */
+    __syncthreads();
+    const uint32_t nc = Lc.n, ns = Ls.n;
+    __syncthreads();
+    if (nc > SCAN_LDS - TILE) lds_flush(Lc, cand, cand_cap, &ctr->n_cand, &ctr->err);
+    if (ns > SPILL_LDS - TILE) lds_flush(Ls, spilled, spilled_cap, &ctr->n_spill, &ctr->err);
   }
-  block_add(nb, &ctr->n_branch);
-  __syncthreads();
-  block_add(nc, &ctr->n_chain);
+  lds_flush(Lc, cand, cand_cap, &ctr->n_cand, &ctr->err);
+  lds_flush(Ls, spilled, spilled_cap, &ctr->n_spill, &ctr->err);
+}
+
+/* Pass 2: local_insert_by_freqencies on the candidate slots (all k2
+ * entries in the slot), node classification, record written in place and
+ * the node appended to the chain or branch list. */
+template <typename KT>
+__global__ __launch_bounds__(BLOCK) void k_filter_cand(Slot<KT>* T, const uint64_t* cand, uint64_t n_cand, int K,
+                                                       int thresholding, Thresh th, NodeLists nl, Counters* ctr) {
+  __shared__ ReserveSmem sm;
+  __shared__ ReserveSmem sm2;
+  kops<KT> ops(K);
+  const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+  for (uint64_t base = (uint64_t)blockIdx.x * BLOCK; base < n_cand; base += stride) {
+    const uint64_t i = base + threadIdx.x;
+    bool kept = false;
+    uint64_t s = 0;
+    NodeRec rec;
+    rec.cls = 0;
+    if (i < n_cand) {
+      s = cand[i];
+      Slot<KT>* p = T + s;
+      K2L<Slot<KT>::NSUB> L;
+      sub_entries<KT, Slot<KT>::NSUB>(p, L);
+      KT k = SlotOps<KT>::kmer(p);
+      uint16_t c[9];
+      if (thresholding) {
+        kept = p2_counts<KT, Slot<KT>::NSUB>(ops, k, L, th, c);
+      } else {
+        p1_counts(L, c);
+        kept = true;
+      }
+      if (kept) rec = make_node<KT>(ops, k, c);
+    }
+    const bool ch = kept && rec.cls == CLS_CHAIN, br = kept && rec.cls == CLS_BRANCH;
+    unsigned long long pc = block_reserve(ch, &ctr->n_chain, sm);
+    unsigned long long pb = block_reserve(br, &ctr->n_branch, sm2);
+    if (kept) store_node<KT>(T, s, rec, ch ? pc : pb, nl, ctr);
+  }
 }
 
 /* Slots with spilled k2 entries: gather up to 25 entries from the slot and
@@ -121,68 +231,36 @@ __global__ __launch_bounds__(BLOCK) void k_filter(Slot<KT>* T, uint64_t cap, int
 template <typename KT>
 __global__ __launch_bounds__(BLOCK) void k_filter_spilled(Slot<KT>* T, const Spill* sp, uint64_t spcap, int K,
                                                           int thresholding, Thresh th, const uint64_t* spilled,
-                                                          uint64_t n_spilled, uint64_t* present, uint64_t present_cap,
-                                                          Counters* ctr) {
-  kops<KT> ops(K);
-  unsigned long long nb = 0, nc = 0;
-  const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
-  for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < n_spilled; i += stride) {
-    const uint64_t s = spilled[i];
-    Slot<KT>* p = T + s;
-    K2List L;
-    gather_k2<KT>(p, s, sp, spcap, L);
-    KT k = SlotOps<KT>::kmer(p);
-    uint16_t c[9];
-    bool keep;
-    if (thresholding) {
-      keep = p2_counts<KT, 25>(ops, k, L, th, c);
-    } else {
-      p1_counts(L, c);
-      keep = true;
-    }
-    if (!keep) continue;
-    uint8_t cls = finish_node<KT>(p, ops, k, c);
-    nb += (cls == CLS_BRANCH);
-    nc += (cls == CLS_CHAIN);
-    unsigned long long pos = atomicAdd(&ctr->n_present, 1ULL);
-    if (pos < present_cap) present[pos] = s;
-    else atomicOr(&ctr->err, ERR_LIST_FULL);
-  }
-  block_add(nb, &ctr->n_branch);
-  __syncthreads();
-  block_add(nc, &ctr->n_chain);
-}
-
-/* split present slots into branch and chain lists; chain index into slot.
- * In a batch, each node's job is the table segment holding its slot. */
-template <typename KT>
-__global__ __launch_bounds__(BLOCK) void k_split(Slot<KT>* T, const uint64_t* present, uint64_t n_present,
-                                                 uint64_t* chain_slot, uint64_t* branch_slot, const uint64_t* seg,
-                                                 uint32_t n_jobs, uint32_t* chain_job, uint32_t* branch_job,
-                                                 Counters* ctr) {
+                                                          uint64_t n_spilled, NodeLists nl, Counters* ctr) {
   __shared__ ReserveSmem sm;
   __shared__ ReserveSmem sm2;
+  kops<KT> ops(K);
   const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
-  for (uint64_t base = (uint64_t)blockIdx.x * BLOCK; base < n_present; base += stride) {
+  for (uint64_t base = (uint64_t)blockIdx.x * BLOCK; base < n_spilled; base += stride) {
     const uint64_t i = base + threadIdx.x;
-    uint8_t cls = 0;
+    bool kept = false;
     uint64_t s = 0;
-    if (i < n_present) {
-      s = present[i];
-      cls = ((NodeRec*)&T[s].sub[0])->cls;
+    NodeRec rec;
+    rec.cls = 0;
+    if (i < n_spilled) {
+      s = spilled[i];
+      Slot<KT>* p = T + s;
+      K2List L;
+      gather_k2<KT>(p, s, sp, spcap, L);
+      KT k = SlotOps<KT>::kmer(p);
+      uint16_t c[9];
+      if (thresholding) {
+        kept = p2_counts<KT, 25>(ops, k, L, th, c);
+      } else {
+        p1_counts(L, c);
+        kept = true;
+      }
+      if (kept) rec = make_node<KT>(ops, k, c);
     }
-    unsigned long long pc = block_reserve(cls == CLS_CHAIN, &ctr->lr_count, sm);
-    unsigned long long pb = block_reserve(cls == CLS_BRANCH, &ctr->lr_next, sm2);
-    const uint32_t job = (cls && n_jobs > 1) ? job_of(seg, n_jobs, s) : 0u;
-    if (cls == CLS_CHAIN) {
-      chain_slot[pc] = s;
-      if (n_jobs > 1) chain_job[pc] = job;
-      ((NodeRec*)&T[s].sub[0])->idx = (uint32_t)pc;
-    } else if (cls == CLS_BRANCH) {
-      branch_slot[pb] = s;
-      if (n_jobs > 1) branch_job[pb] = job;
-      ((NodeRec*)&T[s].sub[0])->idx = (uint32_t)pb;
-    }
+    const bool ch = kept && rec.cls == CLS_CHAIN, br = kept && rec.cls == CLS_BRANCH;
+    unsigned long long pc = block_reserve(ch, &ctr->n_chain, sm);
+    unsigned long long pb = block_reserve(br, &ctr->n_branch, sm2);
+    if (kept) store_node<KT>(T, s, rec, ch ? pc : pb, nl, ctr);
   }
 }
 
@@ -1022,8 +1100,7 @@ struct bruno_job {
   std::vector<uint64_t> job_cycle, job_isolated;
   std::vector<uint64_t> chain_start, fasta_off, branch_start; /* n_jobs + 1 each, after compact */
 
-  /* after filter */
-  uint64_t* present = nullptr;
+  /* after filter: the nodes are the chain list followed by the branch list */
   uint64_t n_present = 0, n_branch = 0, n_chain = 0, n_virt = 0;
   uint64_t* chain_slot = nullptr;
   uint64_t* branch_slot = nullptr;
@@ -1185,42 +1262,59 @@ bruno_status parse_count_t(bruno_job* j, const char* d_bases, const uint64_t* d_
 template <typename KT>
 bruno_status filter_t(bruno_job* j) {
   HIP_CHECK(hipMemsetAsync(&j->ctr->n_present, 0, 3 * sizeof(unsigned long long), j->stream));
-  /* present list: at most the distinct k-mers */
-  uint64_t n_distinct = j->hctr->n_distinct;
-  TRY(dev_alloc(j, (void**)&j->present, std::max<uint64_t>(1, n_distinct) * sizeof(uint64_t)));
-  /* slots with spilled entries: at most the side-table entries */
-  uint64_t n_spill_cap = std::max<uint64_t>(1, std::min<uint64_t>(j->hctr->n_k2, j->spcap));
-  uint64_t* spilled = nullptr;
-  TRY(dev_alloc(j, (void**)&spilled, n_spill_cap * sizeof(uint64_t)));
   HIP_CHECK(hipMemsetAsync(&j->ctr->n_spill, 0, sizeof(unsigned long long), j->stream));
-  hipLaunchKernelGGL((k_filter<KT>), dim3(grid_for((j->cap + FILTER_SPT - 1) / FILTER_SPT, BLOCK, 1u << 15)),
-                     dim3(BLOCK), 0, j->stream, (Slot<KT>*)j->table, j->cap, j->K, j->thresholding, j->th,
-                     j->present, std::max<uint64_t>(1, n_distinct), spilled, n_spill_cap, j->ctr);
+  HIP_CHECK(hipMemsetAsync(&j->ctr->n_cand, 0, sizeof(unsigned long long), j->stream));
+  /* candidates: at most the distinct k-mers; slots with spilled entries: at
+   * most the side-table entries */
+  const uint64_t n_distinct = std::max<uint64_t>(1, j->hctr->n_distinct);
+  const uint64_t n_spill_cap = std::max<uint64_t>(1, std::min<uint64_t>(j->hctr->n_k2, j->spcap));
+  uint64_t *cand = nullptr, *spilled = nullptr;
+  TRY(dev_alloc(j, (void**)&cand, n_distinct * sizeof(uint64_t)));
+  TRY(dev_alloc(j, (void**)&spilled, n_spill_cap * sizeof(uint64_t)));
+  hipLaunchKernelGGL((k_filter_scan<KT>), dim3(grid_for((j->cap + FILTER_SPT - 1) / FILTER_SPT, BLOCK, 1u << 14)),
+                     dim3(BLOCK), 0, j->stream, (const Slot<KT>*)j->table, j->cap, j->thresholding, j->th.t1lo, cand,
+                     n_distinct, spilled, n_spill_cap, j->ctr);
   TRY(sync_counters(j));
-  uint64_t n_spilled = j->hctr->n_spill;
-  if (n_spilled)
+  const uint64_t n_cand = j->hctr->n_cand, n_spilled = j->hctr->n_spill;
+  /* nodes: at most the candidates plus the spilled slots, in either list */
+  const uint64_t lcap = std::max<uint64_t>(1, n_cand + n_spilled);
+  TRY(dev_alloc(j, (void**)&j->chain_slot, lcap * sizeof(uint64_t)));
+  TRY(dev_alloc(j, (void**)&j->branch_slot, lcap * sizeof(uint64_t)));
+  if (j->n_jobs > 1) {
+    TRY(dev_alloc(j, (void**)&j->chain_job, lcap * sizeof(uint32_t)));
+    TRY(dev_alloc(j, (void**)&j->branch_job, lcap * sizeof(uint32_t)));
+  }
+  NodeLists nl{j->chain_slot, j->branch_slot, j->chain_job, j->branch_job, lcap, j->d_seg, j->n_jobs};
+  if (n_cand)
+    hipLaunchKernelGGL((k_filter_cand<KT>), dim3(grid_for(n_cand)), dim3(BLOCK), 0, j->stream, (Slot<KT>*)j->table,
+                       cand, n_cand, j->K, j->thresholding, j->th, nl, j->ctr);
+  if (n_spilled) {
+    /* slot order: neighbouring threads then read neighbouring slots */
+    uint64_t* sorted = nullptr;
+    TRY(dev_alloc(j, (void**)&sorted, n_spilled * sizeof(uint64_t)));
+    int bits = 1;
+    while (bits < 64 && (1ULL << bits) < j->cap) ++bits;
+    size_t tmp = 0;
+    HIP_CHECK(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp, spilled, sorted, (int)n_spilled, 0, bits, j->stream));
+    void* t = nullptr;
+    TRY(dev_alloc(j, &t, tmp));
+    HIP_CHECK(hipcub::DeviceRadixSort::SortKeys(t, tmp, spilled, sorted, (int)n_spilled, 0, bits, j->stream));
     hipLaunchKernelGGL((k_filter_spilled<KT>), dim3(grid_for(n_spilled)), dim3(BLOCK), 0, j->stream,
-                       (Slot<KT>*)j->table, (const Spill*)j->spill, j->spcap, j->K, j->thresholding, j->th, spilled,
-                       n_spilled, j->present, std::max<uint64_t>(1, n_distinct), j->ctr);
+                       (Slot<KT>*)j->table, (const Spill*)j->spill, j->spcap, j->K, j->thresholding, j->th, sorted,
+                       n_spilled, nl, j->ctr);
+    dev_free(j, t, tmp);
+    void* a = sorted;
+    dev_free(j, a, n_spilled * sizeof(uint64_t));
+  }
   TRY(sync_counters(j));
-  j->n_present = j->hctr->n_present;
+  void* fa = cand;
+  dev_free(j, fa, n_distinct * sizeof(uint64_t));
+  fa = spilled;
+  dev_free(j, fa, n_spill_cap * sizeof(uint64_t));
   j->n_branch = j->hctr->n_branch;
   j->n_chain = j->hctr->n_chain;
-  TRY(dev_alloc(j, (void**)&j->chain_slot, std::max<uint64_t>(1, j->n_chain) * sizeof(uint64_t)));
-  TRY(dev_alloc(j, (void**)&j->branch_slot, std::max<uint64_t>(1, j->n_branch) * sizeof(uint64_t)));
-  /* lr_count / lr_next reused as list counters */
-  j->hctr->lr_count = 0;
-  j->hctr->lr_next = 0;
-  HIP_CHECK(hipMemcpyAsync(&j->ctr->lr_count, &j->hctr->lr_count, 2 * sizeof(unsigned long long),
-                           hipMemcpyHostToDevice, j->stream));
-  if (j->n_jobs > 1) {
-    TRY(dev_alloc(j, (void**)&j->chain_job, std::max<uint64_t>(1, j->n_chain) * sizeof(uint32_t)));
-    TRY(dev_alloc(j, (void**)&j->branch_job, std::max<uint64_t>(1, j->n_branch) * sizeof(uint32_t)));
-  }
-  hipLaunchKernelGGL((k_split<KT>), dim3(grid_for(j->n_present)), dim3(BLOCK), 0, j->stream, (Slot<KT>*)j->table,
-                     j->present, j->n_present, j->chain_slot, j->branch_slot, j->d_seg, j->n_jobs, j->chain_job,
-                     j->branch_job, j->ctr);
-  return sync_counters(j);
+  j->n_present = j->n_branch + j->n_chain;
+  return BRUNO_OK;
 }
 
 template <typename KT>
@@ -1810,7 +1904,7 @@ bruno_status bruno_job_reset(bruno_job* j) {
   j->allocs = keep;
   j->device_bytes = persistent_bytes(j);
   j->peak_bytes = j->device_bytes;
-  j->present = j->chain_slot = j->branch_slot = nullptr;
+  j->chain_slot = j->branch_slot = nullptr;
   j->ckmer = j->vkmer = j->bkmer = j->out_k = j->out_rep = j->bsorted = nullptr;
   j->lr = nullptr;
   j->list[0] = j->list[1] = nullptr;
@@ -2217,6 +2311,8 @@ bruno_status bruno_batch_get_nodes(bruno_job* j, uint32_t idx, bruno_node* buf, 
   if (!valid_idx(j, idx) || !n) return BRUNO_BAD_INPUT;
   if (j->stage < ST_FILTERED) return BRUNO_STATE;
   const uint64_t np = j->n_present;
+  const uint64_t* lists[2] = {j->chain_slot, j->branch_slot};
+  const uint64_t lens[2] = {j->n_chain, j->n_branch};
   if (j->n_jobs == 1) {
     if (!buf) {
       *n = np;
@@ -2225,18 +2321,21 @@ bruno_status bruno_batch_get_nodes(bruno_job* j, uint32_t idx, bruno_node* buf, 
     if (*n < np) return BRUNO_BAD_INPUT;
     bruno_node* d = nullptr;
     TRY(dev_alloc(j, (void**)&d, std::max<uint64_t>(1, np) * sizeof(bruno_node)));
-    if (np) {
+    uint64_t at = 0;
+    for (int l = 0; l < 2; ++l) {
+      if (!lens[l]) continue;
       if (j->words == 1)
-        hipLaunchKernelGGL((k_get_nodes<uint64_t>), dim3(grid_for(np, BLOCK, 1u << 30)), dim3(BLOCK), 0, j->stream,
-                           (const Slot<uint64_t>*)j->table, j->present, np, d);
+        hipLaunchKernelGGL((k_get_nodes<uint64_t>), dim3(grid_for(lens[l], BLOCK, 1u << 30)), dim3(BLOCK), 0,
+                           j->stream, (const Slot<uint64_t>*)j->table, lists[l], lens[l], d + at);
       else
-        hipLaunchKernelGGL((k_get_nodes<u128>), dim3(grid_for(np, BLOCK, 1u << 30)), dim3(BLOCK), 0, j->stream,
-                           (const Slot<u128>*)j->table, j->present, np, d);
-      HIP_CHECK(hipMemcpyAsync(buf, d, np * sizeof(bruno_node), hipMemcpyDeviceToHost, j->stream));
+        hipLaunchKernelGGL((k_get_nodes<u128>), dim3(grid_for(lens[l], BLOCK, 1u << 30)), dim3(BLOCK), 0, j->stream,
+                           (const Slot<u128>*)j->table, lists[l], lens[l], d + at);
+      at += lens[l];
     }
-    HIP_CHECK(hipStreamSynchronize(j->stream));
+    hipError_t e = np ? copy_out(j, buf, d, np * sizeof(bruno_node)) : hipSuccess;
     void* a = d;
     dev_free(j, a, std::max<uint64_t>(1, np) * sizeof(bruno_node));
+    if (e != hipSuccess) return BRUNO_INTERNAL;
     *n = np;
     return BRUNO_OK;
   }
@@ -2246,22 +2345,20 @@ bruno_status bruno_batch_get_nodes(bruno_job* j, uint32_t idx, bruno_node* buf, 
   if (buf) TRY(dev_alloc(j, (void**)&d, std::max<uint64_t>(1, cap_out) * sizeof(bruno_node)));
   j->hctr->n_out = 0;
   HIP_CHECK(hipMemcpyAsync(&j->ctr->n_out, &j->hctr->n_out, 8, hipMemcpyHostToDevice, j->stream));
-  if (np) {
+  for (int l = 0; l < 2; ++l) {
+    if (!lens[l]) continue;
     if (j->words == 1)
-      hipLaunchKernelGGL((k_get_nodes_range<uint64_t>), dim3(grid_for(np)), dim3(BLOCK), 0, j->stream,
-                         (const Slot<uint64_t>*)j->table, j->present, np, s0, s1, d, cap_out, j->ctr);
+      hipLaunchKernelGGL((k_get_nodes_range<uint64_t>), dim3(grid_for(lens[l])), dim3(BLOCK), 0, j->stream,
+                         (const Slot<uint64_t>*)j->table, lists[l], lens[l], s0, s1, d, cap_out, j->ctr);
     else
-      hipLaunchKernelGGL((k_get_nodes_range<u128>), dim3(grid_for(np)), dim3(BLOCK), 0, j->stream,
-                         (const Slot<u128>*)j->table, j->present, np, s0, s1, d, cap_out, j->ctr);
+      hipLaunchKernelGGL((k_get_nodes_range<u128>), dim3(grid_for(lens[l])), dim3(BLOCK), 0, j->stream,
+                         (const Slot<u128>*)j->table, lists[l], lens[l], s0, s1, d, cap_out, j->ctr);
   }
   bruno_status s = sync_counters(j);
   const uint64_t got = j->hctr->n_out;
   if (s == BRUNO_CAPACITY) s = BRUNO_BAD_INPUT;
   if (s == BRUNO_OK && buf && got) {
-    if (copy_out(j, buf, d, got * sizeof(bruno_node)) != hipSuccess) {
-      (void)hipGetLastError();
-      s = BRUNO_INTERNAL;
-    }
+    if (copy_out(j, buf, d, got * sizeof(bruno_node)) != hipSuccess) s = BRUNO_INTERNAL;
   }
   if (s == BRUNO_BAD_INPUT) {
     j->hctr->err = 0;

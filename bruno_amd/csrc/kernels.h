@@ -68,6 +68,7 @@ struct Counters {
   unsigned long long n_present, n_branch, n_chain, n_virt;
   unsigned long long lr_count, lr_next, lr_cycle, lr_conflict;
   unsigned long long n_cycle, n_isolated, n_out, n_spill;
+  unsigned long long n_cand;
   uint32_t err;
   uint32_t pad;
 };
