@@ -34,10 +34,31 @@ HBM_PEAK_GBS = 8000.0
 POOL_FRACTION = 0.9
 
 
+def filter_bytes(stats, k):
+    """Algorithmic bytes of the filter: every distinct (k-mer, biedge) pair
+    read once and every kept k-mer's node record written once."""
+    r, w = (16, 32) if k <= 31 else (24, 48)
+    return stats["n_k2"] * r + stats["n_nodes"] * w
+
+
 def count_bytes(n_bases, n_inst):
     """Algorithmic bytes of the count kernel: every base read once plus one
     random read-modify-write of a 64-byte line per k-mer instance."""
     return n_bases * 1 + n_inst * 2 * 64
+
+
+def measured_traffic(config, scale, kernel):
+    """HBM bytes per launch of a kernel on this workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs (profiles/hbm_traffic.json), or
+    None when no counter run covers the workload."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "hbm_traffic.json")) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("workload") != config or float(t.get("scale", -1)) != float(scale):
+        return None
+    return t["bytes_per_launch"].get(kernel)
 
 
 def capacity_for(synth, k, load_factor=0.5):
@@ -86,6 +107,9 @@ def main():
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--rate", type=float, default=40.0, help="config5: arrivals per second")
     ap.add_argument("--workers", type=int, default=2, help="config5: batches in flight")
+    ap.add_argument("--autotune", action="store_true",
+                    help="tune load factor and launch knobs on this workload first (untimed)")
+    ap.add_argument("--tuning", default="", help="apply a tuning saved by bruno_amd.autotune")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the timed steps, copy 4 GiB device-to-device once (a plain copy kernel of "
                          "known size, to scale FETCH_SIZE / WRITE_SIZE counters)")
@@ -117,7 +141,18 @@ def main():
     synth_reads_device(s, bases.ptr, offs.ptr)
     synchronize()
 
-    cap = capacity_for(s, k, args.load_factor)
+    lf = args.load_factor
+    tuned = None
+    if args.tuning:
+        from bruno_amd.autotune import apply
+        tuned = apply(args.tuning)
+        lf = tuned["load_factor"]
+    if args.autotune:
+        from bruno_amd.autotune import tune
+        tuned = tune(k, cfg["thresholds"], bases.ptr, offs.ptr, s.n_reads, lambda x: capacity_for(s, k, x),
+                     log=lambda *a: print("autotune:", *a, file=sys.stderr, flush=True))
+        lf = tuned["load_factor"]
+    cap = capacity_for(s, k, lf)
     job = bruno_amd.GraphJob(k, cfg["thresholds"], capacity=cap)
     times, phases = [], []
     stats = None
@@ -160,7 +195,7 @@ def main():
                            s.n_families, 100 * s.repeat_fraction, k),
             "k": k, "n_reads": s.n_reads, "read_len": s.read_len, "genome_bp": s.genome_len,
             "k_mer_instances": n_inst, "list_rank": "min_update" if args.variant else "list_rank",
-            "load_factor": args.load_factor,
+            "load_factor": lf,
         },
         "latency_ms": {"p50": float(np.percentile(times, 50) * 1e3), "p99": float(np.percentile(times, 99) * 1e3)},
         "phases_ms": ph,
@@ -174,9 +209,24 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(args.config, args.scale, "k_parse") if lf == 0.5 else None,
+            "bytes_per_unit": "1 B per base + 128 B per k-mer instance (one 64-byte slot read and write)",
+        },
+        "roofline_filter": {
+            "kernel": "filter phase (table scan + candidate and spilled-slot passes)",
+            "bound": "hbm",
+            "achieved": filter_bytes(stats, k) / (ph["filter_ms"] * 1e-3) / 1e9,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": filter_bytes(stats, k) / (ph["filter_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "table_stream_gbs": stats["capacity"] * (32 if k <= 31 else 64) / (ph["filter_ms"] * 1e-3) / 1e9,
+            "bytes_per_unit": "16 B per distinct k2-mer read (24 B for k > 31) + 32 B per kept k-mer written (48 B)",
         },
     }
+    if tuned is not None:
+        from bruno_amd.autotune import get_tuning
+        out["tuning"] = {"load_factor": lf, "knobs": get_tuning(),
+                         "source": "autotuned on this workload" if args.autotune else args.tuning}
     job.close()
     bases.free()
     offs.free()

@@ -44,6 +44,11 @@ class SynthDesc(ctypes.Structure):
                 ("indel_rate", ctypes.c_double)]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("parse_block", ctypes.c_uint32), ("parse_grid", ctypes.c_uint32),
+                ("scan_grid", ctypes.c_uint32), ("ruler_sample", ctypes.c_uint32)]
+
+
 class ServerDesc(ctypes.Structure):
     _fields_ = [("hbm_budget", ctypes.c_uint64), ("n_workers", ctypes.c_uint32),
                 ("max_batch_jobs", ctypes.c_uint32), ("variant", ctypes.c_int), ("reserve_pool", ctypes.c_int)]
@@ -93,7 +98,7 @@ EXPORTS = (
     "bruno_synth_batch_host", "bruno_pool_reserve", "bruno_pool_trim", "bruno_pool_stats",
     "bruno_server_create", "bruno_server_submit", "bruno_server_poll", "bruno_server_wait",
     "bruno_server_get_unitigs", "bruno_server_get_branch_fasta", "bruno_server_release",
-    "bruno_server_get_stats", "bruno_server_destroy",
+    "bruno_server_get_stats", "bruno_server_destroy", "bruno_set_tuning", "bruno_get_tuning",
 )
 
 _LIB = None
@@ -157,6 +162,8 @@ def lib():
         L.bruno_server_release.argtypes = [vp, u64]
         L.bruno_server_get_stats.argtypes = [vp, ctypes.POINTER(ServerStats)]
         L.bruno_server_destroy.argtypes = [vp]
+        L.bruno_set_tuning.argtypes = [ctypes.POINTER(Tuning)]
+        L.bruno_get_tuning.argtypes = [ctypes.POINTER(Tuning)]
         L.bruno_dev_alloc.argtypes = [u64, ctypes.POINTER(vp)]
         L.bruno_dev_free.argtypes = [vp]
         L.bruno_dev_copy.argtypes = [vp, vp, u64, i]

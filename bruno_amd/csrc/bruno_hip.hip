@@ -7,6 +7,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <mutex>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -511,14 +512,15 @@ __global__ __launch_bounds__(BLOCK) void k_lr_recv(LRNode* lr, const uint32_t* l
  * (chain ends plus sampled nodes) walk to the next ruler, heads walk the
  * ruler graph, then every node reads its two chain ends off its rulers. */
 constexpr uint32_t NONE32 = 0xFFFFFFFFu;
-constexpr uint32_t RULER_SAMPLE = 64;
 constexpr uint32_t WALK_MAX = 1u << 16;
 constexpr uint32_t ERR_NOT_WELL_FORMED = 16;
 constexpr uint32_t ERR_WALK = 32;
 
 __device__ __forceinline__ uint32_t end_bit(int e) { return e ? END_OUT : 0u; }
 
-__global__ __launch_bounds__(BLOCK) void k_lr_check(const LRNode* lr, uint64_t n_chain, uint8_t* ruler, Counters* ctr) {
+/* ruler_mask + 1 = sampling period of the rulers (a power of two) */
+__global__ __launch_bounds__(BLOCK) void k_lr_check(const LRNode* lr, uint64_t n_chain, uint8_t* ruler,
+                                                    uint64_t ruler_mask, Counters* ctr) {
   const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
   unsigned long long bad = 0;
   for (uint64_t t = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; t < n_chain; t += stride) {
@@ -540,7 +542,7 @@ __global__ __launch_bounds__(BLOCK) void k_lr_check(const LRNode* lr, uint64_t n
       const LRNode o = lr[j];
       if (o.d[e] != 1u || o.p[e] != ((uint32_t)t | end_bit(E))) ++bad;
     }
-    bool sampled = (fmix64(t * 0x9e3779b97f4a7c15ULL + 0x1234567ULL) % RULER_SAMPLE) == 0;
+    bool sampled = (fmix64(t * 0x9e3779b97f4a7c15ULL + 0x1234567ULL) & ruler_mask) == 0;
     ruler[t] = (head || sampled) ? 1 : 0;
   }
   block_add(bad, &ctr->lr_conflict);
@@ -1138,6 +1140,15 @@ struct bruno_job {
 
 namespace {
 
+/* Process-wide tuning knobs (bruno_set_tuning). */
+std::mutex g_tuning_mu;
+/* defaults: the autotuner's picks on the 100 Mbp / 33 M-read workload */
+bruno_tuning g_tuning = {256, 65536, 32768, 64};
+bruno_tuning tuning() {
+  std::lock_guard<std::mutex> g(g_tuning_mu);
+  return g_tuning;
+}
+
 /* All device memory comes from the per-device pool. */
 hipError_t pool_alloc(void** p, uint64_t bytes) { return DevicePool::current().alloc(p, bytes); }
 void pool_free(void* p) { DevicePool::current().release(p); }
@@ -1253,9 +1264,15 @@ bruno_status parse_count_t(bruno_job* j, const char* d_bases, const uint64_t* d_
   a.spill = j->spill;
   a.spcap = j->spcap;
   a.ctr = j->ctr;
-  uint64_t waves = (n_reads + 3) / 4;
-  unsigned g = grid_for(waves, 1, 8192);
-  hipLaunchKernelGGL((k_parse<KT, 2>), dim3(g), dim3(BLOCK), 0, j->stream, a);
+  const bruno_tuning t = tuning();
+  const unsigned wpb = t.parse_block / 64;
+  const uint64_t waves = std::min<uint64_t>(n_reads, t.parse_grid);
+  const unsigned g = (unsigned)std::max<uint64_t>(1, (waves + wpb - 1) / wpb);
+  switch (t.parse_block) {
+    case 128: hipLaunchKernelGGL((k_parse<KT, 2, 128>), dim3(g), dim3(128), 0, j->stream, a); break;
+    case 512: hipLaunchKernelGGL((k_parse<KT, 2, 512>), dim3(g), dim3(512), 0, j->stream, a); break;
+    default: hipLaunchKernelGGL((k_parse<KT, 2, 256>), dim3(g), dim3(256), 0, j->stream, a); break;
+  }
   return BRUNO_OK;
 }
 
@@ -1271,7 +1288,8 @@ bruno_status filter_t(bruno_job* j) {
   uint64_t *cand = nullptr, *spilled = nullptr;
   TRY(dev_alloc(j, (void**)&cand, n_distinct * sizeof(uint64_t)));
   TRY(dev_alloc(j, (void**)&spilled, n_spill_cap * sizeof(uint64_t)));
-  hipLaunchKernelGGL((k_filter_scan<KT>), dim3(grid_for((j->cap + FILTER_SPT - 1) / FILTER_SPT, BLOCK, 1u << 14)),
+  hipLaunchKernelGGL((k_filter_scan<KT>),
+                     dim3(grid_for((j->cap + FILTER_SPT - 1) / FILTER_SPT, BLOCK, tuning().scan_grid)),
                      dim3(BLOCK), 0, j->stream, (const Slot<KT>*)j->table, j->cap, j->thresholding, j->th.t1lo, cand,
                      n_distinct, spilled, n_spill_cap, j->ctr);
   TRY(sync_counters(j));
@@ -1383,7 +1401,8 @@ bruno_status list_rank_fast(bruno_job* j, bool* used) {
   j->hctr->lr_cycle = 0;
   HIP_CHECK(hipMemcpyAsync(&j->ctr->lr_count, &j->hctr->lr_count, 4 * sizeof(unsigned long long),
                            hipMemcpyHostToDevice, j->stream));
-  hipLaunchKernelGGL(k_lr_check, dim3(grid_for(nc)), dim3(BLOCK), 0, j->stream, j->lr, nc, ruler, j->ctr);
+  hipLaunchKernelGGL(k_lr_check, dim3(grid_for(nc)), dim3(BLOCK), 0, j->stream, j->lr, nc, ruler,
+                     (uint64_t)tuning().ruler_sample - 1, j->ctr);
   TRY(sync_counters(j));
   bool well_formed = j->hctr->lr_conflict == 0;
   if (!well_formed) {
@@ -2557,6 +2576,23 @@ bruno_status bruno_dev_free(void* p) {
     return BRUNO_INTERNAL;
   }
   pool_free(p);
+  return BRUNO_OK;
+}
+
+bruno_status bruno_set_tuning(const bruno_tuning* t) {
+  if (!t) return BRUNO_BAD_INPUT;
+  const bool pow2 = t->ruler_sample >= 8 && t->ruler_sample <= 4096 && !(t->ruler_sample & (t->ruler_sample - 1));
+  if ((t->parse_block != 128 && t->parse_block != 256 && t->parse_block != 512) || t->parse_grid < 64 ||
+      t->scan_grid < 1 || !pow2)
+    return BRUNO_BAD_INPUT;
+  std::lock_guard<std::mutex> g(g_tuning_mu);
+  g_tuning = *t;
+  return BRUNO_OK;
+}
+
+bruno_status bruno_get_tuning(bruno_tuning* t) {
+  if (!t) return BRUNO_BAD_INPUT;
+  *t = tuning();
   return BRUNO_OK;
 }
 

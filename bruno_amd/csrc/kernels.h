@@ -216,10 +216,17 @@ struct ParseArgs {
   Counters* ctr;
 };
 
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
 /* MODE 0: count records per read; MODE 1: write records; MODE 2: count into
- * the table. */
-template <typename KT, int MODE>
-__global__ __launch_bounds__(BLOCK) void k_parse(ParseArgs a) {
+ * the table.  BS is the workgroup size (the work is per wave, so any
+ * multiple of 64 works; it is a tuning knob). */
+template <typename KT, int MODE, int BS = BLOCK>
+__global__ __launch_bounds__(BS) void k_parse(ParseArgs a) {
   const int lane = threadIdx.x & 63;
   const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
@@ -318,13 +325,15 @@ __global__ __launch_bounds__(BLOCK) void k_parse(ParseArgs a) {
   }
   if (MODE == 2 && a.n_jobs > 1 && lane == 0 && j_inst) atomicAdd(&a.job_inst[cj], j_inst);
   if (MODE == 2) {
-    block_add(my_inst, &a.ctr->n_instances);
-    __syncthreads();
-    block_add(my_drop, &a.ctr->n_dropped);
-    __syncthreads();
-    block_add(my_new, &a.ctr->n_distinct);
-    __syncthreads();
-    block_add(my_k2, &a.ctr->n_k2);
+    /* one atomic per wave and counter */
+    const unsigned long long si = wave_sum(my_inst), sd = wave_sum(my_drop), sn = wave_sum(my_new),
+                             sk = wave_sum(my_k2);
+    if (lane == 0) {
+      if (si) atomicAdd(&a.ctr->n_instances, si);
+      if (sd) atomicAdd(&a.ctr->n_dropped, sd);
+      if (sn) atomicAdd(&a.ctr->n_distinct, sn);
+      if (sk) atomicAdd(&a.ctr->n_k2, sk);
+    }
   }
 }
 

@@ -240,6 +240,21 @@ bruno_status bruno_dev_sync(void);
  * below counts as used) */
 bruno_status bruno_dev_mem_info(uint64_t* free_bytes, uint64_t* total_bytes);
 
+/* ---- Tuning knobs ---------------------------------------------------------
+ * Process-wide launch parameters, picked by the autotuner
+ * (bruno_amd/autotune.py) on the box that runs the work.  They change speed
+ * only, never results.  No counterpart in the reference. */
+typedef struct {
+  uint32_t parse_block;    /* workgroup size of the parse + count kernel: 128, 256 or 512    */
+  uint32_t parse_grid;     /* most waves of the parse + count launch (one read per wave)     */
+  uint32_t scan_grid;      /* most workgroups of the filter's table scan                     */
+  uint32_t ruler_sample;   /* list ranking: one sampled ruler per this many chain nodes on
+                              average (power of two, 8..4096): the walk length between
+                              rulers and the size of the contracted list                    */
+} bruno_tuning;
+bruno_status bruno_set_tuning(const bruno_tuning* t);
+bruno_status bruno_get_tuning(bruno_tuning* t);
+
 /* ---- Device memory pool --------------------------------------------------
  * Every device allocation of the library (job tables, phase buffers,
  * bruno_dev_alloc) is a block of a per-device pool, so a steady-state run

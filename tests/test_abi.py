@@ -37,3 +37,16 @@ def test_host_generator_is_deterministic():
     b = bruno_amd.synth_reads_host(c)
     assert (a[0] == b[0]).all() and (a[1] == b[1]).all()
     assert set(bytes(a[0])) <= set(b"ACGT")
+
+
+def test_tuning_knobs_validated_on_the_host():
+    from bruno_amd.autotune import get_tuning, set_tuning
+    import pytest
+    before = get_tuning()
+    for bad in ({"parse_block": 192}, {"ruler_sample": 100}, {"ruler_sample": 4}, {"parse_grid": 1}):
+        with pytest.raises(bruno_amd.BrunoError):
+            set_tuning(**bad)
+    assert get_tuning() == before
+    set_tuning(parse_block=512, ruler_sample=128)
+    assert get_tuning()["parse_block"] == 512 and get_tuning()["ruler_sample"] == 128
+    set_tuning(**before)

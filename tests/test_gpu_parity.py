@@ -256,3 +256,25 @@ def test_kmer_only_threshold(case, variant):
     _, reads = random_reads(seed, G, n, L, err)
     b, off = pack(reads)
     compare_all(k, [3, 65536, 0, 65536, 0, 65536], b, off, variant=variant)
+
+
+@pytest.mark.parametrize("knobs", [dict(parse_block=128, parse_grid=64, ruler_sample=8, scan_grid=1),
+                                   dict(parse_block=512, parse_grid=65536, ruler_sample=4096, scan_grid=32768),
+                                   dict(parse_block=256, parse_grid=1000, ruler_sample=16, scan_grid=7)])
+def test_tuning_knobs_do_not_change_results(knobs):
+    """Every knob the autotuner may set leaves every parity object unchanged
+    (tiny grids included: every read, slot and ruler walk is still covered)."""
+    from bruno_amd.autotune import get_tuning, set_tuning
+    before = get_tuning()
+    try:
+        set_tuning(**knobs)
+        for case in (CASES[2], CASES[8]):
+            seed, G, n, L, err, k = case
+            _, reads = random_reads(seed, G, n, L, err)
+            b, off = pack(reads)
+            compare_all(k, thresholds_L(2), b, off, variant=1)
+        _, reads = random_reads(3, 30000, 3000, 150, 0.0)
+        b, off = pack(reads)
+        compare_all(31, None, b, off, variant=1, stages=False)
+    finally:
+        set_tuning(**before)
