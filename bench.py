@@ -76,7 +76,8 @@ def cpu_baseline(cfg, seconds_target=15.0, threads=None):
     s = cfg["synth"]
     threads = threads or min(16, os.cpu_count() or 1)
     # reads 0..S-1 of the generator are the first S reads of the full workload
-    n = max(1000, min(s.n_reads, int(2_000_000 * seconds_target / 15.0)))
+    # ~0.8 M reads per 15 s of work on the box's 16 host threads
+    n = max(1000, min(s.n_reads, int(800_000 * seconds_target / 15.0)))
     sample = SynthConfig(**{**s.__dict__, "n_reads": n})
     b, o = bruno_amd.synth_reads_host(sample)
     t0 = time.time()
@@ -480,7 +481,7 @@ def cpu_baseline_jobs(k, th, jobs, seconds_target=15.0, threads=None):
     from bruno_amd.pipeline import SynthConfig
     from oracle.oracle import Oracle  # the checker, timed as the baseline
     threads = threads or min(16, os.cpu_count() or 1)
-    budget = int(240_000_000 * seconds_target / 15.0)
+    budget = int(96_000_000 * seconds_target / 15.0)
     inst, dt, used = 0, 0.0, []
     for c in jobs:
         if inst >= budget:
