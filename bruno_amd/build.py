@@ -4,7 +4,7 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRCS = [os.path.join(HERE, "csrc", "bruno_hip.hip"), os.path.join(HERE, "csrc", "server.hip")]
+SRCS = [os.path.join(HERE, "csrc", n) for n in ("bruno_hip.hip", "server.hip", "fastq.hip")]
 OUT = os.path.join(HERE, "libbruno_hip.so")
 
 HIPCC_FLAGS = [

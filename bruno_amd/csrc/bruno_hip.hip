@@ -2060,6 +2060,42 @@ bruno_status bruno_parse_count_host(bruno_job* j, const char* bases, const uint6
   return s;
 }
 
+bruno_status bruno_parse_count_fastq(bruno_job* j, const char* d_fastq, uint64_t bytes) {
+  if (!valid_job(j) || (bytes && !d_fastq)) return BRUNO_BAD_INPUT;
+  if (j->n_jobs != 1) return BRUNO_BAD_INPUT;
+  if (j->stage > ST_COUNTED) return BRUNO_STATE;
+  uint64_t nr = 0, nb = 0;
+  TRY(bruno_fastq_split(d_fastq, bytes, nullptr, nullptr, &nr, &nb, j->stream));
+  char* db = nullptr;
+  uint64_t* doff = nullptr;
+  TRY(dev_alloc(j, (void**)&db, nb + 1));
+  TRY(dev_alloc(j, (void**)&doff, (nr + 1) * 8));
+  uint64_t nr2 = nr, nb2 = nb;
+  bruno_status s = bruno_fastq_split(d_fastq, bytes, db, doff, &nr2, &nb2, j->stream);
+  if (s == BRUNO_OK) s = bruno_parse_count(j, db, doff, nr2);
+  void* a = db;
+  dev_free(j, a, nb + 1);
+  a = doff;
+  dev_free(j, a, (nr + 1) * 8);
+  return s;
+}
+
+bruno_status bruno_parse_count_fastq_host(bruno_job* j, const char* fastq, uint64_t bytes) {
+  if (!valid_job(j) || (bytes && !fastq)) return BRUNO_BAD_INPUT;
+  char* d = nullptr;
+  TRY(dev_alloc(j, (void**)&d, bytes + 1));
+  bruno_status s = BRUNO_OK;
+  if (bytes && (hipMemcpyAsync(d, fastq, bytes, hipMemcpyHostToDevice, j->stream) != hipSuccess ||
+                hipStreamSynchronize(j->stream) != hipSuccess)) {
+    (void)hipGetLastError();
+    s = BRUNO_INTERNAL;
+  }
+  if (s == BRUNO_OK) s = bruno_parse_count_fastq(j, d, bytes);
+  void* a = d;
+  dev_free(j, a, bytes + 1);
+  return s;
+}
+
 bruno_status bruno_filter(bruno_job* j) {
   if (!valid_job(j)) return BRUNO_BAD_INPUT;
   if (j->stage != ST_COUNTED) return BRUNO_STATE;

@@ -1,0 +1,227 @@
+/*
+ * fastq.hip -- FASTQ records to concatenated reads, on the device.
+ *
+ * Replaces the reader under parse_file_data (compact_dbg_io.hpp:165-217: the
+ * file bytes go to kmerind's FASTQ parser, one sequence per 4-line record,
+ * compact_dbg_freq_construct_clean.cpp:210-219).  The file's bytes are in
+ * device memory; the output is what bruno_parse_count takes: the sequence
+ * lines back to back and n_reads + 1 byte offsets.  Reads with a non-ACGT
+ * base are not removed here: the parse kernel drops them, as
+ * NFilterSequencesIterator does.
+ *
+ * Format accepted: records of exactly four lines (@header, sequence,
+ * +[header], quality); '\n' or "\r\n" line ends; the last line may lack its
+ * newline; blank lines after the last record are ignored.  Anything else
+ * (a header not starting with '@', a separator not starting with '+', text
+ * after the last complete record) is BRUNO_BAD_INPUT.
+ *
+ * Three passes: newlines per 16 KB chunk; a prefix sum gives every chunk the
+ * number of the first line it contains; each chunk, staged in LDS, is walked
+ * by 256 threads over 64-byte runs (a block scan of the per-run newline
+ * counts numbers every line), which records where each sequence line starts
+ * and ends and checks the record structure.  A prefix sum of the sequence
+ * lengths gives the offsets, and one wave per read copies its bases.
+ */
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <vector>
+
+#include "../../include/bruno_hip.h"
+#include "pool.h"
+
+namespace {
+
+constexpr int FQ_BLOCK = 256;
+constexpr uint64_t FQ_RUN = 64;                     /* bytes per thread */
+constexpr uint64_t FQ_CHUNK = FQ_BLOCK * FQ_RUN;    /* bytes per workgroup */
+
+__global__ __launch_bounds__(FQ_BLOCK) void k_fq_count(const uint8_t* s, uint64_t n, unsigned long long* nl) {
+  const uint64_t c0 = (uint64_t)blockIdx.x * FQ_CHUNK;
+  const uint64_t len = n - c0 < FQ_CHUNK ? n - c0 : FQ_CHUNK;
+  unsigned long long c = 0;
+  for (uint64_t i = threadIdx.x; i < len; i += FQ_BLOCK) c += (s[c0 + i] == '\n');
+  typedef hipcub::BlockReduce<unsigned long long, FQ_BLOCK> BR;
+  __shared__ typename BR::TempStorage tmp;
+  unsigned long long t = BR(tmp).Sum(c);
+  if (threadIdx.x == 0) nl[blockIdx.x] = t;
+}
+
+struct FqOut {
+  uint64_t* seq_start;
+  uint64_t* seq_end;
+  uint64_t n_records;   /* complete records R */
+  uint64_t n_lines;     /* lines T, the last one possibly without newline */
+  uint32_t* err;
+};
+
+/* A line ends at p (a newline, or the end of the file for the last line);
+ * line is its number. */
+__device__ __forceinline__ void fq_line_end(const uint8_t* s, uint64_t n, uint64_t p, uint64_t line, const FqOut& o) {
+  const uint64_t R = o.n_records;
+  if (line < 4 * R) {
+    const uint64_t r = line >> 2;
+    const uint32_t m = (uint32_t)(line & 3);
+    if (m == 0) o.seq_start[r] = p + 1;
+    if (m == 1) o.seq_end[r] = (p > 0 && s[p - 1] == '\r') ? p - 1 : p;
+  }
+  /* the line after this one must start as its role says */
+  const uint64_t next = line + 1;
+  if (next < 4 * R && p + 1 < n) {
+    const uint8_t c = s[p + 1];
+    if ((next & 3) == 0 && c != '@') atomicOr(o.err, 1u);
+    if ((next & 3) == 2 && c != '+') atomicOr(o.err, 1u);
+  }
+}
+
+__global__ __launch_bounds__(FQ_BLOCK) void k_fq_mark(const uint8_t* s, uint64_t n, const unsigned long long* base,
+                                                      FqOut o) {
+  __shared__ uint8_t buf[FQ_CHUNK];
+  typedef hipcub::BlockScan<unsigned long long, FQ_BLOCK> BS;
+  __shared__ typename BS::TempStorage tmp;
+  const uint64_t c0 = (uint64_t)blockIdx.x * FQ_CHUNK;
+  const uint64_t len = n - c0 < FQ_CHUNK ? n - c0 : FQ_CHUNK;
+  for (uint64_t i = threadIdx.x; i < len; i += FQ_BLOCK) buf[i] = s[c0 + i];
+  __syncthreads();
+  const uint64_t r0 = (uint64_t)threadIdx.x * FQ_RUN;
+  const uint64_t r1 = r0 + FQ_RUN < len ? r0 + FQ_RUN : len;
+  unsigned long long cnt = 0;
+  for (uint64_t i = r0; i < r1; ++i) cnt += (buf[i] == '\n');
+  unsigned long long pre = 0;
+  BS(tmp).ExclusiveSum(cnt, pre);
+  uint64_t line = base[blockIdx.x] + pre;
+  const uint64_t R = o.n_records;
+  for (uint64_t i = r0; i < r1; ++i) {
+    const uint8_t c = buf[i];
+    if (line >= 4 * R && c != '\n' && c != '\r' && c != ' ') atomicOr(o.err, 2u); /* text after the last record */
+    if (c == '\n') {
+      fq_line_end(s, n, c0 + i, line, o);
+      ++line;
+    }
+  }
+  /* the last line without a newline ends at the end of the file */
+  if (r0 < r1 && c0 + r1 == n && buf[r1 - 1] != '\n') fq_line_end(s, n, n, line, o);
+  if (blockIdx.x == 0 && threadIdx.x == 0 && R > 0 && s[0] != '@') atomicOr(o.err, 1u);
+}
+
+__global__ void k_fq_len(const uint64_t* a, const uint64_t* b, uint64_t n, uint64_t* len) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) len[i] = b[i] >= a[i] ? b[i] - a[i] : 0;
+}
+
+/* one wave per read */
+__global__ __launch_bounds__(256) void k_fq_copy(const uint8_t* s, const uint64_t* start, const uint64_t* off,
+                                                 uint64_t n_reads, uint8_t* out) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  for (uint64_t r = wave; r < n_reads; r += nw) {
+    const uint8_t* src = s + start[r];
+    uint8_t* dst = out + off[r];
+    const uint64_t L = off[r + 1] - off[r];
+    for (uint64_t i = lane; i < L; i += 64) dst[i] = src[i];
+  }
+}
+
+hipError_t scan_u64(const uint64_t* in, uint64_t* out_plus1, uint64_t n, hipStream_t st) {
+  /* out[0] = 0, out[i+1] = in[0] + ... + in[i] */
+  br::DevicePool& pool = br::DevicePool::current();
+  hipError_t e = hipMemsetAsync(out_plus1, 0, 8, st);
+  if (e != hipSuccess || n == 0) return e;
+  size_t tmp = 0;
+  e = hipcub::DeviceScan::InclusiveSum(nullptr, tmp, in, out_plus1 + 1, (int)n, st);
+  if (e != hipSuccess) return e;
+  void* t = nullptr;
+  e = pool.alloc(&t, tmp + 16);
+  if (e != hipSuccess) return e;
+  e = hipcub::DeviceScan::InclusiveSum(t, tmp, in, out_plus1 + 1, (int)n, st);
+  hipError_t e2 = hipStreamSynchronize(st);
+  pool.release(t);
+  return e != hipSuccess ? e : e2;
+}
+
+struct Blocks {
+  std::vector<void*> v;
+  ~Blocks() {
+    for (void* p : v) br::DevicePool::current().release(p);
+  }
+  template <typename T>
+  bool get(T** p, uint64_t bytes) {
+    void* q = nullptr;
+    if (br::DevicePool::current().alloc(&q, std::max<uint64_t>(bytes, 16)) != hipSuccess) return false;
+    v.push_back(q);
+    *p = (T*)q;
+    return true;
+  }
+};
+
+}  // namespace
+
+extern "C" bruno_status bruno_fastq_split(const char* d_fastq, uint64_t bytes, char* d_out_bases,
+                                          uint64_t* d_out_offsets, uint64_t* n_reads, uint64_t* n_bases,
+                                          void* stream) {
+  if (!n_reads || !n_bases || (bytes && !d_fastq)) return BRUNO_BAD_INPUT;
+  if ((d_out_bases == nullptr) != (d_out_offsets == nullptr)) return BRUNO_BAD_INPUT;
+  hipStream_t st = (hipStream_t)stream;
+  const uint64_t cap_reads = *n_reads, cap_bases = *n_bases;
+  *n_reads = *n_bases = 0;
+  if (bytes == 0) {
+    if (d_out_offsets && hipMemsetAsync(d_out_offsets, 0, 8, st) != hipSuccess) return BRUNO_INTERNAL;
+    return hipStreamSynchronize(st) == hipSuccess ? BRUNO_OK : BRUNO_INTERNAL;
+  }
+  const uint8_t* s = (const uint8_t*)d_fastq;
+  const uint64_t nchunks = (bytes + FQ_CHUNK - 1) / FQ_CHUNK;
+  Blocks mem;
+  unsigned long long *nl = nullptr, *base = nullptr;
+  uint32_t* err = nullptr;
+  if (!mem.get(&nl, nchunks * 8) || !mem.get(&base, (nchunks + 1) * 8) || !mem.get(&err, 4)) return BRUNO_OOM;
+  hipLaunchKernelGGL(k_fq_count, dim3((unsigned)nchunks), dim3(FQ_BLOCK), 0, st, s, bytes, nl);
+  if (scan_u64((const uint64_t*)nl, (uint64_t*)base, nchunks, st) != hipSuccess) return BRUNO_INTERNAL;
+  unsigned long long total_nl = 0;
+  uint8_t last = 0;
+  if (hipMemcpyAsync(&total_nl, base + nchunks, 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipMemcpyAsync(&last, s + bytes - 1, 1, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipMemsetAsync(err, 0, 4, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess) {
+    (void)hipGetLastError();
+    return BRUNO_INTERNAL;
+  }
+  const uint64_t T = total_nl + (last != '\n' ? 1 : 0);
+  const uint64_t R = T / 4;
+  FqOut o{};
+  o.n_records = R;
+  o.n_lines = T;
+  o.err = err;
+  uint64_t* lens = nullptr;
+  uint64_t* offs = d_out_offsets;
+  if (!mem.get(&o.seq_start, R * 8) || !mem.get(&o.seq_end, R * 8) || !mem.get(&lens, R * 8)) return BRUNO_OOM;
+  if (!offs && !mem.get(&offs, (R + 1) * 8)) return BRUNO_OOM;
+  if (d_out_offsets && R > cap_reads) return BRUNO_CAPACITY;
+  hipLaunchKernelGGL(k_fq_mark, dim3((unsigned)nchunks), dim3(FQ_BLOCK), 0, st, s, bytes,
+                     (const unsigned long long*)base, o);
+  if (R) hipLaunchKernelGGL(k_fq_len, dim3((unsigned)((R + 255) / 256)), dim3(256), 0, st, o.seq_start, o.seq_end, R, lens);
+  if (scan_u64(lens, offs, R, st) != hipSuccess) return BRUNO_INTERNAL;
+  uint32_t herr = 0;
+  uint64_t nb = 0;
+  if (hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipMemcpyAsync(&nb, offs + R, 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess) {
+    (void)hipGetLastError();
+    return BRUNO_INTERNAL;
+  }
+  if (herr) return BRUNO_BAD_INPUT;
+  *n_reads = R;
+  *n_bases = nb;
+  if (!d_out_bases) return BRUNO_OK;
+  if (nb > cap_bases) return BRUNO_CAPACITY;
+  if (R) {
+    const uint64_t waves = std::min<uint64_t>(R, 65536);
+    hipLaunchKernelGGL(k_fq_copy, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st, s, o.seq_start, offs, R,
+                       (uint8_t*)d_out_bases);
+  }
+  if (hipStreamSynchronize(st) != hipSuccess) {
+    (void)hipGetLastError();
+    return BRUNO_INTERNAL;
+  }
+  return BRUNO_OK;
+}

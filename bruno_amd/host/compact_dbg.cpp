@@ -6,13 +6,15 @@
  *
  * Writes <prefix>.branch.fasta and <prefix>.chain.fasta, and unless -B also
  * <prefix>.chain.components (print_chain_nodes, compact_dbg_io.hpp:463-498).
- * Reads with a non-ACGT base are dropped (NFilterSequencesIterator).
+ * Each file's bytes go to the device, where the FASTQ records are split into
+ * reads; reads with a non-ACGT base are dropped (NFilterSequencesIterator).
  */
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <iterator>
 #include <iostream>
 #include <string>
 #include <vector>
@@ -21,16 +23,10 @@
 
 using namespace bruno_amd;
 
-static bool read_fastq(const std::string& fn, read_batch& out) {
-  std::ifstream f(fn);
+static bool read_file(const std::string& fn, std::string& out) {
+  std::ifstream f(fn, std::ios::binary);
   if (!f) return false;
-  std::string h, s, p, q;
-  while (std::getline(f, h)) {
-    if (h.empty() || h[0] != '@') continue;
-    if (!std::getline(f, s) || !std::getline(f, p) || !std::getline(f, q)) break;
-    while (!s.empty() && (s.back() == '\r' || s.back() == ' ')) s.pop_back();
-    out.add(s);
-  }
+  out.assign(std::istreambuf_iterator<char>(f), std::istreambuf_iterator<char>());
   return true;
 }
 
@@ -95,12 +91,13 @@ int main(int argc, char** argv) {
     debruijn_chain_graph chainmap(j);
     /* build_index_thresholded / build_index: one batch per file (:313-330) */
     for (auto& fn : files) {
-      read_batch b;
-      if (!read_fastq(fn, b)) {
+      std::string bytes;
+      if (!read_file(fn, bytes)) {
         std::cerr << "cannot read " << fn << "\n";
         return 1;
       }
-      idx.get_map().compute_biedge_freqencies(b);
+      /* the FASTQ records are split into reads on the device */
+      idx.get_map().compute_biedge_freqencies_fastq(bytes);
     }
     idx.get_map().local_insert_by_freqencies();
     chainmap.extract_chains(idx);

@@ -91,6 +91,13 @@ class debruijn_graph_map {
           "compute_biedge_freqencies");
     return k2_size();
   }
+  /* the same from a whole FASTQ file's bytes, split into reads on the device
+   * (parse_file_data's reader, compact_dbg_io.hpp:165-217) */
+  size_t compute_biedge_freqencies_fastq(const std::string& fastq_bytes) {
+    check(bruno_parse_count_fastq_host(j_.get(), fastq_bytes.data(), fastq_bytes.size()),
+          "compute_biedge_freqencies (FASTQ)");
+    return k2_size();
+  }
   /* insert (:871-921), the unthresholded build_index path */
   size_t insert(const read_batch& reads) { return compute_biedge_freqencies(reads); }
   /* local_insert_by_freqencies (:1258-1623); thresholds come from the job */

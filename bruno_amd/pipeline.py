@@ -128,6 +128,15 @@ class GraphJob(_Views):
         check(lib().bruno_parse_count(self.h, ctypes.c_void_p(bases_ptr), ctypes.c_void_p(offsets_ptr),
                                       int(n_reads)), "bruno_parse_count")
 
+    def parse_count_fastq(self, data):
+        """A whole FASTQ file's bytes (host), split into reads on the device."""
+        b = np.frombuffer(data, np.uint8) if isinstance(data, (bytes, bytearray)) else np.ascontiguousarray(data, np.uint8)
+        check(lib().bruno_parse_count_fastq_host(self.h, _ptr(b) if b.size else None, b.size),
+              "bruno_parse_count_fastq_host")
+
+    def parse_count_fastq_device(self, ptr, nbytes):
+        check(lib().bruno_parse_count_fastq(self.h, ctypes.c_void_p(ptr), int(nbytes)), "bruno_parse_count_fastq")
+
     def filter(self):
         check(lib().bruno_filter(self.h), "bruno_filter")
 
@@ -300,3 +309,22 @@ def job_first_read(cfgs):
     f = np.zeros(len(cfgs) + 1, np.uint64)
     f[1:] = np.cumsum([c.n_reads for c in cfgs])
     return f
+
+
+def fastq_split_device(fastq_ptr, nbytes, stream=None):
+    """Split FASTQ bytes in device memory into (bases, offsets) host arrays
+    (through device buffers; for tests)."""
+    from .device import DeviceBuffer
+    L = lib()
+    nr, nb = ctypes.c_uint64(0), ctypes.c_uint64(0)
+    check(L.bruno_fastq_split(ctypes.c_void_p(fastq_ptr), int(nbytes), None, None, ctypes.byref(nr), ctypes.byref(nb),
+                              stream), "bruno_fastq_split")
+    db, do = DeviceBuffer(nb.value + 1), DeviceBuffer((nr.value + 1) * 8)
+    try:
+        check(L.bruno_fastq_split(ctypes.c_void_p(fastq_ptr), int(nbytes), ctypes.c_void_p(db.ptr),
+                                  ctypes.c_void_p(do.ptr), ctypes.byref(nr), ctypes.byref(nb), stream),
+              "bruno_fastq_split")
+        return db.to_numpy(np.uint8, nb.value), do.to_numpy(np.uint64, nr.value + 1)
+    finally:
+        db.free()
+        do.free()

@@ -134,6 +134,22 @@ bruno_status bruno_parse_count(bruno_job* job, const char* d_bases, const uint64
 bruno_status bruno_parse_count_host(bruno_job* job, const char* bases, const uint64_t* offsets,
                                     uint64_t n_reads);
 
+/* FASTQ ingest on the device.  Replaces the reader under parse_file_data
+ * (compact_dbg_io.hpp:165-217; kmerind's FASTQ parser, one sequence per
+ * record): d_fastq holds a FASTQ file's bytes (4-line records, '\n' or
+ * "\r\n" line ends, last newline optional, blank lines at the end
+ * ignored).  Writes the sequence lines back to back to d_out_bases and
+ * n_reads + 1 offsets to d_out_offsets.  On entry *n_reads / *n_bases are
+ * the capacities of the two outputs (CAPACITY if exceeded); pass both
+ * outputs NULL to learn the sizes.  A malformed file is BAD_INPUT.  Reads
+ * with a non-ACGT base stay in the output; bruno_parse_count drops them. */
+bruno_status bruno_fastq_split(const char* d_fastq, uint64_t bytes, char* d_out_bases, uint64_t* d_out_offsets,
+                               uint64_t* n_reads, uint64_t* n_bases, void* stream);
+/* parse_file_data + compute_biedge_freqencies on a whole FASTQ file:
+ * device bytes, or host bytes copied first. */
+bruno_status bruno_parse_count_fastq(bruno_job* job, const char* d_fastq, uint64_t bytes);
+bruno_status bruno_parse_count_fastq_host(bruno_job* job, const char* fastq, uint64_t bytes);
+
 /* Replaces local_insert_by_freqencies(k2_counter, threshes)
  * (debruijn_graph_map.hpp:1258-1623); in the unthresholded mode it
  * materialises the count graph (local_insert :175-197).  Also classifies
