@@ -68,6 +68,21 @@ def capacity_for(synth, k, load_factor=0.5):
     return capacity(synth, k, load_factor)
 
 
+def fastq_bytes_on_device(bases, s):
+    """The workload's reads as a FASTQ file ("@\n", read, "\n+\n", 'I' x
+    read length, "\n" per record), built on the host and copied to HBM."""
+    import numpy as np
+    from bruno_amd.device import DeviceBuffer
+    L, n = s.read_len, s.n_reads
+    rec = np.empty((n, 2 * L + 6), np.uint8)
+    rec[:, 0], rec[:, 1] = ord("@"), ord("\n")
+    rec[:, 2:2 + L] = bases.to_numpy(np.uint8, n * L).reshape(n, L)
+    rec[:, 2 + L], rec[:, 3 + L], rec[:, 4 + L] = ord("\n"), ord("+"), ord("\n")
+    rec[:, 5 + L:5 + 2 * L] = ord("I")
+    rec[:, 5 + 2 * L] = ord("\n")
+    return DeviceBuffer.from_numpy(rec.reshape(-1))
+
+
 def cpu_baseline(cfg, seconds_target=15.0, threads=None):
     import numpy as np
     import bruno_amd
@@ -111,6 +126,8 @@ def main():
     ap.add_argument("--autotune", action="store_true",
                     help="tune load factor and launch knobs on this workload first (untimed)")
     ap.add_argument("--tuning", default="", help="apply a tuning saved by bruno_amd.autotune")
+    ap.add_argument("--fastq", action="store_true",
+                    help="feed the reads as FASTQ bytes in HBM (split on the device inside each step)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the timed steps, copy 4 GiB device-to-device once (a plain copy kernel of "
                          "known size, to scale FETCH_SIZE / WRITE_SIZE counters)")
@@ -141,6 +158,9 @@ def main():
     offs = DeviceBuffer((s.n_reads + 1) * 8)
     synth_reads_device(s, bases.ptr, offs.ptr)
     synchronize()
+    fq = None
+    if args.fastq:
+        fq = fastq_bytes_on_device(bases, s)
 
     lf = args.load_factor
     tuned = None
@@ -161,7 +181,10 @@ def main():
         synchronize()
         t0 = time.perf_counter()
         job.reset()
-        job.parse_count_device(bases.ptr, offs.ptr, s.n_reads)
+        if fq is not None:
+            job.parse_count_fastq_device(fq.ptr, fq.nbytes)
+        else:
+            job.parse_count_device(bases.ptr, offs.ptr, s.n_reads)
         job.run(args.variant)
         synchronize()
         dt = time.perf_counter() - t0
@@ -197,6 +220,8 @@ def main():
             "k": k, "n_reads": s.n_reads, "read_len": s.read_len, "genome_bp": s.genome_len,
             "k_mer_instances": n_inst, "list_rank": "min_update" if args.variant else "list_rank",
             "load_factor": lf,
+            "input": "FASTQ bytes in HBM (%.1f GB), split on the device in each step" % (fq.nbytes / 1e9)
+                     if fq is not None else "concatenated bases + offsets in HBM",
         },
         "latency_ms": {"p50": float(np.percentile(times, 50) * 1e3), "p99": float(np.percentile(times, 99) * 1e3)},
         "phases_ms": ph,

@@ -1140,6 +1140,16 @@ struct bruno_job {
 
 namespace {
 
+}  // namespace
+
+/* fastq.hip: sequence-line ranges with arrays from the caller's allocator
+ * (not in the C ABI) */
+bruno_status bruno_fastq_ranges(const char* d_fastq, uint64_t bytes, void* (*alloc)(void*, uint64_t), void* ctx,
+                                uint64_t** starts, uint64_t** ends, uint64_t* n_reads, uint64_t* n_bases,
+                                void* stream);
+
+namespace {
+
 /* Process-wide tuning knobs (bruno_set_tuning). */
 std::mutex g_tuning_mu;
 /* defaults: the autotuner's picks on the 100 Mbp / 33 M-read workload */
@@ -1248,8 +1258,11 @@ uint64_t persistent_bytes(const bruno_job* j) {
 
 template <typename KT>
 bruno_status parse_count_t(bruno_job* j, const char* d_bases, const uint64_t* d_offsets, uint64_t n_reads,
-                           const uint64_t* d_first) {
+                           const uint64_t* d_first, const uint64_t* d_starts = nullptr,
+                           const uint64_t* d_ends = nullptr) {
   ParseArgs a{};
+  a.starts = d_starts;
+  a.ends = d_ends;
   a.n_jobs = j->n_jobs;
   a.job_first = d_first;
   a.seg = j->d_seg;
@@ -1959,10 +1972,11 @@ bruno_status bruno_job_reset(bruno_job* j) {
 }
 
 static bruno_status parse_count_common(bruno_job* j, const char* d_bases, const uint64_t* d_offsets, uint64_t n_reads,
-                                       const uint64_t* d_first) {
+                                       const uint64_t* d_first, const uint64_t* d_starts = nullptr,
+                                       const uint64_t* d_ends = nullptr) {
   ev_record(j, 0);
-  if (j->words == 1) TRY(parse_count_t<uint64_t>(j, d_bases, d_offsets, n_reads, d_first));
-  else TRY(parse_count_t<u128>(j, d_bases, d_offsets, n_reads, d_first));
+  if (j->words == 1) TRY(parse_count_t<uint64_t>(j, d_bases, d_offsets, n_reads, d_first, d_starts, d_ends));
+  else TRY(parse_count_t<u128>(j, d_bases, d_offsets, n_reads, d_first, d_starts, d_ends));
   ev_record(j, 1);
   TRY(sync_counters(j));
   j->ms.parse_count_ms += ev_ms(j, 0, 1);
@@ -2060,23 +2074,36 @@ bruno_status bruno_parse_count_host(bruno_job* j, const char* bases, const uint6
   return s;
 }
 
+static void* job_alloc(void* ctx, uint64_t bytes) {
+  void* p = nullptr;
+  return dev_alloc((bruno_job*)ctx, &p, bytes) == BRUNO_OK ? p : nullptr;
+}
+
 bruno_status bruno_parse_count_fastq(bruno_job* j, const char* d_fastq, uint64_t bytes) {
   if (!valid_job(j) || (bytes && !d_fastq)) return BRUNO_BAD_INPUT;
   if (j->n_jobs != 1) return BRUNO_BAD_INPUT;
   if (j->stage > ST_COUNTED) return BRUNO_STATE;
   uint64_t nr = 0, nb = 0;
-  TRY(bruno_fastq_split(d_fastq, bytes, nullptr, nullptr, &nr, &nb, j->stream));
-  char* db = nullptr;
-  uint64_t* doff = nullptr;
-  TRY(dev_alloc(j, (void**)&db, nb + 1));
-  TRY(dev_alloc(j, (void**)&doff, (nr + 1) * 8));
-  uint64_t nr2 = nr, nb2 = nb;
-  bruno_status s = bruno_fastq_split(d_fastq, bytes, db, doff, &nr2, &nb2, j->stream);
-  if (s == BRUNO_OK) s = bruno_parse_count(j, db, doff, nr2);
-  void* a = db;
-  dev_free(j, a, nb + 1);
-  a = doff;
-  dev_free(j, a, (nr + 1) * 8);
+  uint64_t *st = nullptr, *en = nullptr;
+  /* the sequence lines are parsed where they are in the file: only their
+   * byte ranges are materialised */
+  bruno_status s = bruno_fastq_ranges(d_fastq, bytes, job_alloc, j, &st, &en, &nr, &nb, j->stream);
+  if (s == BRUNO_OK && nr) {
+    if (j->cap == 0) {
+      const uint64_t want = (uint64_t)((double)nb / j->load_factor) + 1024;
+      s = alloc_table(j, &want);
+    }
+    if (s == BRUNO_OK) {
+      j->job_reads[0] += nr;
+      s = parse_count_common(j, d_fastq, nullptr, nr, nullptr, st, en);
+    }
+  } else if (s == BRUNO_OK) {
+    j->stage = ST_COUNTED;
+  }
+  void* a = st;
+  dev_free(j, a, std::max<uint64_t>(1, nr) * 8);
+  a = en;
+  dev_free(j, a, std::max<uint64_t>(1, nr) * 8);
   return s;
 }
 

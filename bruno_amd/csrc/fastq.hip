@@ -37,11 +37,30 @@ constexpr int FQ_BLOCK = 256;
 constexpr uint64_t FQ_RUN = 64;                     /* bytes per thread */
 constexpr uint64_t FQ_CHUNK = FQ_BLOCK * FQ_RUN;    /* bytes per workgroup */
 
+/* number of '\n' bytes in a 32-bit word: x has a zero byte exactly where w
+ * has a newline; the high bit of ((x & 7F..) + 7F..) | x is clear exactly
+ * for zero bytes */
+__device__ __forceinline__ uint32_t newlines4(uint32_t w) {
+  const uint32_t x = w ^ 0x0A0A0A0Au;
+  const uint32_t t = ~(((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x | 0x7F7F7F7Fu);
+  return (uint32_t)__popc(t);
+}
+
+/* whole 16-byte vectors when the chunk is aligned and complete, bytes
+ * otherwise */
 __global__ __launch_bounds__(FQ_BLOCK) void k_fq_count(const uint8_t* s, uint64_t n, unsigned long long* nl) {
   const uint64_t c0 = (uint64_t)blockIdx.x * FQ_CHUNK;
   const uint64_t len = n - c0 < FQ_CHUNK ? n - c0 : FQ_CHUNK;
   unsigned long long c = 0;
-  for (uint64_t i = threadIdx.x; i < len; i += FQ_BLOCK) c += (s[c0 + i] == '\n');
+  if (len == FQ_CHUNK && (((uintptr_t)(s + c0)) & 15) == 0) {
+    const uint4* v = (const uint4*)(s + c0);
+    for (uint64_t i = threadIdx.x; i < FQ_CHUNK / 16; i += FQ_BLOCK) {
+      const uint4 q = v[i];
+      c += newlines4(q.x) + newlines4(q.y) + newlines4(q.z) + newlines4(q.w);
+    }
+  } else {
+    for (uint64_t i = threadIdx.x; i < len; i += FQ_BLOCK) c += (s[c0 + i] == '\n');
+  }
   typedef hipcub::BlockReduce<unsigned long long, FQ_BLOCK> BR;
   __shared__ typename BR::TempStorage tmp;
   unsigned long long t = BR(tmp).Sum(c);
@@ -75,33 +94,66 @@ __device__ __forceinline__ void fq_line_end(const uint8_t* s, uint64_t n, uint64
   }
 }
 
+/* Each thread walks its 64-byte run from registers: four 16-byte loads
+ * (neighbouring threads' runs share cache lines, so the chunk is read from
+ * memory once), a SWAR newline count, a block scan for the line number of
+ * the run's first byte, then one pass over the 64 bytes. */
 __global__ __launch_bounds__(FQ_BLOCK) void k_fq_mark(const uint8_t* s, uint64_t n, const unsigned long long* base,
                                                       FqOut o) {
-  __shared__ uint8_t buf[FQ_CHUNK];
   typedef hipcub::BlockScan<unsigned long long, FQ_BLOCK> BS;
   __shared__ typename BS::TempStorage tmp;
   const uint64_t c0 = (uint64_t)blockIdx.x * FQ_CHUNK;
   const uint64_t len = n - c0 < FQ_CHUNK ? n - c0 : FQ_CHUNK;
-  for (uint64_t i = threadIdx.x; i < len; i += FQ_BLOCK) buf[i] = s[c0 + i];
-  __syncthreads();
   const uint64_t r0 = (uint64_t)threadIdx.x * FQ_RUN;
   const uint64_t r1 = r0 + FQ_RUN < len ? r0 + FQ_RUN : len;
+  uint32_t w[FQ_RUN / 4];
+  const bool whole = r1 - r0 == FQ_RUN && r0 < r1 && (((uintptr_t)(s + c0 + r0)) & 15) == 0;
+  if (whole) {
+    const uint4* v = (const uint4*)(s + c0 + r0);
+#pragma unroll
+    for (int q = 0; q < (int)(FQ_RUN / 16); ++q) {
+      const uint4 x = v[q];
+      w[4 * q] = x.x;
+      w[4 * q + 1] = x.y;
+      w[4 * q + 2] = x.z;
+      w[4 * q + 3] = x.w;
+    }
+  } else {
+    /* partial or unaligned run: bytes, padded with a non-newline */
+#pragma unroll
+    for (int q = 0; q < (int)(FQ_RUN / 4); ++q) {
+      uint32_t x = 0;
+      for (int b = 0; b < 4; ++b) {
+        const uint64_t i = r0 + 4 * q + b;
+        const uint32_t c = i < r1 ? s[c0 + i] : 0u;
+        x |= c << (8 * b);
+      }
+      w[q] = x;
+    }
+  }
   unsigned long long cnt = 0;
-  for (uint64_t i = r0; i < r1; ++i) cnt += (buf[i] == '\n');
+#pragma unroll
+  for (int q = 0; q < (int)(FQ_RUN / 4); ++q) cnt += newlines4(w[q]);
   unsigned long long pre = 0;
   BS(tmp).ExclusiveSum(cnt, pre);
   uint64_t line = base[blockIdx.x] + pre;
   const uint64_t R = o.n_records;
-  for (uint64_t i = r0; i < r1; ++i) {
-    const uint8_t c = buf[i];
-    if (line >= 4 * R && c != '\n' && c != '\r' && c != ' ') atomicOr(o.err, 2u); /* text after the last record */
-    if (c == '\n') {
-      fq_line_end(s, n, c0 + i, line, o);
-      ++line;
+  const uint64_t nb = r1 > r0 ? r1 - r0 : 0;
+#pragma unroll
+  for (int q = 0; q < (int)(FQ_RUN / 4); ++q) {
+    for (int b = 0; b < 4; ++b) {
+      const uint64_t i = 4 * (uint64_t)q + b;
+      if (i >= nb) break;
+      const uint8_t c = (uint8_t)(w[q] >> (8 * b));
+      if (line >= 4 * R && c != '\n' && c != '\r' && c != ' ') atomicOr(o.err, 2u); /* text after the last record */
+      if (c == '\n') {
+        fq_line_end(s, n, c0 + r0 + i, line, o);
+        ++line;
+      }
     }
   }
   /* the last line without a newline ends at the end of the file */
-  if (r0 < r1 && c0 + r1 == n && buf[r1 - 1] != '\n') fq_line_end(s, n, n, line, o);
+  if (nb && c0 + r1 == n && s[n - 1] != '\n') fq_line_end(s, n, n, line, o);
   if (blockIdx.x == 0 && threadIdx.x == 0 && R > 0 && s[0] != '@') atomicOr(o.err, 1u);
 }
 
@@ -141,9 +193,14 @@ hipError_t scan_u64(const uint64_t* in, uint64_t* out_plus1, uint64_t n, hipStre
   return e != hipSuccess ? e : e2;
 }
 
+/* scratch blocks of one call, released when the call returns (after its
+ * stream has finished with them) */
 struct Blocks {
   std::vector<void*> v;
+  hipStream_t st;
+  explicit Blocks(hipStream_t s) : st(s) {}
   ~Blocks() {
+    if (!v.empty()) (void)hipStreamSynchronize(st);
     for (void* p : v) br::DevicePool::current().release(p);
   }
   template <typename T>
@@ -156,23 +213,24 @@ struct Blocks {
   }
 };
 
-}  // namespace
 
-extern "C" bruno_status bruno_fastq_split(const char* d_fastq, uint64_t bytes, char* d_out_bases,
-                                          uint64_t* d_out_offsets, uint64_t* n_reads, uint64_t* n_bases,
-                                          void* stream) {
-  if (!n_reads || !n_bases || (bytes && !d_fastq)) return BRUNO_BAD_INPUT;
-  if ((d_out_bases == nullptr) != (d_out_offsets == nullptr)) return BRUNO_BAD_INPUT;
-  hipStream_t st = (hipStream_t)stream;
-  const uint64_t cap_reads = *n_reads, cap_bases = *n_bases;
+/* One pass.  out_bases / out_offsets are either given (with capacities) or
+ * obtained from alloc(bytes) once the sizes are known. */
+template <typename Alloc>
+bruno_status fastq_split_impl(const uint8_t* s, uint64_t bytes, char** out_bases, uint64_t** out_offs,
+                              uint64_t cap_reads, uint64_t cap_bases, bool have_out, bool copy, Alloc alloc,
+                              uint64_t* n_reads, uint64_t* n_bases, hipStream_t st,
+                              uint64_t** out_start = nullptr, uint64_t** out_end = nullptr) {
   *n_reads = *n_bases = 0;
   if (bytes == 0) {
-    if (d_out_offsets && hipMemsetAsync(d_out_offsets, 0, 8, st) != hipSuccess) return BRUNO_INTERNAL;
+    if (copy && !have_out) {
+      if (!(*out_offs = (uint64_t*)alloc(8)) || !(*out_bases = (char*)alloc(16))) return BRUNO_OOM;
+    }
+    if (*out_offs && hipMemsetAsync(*out_offs, 0, 8, st) != hipSuccess) return BRUNO_INTERNAL;
     return hipStreamSynchronize(st) == hipSuccess ? BRUNO_OK : BRUNO_INTERNAL;
   }
-  const uint8_t* s = (const uint8_t*)d_fastq;
   const uint64_t nchunks = (bytes + FQ_CHUNK - 1) / FQ_CHUNK;
-  Blocks mem;
+  Blocks mem(st);
   unsigned long long *nl = nullptr, *base = nullptr;
   uint32_t* err = nullptr;
   if (!mem.get(&nl, nchunks * 8) || !mem.get(&base, (nchunks + 1) * 8) || !mem.get(&err, 4)) return BRUNO_OOM;
@@ -193,13 +251,31 @@ extern "C" bruno_status bruno_fastq_split(const char* d_fastq, uint64_t bytes, c
   o.n_lines = T;
   o.err = err;
   uint64_t* lens = nullptr;
-  uint64_t* offs = d_out_offsets;
-  if (!mem.get(&o.seq_start, R * 8) || !mem.get(&o.seq_end, R * 8) || !mem.get(&lens, R * 8)) return BRUNO_OOM;
-  if (!offs && !mem.get(&offs, (R + 1) * 8)) return BRUNO_OOM;
-  if (d_out_offsets && R > cap_reads) return BRUNO_CAPACITY;
+  if (out_start) {
+    /* ranges only: the sequence lines stay where they are in the file */
+    if (!(o.seq_start = (uint64_t*)alloc(std::max<uint64_t>(1, R) * 8)) ||
+        !(o.seq_end = (uint64_t*)alloc(std::max<uint64_t>(1, R) * 8)))
+      return BRUNO_OOM;
+    *out_start = o.seq_start;
+    *out_end = o.seq_end;
+  } else if (!mem.get(&o.seq_start, R * 8) || !mem.get(&o.seq_end, R * 8)) {
+    return BRUNO_OOM;
+  }
+  if (!mem.get(&lens, R * 8)) return BRUNO_OOM;
+  uint64_t* offs = nullptr;
+  if (have_out) {
+    if (R > cap_reads) return BRUNO_CAPACITY;
+    offs = *out_offs;
+  } else if (copy) {
+    if (!(offs = (uint64_t*)alloc((R + 1) * 8))) return BRUNO_OOM;
+    *out_offs = offs;
+  } else if (!mem.get(&offs, (R + 1) * 8)) {
+    return BRUNO_OOM;
+  }
   hipLaunchKernelGGL(k_fq_mark, dim3((unsigned)nchunks), dim3(FQ_BLOCK), 0, st, s, bytes,
                      (const unsigned long long*)base, o);
-  if (R) hipLaunchKernelGGL(k_fq_len, dim3((unsigned)((R + 255) / 256)), dim3(256), 0, st, o.seq_start, o.seq_end, R, lens);
+  if (R)
+    hipLaunchKernelGGL(k_fq_len, dim3((unsigned)((R + 255) / 256)), dim3(256), 0, st, o.seq_start, o.seq_end, R, lens);
   if (scan_u64(lens, offs, R, st) != hipSuccess) return BRUNO_INTERNAL;
   uint32_t herr = 0;
   uint64_t nb = 0;
@@ -212,16 +288,69 @@ extern "C" bruno_status bruno_fastq_split(const char* d_fastq, uint64_t bytes, c
   if (herr) return BRUNO_BAD_INPUT;
   *n_reads = R;
   *n_bases = nb;
-  if (!d_out_bases) return BRUNO_OK;
-  if (nb > cap_bases) return BRUNO_CAPACITY;
+  if (!copy) return BRUNO_OK;
+  char* ob = nullptr;
+  if (have_out) {
+    if (nb > cap_bases) return BRUNO_CAPACITY;
+    ob = *out_bases;
+  } else {
+    if (!(ob = (char*)alloc(nb + 1))) return BRUNO_OOM;
+    *out_bases = ob;
+  }
   if (R) {
     const uint64_t waves = std::min<uint64_t>(R, 65536);
     hipLaunchKernelGGL(k_fq_copy, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st, s, o.seq_start, offs, R,
-                       (uint8_t*)d_out_bases);
+                       (uint8_t*)ob);
   }
   if (hipStreamSynchronize(st) != hipSuccess) {
     (void)hipGetLastError();
     return BRUNO_INTERNAL;
   }
   return BRUNO_OK;
+}
+
+}  // namespace
+
+extern "C" bruno_status bruno_fastq_split(const char* d_fastq, uint64_t bytes, char* d_out_bases,
+                                          uint64_t* d_out_offsets, uint64_t* n_reads, uint64_t* n_bases,
+                                          void* stream) {
+  if (!n_reads || !n_bases || (bytes && !d_fastq)) return BRUNO_BAD_INPUT;
+  if ((d_out_bases == nullptr) != (d_out_offsets == nullptr)) return BRUNO_BAD_INPUT;
+  const bool have = d_out_bases != nullptr;
+  char* ob = d_out_bases;
+  uint64_t* oo = d_out_offsets;
+  const uint64_t cr = *n_reads, cb = *n_bases;
+  return fastq_split_impl((const uint8_t*)d_fastq, bytes, &ob, &oo, cr, cb, have, have,
+                          [](uint64_t) -> void* { return nullptr; }, n_reads, n_bases, (hipStream_t)stream);
+}
+
+/* Record ranges only: start / end byte of every sequence line in the file,
+ * in arrays from the caller's allocator (no copy; the parse kernel reads
+ * the sequences where they are; used by bruno_parse_count_fastq). */
+bruno_status bruno_fastq_ranges(const char* d_fastq, uint64_t bytes, void* (*alloc)(void*, uint64_t), void* ctx,
+                                uint64_t** starts, uint64_t** ends, uint64_t* n_reads, uint64_t* n_bases,
+                                void* stream) {
+  if (!n_reads || !n_bases || !starts || !ends || !alloc || (bytes && !d_fastq)) return BRUNO_BAD_INPUT;
+  *starts = *ends = nullptr;
+  char* ob = nullptr;
+  uint64_t* oo = nullptr;
+  if (bytes == 0) {
+    *n_reads = *n_bases = 0;
+    return BRUNO_OK;
+  }
+  return fastq_split_impl((const uint8_t*)d_fastq, bytes, &ob, &oo, 0, 0, false, false,
+                          [&](uint64_t b) { return alloc(ctx, b); }, n_reads, n_bases, (hipStream_t)stream, starts,
+                          ends);
+}
+
+/* Split with outputs allocated by the caller's allocator once their sizes
+ * are known (one pass). */
+bruno_status bruno_fastq_split_alloc(const char* d_fastq, uint64_t bytes, void* (*alloc)(void*, uint64_t), void* ctx,
+                                     char** out_bases, uint64_t** out_offsets, uint64_t* n_reads, uint64_t* n_bases,
+                                     void* stream) {
+  if (!n_reads || !n_bases || !out_bases || !out_offsets || !alloc || (bytes && !d_fastq)) return BRUNO_BAD_INPUT;
+  *out_bases = nullptr;
+  *out_offsets = nullptr;
+  return fastq_split_impl((const uint8_t*)d_fastq, bytes, out_bases, out_offsets, 0, 0, false, true,
+                          [&](uint64_t b) { return alloc(ctx, b); }, n_reads, n_bases, (hipStream_t)stream);
 }

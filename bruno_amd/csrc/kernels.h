@@ -194,6 +194,10 @@ __device__ __forceinline__ uint32_t job_of(const uint64_t* first, uint32_t n, ui
 struct ParseArgs {
   const uint8_t* bases;
   const uint64_t* offsets;
+  /* optional: read r is bases[starts[r], ends[r]) instead of
+   * bases[offsets[r], offsets[r+1]) (reads left in place in a FASTQ file) */
+  const uint64_t* starts;
+  const uint64_t* ends;
   uint64_t n_reads;
   int K;
   /* count mode */
@@ -249,8 +253,8 @@ __global__ __launch_bounds__(BS) void k_parse(ParseArgs a) {
       tbase = a.seg[cj];
       tcap = a.seg[cj + 1] - tbase;
     }
-    const uint64_t off = a.offsets[r];
-    const uint64_t L = a.offsets[r + 1] - off;
+    const uint64_t off = a.starts ? a.starts[r] : a.offsets[r];
+    const uint64_t L = (a.starts ? a.ends[r] : a.offsets[r + 1]) - off;
     const uint8_t* s = a.bases + off;
     /* NFilterSequencesIterator: a read with any non-ACGT base is dropped
      * (compact_dbg_freq_construct_clean.cpp:210-213). */

@@ -108,3 +108,21 @@ def test_graph_from_fastq_bytes_equals_oracle(name):
                    branches=job.branches(), stats=job.stats())
         job.close()
         compare_results(31, res, run_oracle(31, th, b, off, 1), 1)
+
+
+@pytest.mark.parametrize("crlf", [False, True])
+def test_in_place_parse_equals_split_reads(crlf):
+    """bruno_parse_count_fastq parses the sequence lines where they are in
+    the file; the graph equals the one from the split (contiguous) reads."""
+    _, reads = random_reads(12, 30000, 8000, (40, 260), 0.01)
+    data = to_fastq(reads, crlf=crlf, final_newline=False)
+    th = thresholds_L(2)
+    job = bruno_amd.GraphJob(31, th)
+    job.parse_count_fastq(data)
+    job.run(1)
+    got = (job.unitigs(), job.branch_fasta(), job.stats()["n_instances"])
+    job.close()
+    b, off = pack(split(data))
+    o = run_oracle(31, th, b, off, 1)
+    assert got[0] == o.chain_fasta() and got[1] == o.branch_fasta()
+    assert got[2] == o.stat("n_instances")
