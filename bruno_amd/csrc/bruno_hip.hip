@@ -7,6 +7,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <cmath>
 #include <mutex>
 #include <cstdio>
 #include <cstdlib>
@@ -1256,6 +1257,73 @@ uint64_t persistent_bytes(const bruno_job* j) {
          2 * j->n_jobs * sizeof(unsigned long long);
 }
 
+/* HyperLogLog estimate from SKETCH_M registers of a 1-in-64 hash sample,
+ * scaled back to all k-mers; small cardinalities by linear counting. */
+double hll_estimate(const uint32_t* M) {
+  const double m = SKETCH_M;
+  const double alpha = 0.7213 / (1.0 + 1.079 / m);
+  double sum = 0;
+  int zeros = 0;
+  for (int q = 0; q < SKETCH_M; ++q) {
+    sum += std::ldexp(1.0, -(int)M[q]);
+    zeros += M[q] == 0;
+  }
+  double e = alpha * m * m / sum;
+  if (e <= 2.5 * m && zeros > 0) e = m * std::log(m / zeros);
+  return e * (double)(1u << SKETCH_SAMPLE_BITS);
+}
+
+/* Hash slots for an estimated distinct count: a margin of four standard
+ * errors (register and sampling noise), at most one slot per instance. */
+uint64_t capacity_from_estimate(double est, uint64_t n_instances, double load_factor) {
+  const double sampled = std::max(1.0, est / (double)(1u << SKETCH_SAMPLE_BITS));
+  const double rel = 1.04 / std::sqrt((double)SKETCH_M) + 1.0 / std::sqrt(sampled);
+  const double d = std::min((double)n_instances, est * (1.0 + 4.0 * rel));
+  return (uint64_t)(d / load_factor) + 4096;
+}
+
+/* sketch of the canonical k-mers of reads (device pointers) */
+template <typename KT>
+bruno_status sketch_t(int K, const char* d_bases, const uint64_t* d_offsets, uint64_t n_reads, uint32_t* regs,
+                      Counters* ctr, hipStream_t st, const uint64_t* d_starts, const uint64_t* d_ends) {
+  ParseArgs a{};
+  a.bases = (const uint8_t*)d_bases;
+  a.offsets = d_offsets;
+  a.starts = d_starts;
+  a.ends = d_ends;
+  a.n_reads = n_reads;
+  a.K = K;
+  a.n_jobs = 1;
+  a.sketch = regs;
+  a.ctr = ctr;
+  const uint64_t waves = std::min<uint64_t>(n_reads, tuning().parse_grid);
+  const unsigned g = (unsigned)std::max<uint64_t>(1, (waves + 3) / 4);
+  hipLaunchKernelGGL((k_parse<KT, 3, 256>), dim3(g), dim3(256), 0, st, a);
+  return BRUNO_OK;
+}
+
+bruno_status estimate_distinct(int K, const char* d_bases, const uint64_t* d_offsets, uint64_t n_reads, double* est,
+                               hipStream_t st, const uint64_t* d_starts = nullptr, const uint64_t* d_ends = nullptr) {
+  uint32_t* regs = nullptr;
+  if (pool_alloc((void**)&regs, SKETCH_M * 4) != hipSuccess) return BRUNO_OOM;
+  std::vector<uint32_t> h(SKETCH_M);
+  bruno_status s = BRUNO_OK;
+  if (hipMemsetAsync(regs, 0, SKETCH_M * 4, st) != hipSuccess) s = BRUNO_INTERNAL;
+  if (s == BRUNO_OK) {
+    if (K <= 31) s = sketch_t<uint64_t>(K, d_bases, d_offsets, n_reads, regs, nullptr, st, d_starts, d_ends);
+    else s = sketch_t<u128>(K, d_bases, d_offsets, n_reads, regs, nullptr, st, d_starts, d_ends);
+  }
+  if (s == BRUNO_OK && (hipMemcpyAsync(h.data(), regs, SKETCH_M * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                        hipStreamSynchronize(st) != hipSuccess)) {
+    (void)hipGetLastError();
+    s = BRUNO_INTERNAL;
+  }
+  (void)hipStreamSynchronize(st);
+  pool_free(regs);
+  if (s == BRUNO_OK) *est = hll_estimate(h.data());
+  return s;
+}
+
 template <typename KT>
 bruno_status parse_count_t(bruno_job* j, const char* d_bases, const uint64_t* d_offsets, uint64_t n_reads,
                            const uint64_t* d_first, const uint64_t* d_starts = nullptr,
@@ -2006,13 +2074,15 @@ bruno_status bruno_parse_count(bruno_job* j, const char* d_bases, const uint64_t
   }
   if (!d_bases || !d_offsets) return BRUNO_BAD_INPUT;
   if (j->cap == 0) {
-    /* size from the batch: instances <= bases; distinct <= instances */
+    /* size the table from a sketch of this batch's distinct k-mers
+     * (instances <= bases bounds it from above) */
     uint64_t ends[2];
     HIP_CHECK(hipMemcpyAsync(&ends[0], d_offsets, 8, hipMemcpyDeviceToHost, j->stream));
     HIP_CHECK(hipMemcpyAsync(&ends[1], d_offsets + n_reads, 8, hipMemcpyDeviceToHost, j->stream));
     HIP_CHECK(hipStreamSynchronize(j->stream));
-    uint64_t nb = ends[1] - ends[0];
-    uint64_t want = (uint64_t)((double)nb / j->load_factor) + 1024;
+    double est = 0;
+    TRY(estimate_distinct(j->K, d_bases, d_offsets, n_reads, &est, j->stream));
+    uint64_t want = capacity_from_estimate(est, ends[1] - ends[0], j->load_factor);
     TRY(alloc_table(j, &want));
   }
   j->job_reads[0] += n_reads;
@@ -2090,8 +2160,12 @@ bruno_status bruno_parse_count_fastq(bruno_job* j, const char* d_fastq, uint64_t
   bruno_status s = bruno_fastq_ranges(d_fastq, bytes, job_alloc, j, &st, &en, &nr, &nb, j->stream);
   if (s == BRUNO_OK && nr) {
     if (j->cap == 0) {
-      const uint64_t want = (uint64_t)((double)nb / j->load_factor) + 1024;
-      s = alloc_table(j, &want);
+      double est = 0;
+      s = estimate_distinct(j->K, d_fastq, nullptr, nr, &est, j->stream, st, en);
+      if (s == BRUNO_OK) {
+        const uint64_t want = capacity_from_estimate(est, nb, j->load_factor);
+        s = alloc_table(j, &want);
+      }
     }
     if (s == BRUNO_OK) {
       j->job_reads[0] += nr;
@@ -2639,6 +2713,26 @@ bruno_status bruno_dev_free(void* p) {
     return BRUNO_INTERNAL;
   }
   pool_free(p);
+  return BRUNO_OK;
+}
+
+bruno_status bruno_estimate_distinct(int k, const char* d_bases, const uint64_t* d_offsets, uint64_t n_reads,
+                                     double* estimate, uint64_t* capacity, double load_factor, void* stream) {
+  if (k < 3 || k > 63 || !estimate || (n_reads && (!d_bases || !d_offsets))) return BRUNO_BAD_INPUT;
+  *estimate = 0;
+  if (n_reads) TRY(estimate_distinct(k, d_bases, d_offsets, n_reads, estimate, (hipStream_t)stream));
+  if (capacity) {
+    uint64_t ends[2] = {0, 0};
+    if (n_reads &&
+        (hipMemcpyAsync(&ends[0], d_offsets, 8, hipMemcpyDeviceToHost, (hipStream_t)stream) != hipSuccess ||
+         hipMemcpyAsync(&ends[1], d_offsets + n_reads, 8, hipMemcpyDeviceToHost, (hipStream_t)stream) != hipSuccess ||
+         hipStreamSynchronize((hipStream_t)stream) != hipSuccess)) {
+      (void)hipGetLastError();
+      return BRUNO_INTERNAL;
+    }
+    const double lf = (load_factor > 0 && load_factor < 1) ? load_factor : 0.5;
+    *capacity = capacity_from_estimate(*estimate, ends[1] - ends[0], lf);
+  }
   return BRUNO_OK;
 }
 

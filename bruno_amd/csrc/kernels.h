@@ -212,6 +212,8 @@ struct ParseArgs {
   const uint64_t* seg;
   unsigned long long* job_inst; /* per-job k-mer instances (n_jobs > 1) */
   unsigned long long* job_drop; /* per-job dropped reads (n_jobs > 1) */
+  /* sketch mode: SKETCH_M HyperLogLog registers per job */
+  uint32_t* sketch;
   /* record mode */
   const uint64_t* rec_base; /* per-read first record index */
   uint64_t* out_kmers;
@@ -226,9 +228,17 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
   return v;
 }
 
+/* Distinct-k-mer sketch: canonical k-mers whose hash has its top
+ * SKETCH_SAMPLE_BITS bits zero (1 in 64) go to a HyperLogLog of SKETCH_M
+ * registers (index = the next 12 hash bits, value = 1 + leading zeros of
+ * the remaining 46 bits). */
+constexpr int SKETCH_SAMPLE_BITS = 6;
+constexpr int SKETCH_INDEX_BITS = 12;
+constexpr int SKETCH_M = 1 << SKETCH_INDEX_BITS;
+
 /* MODE 0: count records per read; MODE 1: write records; MODE 2: count into
- * the table.  BS is the workgroup size (the work is per wave, so any
- * multiple of 64 works; it is a tuning knob). */
+ * the table; MODE 3: distinct-k-mer sketch only.  BS is the workgroup size
+ * (the work is per wave, so any multiple of 64 works; it is a tuning knob). */
 template <typename KT, int MODE, int BS = BLOCK>
 __global__ __launch_bounds__(BS) void k_parse(ParseArgs a) {
   const int lane = threadIdx.x & 63;
@@ -244,7 +254,7 @@ __global__ __launch_bounds__(BS) void k_parse(ParseArgs a) {
   uint64_t j_lo = 0, j_hi = 0, tbase = 0, tcap = a.cap;
   unsigned long long j_inst = 0;
   for (uint64_t r = wave; r < a.n_reads; r += nwaves) {
-    if (MODE == 2 && a.n_jobs > 1 && !(r >= j_lo && r < j_hi)) {
+    if ((MODE == 2 || MODE == 3) && a.n_jobs > 1 && !(r >= j_lo && r < j_hi)) {
       if (lane == 0 && j_inst) atomicAdd(&a.job_inst[cj], j_inst);
       j_inst = 0;
       cj = job_of(a.job_first, a.n_jobs, r);
@@ -303,8 +313,17 @@ __global__ __launch_bounds__(BS) void k_parse(ParseArgs a) {
           km = rv;
           e = biedge_rc(e);
         }
-        if (MODE == 1) {
-          a.out_kmers[(rbase + i) * kt_traits<KT>::words] = (uint64_t)km;
+        if (MODE == 3) {
+          const uint64_t h = kmer_hash(km);
+          if ((h >> (64 - SKETCH_SAMPLE_BITS)) == 0) {
+            const uint32_t idx = (uint32_t)(h >> (64 - SKETCH_SAMPLE_BITS - SKETCH_INDEX_BITS)) & (SKETCH_M - 1);
+            const uint64_t rest = (h << (SKETCH_SAMPLE_BITS + SKETCH_INDEX_BITS)) |
+                                  (1ULL << (SKETCH_SAMPLE_BITS + SKETCH_INDEX_BITS - 1));
+            const uint32_t rank = (uint32_t)__clzll(rest) + 1;
+            uint32_t* reg = a.sketch + (uint64_t)cj * SKETCH_M + idx;
+            if (rank > __hip_atomic_load(reg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(reg, rank);
+          }
+        } else if (MODE == 1) {
           if (kt_traits<KT>::words == 2)
             a.out_kmers[(rbase + i) * kt_traits<KT>::words + 1] = (uint64_t)((u128)km >> 64);
           a.out_biedges[rbase + i] = e;

@@ -328,3 +328,13 @@ def fastq_split_device(fastq_ptr, nbytes, stream=None):
     finally:
         db.free()
         do.free()
+
+
+def estimate_distinct(k, bases_ptr, offsets_ptr, n_reads, load_factor=0.5, stream=None):
+    """(estimated distinct canonical k-mers, hash slots needed) of reads in
+    device memory, from the library's sketch."""
+    est, cap = ctypes.c_double(0), ctypes.c_uint64(0)
+    check(lib().bruno_estimate_distinct(int(k), ctypes.c_void_p(bases_ptr), ctypes.c_void_p(offsets_ptr),
+                                        int(n_reads), ctypes.byref(est), ctypes.byref(cap), float(load_factor),
+                                        stream), "bruno_estimate_distinct")
+    return est.value, cap.value

@@ -54,7 +54,8 @@ typedef struct {
   uint64_t thresholds[6];   /* {k_lo,k_hi,k1_lo,k1_hi,k2_lo,k2_hi}, reference order
                                (debruijn_graph_map.hpp:1261); -L t = {0,65536,t,65536,0,65536} */
   uint64_t capacity;        /* hash slots (distinct k-mers / load factor);
-                               0 = derive from the first batch's k-mer count      */
+                               0 = size from a sketch of the first batch's
+                               distinct k-mers (bruno_estimate_distinct)           */
   double load_factor;       /* used when capacity == 0; 0 = default 0.5          */
   void* stream;             /* hipStream_t for all of this job's work; NULL = 0   */
 } bruno_job_desc;
@@ -133,6 +134,15 @@ bruno_status bruno_parse_count(bruno_job* job, const char* d_bases, const uint64
 /* Same from host buffers (copies over PCIe first). */
 bruno_status bruno_parse_count_host(bruno_job* job, const char* bases, const uint64_t* offsets,
                                     uint64_t n_reads);
+
+/* Estimated number of distinct canonical k-mers in reads (device pointers),
+ * from a HyperLogLog sketch of a 1-in-64 hash sample (one pass over the
+ * reads, no table); with capacity != NULL also the hash slots a job needs
+ * at load_factor (0 = 0.5), with a margin of four standard errors.  A job
+ * created with capacity 0 sizes its table this way from its first batch.
+ * No counterpart in the reference (its hash maps grow). */
+bruno_status bruno_estimate_distinct(int k, const char* d_bases, const uint64_t* d_offsets, uint64_t n_reads,
+                                     double* estimate, uint64_t* capacity, double load_factor, void* stream);
 
 /* FASTQ ingest on the device.  Replaces the reader under parse_file_data
  * (compact_dbg_io.hpp:165-217; kmerind's FASTQ parser, one sequence per
