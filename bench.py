@@ -175,6 +175,9 @@ def main():
         lf = tuned["load_factor"]
     cap = capacity_for(s, k, lf)
     job = bruno_amd.GraphJob(k, cfg["thresholds"], capacity=cap)
+    # steps repeat one job: its table is cleared beside list ranking and
+    # output of each step, so the next step's reset has nothing left to clear
+    job.set_recycle(True)
     times, phases = [], []
     stats = None
     for it in range(args.warmup + args.steps):

@@ -100,6 +100,7 @@ EXPORTS = (
     "bruno_server_get_unitigs", "bruno_server_get_branch_fasta", "bruno_server_release",
     "bruno_server_get_stats", "bruno_server_destroy", "bruno_set_tuning", "bruno_get_tuning",
     "bruno_fastq_split", "bruno_parse_count_fastq", "bruno_parse_count_fastq_host", "bruno_estimate_distinct",
+    "bruno_job_set_recycle",
 )
 
 _LIB = None
@@ -167,6 +168,7 @@ def lib():
         L.bruno_fastq_split.argtypes = [vp, u64, vp, vp, u64p, u64p, vp]
         L.bruno_parse_count_fastq.argtypes = [vp, vp, u64]
         L.bruno_parse_count_fastq_host.argtypes = [vp, vp, u64]
+        L.bruno_job_set_recycle.argtypes = [vp, i]
         L.bruno_estimate_distinct.argtypes = [i, vp, vp, u64, ctypes.POINTER(ctypes.c_double), u64p,
                                               ctypes.c_double, vp]
         L.bruno_get_tuning.argtypes = [ctypes.POINTER(Tuning)]

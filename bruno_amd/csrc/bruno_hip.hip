@@ -103,14 +103,14 @@ __device__ __forceinline__ void scan_slot(const Slot<u128>* p, uint64_t& fw, uin
 /* Output indices staged in LDS and appended to a global list with one
  * atomic per flush (the filter scan keeps a few percent of the slots, and a
  * global atomic per tile would serialise on the list counter). */
-template <int CAP>
+template <int CAP, typename V = uint64_t>
 struct LdsList {
-  uint64_t v[CAP];
+  V v[CAP];
   uint32_t n;
   unsigned long long base;
 };
-template <int CAP>
-__device__ __forceinline__ void lds_flush(LdsList<CAP>& L, uint64_t* out, uint64_t out_cap, unsigned long long* counter,
+template <int CAP, typename V>
+__device__ __forceinline__ void lds_flush(LdsList<CAP, V>& L, V* out, uint64_t out_cap, unsigned long long* counter,
                                           uint32_t* err) {
   __syncthreads();
   const uint32_t n = L.n;
@@ -519,33 +519,54 @@ constexpr uint32_t ERR_WALK = 32;
 
 __device__ __forceinline__ uint32_t end_bit(int e) { return e ? END_OUT : 0u; }
 
-/* ruler_mask + 1 = sampling period of the rulers (a power of two) */
+constexpr int RULER_LDS = 2048; /* rulers staged per block before an append */
+
+/* ruler_mask + 1 = sampling period of the rulers (a power of two).  Marks
+ * the rulers (chain ends and sampled nodes) and lists them (LDS-staged
+ * appends), and counts ends that break well-formedness. */
 __global__ __launch_bounds__(BLOCK) void k_lr_check(const LRNode* lr, uint64_t n_chain, uint8_t* ruler,
-                                                    uint64_t ruler_mask, Counters* ctr) {
+                                                    uint64_t ruler_mask, uint32_t* rlist, uint64_t rlist_cap,
+                                                    Counters* ctr) {
+  __shared__ LdsList<RULER_LDS, uint32_t> Lr;
+  if (threadIdx.x == 0) Lr.n = 0;
+  __syncthreads();
   const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
   unsigned long long bad = 0;
-  for (uint64_t t = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; t < n_chain; t += stride) {
-    const LRNode st = lr[t];
-    bool head = false;
-    for (int E = 0; E < 2; ++E) {
-      uint32_t d = st.d[E];
-      if (is_terminal(d)) {
-        head = true;
-        continue;
+  for (uint64_t base = (uint64_t)blockIdx.x * BLOCK; base < n_chain; base += stride) {
+    const uint64_t t = base + threadIdx.x;
+    bool is_ruler = false;
+    if (t < n_chain) {
+      const LRNode st = lr[t];
+      bool head = false;
+      for (int E = 0; E < 2; ++E) {
+        uint32_t d = st.d[E];
+        if (is_terminal(d)) {
+          head = true;
+          continue;
+        }
+        /* initial state: a non-terminal end has dist 1 and a resolved pointer */
+        uint32_t j = st.p[E] & IDX_MASK;
+        int e = (st.p[E] & END_OUT) ? 1 : 0;
+        if (d != 1u || j >= n_chain || (j == t && e == E)) {
+          ++bad;
+          continue;
+        }
+        const LRNode o = lr[j];
+        if (o.d[e] != 1u || o.p[e] != ((uint32_t)t | end_bit(E))) ++bad;
       }
-      /* initial state: a non-terminal end has dist 1 and a resolved pointer */
-      uint32_t j = st.p[E] & IDX_MASK;
-      int e = (st.p[E] & END_OUT) ? 1 : 0;
-      if (d != 1u || j >= n_chain || (j == t && e == E)) {
-        ++bad;
-        continue;
-      }
-      const LRNode o = lr[j];
-      if (o.d[e] != 1u || o.p[e] != ((uint32_t)t | end_bit(E))) ++bad;
+      bool sampled = (fmix64(t * 0x9e3779b97f4a7c15ULL + 0x1234567ULL) & ruler_mask) == 0;
+      is_ruler = head || sampled;
+      ruler[t] = is_ruler ? 1 : 0;
     }
-    bool sampled = (fmix64(t * 0x9e3779b97f4a7c15ULL + 0x1234567ULL) & ruler_mask) == 0;
-    ruler[t] = (head || sampled) ? 1 : 0;
+    if (is_ruler) Lr.v[atomicAdd(&Lr.n, 1u)] = (uint32_t)t;
+    /* all threads read the count between two barriers, then keep room for
+     * one more tile */
+    __syncthreads();
+    const uint32_t nr = Lr.n;
+    __syncthreads();
+    if (nr > RULER_LDS - BLOCK) lds_flush(Lr, rlist, rlist_cap, &ctr->lr_next, &ctr->err);
   }
+  lds_flush(Lr, rlist, rlist_cap, &ctr->lr_next, &ctr->err);
   block_add(bad, &ctr->lr_conflict);
 }
 
@@ -553,18 +574,6 @@ struct WalkRec {
   uint32_t who;   /* ruler (node walk) or chain end (ruler walk) */
   uint32_t edist; /* bit 31: end, low 31 bits: distance */
 };
-
-__global__ __launch_bounds__(BLOCK) void k_lr_list_rulers(const uint8_t* ruler, uint64_t n_chain, uint32_t* list,
-                                                          Counters* ctr) {
-  __shared__ ReserveSmem sm;
-  const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
-  for (uint64_t base = (uint64_t)blockIdx.x * BLOCK; base < n_chain; base += stride) {
-    const uint64_t t = base + threadIdx.x;
-    bool r = t < n_chain && ruler[t];
-    unsigned long long pos = block_reserve(r, &ctr->lr_next, sm);
-    if (r) list[pos] = (uint32_t)t;
-  }
-}
 
 /* From every ruler, along each non-terminal end, to the next ruler.  Each
  * visited node records, at the end it was entered through, the ruler and
@@ -1134,6 +1143,13 @@ struct bruno_job {
 
   hipEvent_t ev[12];
   bool ev_ok = false;
+
+  /* recycle: once the chains are built the table is cleared on a side
+   * stream, overlapping list ranking and output; the next reset waits */
+  bool recycle = false;
+  hipStream_t side = nullptr;
+  hipEvent_t ev_chains = nullptr, ev_cleared = nullptr;
+  bool clear_pending = false;
   bruno_phase_ms ms{};
 
   std::vector<void*> allocs;
@@ -1209,9 +1225,19 @@ bruno_status sync_counters(bruno_job* j) {
   return BRUNO_OK;
 }
 
-void fill64(bruno_job* j, void* p, uint64_t n_words, uint64_t v) {
-  hipLaunchKernelGGL(k_fill64, dim3(grid_for(n_words, BLOCK, 1u << 14)), dim3(BLOCK), 0, j->stream, (uint64_t*)p,
-                     n_words, v);
+void fill64(bruno_job* j, void* p, uint64_t n_words, uint64_t v, hipStream_t st) {
+  hipLaunchKernelGGL(k_fill64, dim3(grid_for(n_words, BLOCK, 1u << 14)), dim3(BLOCK), 0, st, (uint64_t*)p, n_words,
+                     v);
+}
+void fill64(bruno_job* j, void* p, uint64_t n_words, uint64_t v) { fill64(j, p, n_words, v, j->stream); }
+
+size_t slot_bytes(const bruno_job* j) { return j->words == 1 ? sizeof(Slot<uint64_t>) : sizeof(Slot<u128>); }
+
+/* table, side table and per-job counters back to empty on stream st */
+void clear_table(bruno_job* j, hipStream_t st) {
+  fill64(j, j->table, j->cap * slot_bytes(j) / 8, EMPTY64, st);
+  fill64(j, j->spill, j->spcap * sizeof(Spill) / 8, 0, st);
+  fill64(j, j->d_jstat, 2 * j->n_jobs, 0, st);
 }
 
 void ev_record(bruno_job* j, int i) {
@@ -1227,7 +1253,6 @@ double ev_ms(bruno_job* j, int a, int b) {
   return ms;
 }
 
-size_t slot_bytes(const bruno_job* j) { return j->words == 1 ? sizeof(Slot<uint64_t>) : sizeof(Slot<u128>); }
 
 /* One table for all jobs: job i gets caps[i] slots from seg[i]. */
 bruno_status alloc_table(bruno_job* j, const uint64_t* caps) {
@@ -1482,18 +1507,18 @@ bruno_status list_rank_fast(bruno_job* j, bool* used) {
   j->hctr->lr_cycle = 0;
   HIP_CHECK(hipMemcpyAsync(&j->ctr->lr_count, &j->hctr->lr_count, 4 * sizeof(unsigned long long),
                            hipMemcpyHostToDevice, j->stream));
+  TRY(dev_alloc(j, (void**)&rlist, nc * 4));
   hipLaunchKernelGGL(k_lr_check, dim3(grid_for(nc)), dim3(BLOCK), 0, j->stream, j->lr, nc, ruler,
-                     (uint64_t)tuning().ruler_sample - 1, j->ctr);
+                     (uint64_t)tuning().ruler_sample - 1, rlist, nc, j->ctr);
   TRY(sync_counters(j));
   bool well_formed = j->hctr->lr_conflict == 0;
   if (!well_formed) {
     void* a = ruler;
     dev_free(j, a, nc);
+    a = rlist;
+    dev_free(j, a, nc * 4);
     return BRUNO_OK;
   }
-  TRY(dev_alloc(j, (void**)&rlist, nc * 4));
-  hipLaunchKernelGGL(k_lr_list_rulers, dim3(grid_for(nc)), dim3(BLOCK), 0, j->stream, ruler, nc, rlist, j->ctr);
-  TRY(sync_counters(j));
   uint64_t nr = j->hctr->lr_next;
   TRY(dev_alloc(j, (void**)&rec, nc * 2 * sizeof(WalkRec)));
   TRY(dev_alloc(j, (void**)&link, nc * 2 * sizeof(WalkRec)));
@@ -1983,6 +2008,12 @@ bruno_status bruno_batch_create(const bruno_job_desc* d, uint32_t n_jobs, const 
 bruno_status bruno_job_destroy(bruno_job* j) {
   if (!j) return BRUNO_BAD_INPUT;
   (void)hipStreamSynchronize(j->stream);
+  if (j->side) {
+    (void)hipStreamSynchronize(j->side);
+    (void)hipStreamDestroy(j->side);
+  }
+  if (j->ev_chains) (void)hipEventDestroy(j->ev_chains);
+  if (j->ev_cleared) (void)hipEventDestroy(j->ev_cleared);
   for (void* p : j->allocs) pool_free(p);
   j->allocs.clear();
   pool_free(j->ctr);
@@ -2029,9 +2060,13 @@ bruno_status bruno_job_reset(bruno_job* j) {
   HIP_CHECK(hipMemsetAsync(j->ctr, 0, sizeof(Counters), j->stream));
   if (j->table) {
     ev_record(j, 10);
-    fill64(j, j->table, j->cap * slot_bytes(j) / 8, EMPTY64);
-    fill64(j, j->spill, j->spcap * sizeof(Spill) / 8, 0);
-    fill64(j, j->d_jstat, 2 * j->n_jobs, 0);
+    if (j->clear_pending) {
+      /* cleared in the background since the chains were built */
+      HIP_CHECK(hipStreamWaitEvent(j->stream, j->ev_cleared, 0));
+      j->clear_pending = false;
+    } else {
+      clear_table(j, j->stream);
+    }
     ev_record(j, 11);
   }
   HIP_CHECK(hipStreamSynchronize(j->stream));
@@ -2217,6 +2252,15 @@ bruno_status bruno_build_chains(bruno_job* j) {
   if (j->words == 1) TRY(chains_t<uint64_t>(j));
   else TRY(chains_t<u128>(j));
   ev_record(j, 5);
+  if (j->recycle && j->table) {
+    /* the table is not read after this point: clear it now, beside the
+     * list ranking and the output */
+    HIP_CHECK(hipEventRecord(j->ev_chains, j->stream));
+    HIP_CHECK(hipStreamWaitEvent(j->side, j->ev_chains, 0));
+    clear_table(j, j->side);
+    HIP_CHECK(hipEventRecord(j->ev_cleared, j->side));
+    j->clear_pending = true;
+  }
   HIP_CHECK(hipStreamSynchronize(j->stream));
   j->ms.chains_ms = ev_ms(j, 4, 5);
   j->stage = ST_CHAINS;
@@ -2465,7 +2509,7 @@ bruno_status bruno_batch_get_k2(bruno_job* j, uint32_t idx, uint64_t* kmers, uin
 
 bruno_status bruno_batch_get_nodes(bruno_job* j, uint32_t idx, bruno_node* buf, uint64_t* n) {
   if (!valid_idx(j, idx) || !n) return BRUNO_BAD_INPUT;
-  if (j->stage < ST_FILTERED) return BRUNO_STATE;
+  if (j->stage < ST_FILTERED || j->clear_pending) return BRUNO_STATE;
   const uint64_t np = j->n_present;
   const uint64_t* lists[2] = {j->chain_slot, j->branch_slot};
   const uint64_t lens[2] = {j->n_chain, j->n_branch};
@@ -2713,6 +2757,20 @@ bruno_status bruno_dev_free(void* p) {
     return BRUNO_INTERNAL;
   }
   pool_free(p);
+  return BRUNO_OK;
+}
+
+bruno_status bruno_job_set_recycle(bruno_job* j, int on) {
+  if (!valid_job(j)) return BRUNO_BAD_INPUT;
+  if (on && !j->side) {
+    if (hipStreamCreateWithFlags(&j->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&j->ev_chains, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&j->ev_cleared, hipEventDisableTiming) != hipSuccess) {
+      (void)hipGetLastError();
+      return BRUNO_INTERNAL;
+    }
+  }
+  j->recycle = on != 0;
   return BRUNO_OK;
 }
 

@@ -324,6 +324,7 @@ __global__ __launch_bounds__(BS) void k_parse(ParseArgs a) {
             if (rank > __hip_atomic_load(reg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(reg, rank);
           }
         } else if (MODE == 1) {
+          a.out_kmers[(rbase + i) * kt_traits<KT>::words] = (uint64_t)km;
           if (kt_traits<KT>::words == 2)
             a.out_kmers[(rbase + i) * kt_traits<KT>::words + 1] = (uint64_t)((u128)km >> 64);
           a.out_biedges[rbase + i] = e;

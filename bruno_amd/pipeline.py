@@ -117,6 +117,12 @@ class GraphJob(_Views):
         """Clear the job for reuse, keeping its hash-table allocation."""
         check(lib().bruno_job_reset(self.h), "bruno_job_reset")
 
+    def set_recycle(self, on=True):
+        """Clear the table in the background once the chains are built (the
+        next reset then has nothing to clear; nodes() is unavailable after
+        build_chains)."""
+        check(lib().bruno_job_set_recycle(self.h, 1 if on else 0), "bruno_job_set_recycle")
+
     # ---- phases ------------------------------------------------------
     def parse_count(self, bases, offsets):
         """Host arrays (bytes / uint8 and uint64 offsets, n_reads+1)."""

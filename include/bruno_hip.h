@@ -123,6 +123,14 @@ bruno_status bruno_job_destroy(bruno_job* job);
  * compact_dbg_freq_construct_clean.cpp:826) so a serving loop reuses HBM. */
 bruno_status bruno_job_reset(bruno_job* job);
 
+/* Recycle mode for jobs run again and again (a serving loop, a benchmark):
+ * as soon as bruno_build_chains has built the chains, the table is cleared
+ * on a side stream, overlapping list ranking and output, and the next
+ * bruno_job_reset waits for that instead of clearing.  In this mode the
+ * filtered graph is gone after bruno_build_chains: bruno_get_nodes returns
+ * BRUNO_STATE.  No counterpart in the reference. */
+bruno_status bruno_job_set_recycle(bruno_job* job, int on);
+
 /* Replaces parse_file_data<debruijn_kmer_simple_biedge_parser> +
  * compute_biedge_freqencies (compact_dbg_freq_construct_clean.cpp:318-328,
  * debruijn_graph_map.hpp:1085-1141), or the unthresholded idx.insert

@@ -11,7 +11,7 @@ import pytest
 
 import bruno_amd
 from bruno_amd.configs import config1, config2, config3
-from gpu_common import compare_all, run_gpu
+from gpu_common import compare_all, run_gpu, run_oracle
 from helpers import REF_DATA, pack, random_reads, read_fastq, thresholds_L
 
 pytestmark = pytest.mark.gpu
@@ -278,3 +278,27 @@ def test_tuning_knobs_do_not_change_results(knobs):
         compare_all(31, None, b, off, variant=1, stages=False)
     finally:
         set_tuning(**before)
+
+
+def test_recycle_mode_repeats_exactly():
+    """Recycle mode clears the table beside list ranking and output; three
+    runs of one job in a row equal the oracle, and the filtered graph is
+    unavailable once the chains are built."""
+    _, reads = random_reads(31, 30000, 6000, 150, 0.01)
+    b, off = pack(reads)
+    th = thresholds_L(2)
+    o = run_oracle(31, th, b, off, 1)
+    job = bruno_amd.GraphJob(31, th, capacity=1 << 20)
+    job.set_recycle(True)
+    for _ in range(3):
+        job.reset()
+        job.parse_count(b, off)
+        job.filter()
+        job.build_chains()
+        with pytest.raises(bruno_amd.BrunoError):
+            job.nodes()
+        job.list_rank(1)
+        job.compact()
+        assert job.unitigs() == o.chain_fasta()
+        assert job.branch_fasta() == o.branch_fasta()
+    job.close()
