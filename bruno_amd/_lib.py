@@ -28,7 +28,7 @@ class PhaseMs(ctypes.Structure):
 STAT_FIELDS = ("n_reads", "n_reads_dropped", "n_instances", "n_distinct", "n_k2", "n_overflow",
                "n_nodes", "n_branches", "n_chain_nodes", "n_cycle_nodes", "n_isolated",
                "n_unitig_nodes", "n_unitigs", "iterations", "n_conflicts", "capacity",
-               "device_bytes", "peak_device_bytes")
+               "device_bytes", "peak_device_bytes", "output_by_chains")
 
 
 class Stats(ctypes.Structure):

@@ -812,11 +812,15 @@ __device__ __forceinline__ KT ptr_kmer(const kops<KT>& ops, uint32_t ptr, const 
   return same ? c : ops.rc(c);
 }
 
-/* to_listranked_chain_node (debruijn_chain_operations.hpp:261-310) */
+/* to_listranked_chain_node (debruijn_chain_operations.hpp:261-310).
+ * out_head / out_len (optional): the chain node at the rep end of the
+ * node's chain (NONE32 when that end is not a chain node) and 1 + the
+ * distance to the far end (the chain's length when the node is the head) */
 template <typename KT>
 __global__ __launch_bounds__(BLOCK) void k_ranked(const LRNode* lr, const KT* ckmer, uint64_t n_chain, const KT* vkmer,
                                                   const uint8_t* alive, int K, const uint32_t* cjob, KT* out_k,
-                                                  KT* out_rep, uint32_t* out_dist, uint32_t* out_job, Counters* ctr) {
+                                                  KT* out_rep, uint32_t* out_dist, uint32_t* out_job,
+                                                  uint32_t* out_head, uint32_t* out_len, Counters* ctr) {
   __shared__ ReserveSmem sm;
   kops<KT> ops(K);
   const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
@@ -824,7 +828,7 @@ __global__ __launch_bounds__(BLOCK) void k_ranked(const LRNode* lr, const KT* ck
     const uint64_t i = base + threadIdx.x;
     bool a = (i < n_chain) && alive[i];
     KT Kv = 0, rep = 0;
-    uint32_t dist = 0;
+    uint32_t dist = 0, head = NONE32, len = 0;
     if (a) {
       const LRNode st = lr[i];
       KT x = ckmer[i];
@@ -833,7 +837,11 @@ __global__ __launch_bounds__(BLOCK) void k_ranked(const LRNode* lr, const KT* ck
       bool sense = !(R < L);
       Kv = sense ? x : ops.rc(x);
       rep = sense ? L : R;
-      dist = cdist(sense ? st.d[0] : st.d[1]);
+      const int rs = sense ? 0 : 1;
+      dist = cdist(st.d[rs]);
+      const uint32_t z = st.p[rs] & IDX_MASK;
+      head = is_terminal(st.d[rs]) ? (uint32_t)i : (z < n_chain ? z : NONE32);
+      len = cdist(st.d[1 - rs]) + 1;
     }
     unsigned long long pos = block_reserve(a, &ctr->n_out, sm);
     if (a) {
@@ -841,8 +849,97 @@ __global__ __launch_bounds__(BLOCK) void k_ranked(const LRNode* lr, const KT* ck
       out_rep[pos] = rep;
       out_dist[pos] = dist;
       if (out_job) out_job[pos] = cjob[i];
+      if (out_head) {
+        out_head[pos] = head;
+        out_len[pos] = len;
+      }
     }
   }
+}
+
+/* positions of the chain heads (dist 0) among the listranked nodes */
+__global__ __launch_bounds__(BLOCK) void k_list_heads_out(const uint32_t* out_dist, uint64_t n, uint32_t* heads,
+                                                          Counters* ctr) {
+  __shared__ ReserveSmem sm;
+  const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+  for (uint64_t base = (uint64_t)blockIdx.x * BLOCK; base < n; base += stride) {
+    const uint64_t p = base + threadIdx.x;
+    const bool h = p < n && out_dist[p] == 0;
+    unsigned long long at = block_reserve(h, &ctr->n_cand, sm);
+    if (h) heads[at] = (uint32_t)p;
+  }
+}
+
+/* per chain, by the node at its rep end */
+template <typename KT>
+struct HeadMeta {
+  KT rep;
+  uint32_t off, len, job, pad;
+};
+
+/* for the heads in (job, rep) order (hs[r] = position of the r-th head):
+ * each chain's range [coff[r], coff[r] + len) and key, stored by head
+ * node; two heads with one key count as bad */
+template <typename KT>
+__global__ void k_head_meta(const uint32_t* hs, const uint64_t* coff, const uint32_t* out_head,
+                            const uint32_t* out_len, const KT* out_rep, const uint32_t* out_job, uint64_t nh,
+                            uint64_t n_chain, HeadMeta<KT>* meta, unsigned long long* bad) {
+  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nh) return;
+  const uint32_t p = hs[r];
+  const uint32_t node = out_head[p];
+  const uint32_t job = out_job ? out_job[p] : 0u;
+  const KT rep = out_rep[p];
+  bool ok = node < n_chain;
+  if (r > 0) {
+    const uint32_t q = hs[r - 1];
+    ok = ok && !(out_rep[q] == rep && (out_job ? out_job[q] : 0u) == job);
+  }
+  if (!ok) {
+    atomicAdd(bad, 1ULL);
+    return;
+  }
+  HeadMeta<KT> m;
+  m.rep = rep;
+  m.off = (uint32_t)coff[r];
+  m.len = out_len[p];
+  m.job = job;
+  m.pad = 0;
+  meta[node] = m;
+}
+
+__global__ void k_take_len(const uint32_t* hs, const uint32_t* out_len, uint64_t nh, uint64_t* len) {
+  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < nh) len[r] = out_len[hs[r]];
+}
+
+/* every node to its chain's offset + its dist; bad counts nodes whose head
+ * is missing, whose key differs from the head's, or whose dist is past the
+ * chain's end (meta.len is 0 for a node that heads no chain) */
+template <typename KT>
+__global__ void k_scatter_ranked(const uint32_t* out_head, const uint32_t* out_dist, const KT* out_rep,
+                                 const uint32_t* out_job, uint64_t n, uint64_t n_chain, const HeadMeta<KT>* meta,
+                                 uint32_t* perm, unsigned long long* bad) {
+  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const uint32_t h = out_head[p];
+  bool ok = h < n_chain;
+  if (ok) {
+    const HeadMeta<KT> m = meta[h];
+    const uint32_t d = out_dist[p];
+    const uint64_t pos = (uint64_t)m.off + d;
+    ok = d < m.len && pos < n && m.rep == out_rep[p] && m.job == (out_job ? out_job[p] : 0u);
+    if (ok) perm[pos] = (uint32_t)p;
+  }
+  if (!ok) atomicAdd(bad, 1ULL);
+}
+
+/* positions no node was placed at (perm is pre-filled with ~0) */
+__global__ __launch_bounds__(BLOCK) void k_count_holes(const uint32_t* perm, uint64_t n, unsigned long long* bad) {
+  const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+  uint32_t holes = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += stride) holes += perm[i] == ~0u;
+  if (holes) atomicAdd(bad, (unsigned long long)holes);
 }
 
 template <typename KT>
@@ -1127,6 +1224,8 @@ struct bruno_job {
   uint64_t vcap = 0;
   uint32_t iterations = 0;
   uint64_t n_cycle = 0, n_isolated = 0, conflicts = 0;
+  bool lr_contracted = false; /* ranked by the contraction path (well-formed chains) */
+  bool out_by_chains = false; /* unitig order placed from chain offsets, not sorted */
 
   /* output */
   uint64_t n_out = 0, n_unitigs = 0;
@@ -1801,6 +1900,98 @@ bruno_status job_bounds(bruno_job* j, const uint32_t* job_of_item, const uint32_
   return BRUNO_OK;
 }
 
+/* chain_rep_less order without sorting every node: after the contraction
+ * ranking each chain's nodes carry distances 0..m-1 from its rep end, so
+ * only the chain heads are sorted (by job, then rep) and every node goes to
+ * its chain's offset + its distance.  The result is the sorted order when
+ * no node is rejected and no position stays empty: n nodes then fill n
+ * places one each, the chains' keys strictly increase and each chain's
+ * nodes sit in dist order.  Otherwise *placed = false and the caller
+ * sorts. */
+template <typename KT>
+bruno_status place_by_chains(bruno_job* j, uint64_t n, const uint32_t* out_head, const uint32_t* out_len, bool* placed) {
+  *placed = false;
+  const uint64_t nc = j->n_chain;
+  const uint32_t nj = j->n_jobs;
+  j->hctr->n_cand = 0;
+  HIP_CHECK(hipMemcpyAsync(&j->ctr->n_cand, &j->hctr->n_cand, 8, hipMemcpyHostToDevice, j->stream));
+  uint32_t* heads = nullptr;
+  TRY(dev_alloc(j, (void**)&heads, n * 4));
+  hipLaunchKernelGGL(k_list_heads_out, dim3(grid_for(n)), dim3(BLOCK), 0, j->stream, j->out_dist, n, heads, j->ctr);
+  TRY(sync_counters(j));
+  const uint64_t nh = j->hctr->n_cand;
+  void* a = nullptr;
+  if (nh == 0 || nh > n) {
+    a = heads;
+    dev_free(j, a, n * 4);
+    return BRUNO_OK;
+  }
+  const dim3 gh(grid_for(nh, BLOCK, 1u << 30)), gn(grid_for(n, BLOCK, 1u << 30));
+  /* heads in (job, rep) order */
+  KT* hkeys = nullptr;
+  uint32_t *hperm = nullptr, *hs = nullptr, *hjob = nullptr;
+  TRY(dev_alloc(j, (void**)&hkeys, nh * sizeof(KT)));
+  TRY(dev_alloc(j, (void**)&hperm, nh * 4));
+  TRY(dev_alloc(j, (void**)&hs, nh * 4));
+  hipLaunchKernelGGL((k_gather_kmer<KT>), gh, dim3(BLOCK), 0, j->stream, (const KT*)j->out_rep, heads, hkeys, nh);
+  TRY(sort_perm_kmer<KT>(j, hkeys, hperm, nh, j->K, false));
+  if (nj > 1) {
+    TRY(dev_alloc(j, (void**)&hjob, nh * 4));
+    hipLaunchKernelGGL(k_gather_u32, gh, dim3(BLOCK), 0, j->stream, j->out_job, heads, hjob, nh);
+    TRY(sort_perm_u32(j, hjob, hperm, nh, job_bits(nj)));
+  }
+  hipLaunchKernelGGL(k_gather_u32, gh, dim3(BLOCK), 0, j->stream, heads, hperm, hs, nh);
+  /* chain offsets in that order */
+  uint64_t *hl = nullptr, *coff = nullptr;
+  HeadMeta<KT>* meta = nullptr;
+  unsigned long long* bad = nullptr;
+  TRY(dev_alloc(j, (void**)&hl, nh * 8));
+  TRY(dev_alloc(j, (void**)&coff, nh * 8));
+  hipLaunchKernelGGL(k_take_len, gh, dim3(BLOCK), 0, j->stream, hs, out_len, nh, hl);
+  size_t tmp = 0;
+  HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, hl, coff, (int)nh, j->stream));
+  void* t = nullptr;
+  TRY(dev_alloc(j, &t, tmp));
+  HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(t, tmp, hl, coff, (int)nh, j->stream));
+  TRY(dev_alloc(j, (void**)&meta, nc * sizeof(HeadMeta<KT>)));
+  TRY(dev_alloc(j, (void**)&bad, 8));
+  /* len 0 marks a node that heads no chain */
+  HIP_CHECK(hipMemsetAsync(meta, 0, nc * sizeof(HeadMeta<KT>), j->stream));
+  HIP_CHECK(hipMemsetAsync(j->perm, 0xFF, n * 4, j->stream));
+  HIP_CHECK(hipMemsetAsync(bad, 0, 8, j->stream));
+  hipLaunchKernelGGL((k_head_meta<KT>), gh, dim3(BLOCK), 0, j->stream, hs, coff, out_head, out_len,
+                     (const KT*)j->out_rep, j->out_job, nh, nc, meta, bad);
+  hipLaunchKernelGGL((k_scatter_ranked<KT>), gn, dim3(BLOCK), 0, j->stream, out_head, j->out_dist,
+                     (const KT*)j->out_rep, j->out_job, n, nc, meta, j->perm, bad);
+  hipLaunchKernelGGL(k_count_holes, dim3(grid_for(n)), dim3(BLOCK), 0, j->stream, j->perm, n, bad);
+  unsigned long long hbad = 0;
+  HIP_CHECK(hipMemcpyAsync(&hbad, bad, 8, hipMemcpyDeviceToHost, j->stream));
+  HIP_CHECK(hipStreamSynchronize(j->stream));
+  *placed = hbad == 0;
+  a = heads;
+  dev_free(j, a, n * 4);
+  a = hkeys;
+  dev_free(j, a, nh * sizeof(KT));
+  a = hperm;
+  dev_free(j, a, nh * 4);
+  a = hs;
+  dev_free(j, a, nh * 4);
+  if (hjob) {
+    a = hjob;
+    dev_free(j, a, nh * 4);
+  }
+  a = hl;
+  dev_free(j, a, nh * 8);
+  a = coff;
+  dev_free(j, a, nh * 8);
+  dev_free(j, t, tmp);
+  a = meta;
+  dev_free(j, a, nc * sizeof(HeadMeta<KT>));
+  a = bad;
+  dev_free(j, a, 8);
+  return BRUNO_OK;
+}
+
 template <typename KT>
 bruno_status compact_t(bruno_job* j) {
   uint64_t nc = j->n_chain;
@@ -1815,16 +2006,24 @@ bruno_status compact_t(bruno_job* j) {
   j->hctr->n_out = 0;
   HIP_CHECK(hipMemcpyAsync(&j->ctr->n_out, &j->hctr->n_out, sizeof(unsigned long long), hipMemcpyHostToDevice,
                            j->stream));
+  uint32_t *out_head = nullptr, *out_len = nullptr;
+  if (j->lr_contracted) {
+    TRY(dev_alloc(j, (void**)&out_head, std::max<uint64_t>(1, nc) * 4));
+    TRY(dev_alloc(j, (void**)&out_len, std::max<uint64_t>(1, nc) * 4));
+  }
   hipLaunchKernelGGL((k_ranked<KT>), dim3(grid_for(nc)), dim3(BLOCK), 0, j->stream, j->lr, (const KT*)j->ckmer, nc,
                      (const KT*)j->vkmer, j->alive, K, j->chain_job, (KT*)j->out_k, (KT*)j->out_rep, j->out_dist,
-                     j->out_job, j->ctr);
+                     j->out_job, out_head, out_len, j->ctr);
   TRY(sync_counters(j));
   uint64_t n = j->hctr->n_out;
   j->n_out = n;
   /* chain_rep_less: by rep, then |dist| -> LSD: dist first, then rep words;
    * a batch adds a last pass on the job, so each job's nodes are contiguous */
   j->fasta_off.assign(nj + 1, 0);
-  if (n > 0) {
+  bool placed = false;
+  if (n > 0 && out_head) TRY(place_by_chains<KT>(j, n, out_head, out_len, &placed));
+  j->out_by_chains = placed;
+  if (n > 0 && !placed) {
     uint32_t* pin = nullptr;
     TRY(dev_alloc(j, (void**)&pin, n * 4));
     hipLaunchKernelGGL(k_iota, dim3(grid_for(n, BLOCK, 1u << 30)), dim3(BLOCK), 0, j->stream, pin, n);
@@ -1844,6 +2043,17 @@ bruno_status compact_t(bruno_job* j) {
     dev_free(j, a, n * 4);
     TRY(sort_perm_kmer<KT>(j, (const KT*)j->out_rep, j->perm, n, K, true));
     if (nj > 1) TRY(sort_perm_u32(j, j->out_job, j->perm, n, job_bits(nj)));
+  }
+  void* a = nullptr;
+  if (out_head) {
+    a = out_head;
+    dev_free(j, a, std::max<uint64_t>(1, nc) * 4);
+    a = out_len;
+    dev_free(j, a, std::max<uint64_t>(1, nc) * 4);
+  }
+  if (n > 0) {
+    void* t = nullptr;
+    size_t tmp = 0;
     TRY(job_bounds(j, j->out_job, j->perm, n, j->chain_start));
     hipLaunchKernelGGL(k_gather_u32, dim3(grid_for(n, BLOCK, 1u << 30)), dim3(BLOCK), 0, j->stream, j->out_dist,
                        j->perm, j->dist_sorted, n);
@@ -2046,6 +2256,7 @@ bruno_status bruno_job_reset(bruno_job* j) {
   j->n_present = j->n_branch = j->n_chain = j->n_virt = 0;
   j->iterations = 0;
   j->n_cycle = j->n_isolated = j->conflicts = 0;
+  j->lr_contracted = j->out_by_chains = false;
   j->n_out = j->n_unitigs = j->fasta_len = j->bfasta_len = 0;
   j->n_reads = 0;
   j->job_reads.assign(j->n_jobs, 0);
@@ -2275,6 +2486,7 @@ bruno_status bruno_list_rank(bruno_job* j, int variant, uint32_t* rounds) {
   const char* force = getenv("BRUNO_LR_ROUNDS");
   if (!(force && force[0] == '1')) TRY(list_rank_fast(j, &used));
   if (!used) TRY(list_rank_rounds(j, variant));
+  j->lr_contracted = used;
   ev_record(j, 7);
   HIP_CHECK(hipStreamSynchronize(j->stream));
   j->ms.list_rank_ms = ev_ms(j, 6, 7);
@@ -2331,6 +2543,7 @@ bruno_status bruno_get_stats(bruno_job* j, bruno_stats* o) {
   o->capacity = j->cap;
   o->device_bytes = j->device_bytes;
   o->peak_device_bytes = j->peak_bytes;
+  o->output_by_chains = j->out_by_chains;
   return BRUNO_OK;
 }
 
@@ -2381,6 +2594,7 @@ bruno_status bruno_batch_get_stats(bruno_job* j, uint32_t idx, bruno_stats* o) {
     o->n_chain_nodes = o->n_unitig_nodes + o->n_cycle_nodes + o->n_isolated;
     o->n_nodes = o->n_branches + o->n_chain_nodes;
     o->n_unitigs = unitigs_of(j, j->fasta_off[idx + 1] - j->fasta_off[idx], o->n_unitig_nodes);
+    o->output_by_chains = j->out_by_chains;
   }
   return BRUNO_OK;
 }

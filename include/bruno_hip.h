@@ -90,6 +90,8 @@ typedef struct {
   uint64_t capacity;        /* hash slots                                        */
   uint64_t device_bytes;    /* HBM held by the job now                           */
   uint64_t peak_device_bytes; /* most HBM held at once since create / reset      */
+  uint64_t output_by_chains;  /* 1: unitig order placed from chain offsets after the
+                                 contraction ranking; 0: full sort (same bytes)  */
 } bruno_stats;
 
 /* Listranked chain node (debruijn_chain_node.hpp:156): K on the rep strand,

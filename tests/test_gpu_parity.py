@@ -63,6 +63,23 @@ def test_random_small(case, tl, variant):
     compare_all(k, None if tl is None else thresholds_L(tl), b, off, variant=variant)
 
 
+@pytest.mark.parametrize("case", [CASES[2], CASES[5], CASES[8], CASES[10]])
+def test_output_order_paths(case, monkeypatch):
+    """Unitig order placed from the chain offsets (after the contraction
+    ranking) and by the full sort (after round emulation): both equal the
+    oracle's sorted output."""
+    seed, G, n, L, err, k = case
+    _, reads = random_reads(seed, G, n, L, err)
+    b, off = pack(reads)
+    g, _ = compare_all(k, None, b, off, variant=1, stages=False)
+    assert g["stats"]["n_unitig_nodes"] > 0
+    assert g["stats"]["output_by_chains"] == 1
+    monkeypatch.setenv("BRUNO_LR_ROUNDS", "1")
+    g2, _ = compare_all(k, None, b, off, variant=1, stages=False)
+    assert g2["stats"]["output_by_chains"] == 0
+    assert g2["fasta"] == g["fasta"]
+
+
 @pytest.mark.parametrize("case", CASES[:8])
 @pytest.mark.parametrize("variant", [0, 1])
 def test_random_small_round_emulation(case, variant, monkeypatch):
