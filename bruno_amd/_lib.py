@@ -28,7 +28,7 @@ class PhaseMs(ctypes.Structure):
 STAT_FIELDS = ("n_reads", "n_reads_dropped", "n_instances", "n_distinct", "n_k2", "n_overflow",
                "n_nodes", "n_branches", "n_chain_nodes", "n_cycle_nodes", "n_isolated",
                "n_unitig_nodes", "n_unitigs", "iterations", "n_conflicts", "capacity",
-               "device_bytes", "peak_device_bytes", "output_by_chains")
+               "device_bytes", "peak_device_bytes", "output_by_chains", "count_by_regions")
 
 
 class Stats(ctypes.Structure):
@@ -46,7 +46,8 @@ class SynthDesc(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("parse_block", ctypes.c_uint32), ("parse_grid", ctypes.c_uint32),
-                ("scan_grid", ctypes.c_uint32), ("ruler_sample", ctypes.c_uint32)]
+                ("scan_grid", ctypes.c_uint32), ("ruler_sample", ctypes.c_uint32),
+                ("count_regions", ctypes.c_uint32)]
 
 
 class ServerDesc(ctypes.Structure):

@@ -17,6 +17,7 @@
 
 #include "../../include/bruno_hip.h"
 #include "kernels.h"
+#include "regions.h"
 #include "kmer.h"
 #include "pool.h"
 #include "synth.h"
@@ -275,13 +276,14 @@ __global__ __launch_bounds__(BLOCK) void k_filter_spilled(Slot<KT>* T, const Spi
  * holding the canonical k-mer (updates to them are dropped, as
  * map.update(..., false, ...) drops keys absent from the chain map). */
 template <typename KT>
-__device__ uint32_t resolve(const Slot<KT>* T, uint64_t tbase, uint64_t tcap, const kops<KT>& ops, KT n,
-                            uint32_t facing_if_canonical, uint64_t n_chain, KT* vkmer, uint64_t vcap, Counters* ctr) {
+__device__ uint32_t resolve(const Slot<KT>* T, uint64_t tbase, uint64_t tcap, uint32_t rbits, const kops<KT>& ops,
+                            KT n, uint32_t facing_if_canonical, uint64_t n_chain, KT* vkmer, uint64_t vcap,
+                            Counters* ctr) {
   KT rn = ops.rc(n);
   bool flipped = !(n < rn);
   KT cn = flipped ? rn : n;
   uint32_t end = flipped ? (facing_if_canonical ^ END_OUT) : facing_if_canonical;
-  uint64_t s = SlotOps<KT>::find(T + tbase, tcap, cn, true);
+  uint64_t s = SlotOps<KT>::find(T + tbase, tcap, rbits, cn, true);
   if (s != ~0ULL) {
     const NodeRec* nr = (const NodeRec*)&T[tbase + s].sub[0];
     if (nr->cls == CLS_CHAIN) return end | nr->idx;
@@ -298,7 +300,8 @@ __device__ uint32_t resolve(const Slot<KT>* T, uint64_t tbase, uint64_t tcap, co
 /* to_simple_biedge (debruijn_chain_node.hpp:84-141): one neighbour per side
  * at dist 1, else term_mask. */
 template <typename KT>
-__global__ __launch_bounds__(BLOCK) void k_chain_init(const Slot<KT>* T, uint64_t cap, int K, const uint64_t* chain_slot,
+__global__ __launch_bounds__(BLOCK) void k_chain_init(const Slot<KT>* T, uint64_t cap, uint32_t rbits, int K,
+                                                      const uint64_t* chain_slot,
                                                       const uint32_t* chain_job, const uint64_t* seg, uint32_t n_jobs,
                                                       uint64_t n_chain, KT* ckmer, LRNode* lr, KT* vkmer, uint64_t vcap,
                                                       Counters* ctr) {
@@ -321,13 +324,13 @@ __global__ __launch_bounds__(BLOCK) void k_chain_init(const Slot<KT>* T, uint64_
     for (int c = 0; c < 4; ++c)
       if (nr->c[4 + c] > 0) {
         /* in neighbour c.k[0..K-2]: when canonical its OUT end faces k */
-        st.p[0] = resolve<KT>(T, tbase, tcap, ops, ops.in_nbr(k, c), END_OUT, n_chain, vkmer, vcap, ctr);
+        st.p[0] = resolve<KT>(T, tbase, tcap, rbits, ops, ops.in_nbr(k, c), END_OUT, n_chain, vkmer, vcap, ctr);
         st.d[0] = 1;
       }
     for (int c = 0; c < 4; ++c)
       if (nr->c[c] > 0) {
         /* out neighbour k[1..].c: when canonical its IN end faces k */
-        st.p[1] = resolve<KT>(T, tbase, tcap, ops, ops.out_nbr(k, c), 0u, n_chain, vkmer, vcap, ctr);
+        st.p[1] = resolve<KT>(T, tbase, tcap, rbits, ops, ops.out_nbr(k, c), 0u, n_chain, vkmer, vcap, ctr);
         st.d[1] = 1;
       }
     ckmer[i] = k;
@@ -338,7 +341,8 @@ __global__ __launch_bounds__(BLOCK) void k_chain_init(const Slot<KT>* T, uint64_
 /* setup_chain_termini: each branch node marks the facing end of its chain
  * neighbours as pointing to a branch (dist 0). */
 template <typename KT>
-__global__ __launch_bounds__(BLOCK) void k_termini(const Slot<KT>* T, uint64_t cap, int K, const uint64_t* branch_slot,
+__global__ __launch_bounds__(BLOCK) void k_termini(const Slot<KT>* T, uint64_t cap, uint32_t rbits, int K,
+                                                   const uint64_t* branch_slot,
                                                    const uint32_t* branch_job, const uint64_t* seg, uint32_t n_jobs,
                                                    uint64_t n_branch, LRNode* lr, KT* bkmer) {
   kops<KT> ops(K);
@@ -364,7 +368,7 @@ __global__ __launch_bounds__(BLOCK) void k_termini(const Slot<KT>* T, uint64_t c
         KT rn = ops.rc(n);
         bool flipped = !(n < rn);
         KT cn = flipped ? rn : n;
-        uint64_t s = SlotOps<KT>::find(T + tbase, tcap, cn, true);
+        uint64_t s = SlotOps<KT>::find(T + tbase, tcap, rbits, cn, true);
         if (s == ~0ULL) continue;
         const NodeRec* q = (const NodeRec*)&T[tbase + s].sub[0];
         if (q->cls != CLS_CHAIN) continue;
@@ -812,10 +816,40 @@ __device__ __forceinline__ KT ptr_kmer(const kops<KT>& ops, uint32_t ptr, const 
   return same ? c : ops.rc(c);
 }
 
-/* to_listranked_chain_node (debruijn_chain_operations.hpp:261-310).
- * out_head / out_len (optional): the chain node at the rep end of the
- * node's chain (NONE32 when that end is not a chain node) and 1 + the
- * distance to the far end (the chain's length when the node is the head) */
+template <typename KT>
+struct RankedRec {
+  KT k, rep;
+  uint32_t dist, head, len, job;
+};
+
+/* to_listranked_chain_node (debruijn_chain_operations.hpp:261-310) for
+ * chain node i: the node on the rep's strand, the rep (the smaller end
+ * k-mer), the distance from the rep end, plus the chain node at the rep end
+ * (NONE32 when that end is not a chain node) and 1 + the distance to the
+ * far end (the chain's length when the node is the head). */
+template <typename KT>
+__device__ __forceinline__ RankedRec<KT> ranked_one(const kops<KT>& ops, const LRNode* lr, const KT* ckmer,
+                                                    uint64_t n_chain, const KT* vkmer, uint64_t i) {
+  RankedRec<KT> r;
+  const LRNode st = lr[i];
+  const KT x = ckmer[i];
+  const KT L = is_terminal(st.d[0]) ? x : ptr_kmer<KT>(ops, st.p[0], ckmer, n_chain, vkmer, true);
+  const KT R = is_terminal(st.d[1]) ? ops.rc(x) : ops.rc(ptr_kmer<KT>(ops, st.p[1], ckmer, n_chain, vkmer, false));
+  const bool sense = !(R < L);
+  r.k = sense ? x : ops.rc(x);
+  r.rep = sense ? L : R;
+  const int rs = sense ? 0 : 1;
+  r.dist = cdist(st.d[rs]);
+  const uint32_t z = st.p[rs] & IDX_MASK;
+  r.head = is_terminal(st.d[rs]) ? (uint32_t)i : (z < n_chain ? z : NONE32);
+  r.len = cdist(st.d[1 - rs]) + 1;
+  return r;
+}
+
+/* Every alive chain node, compacted; RANKED_ITEMS nodes per thread share one
+ * output reservation (one global atomic per block tile).  out_job,
+ * out_head and out_len are optional. */
+constexpr int RANKED_ITEMS = 4;
 template <typename KT>
 __global__ __launch_bounds__(BLOCK) void k_ranked(const LRNode* lr, const KT* ckmer, uint64_t n_chain, const KT* vkmer,
                                                   const uint8_t* alive, int K, const uint32_t* cjob, KT* out_k,
@@ -823,50 +857,58 @@ __global__ __launch_bounds__(BLOCK) void k_ranked(const LRNode* lr, const KT* ck
                                                   uint32_t* out_head, uint32_t* out_len, Counters* ctr) {
   __shared__ ReserveSmem sm;
   kops<KT> ops(K);
-  const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
-  for (uint64_t base = (uint64_t)blockIdx.x * BLOCK; base < n_chain; base += stride) {
-    const uint64_t i = base + threadIdx.x;
-    bool a = (i < n_chain) && alive[i];
-    KT Kv = 0, rep = 0;
-    uint32_t dist = 0, head = NONE32, len = 0;
-    if (a) {
-      const LRNode st = lr[i];
-      KT x = ckmer[i];
-      KT L = is_terminal(st.d[0]) ? x : ptr_kmer<KT>(ops, st.p[0], ckmer, n_chain, vkmer, true);
-      KT R = is_terminal(st.d[1]) ? ops.rc(x) : ops.rc(ptr_kmer<KT>(ops, st.p[1], ckmer, n_chain, vkmer, false));
-      bool sense = !(R < L);
-      Kv = sense ? x : ops.rc(x);
-      rep = sense ? L : R;
-      const int rs = sense ? 0 : 1;
-      dist = cdist(st.d[rs]);
-      const uint32_t z = st.p[rs] & IDX_MASK;
-      head = is_terminal(st.d[rs]) ? (uint32_t)i : (z < n_chain ? z : NONE32);
-      len = cdist(st.d[1 - rs]) + 1;
-    }
-    unsigned long long pos = block_reserve(a, &ctr->n_out, sm);
-    if (a) {
-      out_k[pos] = Kv;
-      out_rep[pos] = rep;
-      out_dist[pos] = dist;
-      if (out_job) out_job[pos] = cjob[i];
-      if (out_head) {
-        out_head[pos] = head;
-        out_len[pos] = len;
+  const uint64_t tile = (uint64_t)BLOCK * RANKED_ITEMS;
+  for (uint64_t base = (uint64_t)blockIdx.x * tile; base < n_chain; base += (uint64_t)gridDim.x * tile) {
+    RankedRec<KT> r[RANKED_ITEMS];
+    bool a[RANKED_ITEMS];
+    uint32_t m = 0;
+#pragma unroll
+    for (int q = 0; q < RANKED_ITEMS; ++q) {
+      const uint64_t i = base + (uint64_t)q * BLOCK + threadIdx.x;
+      a[q] = i < n_chain && alive[i];
+      if (a[q]) {
+        r[q] = ranked_one<KT>(ops, lr, ckmer, n_chain, vkmer, i);
+        r[q].job = cjob ? cjob[i] : 0u;
       }
+      m += a[q];
     }
+    unsigned long long pos = block_reserve(m, &ctr->n_out, sm);
+#pragma unroll
+    for (int q = 0; q < RANKED_ITEMS; ++q)
+      if (a[q]) {
+        out_k[pos] = r[q].k;
+        out_rep[pos] = r[q].rep;
+        out_dist[pos] = r[q].dist;
+        if (out_job) out_job[pos] = r[q].job;
+        if (out_head) {
+          out_head[pos] = r[q].head;
+          out_len[pos] = r[q].len;
+        }
+        ++pos;
+      }
   }
 }
 
-/* positions of the chain heads (dist 0) among the listranked nodes */
+/* positions of the chain heads (dist 0) among the listranked nodes;
+ * HEAD_ITEMS positions per thread share one reservation */
+constexpr int HEAD_ITEMS = 16;
 __global__ __launch_bounds__(BLOCK) void k_list_heads_out(const uint32_t* out_dist, uint64_t n, uint32_t* heads,
                                                           Counters* ctr) {
   __shared__ ReserveSmem sm;
-  const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
-  for (uint64_t base = (uint64_t)blockIdx.x * BLOCK; base < n; base += stride) {
-    const uint64_t p = base + threadIdx.x;
-    const bool h = p < n && out_dist[p] == 0;
-    unsigned long long at = block_reserve(h, &ctr->n_cand, sm);
-    if (h) heads[at] = (uint32_t)p;
+  const uint64_t tile = (uint64_t)BLOCK * HEAD_ITEMS;
+  for (uint64_t base = (uint64_t)blockIdx.x * tile; base < n; base += (uint64_t)gridDim.x * tile) {
+    uint32_t hm = 0, m = 0;
+#pragma unroll
+    for (int q = 0; q < HEAD_ITEMS; ++q) {
+      const uint64_t p = base + (uint64_t)q * BLOCK + threadIdx.x;
+      const bool h = p < n && out_dist[p] == 0;
+      hm |= (uint32_t)h << q;
+      m += h;
+    }
+    unsigned long long at = block_reserve(m, &ctr->n_cand, sm);
+#pragma unroll
+    for (int q = 0; q < HEAD_ITEMS; ++q)
+      if (hm >> q & 1u) heads[at++] = (uint32_t)(base + (uint64_t)q * BLOCK + threadIdx.x);
   }
 }
 
@@ -954,6 +996,9 @@ template <typename KT>
 __global__ void k_gather_kmer(const KT* src, const uint32_t* perm, KT* dst, uint64_t n) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) dst[i] = src[perm[i]];
+}
+__global__ void k_add_u64(unsigned long long* dst, const unsigned long long* src, uint32_t n) {
+  if (threadIdx.x < n) dst[threadIdx.x] += src[threadIdx.x];
 }
 __global__ void k_gather_u32(const uint32_t* src, const uint32_t* perm, uint32_t* dst, uint64_t n) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1188,6 +1233,16 @@ struct bruno_job {
 
   uint64_t cap = 0, spcap = 0;
   void* table = nullptr;
+  /* Region layout (plain jobs with k <= 31 and a large table): 2^rbits
+   * regions of rspan slots, each small enough for one workgroup's LDS, so
+   * a count can build every region on chip from instance records grouped
+   * by region.  rbits == 0: one probe window over the whole table (or a
+   * batch job's segment). */
+  uint32_t rbits = 0;
+  uint64_t rspan = 0;
+  bool region_ok = false;     /* plain job: the region layout may be chosen */
+  bool table_valid = false;   /* every slot holds a key or EMPTY (else it is stale) */
+  bool by_regions = false;    /* the last count built the table region by region */
   Spill* spill = nullptr;
   Counters* ctr = nullptr;      /* device */
   Counters* hctr = nullptr;     /* pinned host mirror */
@@ -1269,7 +1324,7 @@ namespace {
 /* Process-wide tuning knobs (bruno_set_tuning). */
 std::mutex g_tuning_mu;
 /* defaults: the autotuner's picks on the 100 Mbp / 33 M-read workload */
-bruno_tuning g_tuning = {256, 65536, 32768, 64};
+bruno_tuning g_tuning = {256, 65536, 32768, 64, 1};
 bruno_tuning tuning() {
   std::lock_guard<std::mutex> g(g_tuning_mu);
   return g_tuning;
@@ -1332,9 +1387,16 @@ void fill64(bruno_job* j, void* p, uint64_t n_words, uint64_t v) { fill64(j, p, 
 
 size_t slot_bytes(const bruno_job* j) { return j->words == 1 ? sizeof(Slot<uint64_t>) : sizeof(Slot<u128>); }
 
-/* table, side table and per-job counters back to empty on stream st */
+/* table, side table and per-job counters back to empty on stream st.  A
+ * region-layout table is only marked stale: the next count writes every
+ * slot (or ensure_table_valid fills it). */
 void clear_table(bruno_job* j, hipStream_t st) {
-  fill64(j, j->table, j->cap * slot_bytes(j) / 8, EMPTY64, st);
+  if (j->rbits) {
+    j->table_valid = false;
+  } else {
+    fill64(j, j->table, j->cap * slot_bytes(j) / 8, EMPTY64, st);
+    j->table_valid = true;
+  }
   fill64(j, j->spill, j->spcap * sizeof(Spill) / 8, 0, st);
   fill64(j, j->d_jstat, 2 * j->n_jobs, 0, st);
 }
@@ -1353,11 +1415,26 @@ double ev_ms(bruno_job* j, int a, int b) {
 }
 
 
-/* One table for all jobs: job i gets caps[i] slots from seg[i]. */
+/* Tables of at least this many slots get the region layout. */
+constexpr uint64_t REGION_MIN_CAP = 1ULL << 22;
+
+/* One table for all jobs: job i gets caps[i] slots from seg[i].  A plain
+ * k <= 31 job with a large table gets 2^rbits regions of at most
+ * REGION_MAX_SLOTS slots (the capacity rounds up to a whole number of
+ * regions); its slots are written by the first count, not filled here. */
 bruno_status alloc_table(bruno_job* j, const uint64_t* caps) {
   const uint32_t nj = j->n_jobs;
   j->seg.assign(nj + 1, 0);
   for (uint32_t i = 0; i < nj; ++i) j->seg[i + 1] = j->seg[i] + caps[i];
+  j->rbits = 0;
+  j->rspan = 0;
+  if (nj == 1 && j->region_ok && j->words == 1 && tuning().count_regions && j->seg[1] >= REGION_MIN_CAP) {
+    uint32_t b = REC_BUCKET_BITS;
+    while ((j->seg[1] + (1ULL << b) - 1) >> b > REGION_MAX_SLOTS) ++b;
+    j->rbits = b;
+    j->rspan = (j->seg[1] + (1ULL << b) - 1) >> b;
+    j->seg[1] = j->rspan << b;
+  }
   const uint64_t cap = j->seg[nj];
   j->cap = cap;
   TRY(dev_alloc(j, &j->table, cap * slot_bytes(j)));
@@ -1366,10 +1443,25 @@ bruno_status alloc_table(bruno_job* j, const uint64_t* caps) {
   TRY(dev_alloc(j, (void**)&j->d_seg, (nj + 1) * sizeof(uint64_t)));
   TRY(dev_alloc(j, (void**)&j->d_jstat, 2 * nj * sizeof(unsigned long long)));
   HIP_CHECK(hipMemcpyAsync(j->d_seg, j->seg.data(), (nj + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, j->stream));
-  fill64(j, j->table, cap * slot_bytes(j) / 8, EMPTY64);
+  j->table_valid = false;
+  if (!j->rbits) {
+    fill64(j, j->table, cap * slot_bytes(j) / 8, EMPTY64);
+    j->table_valid = true;
+  }
   fill64(j, j->spill, j->spcap * sizeof(Spill) / 8, 0);
   fill64(j, j->d_jstat, 2 * nj, 0);
   return BRUNO_OK;
+}
+
+/* probe span of a plain job: one region, or the whole table */
+uint64_t probe_span(const bruno_job* j) { return j->rbits ? j->rspan : j->cap; }
+
+/* every slot EMPTY unless a count has written the table since the reset */
+void ensure_table_valid(bruno_job* j) {
+  if (j->table && !j->table_valid) {
+    fill64(j, j->table, j->cap * slot_bytes(j) / 8, EMPTY64);
+    j->table_valid = true;
+  }
 }
 
 bool persistent(const bruno_job* j, const void* p) {
@@ -1448,6 +1540,119 @@ bruno_status estimate_distinct(int K, const char* d_bases, const uint64_t* d_off
   return s;
 }
 
+/* Count one batch of reads into a region-layout table (regions.h): bucketed
+ * instance records, a sort of each bucket by region, one on-chip build per
+ * region.  *done = false (and nothing counted) when the record buffers do
+ * not fit or a bucket overflowed its share; the caller then counts with
+ * atomics. */
+bruno_status count_by_regions(bruno_job* j, const ParseArgs& a, bool* done) {
+  *done = false;
+  const uint64_t n_reads = a.n_reads;
+  /* instance bound -> per-bucket capacity (hash buckets are even up to
+   * sampling noise; a quarter more plus a constant leaves a wide margin,
+   * plus the padding of one open chunk per workgroup) */
+  constexpr int NS = 2 * REC_BUCKETS + 3;
+  unsigned long long* scratch = nullptr;
+  TRY(dev_alloc(j, (void**)&scratch, NS * 8));
+  unsigned long long *front = scratch, *back = scratch + REC_BUCKETS, *stat = scratch + 2 * REC_BUCKETS,
+                     *bound = scratch + 2 * REC_BUCKETS + 2;
+  HIP_CHECK(hipMemsetAsync(scratch, 0, NS * 8, j->stream));
+  hipLaunchKernelGGL(k_count_instances, dim3(grid_for(n_reads)), dim3(BLOCK), 0, j->stream, a, bound);
+  unsigned long long n_bound = 0;
+  HIP_CHECK(hipMemcpyAsync(&n_bound, bound, 8, hipMemcpyDeviceToHost, j->stream));
+  HIP_CHECK(hipStreamSynchronize(j->stream));
+  /* one workgroup per CU slot (persistent); chunks of up to 2048 records,
+   * smaller for small inputs so padding stays under ~1/8 of the records */
+  const uint64_t groups = std::max<uint64_t>(1, std::min<uint64_t>((n_reads + REC_BLOCK / 64 - 1) / (REC_BLOCK / 64),
+                                                                     REC_GRID));
+  uint32_t chunk = 2048;
+  while (chunk > (uint32_t)REC_STAGE && (uint64_t)chunk * groups * REC_BUCKETS * 8 > n_bound) chunk >>= 1;
+  const uint64_t bcap = n_bound / REC_BUCKETS + n_bound / (4 * REC_BUCKETS) + 65536 + groups * chunk;
+  void* sp = scratch;
+  uint64_t* recs = nullptr;
+  if (dev_alloc(j, (void**)&recs, REC_BUCKETS * bcap * 8) != BRUNO_OK) {
+    dev_free(j, sp, NS * 8);
+    return BRUNO_OK;
+  }
+  void* rp = recs;
+  const uint32_t rb = j->rbits - REC_BUCKET_BITS; /* region bits inside a bucket */
+  RecArgs ra{recs, bcap, front, back, stat, chunk, (int)rb};
+  hipLaunchKernelGGL(k_parse_records, dim3((unsigned)groups), dim3(REC_BLOCK), 0, j->stream, a, ra);
+  unsigned long long hcur[2 * REC_BUCKETS];
+  HIP_CHECK(hipMemcpyAsync(hcur, front, 2 * REC_BUCKETS * 8, hipMemcpyDeviceToHost, j->stream));
+  HIP_CHECK(hipStreamSynchronize(j->stream));
+  uint64_t nmax = 0;
+  bool over = false;
+  for (int b = 0; b < REC_BUCKETS; ++b) {
+    over |= hcur[b] + hcur[REC_BUCKETS + b] > bcap;
+    nmax = std::max<uint64_t>(nmax, hcur[b] + hcur[REC_BUCKETS + b]);
+  }
+  if (over) {
+    dev_free(j, rp, REC_BUCKETS * bcap * 8);
+    dev_free(j, sp, NS * 8);
+    return BRUNO_OK;
+  }
+  /* records appended from the back join the chunks */
+  for (int b = 0; b < REC_BUCKETS; ++b) {
+    const uint64_t nf = hcur[b], nb = hcur[REC_BUCKETS + b];
+    if (!nb) continue;
+    uint64_t* tmpb = nullptr;
+    TRY(dev_alloc(j, (void**)&tmpb, nb * 8));
+    uint64_t* area = recs + (uint64_t)b * bcap;
+    HIP_CHECK(hipMemcpyAsync(tmpb, area + bcap - nb, nb * 8, hipMemcpyDeviceToDevice, j->stream));
+    HIP_CHECK(hipMemcpyAsync(area + nf, tmpb, nb * 8, hipMemcpyDeviceToDevice, j->stream));
+    void* tp = tmpb;
+    dev_free(j, tp, nb * 8);
+    hcur[b] = nf + nb;
+  }
+  /* sort each bucket on its region bits (the low rb bits of a record),
+   * find region ranges, build */
+  const uint64_t nreg = 1ULL << rb;
+  uint64_t *alt = nullptr, *start = nullptr;
+  TRY(dev_alloc(j, (void**)&alt, std::max<uint64_t>(1, nmax) * 8));
+  TRY(dev_alloc(j, (void**)&start, (nreg + 1) * 8));
+  size_t tmp = 0;
+  {
+    hipcub::DoubleBuffer<uint64_t> db(recs, alt);
+    HIP_CHECK(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp, db, (int)std::max<uint64_t>(1, nmax), 0, (int)rb,
+                                                j->stream));
+  }
+  void* t = nullptr;
+  TRY(dev_alloc(j, &t, tmp));
+  const int valid = j->table_valid ? 1 : 0;
+  for (int b = 0; b < REC_BUCKETS; ++b) {
+    const uint64_t n = hcur[b];
+    const uint64_t* sorted = recs + (uint64_t)b * bcap;
+    if (n > 1 && rb > 0) {
+      hipcub::DoubleBuffer<uint64_t> db(recs + (uint64_t)b * bcap, alt);
+      size_t tb = tmp;
+      HIP_CHECK(hipcub::DeviceRadixSort::SortKeys(t, tb, db, (int)n, 0, (int)rb, j->stream));
+      sorted = db.Current();
+    }
+    if (n == 0 && valid) continue;
+    hipLaunchKernelGGL(k_region_starts, dim3(grid_for(n + 1)), dim3(BLOCK), 0, j->stream, sorted, n, nreg, start);
+    RegionArgs g{(Slot<uint64_t>*)j->table, j->rspan, j->rbits, (uint32_t)b, nreg, sorted, start, valid,
+                 j->spill, j->spcap, j->ctr};
+    hipLaunchKernelGGL(k_region_build, dim3((unsigned)std::min<uint64_t>(nreg, 1024)), dim3(REGION_BLOCK), 0,
+                       j->stream, g);
+  }
+  /* instances and dropped reads into the job's counters */
+  hipLaunchKernelGGL(k_add_u64, dim3(1), dim3(64), 0, j->stream, (unsigned long long*)&j->ctr->n_instances, stat, 1);
+  hipLaunchKernelGGL(k_add_u64, dim3(1), dim3(64), 0, j->stream, (unsigned long long*)&j->ctr->n_dropped, stat + 1,
+                     1);
+  HIP_CHECK(hipStreamSynchronize(j->stream));
+  dev_free(j, t, tmp);
+  void* p = start;
+  dev_free(j, p, (nreg + 1) * 8);
+  p = alt;
+  dev_free(j, p, std::max<uint64_t>(1, nmax) * 8);
+  dev_free(j, rp, REC_BUCKETS * bcap * 8);
+  dev_free(j, sp, NS * 8);
+  j->table_valid = true;
+  *done = true;
+  return BRUNO_OK;
+}
+
 template <typename KT>
 bruno_status parse_count_t(bruno_job* j, const char* d_bases, const uint64_t* d_offsets, uint64_t n_reads,
                            const uint64_t* d_first, const uint64_t* d_starts = nullptr,
@@ -1465,11 +1670,22 @@ bruno_status parse_count_t(bruno_job* j, const char* d_bases, const uint64_t* d_
   a.n_reads = n_reads;
   a.K = j->K;
   a.table = j->table;
-  a.cap = j->cap;
+  a.cap = probe_span(j);
+  a.rbits = j->rbits;
   a.spill = j->spill;
   a.spcap = j->spcap;
   a.ctr = j->ctr;
   const bruno_tuning t = tuning();
+  j->by_regions = false;
+  if (std::is_same<KT, uint64_t>::value && j->rbits && t.count_regions) {
+    bool done = false;
+    TRY(count_by_regions(j, a, &done));
+    if (done) {
+      j->by_regions = true;
+      return BRUNO_OK;
+    }
+  }
+  ensure_table_valid(j);
   const unsigned wpb = t.parse_block / 64;
   const uint64_t waves = std::min<uint64_t>(n_reads, t.parse_grid);
   const unsigned g = (unsigned)std::max<uint64_t>(1, (waves + wpb - 1) / wpb);
@@ -1483,6 +1699,7 @@ bruno_status parse_count_t(bruno_job* j, const char* d_bases, const uint64_t* d_
 
 template <typename KT>
 bruno_status filter_t(bruno_job* j) {
+  ensure_table_valid(j);
   HIP_CHECK(hipMemsetAsync(&j->ctr->n_present, 0, 3 * sizeof(unsigned long long), j->stream));
   HIP_CHECK(hipMemsetAsync(&j->ctr->n_spill, 0, sizeof(unsigned long long), j->stream));
   HIP_CHECK(hipMemsetAsync(&j->ctr->n_cand, 0, sizeof(unsigned long long), j->stream));
@@ -1549,11 +1766,11 @@ bruno_status chains_t(bruno_job* j) {
   TRY(dev_alloc(j, &j->bkmer, std::max<uint64_t>(1, j->n_branch) * sizeof(KT)));
   TRY(dev_alloc(j, (void**)&j->lr, std::max<uint64_t>(1, nc) * sizeof(LRNode)));
   hipLaunchKernelGGL((k_chain_init<KT>), dim3(grid_for(nc)), dim3(BLOCK), 0, j->stream, (const Slot<KT>*)j->table,
-                     j->cap, j->K, j->chain_slot, j->chain_job, j->d_seg, j->n_jobs, nc, (KT*)j->ckmer, j->lr,
-                     (KT*)j->vkmer, j->vcap, j->ctr);
+                     probe_span(j), j->rbits, j->K, j->chain_slot, j->chain_job, j->d_seg, j->n_jobs, nc,
+                     (KT*)j->ckmer, j->lr, (KT*)j->vkmer, j->vcap, j->ctr);
   hipLaunchKernelGGL((k_termini<KT>), dim3(grid_for(j->n_branch)), dim3(BLOCK), 0, j->stream,
-                     (const Slot<KT>*)j->table, j->cap, j->K, j->branch_slot, j->branch_job, j->d_seg, j->n_jobs,
-                     j->n_branch, j->lr, (KT*)j->bkmer);
+                     (const Slot<KT>*)j->table, probe_span(j), j->rbits, j->K, j->branch_slot, j->branch_job,
+                     j->d_seg, j->n_jobs, j->n_branch, j->lr, (KT*)j->bkmer);
   TRY(sync_counters(j));
   j->n_virt = j->hctr->n_virt;
   if (nc + j->n_virt >= (1ULL << 31)) return BRUNO_CAPACITY;
@@ -1917,7 +2134,8 @@ bruno_status place_by_chains(bruno_job* j, uint64_t n, const uint32_t* out_head,
   HIP_CHECK(hipMemcpyAsync(&j->ctr->n_cand, &j->hctr->n_cand, 8, hipMemcpyHostToDevice, j->stream));
   uint32_t* heads = nullptr;
   TRY(dev_alloc(j, (void**)&heads, n * 4));
-  hipLaunchKernelGGL(k_list_heads_out, dim3(grid_for(n)), dim3(BLOCK), 0, j->stream, j->out_dist, n, heads, j->ctr);
+  hipLaunchKernelGGL(k_list_heads_out, dim3(grid_for((n + HEAD_ITEMS - 1) / HEAD_ITEMS)), dim3(BLOCK), 0, j->stream,
+                     j->out_dist, n, heads, j->ctr);
   TRY(sync_counters(j));
   const uint64_t nh = j->hctr->n_cand;
   void* a = nullptr;
@@ -2011,9 +2229,9 @@ bruno_status compact_t(bruno_job* j) {
     TRY(dev_alloc(j, (void**)&out_head, std::max<uint64_t>(1, nc) * 4));
     TRY(dev_alloc(j, (void**)&out_len, std::max<uint64_t>(1, nc) * 4));
   }
-  hipLaunchKernelGGL((k_ranked<KT>), dim3(grid_for(nc)), dim3(BLOCK), 0, j->stream, j->lr, (const KT*)j->ckmer, nc,
-                     (const KT*)j->vkmer, j->alive, K, j->chain_job, (KT*)j->out_k, (KT*)j->out_rep, j->out_dist,
-                     j->out_job, out_head, out_len, j->ctr);
+  hipLaunchKernelGGL((k_ranked<KT>), dim3(grid_for((nc + RANKED_ITEMS - 1) / RANKED_ITEMS)), dim3(BLOCK), 0,
+                     j->stream, j->lr, (const KT*)j->ckmer, nc, (const KT*)j->vkmer, j->alive, K, j->chain_job,
+                     (KT*)j->out_k, (KT*)j->out_rep, j->out_dist, j->out_job, out_head, out_len, j->ctr);
   TRY(sync_counters(j));
   uint64_t n = j->hctr->n_out;
   j->n_out = n;
@@ -2149,7 +2367,8 @@ const char* bruno_status_string(bruno_status s) {
   return "unknown";
 }
 
-static bruno_status create_common(const bruno_job_desc* d, uint32_t n_jobs, const uint64_t* caps, bruno_job** out) {
+static bruno_status create_common(const bruno_job_desc* d, uint32_t n_jobs, const uint64_t* caps, bruno_job** out,
+                                  bool plain = false) {
   if (!d || !out) return BRUNO_BAD_INPUT;
   *out = nullptr;
   if (d->k < 3 || d->k > 63 || n_jobs == 0) return BRUNO_BAD_INPUT;
@@ -2168,6 +2387,7 @@ static bruno_status create_common(const bruno_job_desc* d, uint32_t n_jobs, cons
   j->load_factor = (d->load_factor > 0 && d->load_factor < 1) ? d->load_factor : 0.5;
   j->stream = (hipStream_t)d->stream;
   j->n_jobs = n_jobs;
+  j->region_ok = plain;
   j->job_reads.assign(n_jobs, 0);
   if (pool_alloc((void**)&j->ctr, sizeof(Counters)) != hipSuccess ||
       hipHostMalloc((void**)&j->hctr, sizeof(Counters), hipHostMallocDefault) != hipSuccess) {
@@ -2204,7 +2424,7 @@ static bruno_status create_common(const bruno_job_desc* d, uint32_t n_jobs, cons
 bruno_status bruno_job_create(const bruno_job_desc* d, bruno_job** out) {
   if (!d) return BRUNO_BAD_INPUT;
   const uint64_t cap = d->capacity;
-  return create_common(d, 1, cap ? &cap : nullptr, out);
+  return create_common(d, 1, cap ? &cap : nullptr, out, true);
 }
 
 bruno_status bruno_batch_create(const bruno_job_desc* d, uint32_t n_jobs, const uint64_t* capacities,
@@ -2544,6 +2764,7 @@ bruno_status bruno_get_stats(bruno_job* j, bruno_stats* o) {
   o->device_bytes = j->device_bytes;
   o->peak_device_bytes = j->peak_bytes;
   o->output_by_chains = j->out_by_chains;
+  o->count_by_regions = j->by_regions;
   return BRUNO_OK;
 }
 
@@ -2602,6 +2823,7 @@ bruno_status bruno_batch_get_stats(bruno_job* j, uint32_t idx, bruno_stats* o) {
 bruno_status bruno_batch_get_counts(bruno_job* j, uint32_t idx, bruno_node* buf, uint64_t* n) {
   if (!valid_idx(j, idx) || !n) return BRUNO_BAD_INPUT;
   if (j->stage != ST_COUNTED) return BRUNO_STATE;
+  ensure_table_valid(j);
   const uint64_t s0 = j->seg.empty() ? 0 : j->seg[idx], s1 = j->seg.empty() ? 0 : j->seg[idx + 1];
   const bool one = j->n_jobs == 1;
   if (!buf) {
@@ -2658,6 +2880,7 @@ bruno_status bruno_batch_get_k2(bruno_job* j, uint32_t idx, uint64_t* kmers, uin
                                 uint64_t* n) {
   if (!valid_idx(j, idx) || !n) return BRUNO_BAD_INPUT;
   if (j->stage != ST_COUNTED) return BRUNO_STATE;
+  ensure_table_valid(j);
   const uint64_t s0 = j->seg.empty() ? 0 : j->seg[idx], s1 = j->seg.empty() ? 0 : j->seg[idx + 1];
   const bool one = j->n_jobs == 1;
   if (!kmers) {
@@ -3012,7 +3235,7 @@ bruno_status bruno_set_tuning(const bruno_tuning* t) {
   if (!t) return BRUNO_BAD_INPUT;
   const bool pow2 = t->ruler_sample >= 8 && t->ruler_sample <= 4096 && !(t->ruler_sample & (t->ruler_sample - 1));
   if ((t->parse_block != 128 && t->parse_block != 256 && t->parse_block != 512) || t->parse_grid < 64 ||
-      t->scan_grid < 1 || !pow2)
+      t->scan_grid < 1 || !pow2 || t->count_regions > 1)
     return BRUNO_BAD_INPUT;
   std::lock_guard<std::mutex> g(g_tuning_mu);
   g_tuning = *t;

@@ -200,9 +200,11 @@ struct ParseArgs {
   const uint64_t* ends;
   uint64_t n_reads;
   int K;
-  /* count mode */
+  /* count mode: cap is the probe span (the whole table, or one region
+   * when rbits > 0; a batch job's span is its segment) */
   void* table;
   uint64_t cap;
+  uint32_t rbits;
   Spill* spill;
   uint64_t spcap;
   /* batches: job j owns reads [job_first[j], job_first[j+1]) of this call
@@ -235,6 +237,50 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 constexpr int SKETCH_SAMPLE_BITS = 6;
 constexpr int SKETCH_INDEX_BITS = 12;
 constexpr int SKETCH_M = 1 << SKETCH_INDEX_BITS;
+
+/* The canonical (k-mer, biedge) instances of one read of L >= K + 2 bases
+ * at s, parsed by one wave: lane l takes instances p0 + l of a sliding
+ * window of four packed words covering bases [p0, p0 + 128).  f(i, km, e)
+ * runs for each instance i < L - K + 1. */
+template <typename KT, typename F>
+__device__ __forceinline__ void for_each_instance(const uint8_t* s, uint64_t L, int K, int lane, const kops<KT>& ops,
+                                                  F&& f) {
+  const uint64_t nk = L - K + 1;
+  uint64_t A, B, C, D, prevB = 0;
+  {
+    int c0 = (lane < L) ? base_code(s[lane]) : 0;
+    pack64(c0, A, B);
+    int c1 = (64 + lane < L) ? base_code(s[64 + lane]) : 0;
+    pack64(c1, C, D);
+  }
+  for (uint64_t p0 = 0; p0 < nk; p0 += 64) {
+    const uint64_t i = p0 + lane;
+    if (i < nk) {
+      KT fw = window_kmer<KT>(lane, K, A, B, C, D);
+      int inb = (lane > 0) ? window_base(lane - 1, A, B, C, D) : (int)(prevB & 3);
+      int outb = window_base(lane + K, A, B, C, D);
+      uint8_t left = (i > 0) ? (uint8_t)(1u << inb) : (uint8_t)0;
+      uint8_t right = (i + K < L) ? (uint8_t)(1u << outb) : (uint8_t)0;
+      uint8_t e = (uint8_t)((left << 4) | right);
+      /* lex_less (debruijn_common.hpp:53-58): keep if x < rc(x), else
+       * rc the k-mer and the biedge. */
+      KT rv = ops.rc(fw);
+      KT km = fw;
+      if (!(fw < rv)) {
+        km = rv;
+        e = biedge_rc(e);
+      }
+      f(i, km, e);
+    }
+    /* advance the window by 64 bases */
+    prevB = B;
+    A = C;
+    B = D;
+    uint64_t nxt = p0 + 128 + lane;
+    int cn = (nxt < L) ? base_code(s[nxt]) : 0;
+    pack64(cn, C, D);
+  }
+}
 
 /* MODE 0: count records per read; MODE 1: write records; MODE 2: count into
  * the table; MODE 3: distinct-k-mer sketch only.  BS is the workgroup size
@@ -288,64 +334,33 @@ __global__ __launch_bounds__(BS) void k_parse(ParseArgs a) {
     }
     uint64_t rbase = (MODE == 1) ? a.rec_base[r] : 0;
     if (MODE == 2 && a.n_jobs > 1) j_inst += nk;
-    /* sliding window of 4 packed words covering bases [p0, p0 + 128) */
-    uint64_t A, B, C, D, prevB = 0;
-    {
-      int c0 = (lane < L) ? base_code(s[lane]) : 0;
-      pack64(c0, A, B);
-      int c1 = (64 + lane < L) ? base_code(s[64 + lane]) : 0;
-      pack64(c1, C, D);
-    }
-    for (uint64_t p0 = 0; p0 < nk; p0 += 64) {
-      const uint64_t i = p0 + lane;
-      if (i < nk) {
-        KT fw = window_kmer<KT>(lane, K, A, B, C, D);
-        int inb = (lane > 0) ? window_base(lane - 1, A, B, C, D) : (int)(prevB & 3);
-        int outb = window_base(lane + K, A, B, C, D);
-        uint8_t left = (i > 0) ? (uint8_t)(1u << inb) : (uint8_t)0;
-        uint8_t right = (i + K < L) ? (uint8_t)(1u << outb) : (uint8_t)0;
-        uint8_t e = (uint8_t)((left << 4) | right);
-        /* lex_less (debruijn_common.hpp:53-58): keep if x < rc(x), else
-         * rc the k-mer and the biedge. */
-        KT rv = ops.rc(fw);
-        KT km = fw;
-        if (!(fw < rv)) {
-          km = rv;
-          e = biedge_rc(e);
+    for_each_instance<KT>(s, L, K, lane, ops, [&](uint64_t i, KT km, uint8_t e) {
+      if (MODE == 3) {
+        const uint64_t h = kmer_hash(km);
+        if ((h >> (64 - SKETCH_SAMPLE_BITS)) == 0) {
+          const uint32_t idx = (uint32_t)(h >> (64 - SKETCH_SAMPLE_BITS - SKETCH_INDEX_BITS)) & (SKETCH_M - 1);
+          const uint64_t rest = (h << (SKETCH_SAMPLE_BITS + SKETCH_INDEX_BITS)) |
+                                (1ULL << (SKETCH_SAMPLE_BITS + SKETCH_INDEX_BITS - 1));
+          const uint32_t rank = (uint32_t)__clzll(rest) + 1;
+          uint32_t* reg = a.sketch + (uint64_t)cj * SKETCH_M + idx;
+          if (rank > __hip_atomic_load(reg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(reg, rank);
         }
-        if (MODE == 3) {
-          const uint64_t h = kmer_hash(km);
-          if ((h >> (64 - SKETCH_SAMPLE_BITS)) == 0) {
-            const uint32_t idx = (uint32_t)(h >> (64 - SKETCH_SAMPLE_BITS - SKETCH_INDEX_BITS)) & (SKETCH_M - 1);
-            const uint64_t rest = (h << (SKETCH_SAMPLE_BITS + SKETCH_INDEX_BITS)) |
-                                  (1ULL << (SKETCH_SAMPLE_BITS + SKETCH_INDEX_BITS - 1));
-            const uint32_t rank = (uint32_t)__clzll(rest) + 1;
-            uint32_t* reg = a.sketch + (uint64_t)cj * SKETCH_M + idx;
-            if (rank > __hip_atomic_load(reg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(reg, rank);
-          }
-        } else if (MODE == 1) {
-          a.out_kmers[(rbase + i) * kt_traits<KT>::words] = (uint64_t)km;
-          if (kt_traits<KT>::words == 2)
-            a.out_kmers[(rbase + i) * kt_traits<KT>::words + 1] = (uint64_t)((u128)km >> 64);
-          a.out_biedges[rbase + i] = e;
-        } else {
-          int claimed = 0;
-          uint64_t slot = SlotOps<KT>::find_or_insert((Slot<KT>*)a.table + tbase, tcap, km, &a.ctr->err, &claimed);
-          if (slot != ~0ULL) {
-            my_new += claimed;
-            my_k2 += add_k2<KT>((Slot<KT>*)a.table, tbase + slot, e, a.spill, a.spcap, &a.ctr->err);
-          }
+      } else if (MODE == 1) {
+        a.out_kmers[(rbase + i) * kt_traits<KT>::words] = (uint64_t)km;
+        if (kt_traits<KT>::words == 2)
+          a.out_kmers[(rbase + i) * kt_traits<KT>::words + 1] = (uint64_t)((u128)km >> 64);
+        a.out_biedges[rbase + i] = e;
+      } else {
+        int claimed = 0;
+        uint64_t slot =
+            SlotOps<KT>::find_or_insert((Slot<KT>*)a.table + tbase, tcap, a.rbits, km, &a.ctr->err, &claimed);
+        if (slot != ~0ULL) {
+          my_new += claimed;
+          my_k2 += add_k2<KT>((Slot<KT>*)a.table, tbase + slot, e, a.spill, a.spcap, &a.ctr->err);
         }
-        ++my_inst;
       }
-      /* advance the window by 64 bases */
-      prevB = B;
-      A = C;
-      B = D;
-      uint64_t nxt = p0 + 128 + lane;
-      int cn = (nxt < L) ? base_code(s[nxt]) : 0;
-      pack64(cn, C, D);
-    }
+      ++my_inst;
+    });
   }
   if (MODE == 2 && a.n_jobs > 1 && lane == 0 && j_inst) atomicAdd(&a.job_inst[cj], j_inst);
   if (MODE == 2) {

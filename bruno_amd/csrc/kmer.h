@@ -39,6 +39,16 @@ BR_HD uint64_t fmix64(uint64_t x) {
 }
 
 BR_HD uint64_t kmer_hash(uint64_t v) { return fmix64(v * 0x9e3779b97f4a7c15ULL + 0x632be59bd9b4e019ULL); }
+/* kmer_hash is a bijection of 64-bit words (odd multipliers, xor-shifts by
+ * >= 32 bits undo themselves); its inverse recovers a k-mer from its hash */
+BR_HD uint64_t kmer_unhash(uint64_t h) {
+  h ^= h >> 33;
+  h *= 0x9cb4b2f8129337dbULL; /* 0xc4ceb9fe1a85ec53^-1 mod 2^64 */
+  h ^= h >> 33;
+  h *= 0x4f74430c22a54005ULL; /* 0xff51afd7ed558ccd^-1 */
+  h ^= h >> 33;
+  return (h - 0x632be59bd9b4e019ULL) * 0xf1de83e19937733dULL; /* 0x9e3779b97f4a7c15^-1 */
+}
 BR_HD uint64_t kmer_hash(u128 v) {
   return fmix64((uint64_t)v ^ fmix64((uint64_t)(v >> 64) + 0x9e3779b97f4a7c15ULL));
 }

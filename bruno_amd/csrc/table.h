@@ -93,6 +93,21 @@ __device__ __forceinline__ void or64(uint64_t* p, uint64_t v) {
 
 __device__ __forceinline__ uint64_t home_slot(uint64_t h, uint64_t cap) { return __umul64hi(h, cap); }
 
+/* Probe window of a k-mer hash.  rbits == 0: the whole table of span slots
+ * (home = fastrange(h, span), wrapping at span).  rbits > 0: the table is
+ * 2^rbits regions of span slots each; the top rbits hash bits pick the
+ * region, the rest the home slot inside it, and probes wrap inside the
+ * region, so every slot a k-mer can occupy lies in one region. */
+struct Probe {
+  uint64_t lo, hi, s;
+  __device__ __forceinline__ Probe(uint64_t h, uint64_t span, uint32_t rbits) {
+    lo = rbits ? (h >> (64 - rbits)) * span : 0;
+    hi = lo + span;
+    s = lo + __umul64hi(h << rbits, span);
+  }
+  __device__ __forceinline__ void next() { s = (s + 1 == hi) ? lo : s + 1; }
+};
+
 /* ------------------------------------------------------------------ */
 /* key access per slot type                                             */
 /* ------------------------------------------------------------------ */
@@ -111,39 +126,38 @@ template <> struct SlotOps<uint64_t> {
   typedef Slot<uint64_t> S;
   /* Find or claim the slot of canonical k-mer v.  Returns the slot index or
    * ~0 on a full table; *claimed = 1 when this call inserted the key. */
-  static __device__ uint64_t find_or_insert(S* T, uint64_t cap, uint64_t v, uint32_t* err, int* claimed) {
-    uint64_t s = home_slot(kmer_hash(v), cap);
-    const uint64_t lim = cap < MAX_PROBE ? cap : MAX_PROBE;
-    for (uint64_t probe = 0; probe < lim; ++probe) {
-      S* p = T + s;
+  static __device__ uint64_t find_or_insert(S* T, uint64_t span, uint32_t rbits, uint64_t v, uint32_t* err,
+                                            int* claimed) {
+    Probe q(kmer_hash(v), span, rbits);
+    const uint64_t lim = span < MAX_PROBE ? span : MAX_PROBE;
+    for (uint64_t probe = 0; probe < lim; ++probe, q.next()) {
+      S* p = T + q.s;
       uint64_t k0 = ld_relaxed(&p->key);
       if (k0 == EMPTY64) {
         uint64_t old = cas64(&p->key, EMPTY64, v);
         if (old == EMPTY64) {
           *claimed = 1;
-          return s;
+          return q.s;
         }
         k0 = old;
       }
-      if ((k0 & KEY_BITS) == v) return s;
-      s = (s + 1 == cap) ? 0 : s + 1;
+      if ((k0 & KEY_BITS) == v) return q.s;
     }
     atomicOr(err, ERR_TABLE_FULL);
     return ~0ULL;
   }
   /* Lookup only (separate kernels after counting); ~0 when absent.
    * require_present: only nodes that survived the filter. */
-  static __device__ uint64_t find(const S* T, uint64_t cap, uint64_t v, bool require_present) {
-    uint64_t s = home_slot(kmer_hash(v), cap);
-    const uint64_t lim = cap < MAX_PROBE ? cap : MAX_PROBE;
-    for (uint64_t probe = 0; probe < lim; ++probe) {
-      uint64_t k0 = T[s].key;
+  static __device__ uint64_t find(const S* T, uint64_t span, uint32_t rbits, uint64_t v, bool require_present) {
+    Probe q(kmer_hash(v), span, rbits);
+    const uint64_t lim = span < MAX_PROBE ? span : MAX_PROBE;
+    for (uint64_t probe = 0; probe < lim; ++probe, q.next()) {
+      uint64_t k0 = T[q.s].key;
       if (k0 == EMPTY64) return ~0ULL;
       if ((k0 & KEY_BITS) == v) {
         if (require_present && !(k0 & F_PRESENT)) return ~0ULL;
-        return s;
+        return q.s;
       }
-      s = (s + 1 == cap) ? 0 : s + 1;
     }
     return ~0ULL;
   }
@@ -160,11 +174,13 @@ template <> struct SlotOps<uint64_t> {
  * read lo and hi. */
 template <> struct SlotOps<u128> {
   typedef Slot<u128> S;
-  static __device__ uint64_t find_or_insert(S* T, uint64_t cap, u128 v, uint32_t* err, int* claimed) {
+  static __device__ uint64_t find_or_insert(S* T, uint64_t span, uint32_t rbits, u128 v, uint32_t* err,
+                                            int* claimed) {
     uint64_t lo = (uint64_t)v, hi = (uint64_t)(v >> 64);
-    uint64_t s = home_slot(kmer_hash(v), cap);
-    const uint64_t lim = cap < MAX_PROBE ? cap : MAX_PROBE;
-    for (uint64_t probe = 0; probe < lim; ++probe) {
+    Probe q(kmer_hash(v), span, rbits);
+    const uint64_t lim = span < MAX_PROBE ? span : MAX_PROBE;
+    for (uint64_t probe = 0; probe < lim; ++probe, q.next()) {
+      const uint64_t s = q.s;
       S* p = T + s;
       uint64_t m = ld_relaxed(&p->meta);
       if (m == EMPTY64) {
@@ -192,23 +208,22 @@ template <> struct SlotOps<u128> {
         return ~0ULL;
       }
       if (ld_relaxed(&p->lo) == lo && ld_relaxed(&p->hi) == hi) return s;
-      s = (s + 1 == cap) ? 0 : s + 1;
     }
     atomicOr(err, ERR_TABLE_FULL);
     return ~0ULL;
   }
-  static __device__ uint64_t find(const S* T, uint64_t cap, u128 v, bool require_present) {
+  static __device__ uint64_t find(const S* T, uint64_t span, uint32_t rbits, u128 v, bool require_present) {
     uint64_t lo = (uint64_t)v, hi = (uint64_t)(v >> 64);
-    uint64_t s = home_slot(kmer_hash(v), cap);
-    const uint64_t lim = cap < MAX_PROBE ? cap : MAX_PROBE;
-    for (uint64_t probe = 0; probe < lim; ++probe) {
+    Probe q(kmer_hash(v), span, rbits);
+    const uint64_t lim = span < MAX_PROBE ? span : MAX_PROBE;
+    for (uint64_t probe = 0; probe < lim; ++probe, q.next()) {
+      const uint64_t s = q.s;
       uint64_t m = T[s].meta;
       if (m == EMPTY64) return ~0ULL;
       if (T[s].lo == lo && T[s].hi == hi) {
         if (require_present && !(m & F_PRESENT)) return ~0ULL;
         return s;
       }
-      s = (s + 1 == cap) ? 0 : s + 1;
     }
     return ~0ULL;
   }
@@ -217,6 +232,32 @@ template <> struct SlotOps<u128> {
   static __device__ __forceinline__ u128 kmer(const S* p) { return ((u128)p->hi << 64) | p->lo; }
   static __device__ __forceinline__ uint64_t flags(const S* p) { return p->meta; }
 };
+
+/* Count one instance of side-table key (slot << 9) | code.  Returns 1 if
+ * the entry is new. */
+__device__ __forceinline__ int spill_add(Spill* sp, uint64_t spcap, uint64_t key, uint32_t* err) {
+  uint64_t h = home_slot(fmix64(key), spcap);
+  const uint64_t lim = spcap < MAX_PROBE ? spcap : MAX_PROBE;
+  for (uint64_t probe = 0; probe < lim; ++probe) {
+    Spill* q = sp + h;
+    uint64_t k0 = ld_relaxed(&q->key);
+    if (k0 == 0) {
+      uint64_t old = cas64(&q->key, 0, key);
+      if (old == 0) {
+        add64(&q->cnt, 1);
+        return 1;
+      }
+      k0 = old;
+    }
+    if (k0 == key) {
+      add64(&q->cnt, 1);
+      return 0;
+    }
+    h = (h + 1 == spcap) ? 0 : h + 1;
+  }
+  atomicOr(err, ERR_SPILL_FULL);
+  return 0;
+}
 
 /* Add one instance of biedge e to slot s.  Returns 1 if a new k2 entry was
  * created. */
@@ -240,28 +281,7 @@ __device__ int add_k2(Slot<KT>* T, uint64_t s, uint8_t e, Spill* sp, uint64_t sp
   /* spill: mark the slot, then count in the side table */
   uint64_t* fw = SlotOps<KT>::flag_word(p);
   if (!(ld_relaxed(fw) & F_SPILL)) or64(fw, F_SPILL);
-  const uint64_t key = (s << 9) | code;
-  uint64_t h = home_slot(fmix64(key), spcap);
-  const uint64_t lim = spcap < MAX_PROBE ? spcap : MAX_PROBE;
-  for (uint64_t probe = 0; probe < lim; ++probe) {
-    Spill* q = sp + h;
-    uint64_t k0 = ld_relaxed(&q->key);
-    if (k0 == 0) {
-      uint64_t old = cas64(&q->key, 0, key);
-      if (old == 0) {
-        add64(&q->cnt, 1);
-        return 1;
-      }
-      k0 = old;
-    }
-    if (k0 == key) {
-      add64(&q->cnt, 1);
-      return 0;
-    }
-    h = (h + 1 == spcap) ? 0 : h + 1;
-  }
-  atomicOr(err, ERR_SPILL_FULL);
-  return 0;
+  return spill_add(sp, spcap, (s << 9) | code, err);
 }
 
 __device__ __forceinline__ uint64_t spill_find(const Spill* sp, uint64_t spcap, uint64_t key) {

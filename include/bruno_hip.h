@@ -92,6 +92,8 @@ typedef struct {
   uint64_t peak_device_bytes; /* most HBM held at once since create / reset      */
   uint64_t output_by_chains;  /* 1: unitig order placed from chain offsets after the
                                  contraction ranking; 0: full sort (same bytes)  */
+  uint64_t count_by_regions;  /* 1: the last count built the table region by region
+                                 from instance records; 0: atomic counting in HBM */
 } bruno_stats;
 
 /* Listranked chain node (debruijn_chain_node.hpp:156): K on the rep strand,
@@ -287,6 +289,11 @@ typedef struct {
   uint32_t ruler_sample;   /* list ranking: one sampled ruler per this many chain nodes on
                               average (power of two, 8..4096): the walk length between
                               rulers and the size of the contracted list                    */
+  uint32_t count_regions;  /* 1: plain jobs (k <= 31, >= 2^22 slots) lay the table out in
+                              LDS-sized regions and count by writing instance records
+                              grouped by region, then building each region on chip; 0: one
+                              probe window and atomic counting in HBM.  Read when a table
+                              is allocated and at each count.                               */
 } bruno_tuning;
 bruno_status bruno_set_tuning(const bruno_tuning* t);
 bruno_status bruno_get_tuning(bruno_tuning* t);

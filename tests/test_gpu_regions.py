@@ -1,0 +1,170 @@
+"""Counting by table region (bucketed instance records, per-region builds in
+LDS) against the oracle, and against the atomic count on the same inputs.
+Tables of >= 2^22 slots of plain k <= 31 jobs use the region layout."""
+import numpy as np
+import pytest
+
+import bruno_amd
+from bruno_amd.autotune import get_tuning, set_tuning
+from gpu_common import compare_all, compare_results, node_table, run_oracle
+from helpers import pack, random_reads, thresholds_L
+
+pytestmark = pytest.mark.gpu
+
+REGION_CAP = 1 << 22
+
+CASES = [
+    (1, 300, 40, 30, 0.0, 7),
+    (3, 2000, 300, 50, 0.01, 11),
+    (6, 4000, 400, 100, 0.01, 31),
+    (7, 600, 120, (40, 300), 0.01, 31),
+    (11, 5000, 500, (60, 200), 0.02, 25),
+    (13, 30000, 3000, 150, 0.01, 31),
+]
+
+
+@pytest.fixture
+def direct_count():
+    old = get_tuning()
+    set_tuning(count_regions=0)
+    yield
+    set_tuning(**old)
+
+
+def _reads(case):
+    seed, G, n, L, err, k = case
+    _, reads = random_reads(seed, G, n, L, err)
+    return k, reads
+
+
+@pytest.mark.parametrize("case", CASES)
+@pytest.mark.parametrize("tl", [None, 2])
+def test_region_count_matches_oracle(case, tl):
+    k, reads = _reads(case)
+    b, off = pack(reads)
+    g, _ = compare_all(k, None if tl is None else thresholds_L(tl), b, off, capacity=REGION_CAP)
+    assert g["stats"]["count_by_regions"] == 1
+    assert g["stats"]["capacity"] >= REGION_CAP
+
+
+@pytest.mark.parametrize("case", CASES[2:4])
+def test_atomic_count_on_large_table(case, direct_count):
+    """The knob off: one probe window, atomic counting, same results."""
+    k, reads = _reads(case)
+    b, off = pack(reads)
+    g, _ = compare_all(k, thresholds_L(2), b, off, capacity=REGION_CAP)
+    assert g["stats"]["count_by_regions"] == 0
+
+
+def test_counts_accumulate_over_calls():
+    """Three parse_count calls: later calls merge into regions the first
+    call wrote."""
+    k, reads = _reads(CASES[5])
+    job = bruno_amd.GraphJob(k, thresholds_L(2), capacity=REGION_CAP)
+    parts = np.array_split(np.arange(len(reads)), 3)
+    for idx in parts:
+        pb, po = pack([reads[i] for i in idx])
+        job.parse_count(pb, po)
+        assert job.stats()["count_by_regions"] == 1
+    b, off = pack(reads)
+    o = run_oracle(k, thresholds_L(2), b, off)
+    g = {"counts": job.counts(), "k2": job.k2()}
+    job.filter()
+    g["nodes"] = job.nodes()
+    job.build_chains()
+    g["rounds"] = job.list_rank(1)
+    job.compact()
+    g.update(fasta=job.unitigs(), branch_fasta=job.branch_fasta(), chain_nodes=job.chain_nodes(),
+             branches=job.branches(), stats=job.stats())
+    job.close()
+    compare_results(k, g, o, 1, True)
+
+
+def test_skewed_input_falls_back_to_atomic_count():
+    """One k-mer in most instances overfills its record bucket: the call
+    is counted with atomics instead, with the same results."""
+    rng = np.random.default_rng(9)
+    reads = ["A" * 200] * 800 + ["".join(rng.choice(list("ACGT"), 150)) for _ in range(200)]
+    b, off = pack(reads)
+    g, _ = compare_all(31, None, b, off, capacity=REGION_CAP)
+    assert g["stats"]["count_by_regions"] == 0
+    g, _ = compare_all(31, thresholds_L(1), b, off, capacity=REGION_CAP)
+
+
+def test_edge_reads():
+    """Empty, short, N-containing reads and one 20 kbp read (its records
+    overflow the workgroup's staging and are appended one by one)."""
+    long_read = "".join(np.random.default_rng(3).choice(list("ACGT"), 20000))
+    reads = ["", "ACGT" * 5, "A" * 33, "ACGTN" * 10, "C" * 31 + "A", long_read, long_read[5000:9000],
+             long_read[12000:15000][::-1].translate(str.maketrans("ACGT", "TGCA"))]
+    b, off = pack(reads)
+    for th in (None, thresholds_L(1)):
+        g, _ = compare_all(31, th, b, off, capacity=REGION_CAP)
+        assert g["stats"]["count_by_regions"] == 1
+
+
+def test_empty_and_reused_job():
+    """A job that counts nothing, then reset and reused with reads, then
+    reset with recycling on: each run equals a fresh job's."""
+    job = bruno_amd.GraphJob(31, thresholds_L(2), capacity=REGION_CAP)
+    job.parse_count(b"", np.zeros(1, np.uint64))
+    job.run()
+    assert job.unitigs() == b"" and job.branch_fasta() == b""
+    for case, recycle in ((CASES[2], False), (CASES[4], True), (CASES[5], True)):
+        _, reads = _reads(case)
+        b, off = pack(reads)
+        job.reset()
+        job.set_recycle(recycle)
+        job.parse_count(b, off)
+        job.run()
+        fresh = bruno_amd.GraphJob(31, thresholds_L(2), capacity=REGION_CAP)
+        fresh.parse_count(b, off)
+        fresh.run()
+        assert job.unitigs() == fresh.unitigs()
+        assert job.branch_fasta() == fresh.branch_fasta()
+        assert job.stats()["count_by_regions"] == 1
+        fresh.close()
+    job.close()
+
+
+@pytest.mark.parametrize("k", [15, 31])
+def test_region_vs_atomic_tables_agree(k, direct_count):
+    """Counts, k2 entries and every output of the region build equal the
+    atomic count's on a bigger random genome (side-table spills included)."""
+    _, reads = random_reads(21, 200000, 6000, 150, 0.02)
+    b, off = pack(reads)
+    outs = []
+    for regions in (1, 0):
+        set_tuning(count_regions=regions)
+        job = bruno_amd.GraphJob(k, thresholds_L(2), capacity=REGION_CAP)
+        job.parse_count(b, off)
+        st = job.stats()
+        assert st["count_by_regions"] == regions
+        c = job.counts()
+        kk = job.k2()
+        job.run()
+        k2 = sorted(zip(map(tuple, kk[0].tolist()), kk[1].tolist(), kk[2].tolist()))
+        outs.append((node_table(c), k2, job.unitigs(), job.branch_fasta(), st["n_distinct"], st["n_k2"],
+                     st["n_instances"]))
+        job.close()
+    assert outs[0] == outs[1]
+
+
+def test_spilled_entries_in_regions():
+    """k-mers with more distinct biedges than a slot holds: the region build
+    sends the extra entries to the side table, keyed by table slot."""
+    rng = np.random.default_rng(5)
+    reads = []
+    for _ in range(40):
+        core = "".join(rng.choice(list("ACGT"), 31))
+        for a in "ACGT":
+            for c in "ACGT":
+                r = "".join(rng.choice(list("ACGT"), 10)) + a + core + c + "".join(rng.choice(list("ACGT"), 10))
+                reads += [r] * 5 + [r.translate(str.maketrans("ACGT", "TGCA"))[::-1]] * 2
+    order = rng.permutation(len(reads))
+    reads = [reads[i] for i in order]
+    b, off = pack(reads)
+    for th in (None, thresholds_L(2)):
+        g, _ = compare_all(31, th, b, off, capacity=REGION_CAP)
+        assert g["stats"]["count_by_regions"] == 1
+        assert g["stats"]["n_overflow"] >= 40
