@@ -1606,11 +1606,10 @@ bruno_status count_by_regions(bruno_job* j, const ParseArgs& a, bool* done) {
     hcur[b] = nf + nb;
   }
   /* sort each bucket on its region bits (the low rb bits of a record),
-   * find region ranges, build */
+   * then build its regions */
   const uint64_t nreg = 1ULL << rb;
-  uint64_t *alt = nullptr, *start = nullptr;
+  uint64_t* alt = nullptr;
   TRY(dev_alloc(j, (void**)&alt, std::max<uint64_t>(1, nmax) * 8));
-  TRY(dev_alloc(j, (void**)&start, (nreg + 1) * 8));
   size_t tmp = 0;
   {
     hipcub::DoubleBuffer<uint64_t> db(recs, alt);
@@ -1623,17 +1622,16 @@ bruno_status count_by_regions(bruno_job* j, const ParseArgs& a, bool* done) {
   for (int b = 0; b < REC_BUCKETS; ++b) {
     const uint64_t n = hcur[b];
     const uint64_t* sorted = recs + (uint64_t)b * bcap;
-    if (n > 1 && rb > 0) {
+    if (n > 1) {
       hipcub::DoubleBuffer<uint64_t> db(recs + (uint64_t)b * bcap, alt);
       size_t tb = tmp;
       HIP_CHECK(hipcub::DeviceRadixSort::SortKeys(t, tb, db, (int)n, 0, (int)rb, j->stream));
       sorted = db.Current();
     }
     if (n == 0 && valid) continue;
-    hipLaunchKernelGGL(k_region_starts, dim3(grid_for(n + 1)), dim3(BLOCK), 0, j->stream, sorted, n, nreg, start);
-    RegionArgs g{(Slot<uint64_t>*)j->table, j->rspan, j->rbits, (uint32_t)b, nreg, sorted, start, valid,
+    RegionArgs g{(Slot<uint64_t>*)j->table, j->rspan, j->rbits, (uint32_t)b, nreg, sorted, n, valid,
                  j->spill, j->spcap, j->ctr};
-    hipLaunchKernelGGL(k_region_build, dim3((unsigned)std::min<uint64_t>(nreg, 1024)), dim3(REGION_BLOCK), 0,
+    hipLaunchKernelGGL(k_region_build, dim3((unsigned)std::min<uint64_t>(nreg, REGION_GRID)), dim3(REGION_BLOCK), 0,
                        j->stream, g);
   }
   /* instances and dropped reads into the job's counters */
@@ -1642,9 +1640,7 @@ bruno_status count_by_regions(bruno_job* j, const ParseArgs& a, bool* done) {
                      1);
   HIP_CHECK(hipStreamSynchronize(j->stream));
   dev_free(j, t, tmp);
-  void* p = start;
-  dev_free(j, p, (nreg + 1) * 8);
-  p = alt;
+  void* p = alt;
   dev_free(j, p, std::max<uint64_t>(1, nmax) * 8);
   dev_free(j, rp, REC_BUCKETS * bcap * 8);
   dev_free(j, sp, NS * 8);

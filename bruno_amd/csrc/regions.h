@@ -10,15 +10,14 @@
  *      rotated left so the region bits below the bucket bits come lowest,
  *      into one of 32 buckets picked by hash bits 59..63.  A workgroup stages
  *      records per bucket in LDS and appends runs of >= 128 with one atomic.
- *   2. a radix sort of each bucket on its low (region) bits, and
- *      k_region_starts: the record range of every region of the bucket.
+ *   2. a radix sort of each bucket on its low (region) bits.
  *      (The region bits are put lowest because the library radix sort
  *      returned wrong orders for bit ranges starting above bit 0 on more
  *      than a block of keys.)
- *   3. k_region_build: one workgroup per region loads the region's slots
- *      into LDS (or starts from EMPTY when the table holds nothing yet), adds
- *      the region's records with LDS atomics, and writes the region back
- *      with coalesced stores.  Entries beyond a slot's NSUB sub-entries go to
+ *   3. k_region_build: a workgroup walks the sorted records of a run of
+ *      regions; per region it loads the slots into LDS (or starts from EMPTY
+ *      when the table holds nothing yet), adds the region's records with LDS
+ *      atomics, and writes the region back with coalesced stores.  Entries beyond a slot's NSUB sub-entries go to
  *      the HBM side table exactly as in the atomic count.
  *
  * The table contents equal those of the atomic count up to the order of
@@ -35,11 +34,12 @@ constexpr int REC_BUCKET_BITS = 5;
 constexpr int REC_BUCKETS = 1 << REC_BUCKET_BITS;
 constexpr uint32_t REGION_MAX_SLOTS = 2560;  /* 80 KB of 32-byte slots */
 constexpr int REGION_BLOCK = 512;
+constexpr uint64_t REGION_GRID = 1024;       /* workgroups of a bucket's build (2 per CU) */
 constexpr int REC_BLOCK = 512;
 constexpr uint64_t REC_GRID = 1024;          /* workgroups of the record parser (4 per CU) */
 constexpr int REC_STAGE = 128;               /* staged records per bucket (32 KB per workgroup) */
 constexpr int REC_FLUSH = 64;                /* a bucket is appended once it holds this many */
-constexpr int REGION_ITEMS = 12;             /* records per thread loaded at once by the build */
+constexpr int REGION_ITEMS = 8;              /* records per thread loaded at once by the build */
 
 /* biedge byte (nibbles 0, 1, 2, 4, 8 per side) -> index into BIEDGE25 */
 __device__ __forceinline__ uint32_t biedge_index(uint8_t e) {
@@ -133,13 +133,13 @@ __global__ __launch_bounds__(REC_BLOCK) void k_parse_records(ParseArgs a, RecArg
       }
       if (n) cnt[t] = 0;
     }
-    __syncthreads();
+    lds_barrier();
     for (int b = w; b < REC_BUCKETS; b += WPB) {
       const uint32_t m1 = fl_n1[b], m2 = fl_n2[b];
       const unsigned long long d1 = fl_d1[b], d2 = fl_d2[b];
       for (uint32_t i = lane; i < m1 + m2; i += 64) rec_put(ra, b, i < m1 ? d1 + i : d2 + (i - m1), stg[b][i]);
     }
-    __syncthreads();
+    lds_barrier();
   };
   for (uint64_t r0 = (uint64_t)blockIdx.x * WPB; r0 < a.n_reads; r0 += (uint64_t)gridDim.x * WPB) {
     const uint64_t r = r0 + w;
@@ -169,7 +169,7 @@ __global__ __launch_bounds__(REC_BLOCK) void k_parse_records(ParseArgs a, RecArg
         });
       }
     }
-    __syncthreads();
+    lds_barrier();
     flush(REC_FLUSH);
   }
   flush(1);
@@ -186,19 +186,6 @@ __global__ __launch_bounds__(REC_BLOCK) void k_parse_records(ParseArgs a, RecArg
   }
 }
 
-/* start[q] = first record of region q of a bucket sorted by region
- * (region = rec & (n_regions - 1)), start[n_regions] = n.  Record i starts
- * every region in (region(i-1), region(i)]; the end starts the rest. */
-__global__ __launch_bounds__(BLOCK) void k_region_starts(const uint64_t* recs, uint64_t n, uint64_t n_regions,
-                                                        uint64_t* start) {
-  const uint64_t m = n_regions - 1;
-  for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i <= n; i += (uint64_t)gridDim.x * BLOCK) {
-    const uint64_t lo = i == 0 ? 0 : (recs[i - 1] & m) + 1;
-    const uint64_t hi = i == n ? n_regions : (recs[i] & m);
-    for (uint64_t q = lo; q <= hi && q <= n_regions; ++q) start[q] = i;
-  }
-}
-
 struct RegionArgs {
   Slot<uint64_t>* table;
   uint64_t span;             /* slots per region */
@@ -206,7 +193,7 @@ struct RegionArgs {
   uint32_t bucket;
   uint64_t n_regions;        /* regions per bucket */
   const uint64_t* recs;      /* the bucket's records sorted by region */
-  const uint64_t* start;     /* n_regions + 1 */
+  uint64_t n;                /* records of the bucket */
   int valid;                 /* the table holds earlier counts (else start from EMPTY) */
   Spill* spill;
   uint64_t spcap;
@@ -278,43 +265,101 @@ __device__ __forceinline__ void region_add(const RegionArgs& g, Slot<uint64_t>* 
   my_k2 += lds_add_k2(L + s, BIEDGE25[code], base + s, g.spill, g.spcap, &g.ctr->err);
 }
 
-/* One workgroup per region of one bucket (grid-stride over regions).  The
- * first REGION_BLOCK * REGION_ITEMS records of a region are loaded with all
- * loads in flight before the slots are set up, so a region costs about one
- * memory latency plus its stores. */
-__global__ __launch_bounds__(REGION_BLOCK) void k_region_build(RegionArgs g) {
+/* Workgroup-wide count of pred, through one LDS word the caller is not
+ * using (the build's slot array between regions: two workgroups of 80 KB
+ * fill a CU's LDS, so no other LDS is available). */
+__device__ __forceinline__ uint32_t block_count(bool pred, uint32_t* word) {
+  if (threadIdx.x == 0) *word = 0;
+  lds_barrier();
+  const uint64_t bal = __ballot(pred);
+  if ((threadIdx.x & 63) == 0 && bal) atomicAdd(word, (uint32_t)__popcll(bal));
+  lds_barrier();
+  const uint32_t c = *word;
+  lds_barrier();
+  return c;
+}
+
+/* First index of the sorted records whose region (rec & m) is >= target,
+ * found by the whole workgroup: each round samples REGION_BLOCK evenly
+ * spaced records and keeps the gap holding the boundary. */
+__device__ __forceinline__ uint64_t block_lower_bound(const uint64_t* recs, uint64_t n, uint64_t m, uint64_t target,
+                                                      uint32_t* word) {
+  uint64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint64_t step = (hi - lo + REGION_BLOCK - 1) / REGION_BLOCK;
+    const uint64_t pt = lo + (uint64_t)threadIdx.x * step;
+    const uint32_t c = block_count(pt < hi && (recs[pt] & m) < target, word);
+    if (c == 0) return lo;
+    const uint64_t last_less = lo + (uint64_t)(c - 1) * step, first_ge = lo + (uint64_t)c * step;
+    lo = last_less + 1;
+    if (first_ge < hi) hi = first_ge;
+  }
+  return lo;
+}
+
+/* Build regions [qa, qb) of one bucket in LDS, workgroup i taking an even
+ * share of the bucket's regions.  Its records are contiguous in the sorted
+ * bucket: it finds the first with a binary search, then walks tiles of
+ * REGION_BLOCK * REGION_ITEMS records (all loads in flight at once),
+ * closing a region when the records move past it.  Regions without
+ * records are written EMPTY when the table holds nothing yet and are
+ * skipped otherwise.  Barriers order LDS only, so the write-back of one
+ * region overlaps the next. */
+__global__ __launch_bounds__(REGION_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_region_build(RegionArgs g) {
   __shared__ Slot<uint64_t> L[REGION_MAX_SLOTS];
   const uint32_t span = (uint32_t)g.span;
   const uint32_t words = span * 4;
   const int rot = (int)(g.rbits - REC_BUCKET_BITS);
+  const uint64_t nreg = g.n_regions, m = nreg - 1, n = g.n;
+  const uint64_t qa = (uint64_t)blockIdx.x * nreg / gridDim.x, qb = ((uint64_t)blockIdx.x + 1) * nreg / gridDim.x;
   unsigned long long my_new = 0, my_k2 = 0;
-  for (uint64_t q = blockIdx.x; q < g.n_regions; q += gridDim.x) {
-    const uint64_t r0 = g.start[q], r1 = g.start[q + 1];
-    if (r0 == r1 && g.valid) continue; /* nothing to add to a region that is already written */
-    const uint64_t region = ((uint64_t)g.bucket << rot) | q;
-    const uint64_t base = region * g.span;
+  uint64_t* Lw = (uint64_t*)L;
+  uint64_t q = qa;
+  uint64_t p = qa < qb ? block_lower_bound(g.recs, n, m, qa, (uint32_t*)L) : n;
+  bool open = false; /* LDS holds region q */
+  while (q < qb) {
     uint64_t rr[REGION_ITEMS];
 #pragma unroll
     for (int k = 0; k < REGION_ITEMS; ++k) {
-      const uint64_t i = r0 + (uint64_t)k * REGION_BLOCK + threadIdx.x;
-      rr[k] = i < r1 ? g.recs[i] : ~0ULL;
+      const uint64_t i = p + (uint64_t)k * REGION_BLOCK + threadIdx.x;
+      rr[k] = i < n ? g.recs[i] : ~0ULL;
     }
-    uint64_t* Lw = (uint64_t*)L;
-    ulonglong2* T2 = (ulonglong2*)(g.table + base);
-    if (g.valid) {
-      for (uint32_t i = threadIdx.x; i < words / 2; i += REGION_BLOCK) ((ulonglong2*)Lw)[i] = T2[i];
-    } else {
-      for (uint32_t i = threadIdx.x; i < words; i += REGION_BLOCK) Lw[i] = EMPTY64;
-    }
-    __syncthreads();
+    const uint64_t tile_end = p + (uint64_t)REGION_ITEMS * REGION_BLOCK < n ? p + (uint64_t)REGION_ITEMS * REGION_BLOCK : n;
+    /* region of the tile's last record; past the bucket's end: none */
+    const uint64_t last = tile_end < n ? (g.recs[tile_end - 1] & m) : nreg;
+    while (q < qb && q <= last) {
+      if (!open) {
+        if (g.valid) {
+          bool has = false;
 #pragma unroll
-    for (int k = 0; k < REGION_ITEMS; ++k)
-      if (rr[k] != ~0ULL) region_add(g, L, span, rot, base, rr[k], my_new, my_k2);
-    for (uint64_t i = r0 + (uint64_t)REGION_ITEMS * REGION_BLOCK + threadIdx.x; i < r1; i += REGION_BLOCK)
-      region_add(g, L, span, rot, base, g.recs[i], my_new, my_k2);
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < words / 2; i += REGION_BLOCK) T2[i] = ((const ulonglong2*)Lw)[i];
-    __syncthreads();
+          for (int k = 0; k < REGION_ITEMS; ++k) has |= rr[k] != ~0ULL && (rr[k] & m) == q;
+          if (!block_count(has, (uint32_t*)L)) { /* no records: the region stays as it is */
+            ++q;
+            continue;
+          }
+        }
+        ulonglong2* T2 = (ulonglong2*)(g.table + (((uint64_t)g.bucket << rot) | q) * g.span);
+        if (g.valid) {
+          for (uint32_t i = threadIdx.x; i < words / 2; i += REGION_BLOCK) ((ulonglong2*)Lw)[i] = T2[i];
+        } else {
+          for (uint32_t i = threadIdx.x; i < words; i += REGION_BLOCK) Lw[i] = EMPTY64;
+        }
+        lds_barrier();
+        open = true;
+      }
+      const uint64_t base = (((uint64_t)g.bucket << rot) | q) * g.span;
+#pragma unroll
+      for (int k = 0; k < REGION_ITEMS; ++k)
+        if (rr[k] != ~0ULL && (rr[k] & m) == q) region_add(g, L, span, rot, base, rr[k], my_new, my_k2);
+      if (q == last) break; /* the region continues in the next tile */
+      lds_barrier();
+      ulonglong2* T2 = (ulonglong2*)(g.table + base);
+      for (uint32_t i = threadIdx.x; i < words / 2; i += REGION_BLOCK) T2[i] = ((const ulonglong2*)Lw)[i];
+      lds_barrier();
+      open = false;
+      ++q;
+    }
+    p = tile_end;
   }
   const unsigned long long sn = wave_sum(my_new), sk = wave_sum(my_k2);
   if ((threadIdx.x & 63) == 0) {
