@@ -1218,6 +1218,17 @@ inline unsigned grid_for(uint64_t n, unsigned block = BLOCK, unsigned maxg = 1u 
   return (unsigned)g;
 }
 
+/* compute units of the current device (256 on MI355X) */
+inline uint32_t device_cus() {
+  int dev = 0, n = 0;
+  if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      n <= 0) {
+    (void)hipGetLastError();
+    return 256;
+  }
+  return (uint32_t)n;
+}
+
 enum Stage { ST_CREATED = 0, ST_COUNTED = 1, ST_FILTERED = 2, ST_CHAINS = 3, ST_RANKED = 4, ST_COMPACTED = 5 };
 
 }  // namespace
@@ -1303,6 +1314,7 @@ struct bruno_job {
   bool recycle = false;
   hipStream_t side = nullptr;
   hipEvent_t ev_chains = nullptr, ev_cleared = nullptr;
+  hipEvent_t ev_region[5] = {nullptr, nullptr, nullptr, nullptr, nullptr}; /* count by regions: two streams */
   bool clear_pending = false;
   bruno_phase_ms ms{};
 
@@ -1324,7 +1336,7 @@ namespace {
 /* Process-wide tuning knobs (bruno_set_tuning). */
 std::mutex g_tuning_mu;
 /* defaults: the autotuner's picks on the 100 Mbp / 33 M-read workload */
-bruno_tuning g_tuning = {256, 65536, 32768, 64, 1};
+bruno_tuning g_tuning = {256, 65536, 32768, 64, 1, 2560};
 bruno_tuning tuning() {
   std::lock_guard<std::mutex> g(g_tuning_mu);
   return g_tuning;
@@ -1420,7 +1432,7 @@ constexpr uint64_t REGION_MIN_CAP = 1ULL << 22;
 
 /* One table for all jobs: job i gets caps[i] slots from seg[i].  A plain
  * k <= 31 job with a large table gets 2^rbits regions of at most
- * REGION_MAX_SLOTS slots (the capacity rounds up to a whole number of
+ * region_slots slots (the capacity rounds up to a whole number of
  * regions); its slots are written by the first count, not filled here. */
 bruno_status alloc_table(bruno_job* j, const uint64_t* caps) {
   const uint32_t nj = j->n_jobs;
@@ -1430,7 +1442,8 @@ bruno_status alloc_table(bruno_job* j, const uint64_t* caps) {
   j->rspan = 0;
   if (nj == 1 && j->region_ok && j->words == 1 && tuning().count_regions && j->seg[1] >= REGION_MIN_CAP) {
     uint32_t b = REC_BUCKET_BITS;
-    while ((j->seg[1] + (1ULL << b) - 1) >> b > REGION_MAX_SLOTS) ++b;
+    const uint64_t most = tuning().region_slots;
+    while ((j->seg[1] + (1ULL << b) - 1) >> b > most) ++b;
     j->rbits = b;
     j->rspan = (j->seg[1] + (1ULL << b) - 1) >> b;
     j->seg[1] = j->rspan << b;
@@ -1563,8 +1576,19 @@ bruno_status count_by_regions(bruno_job* j, const ParseArgs& a, bool* done) {
   HIP_CHECK(hipStreamSynchronize(j->stream));
   /* one workgroup per CU slot (persistent); chunks of up to 2048 records,
    * smaller for small inputs so padding stays under ~1/8 of the records */
-  const uint64_t groups = std::max<uint64_t>(1, std::min<uint64_t>((n_reads + REC_BLOCK / 64 - 1) / (REC_BLOCK / 64),
-                                                                     REC_GRID));
+  constexpr int rbs = REC_BS;
+  /* persistent: exactly the workgroups that fit on the device at once */
+  static int per_cu = 0;
+  if (per_cu == 0) {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_parse_records<REC_BS, REC_RPW, REC_STAGE, REC_FLUSH>,
+                                                     rbs, 0) != hipSuccess ||
+        per_cu <= 0) {
+      (void)hipGetLastError();
+      per_cu = 1;
+    }
+  }
+  const uint64_t groups = std::max<uint64_t>(
+      1, std::min<uint64_t>((n_reads + rbs / 64 * REC_RPW - 1) / (rbs / 64 * REC_RPW), (uint64_t)device_cus() * per_cu));
   uint32_t chunk = 2048;
   while (chunk > (uint32_t)REC_STAGE && (uint64_t)chunk * groups * REC_BUCKETS * 8 > n_bound) chunk >>= 1;
   const uint64_t bcap = n_bound / REC_BUCKETS + n_bound / (4 * REC_BUCKETS) + 65536 + groups * chunk;
@@ -1577,7 +1601,8 @@ bruno_status count_by_regions(bruno_job* j, const ParseArgs& a, bool* done) {
   void* rp = recs;
   const uint32_t rb = j->rbits - REC_BUCKET_BITS; /* region bits inside a bucket */
   RecArgs ra{recs, bcap, front, back, stat, chunk, (int)rb};
-  hipLaunchKernelGGL(k_parse_records, dim3((unsigned)groups), dim3(REC_BLOCK), 0, j->stream, a, ra);
+  hipLaunchKernelGGL((k_parse_records<REC_BS, REC_RPW, REC_STAGE, REC_FLUSH>), dim3((unsigned)groups), dim3(rbs), 0,
+                     j->stream, a, ra);
   unsigned long long hcur[2 * REC_BUCKETS];
   HIP_CHECK(hipMemcpyAsync(hcur, front, 2 * REC_BUCKETS * 8, hipMemcpyDeviceToHost, j->stream));
   HIP_CHECK(hipStreamSynchronize(j->stream));
@@ -1605,34 +1630,72 @@ bruno_status count_by_regions(bruno_job* j, const ParseArgs& a, bool* done) {
     dev_free(j, tp, nb * 8);
     hcur[b] = nf + nb;
   }
-  /* sort each bucket on its region bits (the low rb bits of a record),
-   * then build its regions */
+  /* sort each bucket on its region bits (the low rb bits of a record) on
+   * the job's stream and build its regions on the side stream, so the sort
+   * of bucket b + 1 overlaps the build of bucket b (two sort buffers) */
+  if (!j->side && hipStreamCreateWithFlags(&j->side, hipStreamNonBlocking) != hipSuccess) {
+    (void)hipGetLastError();
+    j->side = nullptr;
+  }
+  if (!j->ev_region[0]) {
+    for (int i = 0; i < 5; ++i)
+      if (hipEventCreateWithFlags(&j->ev_region[i], hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
+        j->ev_region[i] = nullptr;
+      }
+  }
+  bool two = j->side != nullptr;
+  for (int i = 0; i < 5; ++i) two = two && j->ev_region[i];
+  hipStream_t bst = two ? j->side : j->stream;
   const uint64_t nreg = 1ULL << rb;
-  uint64_t* alt = nullptr;
-  TRY(dev_alloc(j, (void**)&alt, std::max<uint64_t>(1, nmax) * 8));
+  uint64_t* alt[2] = {nullptr, nullptr};
+  const uint64_t altn = std::max<uint64_t>(1, nmax);
+  TRY(dev_alloc(j, (void**)&alt[0], altn * 8));
+  if (two) TRY(dev_alloc(j, (void**)&alt[1], altn * 8));
+  else alt[1] = alt[0];
   size_t tmp = 0;
   {
-    hipcub::DoubleBuffer<uint64_t> db(recs, alt);
-    HIP_CHECK(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp, db, (int)std::max<uint64_t>(1, nmax), 0, (int)rb,
-                                                j->stream));
+    hipcub::DoubleBuffer<uint64_t> db(recs, alt[0]);
+    HIP_CHECK(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp, db, (int)altn, 0, (int)rb, j->stream));
   }
   void* t = nullptr;
   TRY(dev_alloc(j, &t, tmp));
   const int valid = j->table_valid ? 1 : 0;
+  hipEvent_t* ev_sorted = j->ev_region, *ev_built = j->ev_region + 2;
+  if (two) {
+    /* the build stream starts after everything before on the job's stream */
+    HIP_CHECK(hipEventRecord(j->ev_region[4], j->stream));
+    HIP_CHECK(hipStreamWaitEvent(bst, j->ev_region[4], 0));
+  }
   for (int b = 0; b < REC_BUCKETS; ++b) {
+    const int sl = b & 1;
     const uint64_t n = hcur[b];
+    if (n == 0 && valid) continue;
+    if (two && b >= 2) HIP_CHECK(hipStreamWaitEvent(j->stream, ev_built[sl], 0)); /* alt[sl] free again */
     const uint64_t* sorted = recs + (uint64_t)b * bcap;
     if (n > 1) {
-      hipcub::DoubleBuffer<uint64_t> db(recs + (uint64_t)b * bcap, alt);
+      hipcub::DoubleBuffer<uint64_t> db(recs + (uint64_t)b * bcap, alt[sl]);
       size_t tb = tmp;
       HIP_CHECK(hipcub::DeviceRadixSort::SortKeys(t, tb, db, (int)n, 0, (int)rb, j->stream));
       sorted = db.Current();
     }
-    if (n == 0 && valid) continue;
+    if (two) {
+      HIP_CHECK(hipEventRecord(ev_sorted[sl], j->stream));
+      HIP_CHECK(hipStreamWaitEvent(bst, ev_sorted[sl], 0));
+    }
     RegionArgs g{(Slot<uint64_t>*)j->table, j->rspan, j->rbits, (uint32_t)b, nreg, sorted, n, valid,
                  j->spill, j->spcap, j->ctr};
-    hipLaunchKernelGGL(k_region_build, dim3((unsigned)std::min<uint64_t>(nreg, REGION_GRID)), dim3(REGION_BLOCK), 0,
-                       j->stream, g);
+    if (j->rspan <= REGION_SMALL_SLOTS)
+      hipLaunchKernelGGL((k_region_build<REGION_SMALL_SLOTS, 256>),
+                         dim3((unsigned)std::min<uint64_t>(nreg, 4 * REGION_BUILD_CU_SLOTS)), dim3(256), 0, bst, g);
+    else
+      hipLaunchKernelGGL((k_region_build<REGION_MAX_SLOTS, 512>),
+                         dim3((unsigned)std::min<uint64_t>(nreg, 2 * REGION_BUILD_CU_SLOTS)), dim3(512), 0, bst, g);
+    if (two) HIP_CHECK(hipEventRecord(ev_built[sl], bst));
+  }
+  if (two) {
+    HIP_CHECK(hipEventRecord(j->ev_region[4], bst));
+    HIP_CHECK(hipStreamWaitEvent(j->stream, j->ev_region[4], 0));
   }
   /* instances and dropped reads into the job's counters */
   hipLaunchKernelGGL(k_add_u64, dim3(1), dim3(64), 0, j->stream, (unsigned long long*)&j->ctr->n_instances, stat, 1);
@@ -1640,8 +1703,12 @@ bruno_status count_by_regions(bruno_job* j, const ParseArgs& a, bool* done) {
                      1);
   HIP_CHECK(hipStreamSynchronize(j->stream));
   dev_free(j, t, tmp);
-  void* p = alt;
-  dev_free(j, p, std::max<uint64_t>(1, nmax) * 8);
+  void* p = alt[0];
+  dev_free(j, p, altn * 8);
+  if (two) {
+    p = alt[1];
+    dev_free(j, p, altn * 8);
+  }
   dev_free(j, rp, REC_BUCKETS * bcap * 8);
   dev_free(j, sp, NS * 8);
   j->table_valid = true;
@@ -2439,6 +2506,8 @@ bruno_status bruno_job_destroy(bruno_job* j) {
     (void)hipStreamDestroy(j->side);
   }
   if (j->ev_chains) (void)hipEventDestroy(j->ev_chains);
+  for (int i = 0; i < 5; ++i)
+    if (j->ev_region[i]) (void)hipEventDestroy(j->ev_region[i]);
   if (j->ev_cleared) (void)hipEventDestroy(j->ev_cleared);
   for (void* p : j->allocs) pool_free(p);
   j->allocs.clear();
@@ -3195,10 +3264,10 @@ bruno_status bruno_dev_free(void* p) {
 
 bruno_status bruno_job_set_recycle(bruno_job* j, int on) {
   if (!valid_job(j)) return BRUNO_BAD_INPUT;
-  if (on && !j->side) {
-    if (hipStreamCreateWithFlags(&j->side, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&j->ev_chains, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&j->ev_cleared, hipEventDisableTiming) != hipSuccess) {
+  if (on) {
+    if ((!j->side && hipStreamCreateWithFlags(&j->side, hipStreamNonBlocking) != hipSuccess) ||
+        (!j->ev_chains && hipEventCreateWithFlags(&j->ev_chains, hipEventDisableTiming) != hipSuccess) ||
+        (!j->ev_cleared && hipEventCreateWithFlags(&j->ev_cleared, hipEventDisableTiming) != hipSuccess)) {
       (void)hipGetLastError();
       return BRUNO_INTERNAL;
     }
@@ -3231,7 +3300,8 @@ bruno_status bruno_set_tuning(const bruno_tuning* t) {
   if (!t) return BRUNO_BAD_INPUT;
   const bool pow2 = t->ruler_sample >= 8 && t->ruler_sample <= 4096 && !(t->ruler_sample & (t->ruler_sample - 1));
   if ((t->parse_block != 128 && t->parse_block != 256 && t->parse_block != 512) || t->parse_grid < 64 ||
-      t->scan_grid < 1 || !pow2 || t->count_regions > 1)
+      t->scan_grid < 1 || !pow2 || t->count_regions > 1 ||
+      (t->region_slots != REGION_MAX_SLOTS && t->region_slots != REGION_SMALL_SLOTS))
     return BRUNO_BAD_INPUT;
   std::lock_guard<std::mutex> g(g_tuning_mu);
   g_tuning = *t;

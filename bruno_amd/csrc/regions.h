@@ -32,13 +32,15 @@ namespace br {
 
 constexpr int REC_BUCKET_BITS = 5;
 constexpr int REC_BUCKETS = 1 << REC_BUCKET_BITS;
-constexpr uint32_t REGION_MAX_SLOTS = 2560;  /* 80 KB of 32-byte slots */
-constexpr int REGION_BLOCK = 512;
-constexpr uint64_t REGION_GRID = 1024;       /* workgroups of a bucket's build (2 per CU) */
-constexpr int REC_BLOCK = 512;
-constexpr uint64_t REC_GRID = 1024;          /* workgroups of the record parser (4 per CU) */
-constexpr int REC_STAGE = 128;               /* staged records per bucket (32 KB per workgroup) */
-constexpr int REC_FLUSH = 64;                /* a bucket is appended once it holds this many */
+/* Region sizes: 2560 slots (80 KB of LDS, two 512-thread builds per CU) or
+ * 1280 (40 KB, four 256-thread builds per CU); the region_slots knob picks
+ * one when a table is allocated. */
+constexpr uint32_t REGION_MAX_SLOTS = 2560;
+constexpr uint32_t REGION_SMALL_SLOTS = 1280;
+constexpr uint64_t REGION_BUILD_CU_SLOTS = 512; /* build workgroups resident on the device (x2 for small) */
+/* record parser: workgroup size, reads per wave between appends, staged
+ * records per bucket, append threshold */
+constexpr int REC_BS = 512, REC_RPW = 2, REC_STAGE = 128, REC_FLUSH = 32;
 constexpr int REGION_ITEMS = 8;              /* records per thread loaded at once by the build */
 
 /* biedge byte (nibbles 0, 1, 2, 4, 8 per side) -> index into BIEDGE25 */
@@ -89,14 +91,15 @@ __device__ __forceinline__ uint64_t rec_filler(const RecArgs& ra, uint64_t at) {
 /* Parse (k <= 31) into bucketed instance records.  The read loop is
  * workgroup-uniform (wave w of a workgroup takes read r0 + w) so staged
  * buckets can be appended between reads. */
-__global__ __launch_bounds__(REC_BLOCK) void k_parse_records(ParseArgs a, RecArgs ra) {
-  __shared__ uint64_t stg[REC_BUCKETS][REC_STAGE];
+template <int BS, int RPW, int STAGE, int FLUSH>
+__global__ __launch_bounds__(BS) void k_parse_records(ParseArgs a, RecArgs ra) {
+  __shared__ uint64_t stg[REC_BUCKETS][STAGE];
   __shared__ uint32_t cnt[REC_BUCKETS];
   __shared__ unsigned long long ch_base[REC_BUCKETS]; /* open chunk (~0: none) */
   __shared__ uint32_t ch_fill[REC_BUCKETS];
   __shared__ uint32_t fl_n1[REC_BUCKETS], fl_n2[REC_BUCKETS];
   __shared__ unsigned long long fl_d1[REC_BUCKETS], fl_d2[REC_BUCKETS];
-  constexpr int WPB = REC_BLOCK / 64;
+  constexpr int WPB = BS / 64;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int K = a.K;
   const uint32_t C = ra.chunk;
@@ -114,7 +117,7 @@ __global__ __launch_bounds__(REC_BLOCK) void k_parse_records(ParseArgs a, RecArg
     if (threadIdx.x < REC_BUCKETS) {
       const uint32_t t = threadIdx.x;
       const uint32_t c = cnt[t];
-      uint32_t n = c < (uint32_t)REC_STAGE ? c : (uint32_t)REC_STAGE;
+      uint32_t n = c < (uint32_t)STAGE ? c : (uint32_t)STAGE;
       if (n < at) n = 0;
       const unsigned long long base = ch_base[t];
       const uint32_t fill = ch_fill[t];
@@ -141,8 +144,9 @@ __global__ __launch_bounds__(REC_BLOCK) void k_parse_records(ParseArgs a, RecArg
     }
     lds_barrier();
   };
-  for (uint64_t r0 = (uint64_t)blockIdx.x * WPB; r0 < a.n_reads; r0 += (uint64_t)gridDim.x * WPB) {
-    const uint64_t r = r0 + w;
+  for (uint64_t r0 = (uint64_t)blockIdx.x * WPB * RPW; r0 < a.n_reads; r0 += (uint64_t)gridDim.x * WPB * RPW) {
+    for (int jr = 0; jr < RPW; ++jr) {
+    const uint64_t r = r0 + (uint64_t)jr * WPB + w;
     if (r < a.n_reads) {
       const uint64_t off = a.starts ? a.starts[r] : a.offsets[r];
       const uint64_t L = (a.starts ? a.ends[r] : a.offsets[r + 1]) - off;
@@ -158,7 +162,7 @@ __global__ __launch_bounds__(REC_BLOCK) void k_parse_records(ParseArgs a, RecArg
           const uint32_t b = (uint32_t)(h >> (64 - REC_BUCKET_BITS));
           const uint64_t rec = rotl64((h << REC_BUCKET_BITS) | biedge_index(e), ra.rot);
           const uint32_t pos = atomicAdd(&cnt[b], 1u);
-          if (pos < (uint32_t)REC_STAGE) {
+          if (pos < (uint32_t)STAGE) {
             stg[b][pos] = rec;
           } else {
             /* stage full (a long read): one record from the back */
@@ -169,8 +173,9 @@ __global__ __launch_bounds__(REC_BLOCK) void k_parse_records(ParseArgs a, RecArg
         });
       }
     }
+    }
     lds_barrier();
-    flush(REC_FLUSH);
+    flush(FLUSH);
   }
   flush(1);
   /* pad the open chunks */
@@ -280,13 +285,15 @@ __device__ __forceinline__ uint32_t block_count(bool pred, uint32_t* word) {
 }
 
 /* First index of the sorted records whose region (rec & m) is >= target,
- * found by the whole workgroup: each round samples REGION_BLOCK evenly
+ * found by the whole workgroup: each round samples one record per thread,
+ * evenly
  * spaced records and keeps the gap holding the boundary. */
+template <int BS>
 __device__ __forceinline__ uint64_t block_lower_bound(const uint64_t* recs, uint64_t n, uint64_t m, uint64_t target,
                                                       uint32_t* word) {
   uint64_t lo = 0, hi = n;
   while (lo < hi) {
-    const uint64_t step = (hi - lo + REGION_BLOCK - 1) / REGION_BLOCK;
+    const uint64_t step = (hi - lo + BS - 1) / BS;
     const uint64_t pt = lo + (uint64_t)threadIdx.x * step;
     const uint32_t c = block_count(pt < hi && (recs[pt] & m) < target, word);
     if (c == 0) return lo;
@@ -300,13 +307,15 @@ __device__ __forceinline__ uint64_t block_lower_bound(const uint64_t* recs, uint
 /* Build regions [qa, qb) of one bucket in LDS, workgroup i taking an even
  * share of the bucket's regions.  Its records are contiguous in the sorted
  * bucket: it finds the first with a binary search, then walks tiles of
- * REGION_BLOCK * REGION_ITEMS records (all loads in flight at once),
+ * BS * REGION_ITEMS records (all loads of a tile in flight at once, the
+ * next tile loading while one is processed),
  * closing a region when the records move past it.  Regions without
  * records are written EMPTY when the table holds nothing yet and are
  * skipped otherwise.  Barriers order LDS only, so the write-back of one
  * region overlaps the next. */
-__global__ __launch_bounds__(REGION_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_region_build(RegionArgs g) {
-  __shared__ Slot<uint64_t> L[REGION_MAX_SLOTS];
+template <uint32_t SLOTS, int BS>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) void k_region_build(RegionArgs g) {
+  __shared__ Slot<uint64_t> L[SLOTS];
   const uint32_t span = (uint32_t)g.span;
   const uint32_t words = span * 4;
   const int rot = (int)(g.rbits - REC_BUCKET_BITS);
@@ -315,18 +324,28 @@ __global__ __launch_bounds__(REGION_BLOCK) __attribute__((amdgpu_waves_per_eu(4)
   unsigned long long my_new = 0, my_k2 = 0;
   uint64_t* Lw = (uint64_t*)L;
   uint64_t q = qa;
-  uint64_t p = qa < qb ? block_lower_bound(g.recs, n, m, qa, (uint32_t*)L) : n;
+  uint64_t p = qa < qb ? block_lower_bound<BS>(g.recs, n, m, qa, (uint32_t*)L) : n;
   bool open = false; /* LDS holds region q */
+  /* tile t is processed while tile t + 1 loads */
+  constexpr uint64_t TILE = (uint64_t)REGION_ITEMS * BS;
+  uint64_t rr[REGION_ITEMS];
+#pragma unroll
+  for (int k = 0; k < REGION_ITEMS; ++k) {
+    const uint64_t i = p + (uint64_t)k * BS + threadIdx.x;
+    rr[k] = i < n ? g.recs[i] : ~0ULL;
+  }
+  uint64_t tile_end = p + TILE < n ? p + TILE : n;
+  /* region of the tile's last record; past the bucket's end: none */
+  uint64_t last = tile_end < n ? (g.recs[tile_end - 1] & m) : nreg;
   while (q < qb) {
-    uint64_t rr[REGION_ITEMS];
+    uint64_t rn[REGION_ITEMS];
 #pragma unroll
     for (int k = 0; k < REGION_ITEMS; ++k) {
-      const uint64_t i = p + (uint64_t)k * REGION_BLOCK + threadIdx.x;
-      rr[k] = i < n ? g.recs[i] : ~0ULL;
+      const uint64_t i = tile_end + (uint64_t)k * BS + threadIdx.x;
+      rn[k] = i < n ? g.recs[i] : ~0ULL;
     }
-    const uint64_t tile_end = p + (uint64_t)REGION_ITEMS * REGION_BLOCK < n ? p + (uint64_t)REGION_ITEMS * REGION_BLOCK : n;
-    /* region of the tile's last record; past the bucket's end: none */
-    const uint64_t last = tile_end < n ? (g.recs[tile_end - 1] & m) : nreg;
+    const uint64_t next_end = tile_end + TILE < n ? tile_end + TILE : n;
+    const uint64_t next_last = next_end < n ? (g.recs[next_end - 1] & m) : nreg;
     while (q < qb && q <= last) {
       if (!open) {
         if (g.valid) {
@@ -340,9 +359,9 @@ __global__ __launch_bounds__(REGION_BLOCK) __attribute__((amdgpu_waves_per_eu(4)
         }
         ulonglong2* T2 = (ulonglong2*)(g.table + (((uint64_t)g.bucket << rot) | q) * g.span);
         if (g.valid) {
-          for (uint32_t i = threadIdx.x; i < words / 2; i += REGION_BLOCK) ((ulonglong2*)Lw)[i] = T2[i];
+          for (uint32_t i = threadIdx.x; i < words / 2; i += BS) ((ulonglong2*)Lw)[i] = T2[i];
         } else {
-          for (uint32_t i = threadIdx.x; i < words; i += REGION_BLOCK) Lw[i] = EMPTY64;
+          for (uint32_t i = threadIdx.x; i < words; i += BS) Lw[i] = EMPTY64;
         }
         lds_barrier();
         open = true;
@@ -354,12 +373,15 @@ __global__ __launch_bounds__(REGION_BLOCK) __attribute__((amdgpu_waves_per_eu(4)
       if (q == last) break; /* the region continues in the next tile */
       lds_barrier();
       ulonglong2* T2 = (ulonglong2*)(g.table + base);
-      for (uint32_t i = threadIdx.x; i < words / 2; i += REGION_BLOCK) T2[i] = ((const ulonglong2*)Lw)[i];
+      for (uint32_t i = threadIdx.x; i < words / 2; i += BS) T2[i] = ((const ulonglong2*)Lw)[i];
       lds_barrier();
       open = false;
       ++q;
     }
-    p = tile_end;
+#pragma unroll
+    for (int k = 0; k < REGION_ITEMS; ++k) rr[k] = rn[k];
+    tile_end = next_end;
+    last = next_last;
   }
   const unsigned long long sn = wave_sum(my_new), sk = wave_sum(my_k2);
   if ((threadIdx.x & 63) == 0) {

@@ -294,6 +294,8 @@ typedef struct {
                               grouped by region, then building each region on chip; 0: one
                               probe window and atomic counting in HBM.  Read when a table
                               is allocated and at each count.                               */
+  uint32_t region_slots;   /* most slots of a region when a region table is allocated:
+                              2560 (80 KB of LDS per build workgroup) or 1280 (40 KB)       */
 } bruno_tuning;
 bruno_status bruno_set_tuning(const bruno_tuning* t);
 bruno_status bruno_get_tuning(bruno_tuning* t);
