@@ -40,7 +40,7 @@ constexpr uint32_t REGION_SMALL_SLOTS = 1280;
 constexpr uint64_t REGION_BUILD_CU_SLOTS = 512; /* build workgroups resident on the device (x2 for small) */
 /* record parser: workgroup size, reads per wave between appends, staged
  * records per bucket, append threshold */
-constexpr int REC_BS = 512, REC_RPW = 2, REC_STAGE = 128, REC_FLUSH = 32;
+constexpr int REC_BS = 512, REC_RPW = 4, REC_STAGE = 192, REC_FLUSH = 32;
 constexpr int REGION_ITEMS = 8;              /* records per thread loaded at once by the build */
 
 /* biedge byte (nibbles 0, 1, 2, 4, 8 per side) -> index into BIEDGE25 */
