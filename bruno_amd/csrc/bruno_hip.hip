@@ -191,6 +191,50 @@ __global__ __launch_bounds__(BLOCK) void k_filter_scan(const Slot<KT>* T, uint64
   lds_flush(Ls, spilled, spilled_cap, &ctr->n_spill, &ctr->err);
 }
 
+/* The filter pre-pass from the bits a region build left (regions.h):
+ * candidate and spilled slots of a region-layout table in slot order,
+ * FILTER_BITS_WPT words per thread per reservation. */
+constexpr int FILTER_BITS_WPT = 8;
+__global__ __launch_bounds__(BLOCK) void k_filter_bits(const uint64_t* cbits, const uint64_t* sbits, uint64_t n_regions,
+                                                       uint64_t stride, uint64_t span, uint64_t* cand,
+                                                       uint64_t cand_cap, uint64_t* spilled, uint64_t spilled_cap,
+                                                       Counters* ctr) {
+  __shared__ ReserveSmem sm;
+  __shared__ ReserveSmem sm2;
+  const uint64_t n_words = n_regions * stride, used = (span + 63) / 64;
+  const uint64_t tile = (uint64_t)BLOCK * FILTER_BITS_WPT;
+  for (uint64_t base = (uint64_t)blockIdx.x * tile; base < n_words; base += (uint64_t)gridDim.x * tile) {
+    uint64_t cw[FILTER_BITS_WPT], sw[FILTER_BITS_WPT];
+    uint32_t nc = 0, ns = 0;
+#pragma unroll
+    for (int q = 0; q < FILTER_BITS_WPT; ++q) {
+      const uint64_t w = base + (uint64_t)q * BLOCK + threadIdx.x;
+      cw[q] = sw[q] = 0;
+      if (w < n_words && w % stride < used) {
+        cw[q] = cbits[w];
+        sw[q] = sbits[w];
+      }
+      nc += __popcll(cw[q]);
+      ns += __popcll(sw[q]);
+    }
+    unsigned long long pc = block_reserve(nc, &ctr->n_cand, sm);
+    unsigned long long ps = block_reserve(ns, &ctr->n_spill, sm2);
+#pragma unroll
+    for (int q = 0; q < FILTER_BITS_WPT; ++q) {
+      const uint64_t w = base + (uint64_t)q * BLOCK + threadIdx.x;
+      const uint64_t s0 = (w / stride) * span + (w % stride) * 64;
+      for (uint64_t m = cw[q]; m; m &= m - 1) {
+        if (pc < cand_cap) cand[pc] = s0 + __builtin_ctzll(m);
+        ++pc;
+      }
+      for (uint64_t m = sw[q]; m; m &= m - 1) {
+        if (ps < spilled_cap) spilled[ps] = s0 + __builtin_ctzll(m);
+        ++ps;
+      }
+    }
+  }
+}
+
 /* Pass 2: local_insert_by_freqencies on the candidate slots (all k2
  * entries in the slot), node classification, record written in place and
  * the node appended to the chain or branch list. */
@@ -1254,6 +1298,11 @@ struct bruno_job {
   bool region_ok = false;     /* plain job: the region layout may be chosen */
   bool table_valid = false;   /* every slot holds a key or EMPTY (else it is stale) */
   bool by_regions = false;    /* the last count built the table region by region */
+  /* the only count since the reset built every region and left the filter
+   * scan's candidate / spilled-slot bits */
+  bool bits_fresh = false;
+  uint64_t* cbits = nullptr;
+  uint64_t* sbits = nullptr;
   Spill* spill = nullptr;
   Counters* ctr = nullptr;      /* device */
   Counters* hctr = nullptr;     /* pinned host mirror */
@@ -1590,7 +1639,8 @@ bruno_status count_by_regions(bruno_job* j, const ParseArgs& a, bool* done) {
   const uint64_t groups = std::max<uint64_t>(
       1, std::min<uint64_t>((n_reads + rbs / 64 * REC_RPW - 1) / (rbs / 64 * REC_RPW), (uint64_t)device_cus() * per_cu));
   uint32_t chunk = 2048;
-  while (chunk > (uint32_t)REC_STAGE && (uint64_t)chunk * groups * REC_BUCKETS * 8 > n_bound) chunk >>= 1;
+  /* a flush appends at most REC_STAGE records, so a chunk holds at least that many */
+  while (chunk / 2 >= (uint32_t)REC_STAGE && (uint64_t)chunk * groups * REC_BUCKETS * 8 > n_bound) chunk >>= 1;
   const uint64_t bcap = n_bound / REC_BUCKETS + n_bound / (4 * REC_BUCKETS) + 65536 + groups * chunk;
   void* sp = scratch;
   uint64_t* recs = nullptr;
@@ -1661,6 +1711,17 @@ bruno_status count_by_regions(bruno_job* j, const ParseArgs& a, bool* done) {
   void* t = nullptr;
   TRY(dev_alloc(j, &t, tmp));
   const int valid = j->table_valid ? 1 : 0;
+  /* a first count builds every region: it also leaves the filter scan's
+   * bits (stale once another count follows) */
+  const uint64_t stride = (j->rspan + 63) / 64, nbits = ((uint64_t)1 << j->rbits) * stride;
+  uint64_t *cbits = nullptr, *sbits = nullptr;
+  if (!valid && dev_alloc(j, (void**)&cbits, nbits * 8) == BRUNO_OK &&
+      dev_alloc(j, (void**)&sbits, nbits * 8) != BRUNO_OK) {
+    void* cp = cbits;
+    dev_free(j, cp, nbits * 8);
+    cbits = nullptr;
+  }
+  if (!sbits) cbits = nullptr;
   hipEvent_t* ev_sorted = j->ev_region, *ev_built = j->ev_region + 2;
   if (two) {
     /* the build stream starts after everything before on the job's stream */
@@ -1684,7 +1745,7 @@ bruno_status count_by_regions(bruno_job* j, const ParseArgs& a, bool* done) {
       HIP_CHECK(hipStreamWaitEvent(bst, ev_sorted[sl], 0));
     }
     RegionArgs g{(Slot<uint64_t>*)j->table, j->rspan, j->rbits, (uint32_t)b, nreg, sorted, n, valid,
-                 j->spill, j->spcap, j->ctr};
+                 j->spill, j->spcap, j->ctr, cbits, sbits, stride, j->thresholding, j->th.t1lo};
     if (j->rspan <= REGION_SMALL_SLOTS)
       hipLaunchKernelGGL((k_region_build<REGION_SMALL_SLOTS, 256>),
                          dim3((unsigned)std::min<uint64_t>(nreg, 4 * REGION_BUILD_CU_SLOTS)), dim3(256), 0, bst, g);
@@ -1712,6 +1773,9 @@ bruno_status count_by_regions(bruno_job* j, const ParseArgs& a, bool* done) {
   dev_free(j, rp, REC_BUCKETS * bcap * 8);
   dev_free(j, sp, NS * 8);
   j->table_valid = true;
+  j->cbits = cbits;
+  j->sbits = sbits;
+  j->bits_fresh = cbits != nullptr;
   *done = true;
   return BRUNO_OK;
 }
@@ -1740,6 +1804,7 @@ bruno_status parse_count_t(bruno_job* j, const char* d_bases, const uint64_t* d_
   a.ctr = j->ctr;
   const bruno_tuning t = tuning();
   j->by_regions = false;
+  j->bits_fresh = false;
   if (std::is_same<KT, uint64_t>::value && j->rbits && t.count_regions) {
     bool done = false;
     TRY(count_by_regions(j, a, &done));
@@ -1773,12 +1838,21 @@ bruno_status filter_t(bruno_job* j) {
   uint64_t *cand = nullptr, *spilled = nullptr;
   TRY(dev_alloc(j, (void**)&cand, n_distinct * sizeof(uint64_t)));
   TRY(dev_alloc(j, (void**)&spilled, n_spill_cap * sizeof(uint64_t)));
-  hipLaunchKernelGGL((k_filter_scan<KT>),
-                     dim3(grid_for((j->cap + FILTER_SPT - 1) / FILTER_SPT, BLOCK, tuning().scan_grid)),
-                     dim3(BLOCK), 0, j->stream, (const Slot<KT>*)j->table, j->cap, j->thresholding, j->th.t1lo, cand,
-                     n_distinct, spilled, n_spill_cap, j->ctr);
+  if (j->bits_fresh) {
+    /* the last count built every region and left the scan's choice */
+    const uint64_t nreg = 1ULL << j->rbits, stride = (j->rspan + 63) / 64;
+    hipLaunchKernelGGL(k_filter_bits, dim3(grid_for((nreg * stride + FILTER_BITS_WPT - 1) / FILTER_BITS_WPT)),
+                       dim3(BLOCK), 0, j->stream, j->cbits, j->sbits, nreg, stride, j->rspan, cand, n_distinct,
+                       spilled, n_spill_cap, j->ctr);
+  } else {
+    hipLaunchKernelGGL((k_filter_scan<KT>),
+                       dim3(grid_for((j->cap + FILTER_SPT - 1) / FILTER_SPT, BLOCK, tuning().scan_grid)),
+                       dim3(BLOCK), 0, j->stream, (const Slot<KT>*)j->table, j->cap, j->thresholding, j->th.t1lo,
+                       cand, n_distinct, spilled, n_spill_cap, j->ctr);
+  }
   TRY(sync_counters(j));
   const uint64_t n_cand = j->hctr->n_cand, n_spilled = j->hctr->n_spill;
+  if (n_cand > n_distinct || n_spilled > n_spill_cap) return BRUNO_CAPACITY;
   /* nodes: at most the candidates plus the spilled slots, in either list */
   const uint64_t lcap = std::max<uint64_t>(1, n_cand + n_spilled);
   TRY(dev_alloc(j, (void**)&j->chain_slot, lcap * sizeof(uint64_t)));
@@ -2542,6 +2616,8 @@ bruno_status bruno_job_reset(bruno_job* j) {
   j->iterations = 0;
   j->n_cycle = j->n_isolated = j->conflicts = 0;
   j->lr_contracted = j->out_by_chains = false;
+  j->bits_fresh = false;
+  j->cbits = j->sbits = nullptr;
   j->n_out = j->n_unitigs = j->fasta_len = j->bfasta_len = 0;
   j->n_reads = 0;
   j->job_reads.assign(j->n_jobs, 0);

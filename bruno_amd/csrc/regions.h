@@ -203,6 +203,14 @@ struct RegionArgs {
   Spill* spill;
   uint64_t spcap;
   Counters* ctr;
+  /* filter pre-pass (optional): per region `stride` words of candidate
+   * bits (occupied, no spilled entries, k-mer count >= t1lo when
+   * thresholding) and of spilled-slot bits, bit i of word w = slot 64 w + i */
+  uint64_t* cbits;
+  uint64_t* sbits;
+  uint64_t stride;
+  int thresholding;
+  uint64_t t1lo;
 };
 
 __device__ __forceinline__ uint64_t lds_load(const uint64_t* p) {
@@ -372,6 +380,39 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) void k_
         if (rr[k] != ~0ULL && (rr[k] & m) == q) region_add(g, L, span, rot, base, rr[k], my_new, my_k2);
       if (q == last) break; /* the region continues in the next tile */
       lds_barrier();
+      if (g.cbits) {
+        /* the filter scan's choice for every slot of the region, one
+         * ballot per 64 slots */
+        const uint64_t region = ((uint64_t)g.bucket << rot) | q;
+        for (uint32_t i0 = 0; i0 < span; i0 += BS) {
+          const uint32_t i = i0 + threadIdx.x;
+          bool c = false, sp = false;
+          if (i < span) {
+            const uint64_t key = L[i].key;
+            if (key != EMPTY64) {
+              if (key & F_SPILL) {
+                sp = true;
+              } else if (!g.thresholding) {
+                c = true;
+              } else {
+                uint64_t tot = 0;
+#pragma unroll
+                for (int j = 0; j < Slot<uint64_t>::NSUB; ++j) {
+                  const uint64_t v = L[i].sub[j];
+                  tot += v != EMPTY64 ? (v & 0xFFFFFFFFFFFFULL) : 0;
+                }
+                c = tot >= g.t1lo;
+              }
+            }
+          }
+          const uint64_t bc = __ballot(c), bsp = __ballot(sp);
+          const uint32_t w0 = i0 + (threadIdx.x & ~63u);
+          if ((threadIdx.x & 63) == 0 && w0 < span) {
+            g.cbits[region * g.stride + w0 / 64] = bc;
+            g.sbits[region * g.stride + w0 / 64] = bsp;
+          }
+        }
+      }
       ulonglong2* T2 = (ulonglong2*)(g.table + base);
       for (uint32_t i = threadIdx.x; i < words / 2; i += BS) T2[i] = ((const ulonglong2*)Lw)[i];
       lds_barrier();
