@@ -16,10 +16,12 @@ against the same jobs run one job per launch.
 estimated HBM footprint and batches whatever is waiting on two streams; the
 same arrivals are replayed against a one-job-at-a-time server.
 
-Prints one JSON line with the metric, a roofline object for the count kernel
-(the dominant kernel) and a CPU baseline: the oracle (the C++ restatement of
-bruno's algorithm, hash-partitioned over host threads) timed on a bounded
-sample of the same workload.
+Prints one JSON line with the metric, a roofline object for the count (the
+dominant phase: on k <= 31 region tables a record parser, per-bucket radix
+sorts and on-chip region builds, timed together with device events) and a
+CPU baseline: the oracle (the C++ restatement of bruno's algorithm,
+hash-partitioned over host threads) timed on a bounded sample of the same
+workload.
 """
 import argparse
 import json
@@ -232,23 +234,25 @@ def main():
                                            "n_unitigs", "n_cycle_nodes", "iterations", "capacity", "device_bytes",
                                            "peak_device_bytes")},
         "roofline": {
-            "kernel": "k_parse (parse + k2 count)",
+            "kernel": ("count phase, one launch group per batch: k_parse_records + per-bucket radix sorts + "
+                       "k_region_build (device events around the phase)") if stats["count_by_regions"] else
+                      "k_parse (parse + atomic k2 count)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(args.config, args.scale, "k_parse") if lf == 0.5 else None,
+            "traffic": measured_traffic(args.config, args.scale, "count_phase")
+                       if lf == 0.5 and stats["count_by_regions"] and fq is None else None,
             "bytes_per_unit": "1 B per base + 128 B per k-mer instance (one 64-byte slot read and write)",
         },
         "roofline_filter": {
-            "kernel": "filter phase (table scan + candidate and spilled-slot passes)",
+            "kernel": "filter phase (candidate bits or table scan, then candidate and spilled-slot passes)",
             "bound": "hbm",
             "achieved": filter_bytes(stats, k) / (ph["filter_ms"] * 1e-3) / 1e9,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": filter_bytes(stats, k) / (ph["filter_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
-            "table_stream_gbs": stats["capacity"] * (32 if k <= 31 else 64) / (ph["filter_ms"] * 1e-3) / 1e9,
             "bytes_per_unit": "16 B per distinct k2-mer read (24 B for k > 31) + 32 B per kept k-mer written (48 B)",
         },
     }
