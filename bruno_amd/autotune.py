@@ -3,10 +3,13 @@
 The knobs change speed only, never results:
   load factor   hash slots per distinct k-mer (count probes vs. table size,
                 which the filter scan and the table reset stream);
-  parse_block   workgroup size of the parse + count kernel;
-  parse_grid    waves of the parse + count launch;
-  scan_grid     workgroups of the filter's table scan;
+  region_slots  region size of a region-layout table (k <= 31 plain jobs);
+  parse_block   workgroup size of the atomic parse + count kernel;
+  parse_grid    waves of the atomic parse + count launch;
+  scan_grid     workgroups of the filter's table scan (atomic-count tables);
   ruler_sample  list-ranking ruler density (walk length vs. contracted size).
+The region-count path has no launch knobs of the atomic kernel, so those
+trials run only when the workload counts with atomics.
 
 tune() runs coordinate descent on the caller's own reads (already in device
 memory): each knob in turn, the others fixed at their best so far, each
@@ -25,6 +28,7 @@ PARSE_BLOCKS = (128, 256, 512)
 PARSE_GRIDS = (16384, 32768, 65536, 131072, 262144)
 SCAN_GRIDS = (4096, 8192, 16384, 32768)
 RULERS = (16, 32, 64, 128, 256)
+REGION_SLOTS = (2560, 1280)
 
 
 def get_tuning():
@@ -56,6 +60,8 @@ def tune(k, thresholds, bases_ptr, offsets_ptr, n_reads, capacity_for_lf, reps=2
     say = log or (lambda *a: None)
     trials = []
 
+    regions = {"on": False}
+
     def run(lf, phases):
         job = GraphJob(k, thresholds, capacity=capacity_for_lf(lf))
         best = None
@@ -69,6 +75,7 @@ def tune(k, thresholds, bases_ptr, offsets_ptr, n_reads, capacity_for_lf, reps=2
                 ph = job.phase_ms()
                 v = sum(ph[p] for p in phases) if phases else sum(ph.values())
                 best = v if best is None else min(best, v)
+            regions["on"] = bool(job.stats()["count_by_regions"])
         finally:
             job.close()
         return best
@@ -80,24 +87,35 @@ def tune(k, thresholds, bases_ptr, offsets_ptr, n_reads, capacity_for_lf, reps=2
         trials.append({"knob": "load_factor", "value": lf, "ms": res[lf]})
         say("load factor %.2f: %.1f ms" % (lf, res[lf]))
     lf = min(res, key=res.get)
-    # 2. parse / count launch shape
-    res = {}
-    for b in PARSE_BLOCKS:
-        for g in PARSE_GRIDS:
-            set_tuning(parse_block=b, parse_grid=g)
-            res[(b, g)] = run(lf, ("parse_count_ms",))
-            trials.append({"knob": "parse", "value": [b, g], "ms": res[(b, g)]})
-            say("parse block %d grid %d: %.1f ms" % (b, g, res[(b, g)]))
-    b, g = min(res, key=res.get)
-    set_tuning(parse_block=b, parse_grid=g)
-    # 3. filter scan grid
-    res = {}
-    for sg in SCAN_GRIDS:
-        set_tuning(scan_grid=sg)
-        res[sg] = run(lf, ("filter_ms",))
-        trials.append({"knob": "scan_grid", "value": sg, "ms": res[sg]})
-        say("scan grid %d: %.1f ms" % (sg, res[sg]))
-    set_tuning(scan_grid=min(res, key=res.get))
+    run(lf, ("parse_count_ms",))
+    if regions["on"]:
+        # 2a. region size of the region count (the table layout)
+        res = {}
+        for rs in REGION_SLOTS:
+            set_tuning(region_slots=rs)
+            res[rs] = run(lf, ("parse_count_ms",))
+            trials.append({"knob": "region_slots", "value": rs, "ms": res[rs]})
+            say("region slots %d: %.1f ms" % (rs, res[rs]))
+        set_tuning(region_slots=min(res, key=res.get))
+    else:
+        # 2b. launch shape of the atomic count
+        res = {}
+        for b in PARSE_BLOCKS:
+            for g in PARSE_GRIDS:
+                set_tuning(parse_block=b, parse_grid=g)
+                res[(b, g)] = run(lf, ("parse_count_ms",))
+                trials.append({"knob": "parse", "value": [b, g], "ms": res[(b, g)]})
+                say("parse block %d grid %d: %.1f ms" % (b, g, res[(b, g)]))
+        b, g = min(res, key=res.get)
+        set_tuning(parse_block=b, parse_grid=g)
+        # 3. filter scan grid (a region count leaves the scan's bits instead)
+        res = {}
+        for sg in SCAN_GRIDS:
+            set_tuning(scan_grid=sg)
+            res[sg] = run(lf, ("filter_ms",))
+            trials.append({"knob": "scan_grid", "value": sg, "ms": res[sg]})
+            say("scan grid %d: %.1f ms" % (sg, res[sg]))
+        set_tuning(scan_grid=min(res, key=res.get))
     # 4. list-ranking rulers
     res = {}
     for r in RULERS:

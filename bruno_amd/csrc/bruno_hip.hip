@@ -1273,6 +1273,26 @@ inline uint32_t device_cus() {
   return (uint32_t)n;
 }
 
+/* One side stream per device, shared by every job and kept for the
+ * process: creating and destroying a stream per job costs milliseconds.
+ * Jobs order their work on it only through events, so sharing it can only
+ * serialise two jobs' side work, never deadlock. */
+inline hipStream_t shared_side_stream() {
+  static std::mutex mu;
+  static hipStream_t s[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> g(mu);
+  if (!s[dev] && hipStreamCreateWithFlags(&s[dev], hipStreamNonBlocking) != hipSuccess) {
+    (void)hipGetLastError();
+    s[dev] = nullptr;
+  }
+  return s[dev];
+}
+
 enum Stage { ST_CREATED = 0, ST_COUNTED = 1, ST_FILTERED = 2, ST_CHAINS = 3, ST_RANKED = 4, ST_COMPACTED = 5 };
 
 }  // namespace
@@ -1476,8 +1496,12 @@ double ev_ms(bruno_job* j, int a, int b) {
 }
 
 
-/* Tables of at least this many slots get the region layout. */
-constexpr uint64_t REGION_MIN_CAP = 1ULL << 22;
+/* Tables of at least this many slots get the region layout (count_regions
+ * 1); count_regions 2 lowers the bar to REGION_MIN_CAP_ANY.  Below the
+ * default bar the atomic count is as fast: the region count's fixed cost
+ * (32 bucket sorts and builds, host round trips) dominates small tables. */
+constexpr uint64_t REGION_MIN_CAP = 1ULL << 27;
+constexpr uint64_t REGION_MIN_CAP_ANY = 1ULL << 22;
 
 /* One table for all jobs: job i gets caps[i] slots from seg[i].  A plain
  * k <= 31 job with a large table gets 2^rbits regions of at most
@@ -1489,7 +1513,9 @@ bruno_status alloc_table(bruno_job* j, const uint64_t* caps) {
   for (uint32_t i = 0; i < nj; ++i) j->seg[i + 1] = j->seg[i] + caps[i];
   j->rbits = 0;
   j->rspan = 0;
-  if (nj == 1 && j->region_ok && j->words == 1 && tuning().count_regions && j->seg[1] >= REGION_MIN_CAP) {
+  const uint32_t cr = tuning().count_regions;
+  if (nj == 1 && j->region_ok && j->words == 1 && cr &&
+      j->seg[1] >= (cr >= 2 ? REGION_MIN_CAP_ANY : REGION_MIN_CAP)) {
     uint32_t b = REC_BUCKET_BITS;
     const uint64_t most = tuning().region_slots;
     while ((j->seg[1] + (1ULL << b) - 1) >> b > most) ++b;
@@ -1626,6 +1652,18 @@ bruno_status count_by_regions(bruno_job* j, const ParseArgs& a, bool* done) {
   /* one workgroup per CU slot (persistent); chunks of up to 2048 records,
    * smaller for small inputs so padding stays under ~1/8 of the records */
   constexpr int rbs = REC_BS;
+  /* reads per wave between appends: a round adds ~WPB x (instances per
+   * read) / 32 records to each bucket, which must fit the room left after
+   * an append (a quarter again for spread); reads too long for one per round go
+   * to the atomic count (the stage would overflow into per-record atomics) */
+  const double per_read = (double)n_bound / (double)std::max<uint64_t>(1, n_reads);
+  const double per_round = (REC_BS / 64) * per_read / REC_BUCKETS * 1.25;
+  const uint32_t rpw = (uint32_t)std::min<double>(REC_RPW, (REC_STAGE - REC_FLUSH) / std::max(1.0, per_round));
+  if (rpw == 0) {
+    void* sp0 = scratch;
+    dev_free(j, sp0, NS * 8);
+    return BRUNO_OK;
+  }
   /* persistent: exactly the workgroups that fit on the device at once */
   static int per_cu = 0;
   if (per_cu == 0) {
@@ -1637,7 +1675,7 @@ bruno_status count_by_regions(bruno_job* j, const ParseArgs& a, bool* done) {
     }
   }
   const uint64_t groups = std::max<uint64_t>(
-      1, std::min<uint64_t>((n_reads + rbs / 64 * REC_RPW - 1) / (rbs / 64 * REC_RPW), (uint64_t)device_cus() * per_cu));
+      1, std::min<uint64_t>((n_reads + rbs / 64 * rpw - 1) / (rbs / 64 * rpw), (uint64_t)device_cus() * per_cu));
   uint32_t chunk = 2048;
   /* a flush appends at most REC_STAGE records, so a chunk holds at least that many */
   while (chunk / 2 >= (uint32_t)REC_STAGE && (uint64_t)chunk * groups * REC_BUCKETS * 8 > n_bound) chunk >>= 1;
@@ -1650,7 +1688,7 @@ bruno_status count_by_regions(bruno_job* j, const ParseArgs& a, bool* done) {
   }
   void* rp = recs;
   const uint32_t rb = j->rbits - REC_BUCKET_BITS; /* region bits inside a bucket */
-  RecArgs ra{recs, bcap, front, back, stat, chunk, (int)rb};
+  RecArgs ra{recs, bcap, front, back, stat, chunk, (int)rb, rpw};
   hipLaunchKernelGGL((k_parse_records<REC_BS, REC_RPW, REC_STAGE, REC_FLUSH>), dim3((unsigned)groups), dim3(rbs), 0,
                      j->stream, a, ra);
   unsigned long long hcur[2 * REC_BUCKETS];
@@ -1683,10 +1721,7 @@ bruno_status count_by_regions(bruno_job* j, const ParseArgs& a, bool* done) {
   /* sort each bucket on its region bits (the low rb bits of a record) on
    * the job's stream and build its regions on the side stream, so the sort
    * of bucket b + 1 overlaps the build of bucket b (two sort buffers) */
-  if (!j->side && hipStreamCreateWithFlags(&j->side, hipStreamNonBlocking) != hipSuccess) {
-    (void)hipGetLastError();
-    j->side = nullptr;
-  }
+  if (!j->side) j->side = shared_side_stream();
   if (!j->ev_region[0]) {
     for (int i = 0; i < 5; ++i)
       if (hipEventCreateWithFlags(&j->ev_region[i], hipEventDisableTiming) != hipSuccess) {
@@ -2575,10 +2610,10 @@ bruno_status bruno_batch_create(const bruno_job_desc* d, uint32_t n_jobs, const 
 bruno_status bruno_job_destroy(bruno_job* j) {
   if (!j) return BRUNO_BAD_INPUT;
   (void)hipStreamSynchronize(j->stream);
-  if (j->side) {
-    (void)hipStreamSynchronize(j->side);
-    (void)hipStreamDestroy(j->side);
-  }
+  /* the side stream is shared: wait for this job's last work on it only
+   * (the count joins it back to the job's stream; the table clear records
+   * ev_cleared) */
+  if (j->clear_pending && j->ev_cleared) (void)hipEventSynchronize(j->ev_cleared);
   if (j->ev_chains) (void)hipEventDestroy(j->ev_chains);
   for (int i = 0; i < 5; ++i)
     if (j->ev_region[i]) (void)hipEventDestroy(j->ev_region[i]);
@@ -3341,7 +3376,8 @@ bruno_status bruno_dev_free(void* p) {
 bruno_status bruno_job_set_recycle(bruno_job* j, int on) {
   if (!valid_job(j)) return BRUNO_BAD_INPUT;
   if (on) {
-    if ((!j->side && hipStreamCreateWithFlags(&j->side, hipStreamNonBlocking) != hipSuccess) ||
+    if (!j->side) j->side = shared_side_stream();
+    if (!j->side ||
         (!j->ev_chains && hipEventCreateWithFlags(&j->ev_chains, hipEventDisableTiming) != hipSuccess) ||
         (!j->ev_cleared && hipEventCreateWithFlags(&j->ev_cleared, hipEventDisableTiming) != hipSuccess)) {
       (void)hipGetLastError();
@@ -3376,7 +3412,7 @@ bruno_status bruno_set_tuning(const bruno_tuning* t) {
   if (!t) return BRUNO_BAD_INPUT;
   const bool pow2 = t->ruler_sample >= 8 && t->ruler_sample <= 4096 && !(t->ruler_sample & (t->ruler_sample - 1));
   if ((t->parse_block != 128 && t->parse_block != 256 && t->parse_block != 512) || t->parse_grid < 64 ||
-      t->scan_grid < 1 || !pow2 || t->count_regions > 1 ||
+      t->scan_grid < 1 || !pow2 || t->count_regions > 2 ||
       (t->region_slots != REGION_MAX_SLOTS && t->region_slots != REGION_SMALL_SLOTS))
     return BRUNO_BAD_INPUT;
   std::lock_guard<std::mutex> g(g_tuning_mu);

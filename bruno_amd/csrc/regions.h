@@ -38,8 +38,9 @@ constexpr int REC_BUCKETS = 1 << REC_BUCKET_BITS;
 constexpr uint32_t REGION_MAX_SLOTS = 2560;
 constexpr uint32_t REGION_SMALL_SLOTS = 1280;
 constexpr uint64_t REGION_BUILD_CU_SLOTS = 512; /* build workgroups resident on the device (x2 for small) */
-/* record parser: workgroup size, reads per wave between appends, staged
- * records per bucket, append threshold */
+/* record parser: workgroup size, most reads per wave between appends (the
+ * count picks fewer for longer reads, so a round's records fit the stage),
+ * staged records per bucket, append threshold */
 constexpr int REC_BS = 512, REC_RPW = 4, REC_STAGE = 192, REC_FLUSH = 32;
 constexpr int REGION_ITEMS = 8;              /* records per thread loaded at once by the build */
 
@@ -63,6 +64,7 @@ struct RecArgs {
   unsigned long long* stat;  /* [0] instances, [1] dropped reads */
   uint32_t chunk;            /* power of two, >= REC_STAGE */
   int rot;                   /* region bits inside a bucket (1..59): the record rotation */
+  uint32_t rpw;              /* reads per wave between appends (1..REC_RPW) */
 };
 
 __device__ __forceinline__ uint64_t rotl64(uint64_t x, int r) { return (x << r) | (x >> (64 - r)); }
@@ -144,8 +146,9 @@ __global__ __launch_bounds__(BS) void k_parse_records(ParseArgs a, RecArgs ra) {
     }
     lds_barrier();
   };
-  for (uint64_t r0 = (uint64_t)blockIdx.x * WPB * RPW; r0 < a.n_reads; r0 += (uint64_t)gridDim.x * WPB * RPW) {
-    for (int jr = 0; jr < RPW; ++jr) {
+  const uint32_t rpw = ra.rpw;
+  for (uint64_t r0 = (uint64_t)blockIdx.x * WPB * rpw; r0 < a.n_reads; r0 += (uint64_t)gridDim.x * WPB * rpw) {
+    for (uint32_t jr = 0; jr < rpw; ++jr) {
     const uint64_t r = r0 + (uint64_t)jr * WPB + w;
     if (r < a.n_reads) {
       const uint64_t off = a.starts ? a.starts[r] : a.offsets[r];

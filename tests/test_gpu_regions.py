@@ -1,6 +1,7 @@
 """Counting by table region (bucketed instance records, per-region builds in
 LDS) against the oracle, and against the atomic count on the same inputs.
-Tables of >= 2^22 slots of plain k <= 31 jobs use the region layout."""
+count_regions 2 gives tables of >= 2^22 slots of plain k <= 31 jobs the
+region layout (the default bar, 2^27 slots, is for speed only)."""
 import numpy as np
 import pytest
 
@@ -23,12 +24,18 @@ CASES = [
 ]
 
 
-@pytest.fixture
-def direct_count():
+@pytest.fixture(autouse=True)
+def small_region_tables():
     old = get_tuning()
-    set_tuning(count_regions=0)
+    set_tuning(count_regions=2)
     yield
     set_tuning(**old)
+
+
+@pytest.fixture
+def direct_count():
+    set_tuning(count_regions=0)
+    yield
 
 
 def _reads(case):
@@ -92,11 +99,13 @@ def test_skewed_input_falls_back_to_atomic_count():
 
 
 def test_edge_reads():
-    """Empty, short, N-containing reads and one 20 kbp read (its records
-    overflow the workgroup's staging and are appended one by one)."""
+    """Empty, short, N-containing reads and one 20 kbp read among 3000 short
+    ones (the long read's records overflow the workgroup's staging and are
+    appended one by one)."""
     long_read = "".join(np.random.default_rng(3).choice(list("ACGT"), 20000))
+    _, short = random_reads(17, 40000, 3000, 120, 0.01)
     reads = ["", "ACGT" * 5, "A" * 33, "ACGTN" * 10, "C" * 31 + "A", long_read, long_read[5000:9000],
-             long_read[12000:15000][::-1].translate(str.maketrans("ACGT", "TGCA"))]
+             long_read[12000:15000][::-1].translate(str.maketrans("ACGT", "TGCA"))] + short
     b, off = pack(reads)
     for th in (None, thresholds_L(1)):
         g, _ = compare_all(31, th, b, off, capacity=REGION_CAP)
@@ -134,12 +143,12 @@ def test_region_vs_atomic_tables_agree(k, direct_count):
     _, reads = random_reads(21, 200000, 6000, 150, 0.02)
     b, off = pack(reads)
     outs = []
-    for regions in (1, 0):
+    for regions in (2, 0):
         set_tuning(count_regions=regions)
         job = bruno_amd.GraphJob(k, thresholds_L(2), capacity=REGION_CAP)
         job.parse_count(b, off)
         st = job.stats()
-        assert st["count_by_regions"] == regions
+        assert st["count_by_regions"] == (regions > 0)
         c = job.counts()
         kk = job.k2()
         job.run()
@@ -148,6 +157,15 @@ def test_region_vs_atomic_tables_agree(k, direct_count):
                      st["n_instances"]))
         job.close()
     assert outs[0] == outs[1]
+
+
+def test_long_reads_count_with_atomics():
+    """Reads too long for one per wave between appends go to the atomic
+    count, with the same results."""
+    _, reads = random_reads(19, 30000, 60, 3000, 0.01)
+    b, off = pack(reads)
+    g, _ = compare_all(31, thresholds_L(2), b, off, capacity=REGION_CAP)
+    assert g["stats"]["count_by_regions"] == 0
 
 
 def test_spilled_entries_in_regions():
