@@ -255,6 +255,7 @@ struct oracle_job {
   int K;
   KmerOps ops;
   bool thresholding;
+  bool edge_filter = false;  /* k+1-mer edge-frequency filter (block-construct driver) */
   uint64_t th[6];
   int nthreads;
   uint64_t order_seed = 0;  /* 0: generation order; else shuffle updates */
@@ -420,11 +421,83 @@ struct oracle_job {
     if (have) out.push_back(n);
   }
 
+  /* ---------------- k+1-mer edge-frequency filter ------------------------
+   * The block-construct driver (compact_debruijn_graph_block_construct.cpp
+   * :415-480): compute_edge_frequency counts canonical k+1-mers over the
+   * reads and keeps those with lower <= count < upper (saturating uint16
+   * counts, :151; debruijn_biedge_filters.hpp:128-176);
+   * transform_biedges_by_frequency (:330-418) clears a record's in / out
+   * edge whose canonical k+1-mer is not kept; filter_isolated_nodes (:60-69)
+   * drops records left without edges; the rest are inserted.  Restated per
+   * k-mer block from the k2 list: the windows of canonical k+1-mer a.K are
+   * the records of K with in-edge a, except that a window of an
+   * rc-palindromic a.K (K not a palindrome) yields two such records, so its
+   * window count is half their sum (the sum is even); an rc-palindromic K
+   * (even k) is recorded from the reverse strand, so its in-edge c also
+   * collects the records with out-edge comp(c).  A kept record adds
+   * its surviving edges to the node (the reference's index stores edge
+   * existence; the counts here are those of the kept records).  Thresholds:
+   * lower = th[2], upper = th[3]. */
+  void filter_block_edges(const K2* b, const K2* e, std::vector<Node>& out) const {
+    const uint64_t cnt_max = 65535;
+    const uint64_t lo = th[2], hi = th[3];
+    u128 k = b->kmer;
+    bool k_pal = ops.is_rc_palindrome(k);
+    bool high = ops.km1_high(k);
+    bool low = ops.km1_low(k);
+    uint64_t win_in[4] = {0, 0, 0, 0}, win_out[4] = {0, 0, 0, 0};
+    bool split_in[4], split_out[4];
+    for (int c = 0; c < 4; ++c) {
+      split_in[c] = ops.k1_pal_in(c, k, high) && !k_pal;
+      split_out[c] = ops.k1_pal_out(k, c, low) && !k_pal;
+    }
+    uint64_t s_in[4] = {0, 0, 0, 0}, s_out[4] = {0, 0, 0, 0};
+    for (const K2* it = b; it != e; ++it) {
+      int ch = it->e >> 4;
+      if (ch > 0) s_in[from_dna16(ch)] += it->cnt;
+      ch = it->e & 0xF;
+      if (ch > 0) s_out[from_dna16(ch)] += it->cnt;
+    }
+    for (int c = 0; c < 4; ++c) {
+      if (k_pal) {
+        /* every instance of an rc-palindromic k-mer is recorded from the
+         * reverse strand (lex_less flips ties): the windows written c.K
+         * show up as out-edge comp(c), those written K.comp(c) as in-edge
+         * c; both are the canonical k+1-mer of in-edge c (even k, so no
+         * k+1-mer is a palindrome) */
+        win_in[c] = s_in[c] + s_out[3 - c];
+        win_out[c] = s_out[c] + s_in[3 - c];
+      } else {
+        win_in[c] = split_in[c] ? s_in[c] >> 1 : s_in[c];
+        win_out[c] = split_out[c] ? s_out[c] >> 1 : s_out[c];
+      }
+      win_in[c] = std::min(cnt_max, win_in[c]);
+      win_out[c] = std::min(cnt_max, win_out[c]);
+    }
+    bool have = false;
+    Node n;
+    n.kmer = k;
+    n.c.fill(0);
+    for (const K2* it = b; it != e; ++it) {
+      int in = it->e >> 4, outn = it->e & 0xF;
+      bool in_ok = in > 0 && win_in[from_dna16(in)] >= lo && win_in[from_dna16(in)] < hi;
+      bool out_ok = outn > 0 && win_out[from_dna16(outn)] >= lo && win_out[from_dna16(outn)] < hi;
+      if (!in_ok && !out_ok) continue;
+      uint8_t edge = it->e;
+      if (!in_ok) edge &= 0x0F;
+      if (!out_ok) edge &= 0xF0;
+      node_update_cnt(n.c, edge, (uint16_t)std::min(cnt_max, (uint64_t)it->cnt));
+      have = true;
+    }
+    if (have) out.push_back(n);
+  }
+
   void filter_range(const std::vector<K2>& v, std::vector<Node>& out) const {
     for (size_t i = 0; i < v.size();) {
       size_t j = i;
       while (j < v.size() && v[j].kmer == v[i].kmer) ++j;
-      filter_block(&v[i], &v[0] + j, out);
+      if (edge_filter) filter_block_edges(&v[i], &v[0] + j, out);
+      else filter_block(&v[i], &v[0] + j, out);
       i = j;
     }
   }
@@ -847,6 +920,7 @@ oracle_job* oracle_create(int k, const uint64_t* thresholds, int nthreads) {
   return new oracle_job(k, thresholds, nthreads);
 }
 void oracle_destroy(oracle_job* j) { delete j; }
+void oracle_set_edge_filter(oracle_job* j, int on) { j->edge_filter = on != 0; }
 
 int oracle_add_reads(oracle_job* j, const char* bases, const uint64_t* offsets, uint64_t n_reads) {
   for (uint64_t r = 0; r < n_reads; ++r) {

@@ -56,6 +56,9 @@ typedef struct {
  * nthreads: 1 = sequential restatement; >1 = hash-partitioned shards that
  * mirror MPI ranks for the parse/count/filter phases (same output). */
 oracle_job* oracle_create(int k, const uint64_t* thresholds, int nthreads);
+/* on: the thresholded filter is the k+1-mer edge-frequency filter of the
+ * block-construct driver (lower = thresholds[2], upper = thresholds[3]) */
+void oracle_set_edge_filter(oracle_job* j, int on);
 void oracle_destroy(oracle_job* j);
 
 /* Feed reads: concatenated bases and n_reads+1 offsets (ASCII). */

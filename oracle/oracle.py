@@ -45,6 +45,7 @@ def lib():
         L.oracle_create.restype = vp
         L.oracle_create.argtypes = [ctypes.c_int, vp, ctypes.c_int]
         L.oracle_destroy.argtypes = [vp]
+        L.oracle_set_edge_filter.argtypes = [vp, ctypes.c_int]
         L.oracle_add_reads.argtypes = [vp, ctypes.c_char_p, vp, u64]
         L.oracle_run.argtypes = [vp, ctypes.c_int]
         for name in ["n_instances", "n_k2", "n_nodes", "n_branches", "n_chain_nodes",
@@ -70,7 +71,7 @@ def lib():
 class Oracle:
     """One oracle job: add reads, run, read the parity objects back."""
 
-    def __init__(self, k, thresholds=None, nthreads=1):
+    def __init__(self, k, thresholds=None, nthreads=1, edge_filter=False):
         L = lib()
         self._th = None
         if thresholds is not None:
@@ -80,6 +81,8 @@ class Oracle:
         if not self.h:
             raise ValueError("bad k")
         self.k = k
+        if edge_filter:
+            L.oracle_set_edge_filter(self.h, 1)
 
     def __del__(self):
         if getattr(self, "h", None):

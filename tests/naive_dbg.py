@@ -140,6 +140,36 @@ def filtered(reads, k, th):
     return g
 
 
+def edge_filtered(reads, k, lo, hi):
+    """The block-construct driver's graph: canonical k+1-mer windows counted
+    over the reads (saturating at 65535), an edge kept when its k+1-mer count
+    is in [lo, hi), a record kept when one of its edges is; nodes are the
+    k-mers of kept records with the kept edges (counts of kept records)."""
+    kp1 = kp1_counts(reads, k)
+
+    def count(e):
+        c = kp1[e] + (kp1[rc(e)] if rc(e) != e else 0)
+        return min(c, 65535)
+
+    g = {}
+    k2 = defaultdict(int)
+    for x, a, b in records(reads, k):
+        k2[(x, a, b)] += 1
+    for (x, a, b), c in k2.items():
+        a_ok = a is not None and lo <= count(a + x) < hi
+        b_ok = b is not None and lo <= count(x + b) < hi
+        if not a_ok and not b_ok:
+            continue
+        node = g.setdefault(x, [0] * 9)
+        cnt = min(c, 65535)
+        if b_ok:
+            node[IDX[b]] = min(node[IDX[b]] + cnt, 65535)
+        if a_ok:
+            node[4 + IDX[a]] = min(node[4 + IDX[a]] + cnt, 65535)
+        node[8] = min(node[8] + cnt, 65535)
+    return g
+
+
 def kplus1_pal(x, node):
     """IsKplus1Palindrome with string tests: an existing edge whose k+1-mer is
     an rc-palindrome, or the 'paired' form where the far k-1 bases are an
