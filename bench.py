@@ -85,7 +85,7 @@ def fastq_bytes_on_device(bases, s):
     return DeviceBuffer.from_numpy(rec.reshape(-1))
 
 
-def cpu_baseline(cfg, seconds_target=15.0, threads=None):
+def cpu_baseline(cfg, seconds_target=15.0, threads=None, edge_filter=False):
     import numpy as np
     import bruno_amd
     from bruno_amd.pipeline import SynthConfig
@@ -98,7 +98,7 @@ def cpu_baseline(cfg, seconds_target=15.0, threads=None):
     sample = SynthConfig(**{**s.__dict__, "n_reads": n})
     b, o = bruno_amd.synth_reads_host(sample)
     t0 = time.time()
-    orc = Oracle(cfg["k"], np.array(cfg["thresholds"], np.uint64), threads)
+    orc = Oracle(cfg["k"], np.array(cfg["thresholds"], np.uint64), threads, edge_filter=edge_filter)
     orc.add_reads(b, o)
     orc.run(1)
     dt = time.time() - t0
@@ -130,6 +130,9 @@ def main():
     ap.add_argument("--tuning", default="", help="apply a tuning saved by bruno_amd.autotune")
     ap.add_argument("--fastq", action="store_true",
                     help="feed the reads as FASTQ bytes in HBM (split on the device inside each step)")
+    ap.add_argument("--edge-filter", action="store_true",
+                    help="filter by canonical k+1-mer counts as the block-construct driver does (-L 3 on "
+                         "k+1-mer windows) instead of local_insert_by_freqencies")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the timed steps, copy 4 GiB device-to-device once (a plain copy kernel of "
                          "known size, to scale FETCH_SIZE / WRITE_SIZE counters)")
@@ -176,7 +179,7 @@ def main():
                      log=lambda *a: print("autotune:", *a, file=sys.stderr, flush=True))
         lf = tuned["load_factor"]
     cap = capacity_for(s, k, lf)
-    job = bruno_amd.GraphJob(k, cfg["thresholds"], capacity=cap)
+    job = bruno_amd.GraphJob(k, cfg["thresholds"], capacity=cap, edge_filter=args.edge_filter)
     # steps repeat one job: its table is cleared beside list ranking and
     # output of each step, so the next step's reset has nothing left to clear
     job.set_recycle(True)
@@ -219,9 +222,11 @@ def main():
         "dtype": "u64",
         "data": "synthetic (seeded generator, reads generated in HBM)",
         "config": {
-            "workload": "%s: %.0f Mbp genome, %d x %d bp reads, sub %.3g, indel %.3g, repeats %d families %.0f%%, k=%d, -T -L 3"
+            "workload": "%s: %.0f Mbp genome, %d x %d bp reads, sub %.3g, indel %.3g, repeats %d families %.0f%%, k=%d, %s"
                         % (args.config, s.genome_len / 1e6, s.n_reads, s.read_len, s.sub_rate, s.indel_rate,
-                           s.n_families, 100 * s.repeat_fraction, k),
+                           s.n_families, 100 * s.repeat_fraction, k,
+                           "-T -L 3 on canonical k+1-mer counts (edge-frequency filter)" if args.edge_filter
+                           else "-T -L 3"),
             "k": k, "n_reads": s.n_reads, "read_len": s.read_len, "genome_bp": s.genome_len,
             "k_mer_instances": n_inst, "list_rank": "min_update" if args.variant else "list_rank",
             "load_factor": lf,
@@ -273,7 +278,7 @@ def main():
         src.free()
         dst.free()
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_seconds, edge_filter=args.edge_filter)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))

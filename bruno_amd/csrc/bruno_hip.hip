@@ -258,7 +258,9 @@ __global__ __launch_bounds__(BLOCK) void k_filter_cand(Slot<KT>* T, const uint64
       sub_entries<KT, Slot<KT>::NSUB>(p, L);
       KT k = SlotOps<KT>::kmer(p);
       uint16_t c[9];
-      if (thresholding) {
+      if (thresholding == 2) {
+        kept = edge_counts<KT, Slot<KT>::NSUB>(ops, k, L, th, c);
+      } else if (thresholding) {
         kept = p2_counts<KT, Slot<KT>::NSUB>(ops, k, L, th, c);
       } else {
         p1_counts(L, c);
@@ -296,7 +298,9 @@ __global__ __launch_bounds__(BLOCK) void k_filter_spilled(Slot<KT>* T, const Spi
       gather_k2<KT>(p, s, sp, spcap, L);
       KT k = SlotOps<KT>::kmer(p);
       uint16_t c[9];
-      if (thresholding) {
+      if (thresholding == 2) {
+        kept = edge_counts<KT, 25>(ops, k, L, th, c);
+      } else if (thresholding) {
         kept = p2_counts<KT, 25>(ops, k, L, th, c);
       } else {
         p1_counts(L, c);
@@ -1544,6 +1548,16 @@ bruno_status alloc_table(bruno_job* j, const uint64_t* caps) {
 /* probe span of a plain job: one region, or the whole table */
 uint64_t probe_span(const bruno_job* j) { return j->rbits ? j->rspan : j->cap; }
 
+/* The filter pre-pass keeps a k-mer when its count can reach the lower
+ * bound.  An edge count of the k2 filter is at most the k-mer count, and so
+ * is a canonical k+1-mer window count of a k-mer that is not its own
+ * reverse complement; for one that is (even k only) the window count is at
+ * most twice it (the window c.K shows up once as in-edge c and once as
+ * out-edge comp(c), both in one read "c K comp(c)"). */
+uint64_t scan_lo(const bruno_job* j) {
+  return (j->thresholding == 2 && !(j->K & 1)) ? (j->th.t1lo + 1) / 2 : j->th.t1lo;
+}
+
 /* every slot EMPTY unless a count has written the table since the reset */
 void ensure_table_valid(bruno_job* j) {
   if (j->table && !j->table_valid) {
@@ -1780,7 +1794,7 @@ bruno_status count_by_regions(bruno_job* j, const ParseArgs& a, bool* done) {
       HIP_CHECK(hipStreamWaitEvent(bst, ev_sorted[sl], 0));
     }
     RegionArgs g{(Slot<uint64_t>*)j->table, j->rspan, j->rbits, (uint32_t)b, nreg, sorted, n, valid,
-                 j->spill, j->spcap, j->ctr, cbits, sbits, stride, j->thresholding, j->th.t1lo};
+                 j->spill, j->spcap, j->ctr, cbits, sbits, stride, j->thresholding, scan_lo(j)};
     if (j->rspan <= REGION_SMALL_SLOTS)
       hipLaunchKernelGGL((k_region_build<REGION_SMALL_SLOTS, 256>),
                          dim3((unsigned)std::min<uint64_t>(nreg, 4 * REGION_BUILD_CU_SLOTS)), dim3(256), 0, bst, g);
@@ -1882,7 +1896,7 @@ bruno_status filter_t(bruno_job* j) {
   } else {
     hipLaunchKernelGGL((k_filter_scan<KT>),
                        dim3(grid_for((j->cap + FILTER_SPT - 1) / FILTER_SPT, BLOCK, tuning().scan_grid)),
-                       dim3(BLOCK), 0, j->stream, (const Slot<KT>*)j->table, j->cap, j->thresholding, j->th.t1lo,
+                       dim3(BLOCK), 0, j->stream, (const Slot<KT>*)j->table, j->cap, j->thresholding, scan_lo(j),
                        cand, n_distinct, spilled, n_spill_cap, j->ctr);
   }
   TRY(sync_counters(j));
@@ -2543,7 +2557,7 @@ static bruno_status create_common(const bruno_job_desc* d, uint32_t n_jobs, cons
                                   bool plain = false) {
   if (!d || !out) return BRUNO_BAD_INPUT;
   *out = nullptr;
-  if (d->k < 3 || d->k > 63 || n_jobs == 0) return BRUNO_BAD_INPUT;
+  if (d->k < 3 || d->k > 63 || n_jobs == 0 || d->thresholding < 0 || d->thresholding > 2) return BRUNO_BAD_INPUT;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
     (void)hipGetLastError();
@@ -2553,7 +2567,7 @@ static bruno_status create_common(const bruno_job_desc* d, uint32_t n_jobs, cons
   if (!j) return BRUNO_OOM;
   j->K = d->k;
   j->words = d->k <= 31 ? 1 : 2;
-  j->thresholding = d->thresholding ? 1 : 0;
+  j->thresholding = d->thresholding;
   j->th = Thresh{d->thresholds[0], d->thresholds[1], d->thresholds[2],
                  d->thresholds[3], d->thresholds[4], d->thresholds[5]};
   j->load_factor = (d->load_factor > 0 && d->load_factor < 1) ? d->load_factor : 0.5;

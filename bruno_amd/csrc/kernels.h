@@ -540,6 +540,69 @@ __device__ bool p2_counts(const kops<KT>& ops, KT k, const K2L<N>& L, const Thre
   return have;
 }
 
+/* The k+1-mer edge-frequency filter of the block-construct driver
+ * (compact_debruijn_graph_block_construct.cpp:415-480): an edge is kept when
+ * the count of its canonical k+1-mer over the reads is in [t1lo, t1hi)
+ * (saturating at 65535, debruijn_biedge_filters.hpp:128-176), a k2 entry
+ * when one of its edges is (filter_isolated_nodes, :60-69), with the edges
+ * that failed cleared (transform_biedges_by_frequency, :330-418).  The
+ * window counts come from the k-mer's own k2 entries: the windows of a.K
+ * are its entries with in-edge a, except that a window of an rc-palindromic
+ * a.K (odd k, K not a palindrome) gives two entries (one per strand), so
+ * the sum is halved (it is even); an rc-palindromic K (even k) is recorded
+ * from the reverse strand, so the windows c.K also show up as out-edge
+ * comp(c).  Returns true if any entry is kept. */
+template <typename KT, int N>
+__device__ bool edge_counts(const kops<KT>& ops, KT k, const K2L<N>& L, const Thresh& t, uint16_t (&c)[9]) {
+  const uint64_t cnt_max = 65535;
+  const bool kpal = ops.is_rc_palindrome(k);
+  const bool high = ops.km1_high(k), low = ops.km1_low(k);
+  const int lastb = ops.last(k), firstb = ops.first(k);
+  uint64_t sin[4] = {0, 0, 0, 0}, sout[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    if (j >= L.n) break;
+    const uint64_t cu = L.c[j];
+    int ch = L.e[j] >> 4;
+    if (ch) add_sel4(sin, nib_idx(ch), cu);
+    ch = L.e[j] & 0xF;
+    if (ch) add_sel4(sout, nib_idx(ch), cu);
+  }
+  uint64_t win_in[4], win_out[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if (kpal) {
+      win_in[q] = sin[q] + sout[3 - q];
+      win_out[q] = sout[q] + sin[3 - q];
+    } else {
+      const bool split_in = (ops.K & 1) && lastb == 3 - q && high;
+      const bool split_out = (ops.K & 1) && firstb == 3 - q && low;
+      win_in[q] = split_in ? sin[q] >> 1 : sin[q];
+      win_out[q] = split_out ? sout[q] >> 1 : sout[q];
+    }
+    win_in[q] = win_in[q] < cnt_max ? win_in[q] : cnt_max;
+    win_out[q] = win_out[q] < cnt_max ? win_out[q] : cnt_max;
+  }
+#pragma unroll
+  for (int q = 0; q < 9; ++q) c[q] = 0;
+  bool have = false;
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    if (j >= L.n) break;
+    const int in = L.e[j] >> 4, out = L.e[j] & 0xF;
+    const uint64_t wi = get_sel4(win_in, nib_idx(in), 0), wo = get_sel4(win_out, nib_idx(out), 0);
+    const bool in_ok = in && wi >= t.t1lo && wi < t.t1hi;
+    const bool out_ok = out && wo >= t.t1lo && wo < t.t1hi;
+    if (!in_ok && !out_ok) continue;
+    uint8_t e = L.e[j];
+    if (!in_ok) e &= 0x0F;
+    if (!out_ok) e &= 0xF0;
+    node_update(c, e, (uint32_t)(L.c[j] < cnt_max ? L.c[j] : cnt_max));
+    have = true;
+  }
+  return have;
+}
+
 /* IsKplus1Palindrome (debruijn_graph_filters.hpp:57-95) with
  * is_k1_k1pair_rc_palindrome (kmer_traits.hpp:110-127). */
 template <typename KT>

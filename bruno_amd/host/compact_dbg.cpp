@@ -2,7 +2,12 @@
  * compact_dbg.cpp -- host driver with the reference driver's call order and
  * flags (test/test/compact_dbg_freq_construct_clean.cpp:431-840, :848-1030):
  *
- *   compact_dbg [-T] [-L t | -L k k1 k2] [-U ...] [-R] [-B] [-K k] -O prefix reads.fastq...
+ *   compact_dbg [-T] [-E] [-L t | -L k k1 k2] [-U ...] [-R] [-B] [-K k] -O prefix reads.fastq...
+ *
+ * -E filters as the block-construct driver does
+ * (compact_debruijn_graph_block_construct.cpp:415-480, :686-712): with -T
+ * -L lower -U upper an edge is kept when its canonical k+1-mer occurs
+ * lower <= count < upper times over the reads.
  *
  * Writes <prefix>.branch.fasta and <prefix>.chain.fasta, and unless -B also
  * <prefix>.chain.components (print_chain_nodes, compact_dbg_io.hpp:463-498).
@@ -43,7 +48,7 @@ static std::string canon_string(const std::string& s) {
 
 int main(int argc, char** argv) {
   std::vector<size_t> threshes(6, 65536);
-  bool thresholding = false, benchmark = false, lr_opt = false;
+  bool thresholding = false, benchmark = false, lr_opt = false, edge_filter = false;
   int k = 31;
   std::string prefix = "./output";
   std::vector<std::string> files;
@@ -55,13 +60,14 @@ int main(int argc, char** argv) {
     };
     if (a == "-T") thresholding = true;
     else if (a == "-B") benchmark = true;
+    else if (a == "-E") edge_filter = true;
     else if (a == "-R") lr_opt = true;
     else if (a == "-L") vals(lower);
     else if (a == "-U") vals(upper);
     else if (a == "-K" && i + 1 < argc) k = atoi(argv[++i]);
     else if (a == "-O" && i + 1 < argc) prefix = argv[++i];
     else if (a == "-h" || a == "--help") {
-      std::cout << "usage: compact_dbg [-T] [-L t|-L k k1 k2] [-U ...] [-R] [-B] [-K k] -O prefix reads.fastq...\n";
+      std::cout << "usage: compact_dbg [-T] [-E] [-L t|-L k k1 k2] [-U ...] [-R] [-B] [-K k] -O prefix reads.fastq...\n";
       return 0;
     } else files.push_back(a);
   }
@@ -86,7 +92,7 @@ int main(int argc, char** argv) {
     return 1;
   }
   try {
-    job j(k, thresholding ? &threshes : nullptr);
+    job j(k, thresholding ? &threshes : nullptr, 0, nullptr, edge_filter);
     debruijn_graph idx(j);
     debruijn_chain_graph chainmap(j);
     /* build_index_thresholded / build_index: one batch per file (:313-330) */

@@ -61,10 +61,15 @@ struct kmer_value {
 /* Shared device job (one per graph + chain graph pair). */
 class job {
  public:
-  job(int k, const std::vector<size_t>* threshes, uint64_t capacity = 0, void* stream = nullptr) : k_(k) {
+  /* edge_filter: the block-construct driver's filter (canonical k+1-mer
+   * counts in [threshes[2], threshes[3]), compact_debruijn_graph_block_construct.cpp
+   * :415-480) in place of local_insert_by_freqencies */
+  job(int k, const std::vector<size_t>* threshes, uint64_t capacity = 0, void* stream = nullptr,
+      bool edge_filter = false)
+      : k_(k) {
     bruno_job_desc d{};
     d.k = k;
-    d.thresholding = threshes ? 1 : 0;
+    d.thresholding = threshes ? (edge_filter ? 2 : 1) : 0;
     for (int i = 0; i < 6; ++i) d.thresholds[i] = threshes ? (*threshes)[i] : 65536;
     d.capacity = capacity;
     d.load_factor = 0.5;

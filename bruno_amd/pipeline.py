@@ -82,14 +82,26 @@ class _Views:
         return km.reshape(-1, self.words), e, c
 
 
+def job_desc(k, thresholds=None, edge_filter=False):
+    """bruno_job_desc for k and the -T thresholds (None: unthresholded).
+    edge_filter selects the block-construct driver's filter: an edge is kept
+    when its canonical k+1-mer occurs lower <= count < upper times over the
+    reads, lower/upper = thresholds[2]/[3]
+    (compact_debruijn_graph_block_construct.cpp:415-480)."""
+    if edge_filter and thresholds is None:
+        raise ValueError("the edge-frequency filter needs thresholds")
+    d = _lib.JobDesc()
+    d.k = int(k)
+    d.thresholding = 0 if thresholds is None else (2 if edge_filter else 1)
+    th = thresholds if thresholds is not None else [DEFAULT_THRESH] * 6
+    for i in range(6):
+        d.thresholds[i] = int(th[i])
+    return d
+
+
 class GraphJob(_Views):
-    def __init__(self, k, thresholds=None, capacity=0, load_factor=0.0, stream=None):
-        d = _lib.JobDesc()
-        d.k = int(k)
-        d.thresholding = 0 if thresholds is None else 1
-        th = thresholds if thresholds is not None else [DEFAULT_THRESH] * 6
-        for i in range(6):
-            d.thresholds[i] = int(th[i])
+    def __init__(self, k, thresholds=None, capacity=0, load_factor=0.0, stream=None, edge_filter=False):
+        d = job_desc(k, thresholds, edge_filter)
         d.capacity = int(capacity)
         d.load_factor = float(load_factor)
         d.stream = stream
@@ -200,13 +212,8 @@ class GraphBatch(GraphJob):
     launches; job i owns capacities[i] hash slots.  Results per job via
     job(i), bit-identical to running each job alone."""
 
-    def __init__(self, k, thresholds, capacities, stream=None):
-        d = _lib.JobDesc()
-        d.k = int(k)
-        d.thresholding = 0 if thresholds is None else 1
-        th = thresholds if thresholds is not None else [DEFAULT_THRESH] * 6
-        for i in range(6):
-            d.thresholds[i] = int(th[i])
+    def __init__(self, k, thresholds, capacities, stream=None, edge_filter=False):
+        d = job_desc(k, thresholds, edge_filter)
         d.stream = stream
         caps = np.ascontiguousarray(capacities, np.uint64)
         h = ctypes.c_void_p()

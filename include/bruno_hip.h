@@ -50,7 +50,12 @@ typedef struct bruno_job bruno_job;
 typedef struct {
   int k;                    /* 1..63 (k <= 31 uses one 64-bit word)                 */
   int thresholding;         /* 0: build_index (compact_dbg_build.hpp:35-62)
-                               1: build_index_thresholded (..._clean.cpp:297-348)  */
+                               1: build_index_thresholded (..._clean.cpp:297-348)
+                               2: the block-construct driver's build_index_thresholded
+                                  (compact_debruijn_graph_block_construct.cpp:415-480):
+                                  an edge is kept when its canonical k+1-mer occurs
+                                  thresholds[2] <= count < thresholds[3] times;
+                                  other values: BRUNO_BAD_INPUT                    */
   uint64_t thresholds[6];   /* {k_lo,k_hi,k1_lo,k1_hi,k2_lo,k2_hi}, reference order
                                (debruijn_graph_map.hpp:1261); -L t = {0,65536,t,65536,0,65536} */
   uint64_t capacity;        /* hash slots (distinct k-mers / load factor);

@@ -11,8 +11,8 @@ def kv(words, k):
     return int(words[0]) | (int(words[1]) << 64 if len(words) > 1 else 0)
 
 
-def run_gpu(k, th, bases, offs, variant=1, capacity=0, stages=True):
-    job = bruno_amd.GraphJob(k, th, capacity=capacity)
+def run_gpu(k, th, bases, offs, variant=1, capacity=0, stages=True, edge_filter=False):
+    job = bruno_amd.GraphJob(k, th, capacity=capacity, edge_filter=edge_filter)
     job.parse_count(bases, offs)
     res = {}
     if stages:
@@ -32,8 +32,8 @@ def run_gpu(k, th, bases, offs, variant=1, capacity=0, stages=True):
     return res
 
 
-def run_oracle(k, th, bases, offs, variant=1, nthreads=1):
-    o = Oracle(k, None if th is None else np.array(th, np.uint64), nthreads)
+def run_oracle(k, th, bases, offs, variant=1, nthreads=1, edge_filter=False):
+    o = Oracle(k, None if th is None else np.array(th, np.uint64), nthreads, edge_filter=edge_filter)
     o.add_reads(bases, offs)
     o.run(variant)
     return o
@@ -48,9 +48,9 @@ def oracle_node_table(arr):
     return {(int(r["kmer_lo"]) | (int(r["kmer_hi"]) << 64)): tuple(int(x) for x in r["counts"]) for r in arr}
 
 
-def compare_all(k, th, bases, offs, variant=1, nthreads=1, stages=True, capacity=0):
-    g = run_gpu(k, th, bases, offs, variant, capacity=capacity, stages=stages)
-    o = run_oracle(k, th, bases, offs, variant, nthreads)
+def compare_all(k, th, bases, offs, variant=1, nthreads=1, stages=True, capacity=0, edge_filter=False):
+    g = run_gpu(k, th, bases, offs, variant, capacity=capacity, stages=stages, edge_filter=edge_filter)
+    o = run_oracle(k, th, bases, offs, variant, nthreads, edge_filter=edge_filter)
     compare_results(k, g, o, variant, stages)
     return g, o
 

@@ -30,6 +30,24 @@ def test_status_strings_and_bad_input():
     assert L.bruno_job_create(None, None) == 1
 
 
+def test_filter_modes_on_the_host():
+    """thresholding 0 / 1 / 2 (plain, k2 filter, k+1-mer edge filter); other
+    values are rejected before any device call."""
+    import ctypes
+    import pytest
+    from bruno_amd.pipeline import job_desc, thresholds_L
+    assert job_desc(31).thresholding == 0
+    assert job_desc(31, thresholds_L(2)).thresholding == 1
+    d = job_desc(31, thresholds_L(2), edge_filter=True)
+    assert d.thresholding == 2 and d.thresholds[2] == 2 and d.thresholds[3] == 65536
+    with pytest.raises(ValueError):
+        job_desc(31, None, edge_filter=True)
+    h = ctypes.c_void_p()
+    for bad in (-1, 3):
+        d.thresholding = bad
+        assert bruno_amd.lib().bruno_job_create(ctypes.byref(d), ctypes.byref(h)) == 1
+
+
 def test_host_generator_is_deterministic():
     from bruno_amd.configs import config2
     c = config2(seed=4, scale=0.0005)["synth"]

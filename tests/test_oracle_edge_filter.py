@@ -93,3 +93,15 @@ def test_palindromic_kmer_windows(lo):
     exp = nd.edge_filtered(reads, k, lo, DEFAULT_T)
     assert oracle_graph(o, k, o.nodes()) == exp
     assert o.chain_fasta().decode() == nd.chain_fasta(exp)
+
+
+@pytest.mark.parametrize("lo", [1, 2, 3])
+def test_window_count_twice_the_kmer_count(lo):
+    """One read 'A GAATTC T' (k=6): the palindromic k-mer occurs once, the
+    canonical k+1-mer of its two windows (AGAATTC, GAATTCT) twice."""
+    reads = ["AGAATTCT"]
+    assert nd.kp1_counts(reads, 6)["AGAATTC"] + nd.kp1_counts(reads, 6)["GAATTCT"] == 2
+    o = run(reads, 6, edge_thresholds(lo))
+    exp = nd.edge_filtered(reads, 6, lo, DEFAULT_T)
+    assert oracle_graph(o, 6, o.nodes()) == exp
+    assert ("GAATTC" in exp) == (lo <= 2)
