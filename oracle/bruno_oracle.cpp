@@ -15,6 +15,7 @@
 #include "bruno_oracle.h"
 
 #include <algorithm>
+#include <map>
 #include <array>
 #include <chrono>
 #include <cstring>
@@ -270,6 +271,8 @@ struct oracle_job {
   uint64_t n_cycle = 0, n_isolated = 0, iterations = 0;
   std::vector<oracle_chain_node> ranked;  // P4 (sorted by chain_rep_less)
   std::string branch_fasta, chain_fasta, chain_components;
+  std::string branch_edges, chain_edges;  /* non-benchmark outputs */
+  uint64_t chain_edge_misses = 0;         /* chain ends without a frequency summary */
   double phase[6] = {0, 0, 0, 0, 0, 0};
 
   oracle_job(int k, const uint64_t* t, int nt) : K(k), ops(k), thresholding(t != nullptr), nthreads(nt < 1 ? 1 : nt) {
@@ -894,6 +897,118 @@ struct oracle_job {
     }
   }
 
+  u128 canon(u128 v) const {
+    u128 r = ops.rc(v);
+    return v < r ? v : r;
+  }
+  const Node* node_of(u128 k) const {
+    auto it = std::lower_bound(nodes.begin(), nodes.end(), k, [](const Node& n, u128 v) { return n.kmer < v; });
+    return (it != nodes.end() && it->kmer == k) ? &*it : nullptr;
+  }
+
+  /* The non-benchmark outputs of do_work
+   * (compact_dbg_freq_construct_clean.cpp:528-543, :759-829).
+   *
+   * .branch.edges: print_branch_edge_frequencies (compact_dbg_stats.hpp
+   * :219-262) prints the branch points sorted by k-mer with
+   * print_graph_node (debruijn_graph_operations.hpp:58-78): k-mer, in-edge
+   * counts A C G T, out-edge counts A C G T, self count, tab separated.
+   *
+   * .chain.edges: compute_freq_map (compact_dbg_stats.hpp:783-816) reduces
+   * the self counts of every listranked chain node by chain rep (the map
+   * canonicalises its keys, FreqMapParams = CanonicalHashMapParams,
+   * compact_dbg_freq_construct_clean.cpp:240-253) into (nodes, sum, min,
+   * max) with freq_summary (debruijn_chain_operations.hpp:447-459);
+   * print_chain_frequencies (compact_dbg_stats.hpp:879-1065) takes the chain
+   * map entries that pass IsCanonicalTerminusOrIsolated
+   * (debruijn_chain_filters.hpp:125-147), orients L and R on one strand,
+   * reads L's in-edge and R's out-edge counts (complemented and reversed
+   * on the other strand), the mean as float(sum) / float(nodes) converted to
+   * the 16-bit count type, sorts by L and prints L, R, 4 + 4 counts, mean,
+   * min, max, each followed by a tab.  The counts come from the filtered
+   * graph: the nodes erased before the print (cycles, edgeless nodes,
+   * chain interiors, branch points; :606-610, :806-810) only lose edges
+   * that point into a chain's interior or into a cycle, never the L in /
+   * R out edges printed. */
+  void make_edge_outputs() {
+    branch_edges.clear();
+    for (u128 b : branches) {
+      const Node* n = node_of(b);
+      std::string line = ops.to_string(b);
+      for (int q = 4; q < 8; ++q) line += "\t" + std::to_string(n->c[q]);
+      for (int q = 0; q < 4; ++q) line += "\t" + std::to_string(n->c[q]);
+      line += "\t" + std::to_string(n->c[8]) + "\n";
+      branch_edges += line;
+    }
+    struct Freq {
+      uint64_t n = 0, sum = 0;
+      uint16_t mn = 65535, mx = 0;
+    };
+    std::map<u128, Freq> freq;
+    for (auto& x : ranked) {
+      const uint16_t self = node_of(canon(lo_hi(x.k_lo, x.k_hi)))->c[8];
+      Freq& f = freq[canon(lo_hi(x.rep_lo, x.rep_hi))];
+      f.n += 1;
+      f.sum += self;
+      f.mn = std::min(f.mn, self);
+      f.mx = std::max(f.mx, self);
+    }
+    struct Line {
+      u128 L, R;
+      uint16_t v[11];
+    };
+    std::vector<Line> lines;
+    chain_edge_misses = 0;
+    for (size_t i = 0; i < chain.keys.size(); ++i) {
+      if (!chain.alive[i]) continue;
+      const u128 key = chain.keys[i];
+      const Biedge& b = chain.vals[i];
+      const bool lt = is_chain_terminal(b.din), rt = is_chain_terminal(b.dout);
+      bool sel;
+      if (lt && rt) sel = true;
+      else if (lt) sel = key <= ops.rc(b.out);
+      else if (rt) sel = ops.rc(key) < b.in;
+      else sel = false;
+      if (!sel) continue;
+      Line l;
+      if (lt && rt) {
+        l.L = key;
+        l.R = key;
+      } else if (lt) {
+        l.L = key;
+        l.R = b.out;
+      } else {
+        l.L = ops.rc(key);
+        l.R = ops.rc(b.in);
+      }
+      const u128 cL = canon(l.L), cR = canon(l.R);
+      const Node* nl = node_of(cL);
+      const Node* nr = node_of(cR);
+      for (int c = 0; c < 4; ++c) {
+        l.v[c] = cL == l.L ? nl->c[4 + c] : nl->c[3 - c];
+        l.v[4 + c] = cR == l.R ? nr->c[c] : nr->c[4 + 3 - c];
+      }
+      auto f = freq.find(cL);
+      if (f == freq.end()) {
+        ++chain_edge_misses;
+        l.v[8] = l.v[9] = l.v[10] = 0;
+      } else {
+        const float mean = (float)f->second.sum / (float)f->second.n;
+        l.v[8] = (uint16_t)(int32_t)mean;
+        l.v[9] = f->second.mn;
+        l.v[10] = f->second.mx;
+      }
+      lines.push_back(l);
+    }
+    std::sort(lines.begin(), lines.end(), [](const Line& a, const Line& b) { return a.L < b.L; });
+    chain_edges.clear();
+    for (auto& l : lines) {
+      chain_edges += ops.to_string(l.L) + "\t" + ops.to_string(l.R) + "\t";
+      for (int q = 0; q < 11; ++q) chain_edges += std::to_string(l.v[q]) + "\t";
+      chain_edges += "\n";
+    }
+  }
+
   int run(int variant) {
     double t0 = now_s();
     parse_all();
@@ -907,6 +1022,7 @@ struct oracle_job {
     phase[4] = now_s() - t0;
     t0 = now_s();
     make_output();
+    make_edge_outputs();
     phase[5] = now_s() - t0;
     return 0;
   }
@@ -992,6 +1108,9 @@ static uint64_t copy_str(const std::string& s, char* out, uint64_t cap) {
 }
 uint64_t oracle_get_branch_fasta(oracle_job* j, char* out, uint64_t cap) { return copy_str(j->branch_fasta, out, cap); }
 uint64_t oracle_get_chain_fasta(oracle_job* j, char* out, uint64_t cap) { return copy_str(j->chain_fasta, out, cap); }
+uint64_t oracle_get_branch_edges(oracle_job* j, char* out, uint64_t cap) { return copy_str(j->branch_edges, out, cap); }
+uint64_t oracle_get_chain_edges(oracle_job* j, char* out, uint64_t cap) { return copy_str(j->chain_edges, out, cap); }
+uint64_t oracle_chain_edge_misses(oracle_job* j) { return j->chain_edge_misses; }
 uint64_t oracle_get_chain_components(oracle_job* j, char* out, uint64_t cap) {
   return copy_str(j->chain_components, out, cap);
 }

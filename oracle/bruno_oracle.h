@@ -92,6 +92,12 @@ uint64_t oracle_get_nodes(oracle_job* j, oracle_node* out, uint64_t cap);
 uint64_t oracle_get_branch_fasta(oracle_job* j, char* out, uint64_t cap);
 uint64_t oracle_get_chain_fasta(oracle_job* j, char* out, uint64_t cap);
 uint64_t oracle_get_chain_components(oracle_job* j, char* out, uint64_t cap);
+/* .branch.edges and .chain.edges text (print_branch_edge_frequencies,
+ * print_chain_frequencies); chain ends whose chain has no frequency summary
+ * (none expected) */
+uint64_t oracle_get_branch_edges(oracle_job* j, char* out, uint64_t cap);
+uint64_t oracle_get_chain_edges(oracle_job* j, char* out, uint64_t cap);
+uint64_t oracle_chain_edge_misses(oracle_job* j);
 uint64_t oracle_get_chain_nodes(oracle_job* j, oracle_chain_node* out, uint64_t cap);
 
 /* Wall-clock seconds of the last oracle_run's phases:

@@ -49,7 +49,7 @@ def lib():
         L.oracle_add_reads.argtypes = [vp, ctypes.c_char_p, vp, u64]
         L.oracle_run.argtypes = [vp, ctypes.c_int]
         for name in ["n_instances", "n_k2", "n_nodes", "n_branches", "n_chain_nodes",
-                     "n_cycle_nodes", "n_isolated", "iterations"]:
+                     "n_cycle_nodes", "n_isolated", "iterations", "chain_edge_misses"]:
             f = getattr(L, "oracle_" + name)
             f.restype = u64
             f.argtypes = [vp]
@@ -58,7 +58,8 @@ def lib():
         L.oracle_get_k2.restype = u64
         L.oracle_get_k2.argtypes = [vp, vp, vp, vp, vp, u64]
         for name in ["get_unfiltered", "get_nodes", "get_chain_nodes",
-                     "get_branch_fasta", "get_chain_fasta", "get_chain_components"]:
+                     "get_branch_fasta", "get_chain_fasta", "get_chain_components",
+                     "get_branch_edges", "get_chain_edges"]:
             f = getattr(L, "oracle_" + name)
             f.restype = u64
             f.argtypes = [vp, vp, u64]
@@ -157,6 +158,12 @@ class Oracle:
 
     def chain_components(self):
         return self._str("oracle_get_chain_components")
+
+    def branch_edges(self):
+        return self._str("oracle_get_branch_edges")
+
+    def chain_edges(self):
+        return self._str("oracle_get_chain_edges")
 
     def phase_seconds(self):
         out = np.zeros(6, np.float64)

@@ -287,3 +287,39 @@ def chain_fasta(g):
 def branch_fasta(g):
     br, _ = classify(g)
     return "".join(">%s\n%s\n" % (b, b) for b in sorted(br))
+
+
+def branch_edges(g):
+    """.branch.edges from the string graph: branch k-mers sorted, each with
+    in A C G T, out A C G T and self counts, tab separated."""
+    br, _ = classify(g)
+    return "".join("\t".join([b] + [str(g[b][i]) for i in (4, 5, 6, 7, 0, 1, 2, 3, 8)]) + "\n" for b in sorted(br))
+
+
+def chain_edges(g, k):
+    """.chain.edges from the walked unitigs: per unitig (in the orientation
+    that starts with its rep) the first k-mer L, the last k-mer R, the
+    in-edge counts of L and the out-edge counts of R on that strand, and the
+    float32 mean (truncated), min and max of the self counts of its k-mers;
+    sorted by L, every field followed by a tab."""
+    import numpy as np
+    us, _ = unitigs(g)
+
+    def in_counts(u):
+        x = canon(u)
+        n = g[x]
+        return [n[4 + IDX[c]] if u == x else n[IDX[COMP[c]]] for c in "ACGT"]
+
+    def out_counts(u):
+        x = canon(u)
+        n = g[x]
+        return [n[IDX[c]] if u == x else n[4 + IDX[COMP[c]]] for c in "ACGT"]
+
+    lines = []
+    for _, seq in us:
+        L, R = seq[:k], seq[-k:]
+        selfs = [g[canon(seq[i:i + k])][8] for i in range(len(seq) - k + 1)]
+        mean = int(np.float32(sum(selfs)) / np.float32(len(selfs)))
+        vals = in_counts(L) + out_counts(R) + [mean, min(selfs), max(selfs)]
+        lines.append((L, L + "\t" + R + "\t" + "".join(str(v) + "\t" for v in vals) + "\n"))
+    return "".join(t for _, t in sorted(lines))
