@@ -108,6 +108,28 @@ def cpu_baseline(cfg, seconds_target=15.0, threads=None, edge_filter=False):
                       "all phases, %.1f s" % (n, s.n_reads, inst, dt)}
 
 
+def time_edge_outputs(job, bases, offs, s, fq):
+    """One untimed step that keeps its table, then the two getters of the
+    non-benchmark outputs (device work + copy to the host), wall clock."""
+    from bruno_amd.device import synchronize
+    job.set_recycle(False)
+    job.reset()
+    if fq is not None:
+        job.parse_count_fastq_device(fq.ptr, fq.nbytes)
+    else:
+        job.parse_count_device(bases.ptr, offs.ptr, s.n_reads)
+    job.run(1)
+    synchronize()
+    t0 = time.perf_counter()
+    bn = job.branch_nodes()
+    t1 = time.perf_counter()
+    ce = job.chain_edges()
+    t2 = time.perf_counter()
+    return {"branch_nodes_ms": (t1 - t0) * 1e3, "n_branch_nodes": int(len(bn)),
+            "chain_edges_ms": (t2 - t1) * 1e3, "n_unitigs": int(len(ce)),
+            "chain_nodes_summarised": int(ce["n_nodes"].sum()) if len(ce) else 0}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -133,6 +155,9 @@ def main():
     ap.add_argument("--edge-filter", action="store_true",
                     help="filter by canonical k+1-mer counts as the block-construct driver does (-L 3 on "
                          "k+1-mer windows) instead of local_insert_by_freqencies")
+    ap.add_argument("--edge-outputs", action="store_true",
+                    help="after the timed steps, run one more step without table recycling and time the "
+                         ".branch.edges / .chain.edges getters (non-benchmark outputs)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the timed steps, copy 4 GiB device-to-device once (a plain copy kernel of "
                          "known size, to scale FETCH_SIZE / WRITE_SIZE counters)")
@@ -261,6 +286,8 @@ def main():
             "bytes_per_unit": "16 B per distinct k2-mer read (24 B for k > 31) + 32 B per kept k-mer written (48 B)",
         },
     }
+    if args.edge_outputs:
+        out["edge_outputs"] = time_edge_outputs(job, bases, offs, s, fq)
     if tuned is not None:
         from bruno_amd.autotune import get_tuning
         out["tuning"] = {"load_factor": lf, "knobs": get_tuning(),

@@ -82,7 +82,9 @@ class ServerStats(ctypes.Structure):
 
 NODE_DTYPE = np.dtype([("kmer", "<u8", (2,)), ("counts", "<u2", (9,)), ("pad", "<u2", (3,))])
 CHAIN_DTYPE = np.dtype([("k", "<u8", (2,)), ("rep", "<u8", (2,)), ("dist", "<i4"), ("pad", "<i4")])
-assert NODE_DTYPE.itemsize == 40 and CHAIN_DTYPE.itemsize == 40
+CHAIN_EDGE_DTYPE = np.dtype([("left", "<u8", (2,)), ("right", "<u8", (2,)), ("sum", "<u8"), ("n_nodes", "<u4"),
+                             ("in", "<u2", (4,)), ("out", "<u2", (4,)), ("min", "<u2"), ("max", "<u2")])
+assert NODE_DTYPE.itemsize == 40 and CHAIN_DTYPE.itemsize == 40 and CHAIN_EDGE_DTYPE.itemsize == 64
 
 # every entry point declared in include/bruno_hip.h
 EXPORTS = (
@@ -101,7 +103,8 @@ EXPORTS = (
     "bruno_server_get_unitigs", "bruno_server_get_branch_fasta", "bruno_server_release",
     "bruno_server_get_stats", "bruno_server_destroy", "bruno_set_tuning", "bruno_get_tuning",
     "bruno_fastq_split", "bruno_parse_count_fastq", "bruno_parse_count_fastq_host", "bruno_estimate_distinct",
-    "bruno_job_set_recycle",
+    "bruno_job_set_recycle", "bruno_get_branch_nodes", "bruno_get_chain_edges", "bruno_batch_get_branch_nodes",
+    "bruno_batch_get_chain_edges",
 )
 
 _LIB = None

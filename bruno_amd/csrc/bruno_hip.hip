@@ -568,6 +568,7 @@ constexpr uint32_t NONE32 = 0xFFFFFFFFu;
 constexpr uint32_t WALK_MAX = 1u << 16;
 constexpr uint32_t ERR_NOT_WELL_FORMED = 16;
 constexpr uint32_t ERR_WALK = 32;
+constexpr uint32_t ERR_NO_NODE = 64; /* an output k-mer without its filtered-graph record */
 
 __device__ __forceinline__ uint32_t end_bit(int e) { return e ? END_OUT : 0u; }
 
@@ -1244,6 +1245,96 @@ __global__ void k_get_chain_nodes(const uint32_t* perm, const KT* out_k, const K
   c.dist = (int32_t)dist_sorted[i];
   c.pad = 0;
   out[i] = c;
+}
+
+/* The filtered-graph record of canonical k-mer k in one job's table
+ * segment (null when absent). */
+template <typename KT>
+__device__ __forceinline__ const NodeRec* node_rec(const Slot<KT>* T, uint64_t tbase, uint64_t tcap, uint32_t rbits,
+                                                   KT k) {
+  const uint64_t s = SlotOps<KT>::find(T + tbase, tcap, rbits, k, true);
+  return s == ~0ULL ? nullptr : (const NodeRec*)&T[tbase + s].sub[0];
+}
+
+/* .branch.edges: the sorted branch k-mers with their counts
+ * (print_branch_edge_frequencies, compact_dbg_stats.hpp:219-262). */
+template <typename KT>
+__global__ void k_branch_nodes(const Slot<KT>* T, uint64_t tbase, uint64_t tcap, uint32_t rbits, const KT* sorted,
+                               uint64_t n, bruno_node* out, Counters* ctr) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const NodeRec* r = node_rec<KT>(T, tbase, tcap, rbits, sorted[i]);
+  bruno_node nd;
+  put_kmer<KT>(nd, sorted[i]);
+  for (int q = 0; q < 9; ++q) nd.counts[q] = r ? r->c[q] : 0;
+  nd.pad[0] = nd.pad[1] = nd.pad[2] = 0;
+  if (!r) atomicOr(&ctr->err, ERR_NO_NODE);
+  out[i] = nd;
+}
+
+/* .chain.edges (compute_freq_map, compact_dbg_stats.hpp:783-816, and
+ * print_chain_frequencies, :879-1065), one wave per unitig: the unitig's
+ * sorted chain nodes [starts[u], starts[u+1]) run from its rep L (dist 0)
+ * to its last k-mer R on the rep strand.  The lanes look up the self counts
+ * of the nodes in the table and reduce their sum, min and max; lane 0 reads
+ * L's in-edge and R's out-edge counts, complemented in reverse order when
+ * the k-mer is the reverse strand of its canonical form. */
+template <typename KT>
+__global__ __launch_bounds__(BLOCK) void k_chain_edges(const Slot<KT>* T, uint64_t tbase, uint64_t tcap, uint32_t rbits,
+                                                       int K, const uint32_t* perm, const KT* out_k,
+                                                       const uint32_t* starts, uint64_t n_unitigs,
+                                                       bruno_chain_edge* out, Counters* ctr) {
+  kops<KT> ops(K);
+  const int lane = threadIdx.x & 63;
+  const uint64_t wave = ((uint64_t)blockIdx.x * BLOCK + threadIdx.x) >> 6;
+  const uint64_t n_waves = ((uint64_t)gridDim.x * BLOCK) >> 6;
+  for (uint64_t u = wave; u < n_unitigs; u += n_waves) {
+    const uint32_t b = starts[u], e = starts[u + 1];
+    uint64_t sum = 0;
+    uint32_t mn = 65535, mx = 0, missing = 0;
+    for (uint32_t p = b + lane; p < e; p += 64) {
+      const NodeRec* r = node_rec<KT>(T, tbase, tcap, rbits, ops.canon(out_k[perm[p]]));
+      const uint32_t c = r ? r->c[8] : 0;
+      missing |= r == nullptr;
+      sum += c;
+      mn = c < mn ? c : mn;
+      mx = c > mx ? c : mx;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      sum += __shfl_xor(sum, off);
+      const uint32_t a = __shfl_xor(mn, off), z = __shfl_xor(mx, off);
+      mn = a < mn ? a : mn;
+      mx = z > mx ? z : mx;
+      missing |= __shfl_xor(missing, off);
+    }
+    if (lane == 0) {
+      const KT L = out_k[perm[b]], R = out_k[perm[e - 1]];
+      const KT cL = ops.canon(L), cR = ops.canon(R);
+      const NodeRec* rl = node_rec<KT>(T, tbase, tcap, rbits, cL);
+      const NodeRec* rr = node_rec<KT>(T, tbase, tcap, rbits, cR);
+      bruno_chain_edge o;
+      o.left[0] = (uint64_t)L;
+      o.left[1] = (uint64_t)((u128)L >> 64);
+      o.right[0] = (uint64_t)R;
+      o.right[1] = (uint64_t)((u128)R >> 64);
+      o.sum = sum;
+      o.n_nodes = e - b;
+      for (int c = 0; c < 4; ++c) {
+        o.in[c] = rl ? (cL == L ? rl->c[4 + c] : rl->c[3 - c]) : 0;
+        o.out[c] = rr ? (cR == R ? rr->c[c] : rr->c[7 - c]) : 0;
+      }
+      o.min = (uint16_t)mn;
+      o.max = (uint16_t)mx;
+      if (missing || !rl || !rr) atomicOr(&ctr->err, ERR_NO_NODE);
+      out[u] = o;
+    }
+  }
+}
+
+/* 1 where a sorted chain node starts a unitig (dist 0) */
+__global__ void k_head_flags(const uint32_t* dist_sorted, uint64_t n, uint8_t* flags) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) flags[i] = dist_sorted[i] == 0;
 }
 
 /* reads of one synthetic job at bases, offsets[r] = obase + r * L */
@@ -3273,6 +3364,134 @@ bruno_status bruno_batch_get_branch_fasta(bruno_job* j, uint32_t idx, char* buf,
   return get_text(j, j->bfasta + b0, len, buf, n);
 }
 
+/* table segment of job idx: first slot and probe span */
+static void job_segment(const bruno_job* j, uint32_t idx, uint64_t& tbase, uint64_t& tcap) {
+  if (j->n_jobs == 1) {
+    tbase = 0;
+    tcap = probe_span(j);
+  } else {
+    tbase = j->seg[idx];
+    tcap = j->seg[idx + 1] - j->seg[idx];
+  }
+}
+
+extern "C++" {
+template <typename KT>
+static bruno_status branch_nodes_t(bruno_job* j, uint32_t idx, uint64_t b0, uint64_t nb, bruno_node* buf) {
+  bruno_node* d = nullptr;
+  TRY(dev_alloc(j, (void**)&d, nb * sizeof(bruno_node)));
+  uint64_t tbase, tcap;
+  job_segment(j, idx, tbase, tcap);
+  hipLaunchKernelGGL((k_branch_nodes<KT>), dim3(grid_for(nb, BLOCK, 1u << 30)), dim3(BLOCK), 0, j->stream,
+                     (const Slot<KT>*)j->table, tbase, tcap, j->rbits, (const KT*)j->bsorted + b0, nb, d, j->ctr);
+  bruno_status st = sync_counters(j);
+  if (st == BRUNO_OK && copy_out(j, buf, d, nb * sizeof(bruno_node)) != hipSuccess) st = BRUNO_INTERNAL;
+  void* a = d;
+  dev_free(j, a, nb * sizeof(bruno_node));
+  return st;
+}
+
+}  // extern "C++"
+
+bruno_status bruno_batch_get_branch_nodes(bruno_job* j, uint32_t idx, bruno_node* buf, uint64_t* n) {
+  if (!valid_idx(j, idx) || !n) return BRUNO_BAD_INPUT;
+  if (j->stage != ST_COMPACTED || j->clear_pending) return BRUNO_STATE;
+  const uint64_t b0 = j->branch_start[idx], nb = j->branch_start[idx + 1] - b0;
+  if (!buf) {
+    *n = nb;
+    return BRUNO_OK;
+  }
+  if (*n < nb) return BRUNO_BAD_INPUT;
+  if (nb) TRY(j->words == 1 ? branch_nodes_t<uint64_t>(j, idx, b0, nb, buf) : branch_nodes_t<u128>(j, idx, b0, nb, buf));
+  *n = nb;
+  return BRUNO_OK;
+}
+
+extern "C++" {
+template <typename KT>
+static bruno_status chain_edges_t(bruno_job* j, uint32_t idx, uint64_t c0, uint64_t m, uint64_t nu,
+                                  bruno_chain_edge* buf) {
+  /* unitig starts: positions of the dist-0 nodes among the job's sorted
+   * chain nodes, in order, then m (room for m + 1: the selection is checked
+   * against the unitig count before anything reads it) */
+  uint8_t* flags = nullptr;
+  uint32_t *starts = nullptr, *n_sel = nullptr;
+  bruno_chain_edge* d = nullptr;
+  void* t = nullptr;
+  size_t tmp = 0;
+  bruno_status st = BRUNO_OK;
+  uint32_t got = 0;
+  hipcub::CountingInputIterator<uint32_t> pos(0);
+  TRY(dev_alloc(j, (void**)&flags, m));
+  if ((st = dev_alloc(j, (void**)&starts, (m + 1) * 4)) != BRUNO_OK) goto done;
+  if ((st = dev_alloc(j, (void**)&n_sel, 4)) != BRUNO_OK) goto done;
+  if ((st = dev_alloc(j, (void**)&d, nu * sizeof(bruno_chain_edge))) != BRUNO_OK) goto done;
+  hipLaunchKernelGGL(k_head_flags, dim3(grid_for(m, BLOCK, 1u << 30)), dim3(BLOCK), 0, j->stream,
+                     j->dist_sorted + c0, m, flags);
+  if (hipcub::DeviceSelect::Flagged(nullptr, tmp, pos, flags, starts, n_sel, (int)m, j->stream) != hipSuccess) {
+    st = BRUNO_INTERNAL;
+    goto done;
+  }
+  if ((st = dev_alloc(j, &t, std::max<size_t>(tmp, 1))) != BRUNO_OK) goto done;
+  if (hipcub::DeviceSelect::Flagged(t, tmp, pos, flags, starts, n_sel, (int)m, j->stream) != hipSuccess ||
+      copy_out(j, &got, n_sel, 4) != hipSuccess) {
+    st = BRUNO_INTERNAL;
+    goto done;
+  }
+  if (got != nu) {
+    st = BRUNO_INTERNAL;
+    goto done;
+  }
+  {
+    const uint32_t mm = (uint32_t)m;
+    if (hipMemcpyAsync(starts + nu, &mm, 4, hipMemcpyHostToDevice, j->stream) != hipSuccess) {
+      st = BRUNO_INTERNAL;
+      goto done;
+    }
+    uint64_t tbase, tcap;
+    job_segment(j, idx, tbase, tcap);
+    hipLaunchKernelGGL((k_chain_edges<KT>), dim3(grid_for(nu * 64, BLOCK, 1u << 16)), dim3(BLOCK), 0, j->stream,
+                       (const Slot<KT>*)j->table, tbase, tcap, j->rbits, j->K, j->perm + c0, (const KT*)j->out_k,
+                       starts, nu, d, j->ctr);
+    st = sync_counters(j);
+    if (st == BRUNO_OK && copy_out(j, buf, d, nu * sizeof(bruno_chain_edge)) != hipSuccess) st = BRUNO_INTERNAL;
+  }
+done:
+  void* a = flags;
+  dev_free(j, a, m);
+  if (starts) {
+    a = starts;
+    dev_free(j, a, (m + 1) * 4);
+  }
+  if (n_sel) {
+    a = n_sel;
+    dev_free(j, a, 4);
+  }
+  if (d) {
+    a = d;
+    dev_free(j, a, nu * sizeof(bruno_chain_edge));
+  }
+  if (t) dev_free(j, t, std::max<size_t>(tmp, 1));
+  return st;
+}
+
+}  // extern "C++"
+
+bruno_status bruno_batch_get_chain_edges(bruno_job* j, uint32_t idx, bruno_chain_edge* buf, uint64_t* n) {
+  if (!valid_idx(j, idx) || !n) return BRUNO_BAD_INPUT;
+  if (j->stage != ST_COMPACTED || j->clear_pending) return BRUNO_STATE;
+  const uint64_t c0 = j->chain_start[idx], m = j->chain_start[idx + 1] - c0;
+  const uint64_t nu = unitigs_of(j, j->fasta_off[idx + 1] - j->fasta_off[idx], m);
+  if (!buf) {
+    *n = nu;
+    return BRUNO_OK;
+  }
+  if (*n < nu) return BRUNO_BAD_INPUT;
+  if (nu) TRY(j->words == 1 ? chain_edges_t<uint64_t>(j, idx, c0, m, nu, buf) : chain_edges_t<u128>(j, idx, c0, m, nu, buf));
+  *n = nu;
+  return BRUNO_OK;
+}
+
 /* whole-job getters: the one-job case of the per-job views */
 #define ONE_JOB(j) \
   if (!valid_job(j)) return BRUNO_BAD_INPUT; \
@@ -3305,6 +3524,14 @@ bruno_status bruno_get_unitigs(bruno_job* j, char* buf, uint64_t* n) {
 bruno_status bruno_get_branch_fasta(bruno_job* j, char* buf, uint64_t* n) {
   ONE_JOB(j);
   return bruno_batch_get_branch_fasta(j, 0, buf, n);
+}
+bruno_status bruno_get_branch_nodes(bruno_job* j, bruno_node* buf, uint64_t* n) {
+  ONE_JOB(j);
+  return bruno_batch_get_branch_nodes(j, 0, buf, n);
+}
+bruno_status bruno_get_chain_edges(bruno_job* j, bruno_chain_edge* buf, uint64_t* n) {
+  ONE_JOB(j);
+  return bruno_batch_get_chain_edges(j, 0, buf, n);
 }
 
 bruno_status bruno_parse_records(int k, const char* d_bases, const uint64_t* d_offsets, uint64_t n_reads,

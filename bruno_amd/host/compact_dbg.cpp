@@ -10,7 +10,10 @@
  * lower <= count < upper times over the reads.
  *
  * Writes <prefix>.branch.fasta and <prefix>.chain.fasta, and unless -B also
- * <prefix>.chain.components (print_chain_nodes, compact_dbg_io.hpp:463-498).
+ * <prefix>.chain.components (print_chain_nodes, compact_dbg_io.hpp:463-498),
+ * <prefix>.branch.edges (print_branch_edge_frequencies,
+ * compact_dbg_stats.hpp:219-262) and <prefix>.chain.edges (compute_freq_map +
+ * print_chain_frequencies, :783-1065).
  * Each file's bytes go to the device, where the FASTQ records are split into
  * reads; reads with a non-ACGT base are dropped (NFilterSequencesIterator).
  */
@@ -121,6 +124,8 @@ int main(int argc, char** argv) {
         out += c + "\t" + n.rep.to_string(k) + "\t" + std::to_string(n.dist) + "\t" + (ks == c ? "1" : "0") + "\n";
       }
       write_file(prefix + ".chain.components", out);
+      write_file(prefix + ".branch.edges", chainmap.branch_edges());
+      write_file(prefix + ".chain.edges", chainmap.chain_edges());
     }
     bruno_stats st;
     check(bruno_get_stats(j.get(), &st), "stats");

@@ -187,6 +187,53 @@ class debruijn_chain_graph {
     compact();
     return text(bruno_get_branch_fasta);
   }
+  /* the .branch.edges bytes of print_branch_edge_frequencies
+   * (compact_dbg_stats.hpp:219-262): branch points sorted by k-mer,
+   * print_graph_node (debruijn_graph_operations.hpp:58-78) lines */
+  std::string branch_edges() {
+    compact();
+    uint64_t n = 0;
+    check(bruno_get_branch_nodes(j_.get(), nullptr, &n), "branch_edges");
+    std::vector<bruno_node> v(n);
+    if (n) check(bruno_get_branch_nodes(j_.get(), v.data(), &n), "branch_edges");
+    std::string out;
+    for (auto& b : v) {
+      kmer_value km;
+      km.w[0] = b.kmer[0];
+      km.w[1] = b.kmer[1];
+      out += km.to_string(j_.k());
+      static const int order[9] = {4, 5, 6, 7, 0, 1, 2, 3, 8};
+      for (int q : order) out += "\t" + std::to_string(b.counts[q]);
+      out += "\n";
+    }
+    return out;
+  }
+  /* the .chain.edges bytes of compute_freq_map + print_chain_frequencies
+   * (compact_dbg_stats.hpp:783-816, :879-1065): per unitig L, R, L's in-edge
+   * counts, R's out-edge counts, mean, min, max, each followed by a tab */
+  std::string chain_edges() {
+    compact();
+    uint64_t n = 0;
+    check(bruno_get_chain_edges(j_.get(), nullptr, &n), "chain_edges");
+    std::vector<bruno_chain_edge> v(n);
+    if (n) check(bruno_get_chain_edges(j_.get(), v.data(), &n), "chain_edges");
+    std::string out;
+    for (auto& e : v) {
+      kmer_value L, R;
+      L.w[0] = e.left[0];
+      L.w[1] = e.left[1];
+      R.w[0] = e.right[0];
+      R.w[1] = e.right[1];
+      /* float division, then the conversion to the 16-bit count type */
+      const float mean = (float)e.sum / (float)e.n_nodes;
+      const uint16_t vals[11] = {e.in[0], e.in[1], e.in[2], e.in[3], e.out[0], e.out[1], e.out[2], e.out[3],
+                                 (uint16_t)(int32_t)mean, e.min, e.max};
+      out += L.to_string(j_.k()) + "\t" + R.to_string(j_.k()) + "\t";
+      for (uint16_t x : vals) out += std::to_string(x) + "\t";
+      out += "\n";
+    }
+    return out;
+  }
 
  private:
   size_t rank(int variant) {

@@ -15,7 +15,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
-from ._lib import CHAIN_DTYPE, NODE_DTYPE, check, lib
+from ._lib import CHAIN_DTYPE, CHAIN_EDGE_DTYPE, NODE_DTYPE, check, lib
 
 DEFAULT_THRESH = 65536
 
@@ -60,6 +60,23 @@ class _Views:
     def unitigs(self):
         return self._two_call("unitigs", text=True)
 
+    def branch_nodes(self):
+        """Branch nodes sorted by k-mer, with counts (after compact)."""
+        return self._two_call("branch_nodes", NODE_DTYPE)
+
+    def chain_edges(self):
+        """Per unitig: end k-mers L, R, L's in-edge and R's out-edge counts,
+        self-count sum / n_nodes / min / max (after compact)."""
+        return self._two_call("chain_edges", CHAIN_EDGE_DTYPE)
+
+    def branch_edges_text(self):
+        """The .branch.edges bytes (print_graph_node lines)."""
+        return format_branch_edges(self.branch_nodes(), self.k)
+
+    def chain_edges_text(self):
+        """The .chain.edges bytes (print_chain_frequencies lines)."""
+        return format_chain_edges(self.chain_edges(), self.k)
+
     def branch_fasta(self):
         return self._two_call("branch_fasta", text=True)
 
@@ -80,6 +97,37 @@ class _Views:
         if n.value:
             self._call("k2", _ptr(km), _ptr(e), _ptr(c), ctypes.byref(n))
         return km.reshape(-1, self.words), e, c
+
+
+def kmer_text(words, k):
+    """A k-mer (two 64-bit words, low first) as bases, first base most
+    significant."""
+    v = int(words[0]) | (int(words[1]) << 64)
+    return "".join("ACGT"[(v >> (2 * (k - 1 - i))) & 3] for i in range(k))
+
+
+def format_branch_edges(nodes, k):
+    """print_graph_node lines (debruijn_graph_operations.hpp:58-78): k-mer,
+    in A C G T, out A C G T, self."""
+    out = []
+    for r in nodes:
+        c = [int(x) for x in r["counts"]]
+        out.append("\t".join([kmer_text(r["kmer"], k)] + [str(x) for x in c[4:8] + c[0:4] + [c[8]]]) + "\n")
+    return "".join(out).encode()
+
+
+def format_chain_edges(edges, k):
+    """print_chain_frequencies lines (compact_dbg_stats.hpp:1036-1055): L, R,
+    L's in-edge counts, R's out-edge counts, mean, min, max, each followed by
+    a tab; the mean is float32(sum) / float32(n_nodes) converted to the
+    16-bit count type (truncated)."""
+    out = []
+    for r in edges:
+        mean = int(np.float32(int(r["sum"])) / np.float32(int(r["n_nodes"])))
+        vals = [int(x) for x in r["in"]] + [int(x) for x in r["out"]] + [mean, int(r["min"]), int(r["max"])]
+        out.append(kmer_text(r["left"], k) + "\t" + kmer_text(r["right"], k) + "\t" +
+                   "".join("%d\t" % v for v in vals) + "\n")
+    return "".join(out).encode()
 
 
 def job_desc(k, thresholds=None, edge_filter=False):

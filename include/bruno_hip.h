@@ -118,6 +118,20 @@ typedef struct {
   uint16_t pad[3];
 } bruno_node;
 
+/* One line of .chain.edges (print_chain_frequencies,
+ * compact_dbg_stats.hpp:879-1065), one per unitig, in unitig order (sorted
+ * by L).  The line prints L, R, in[0..3], out[0..3], the mean
+ * (uint16_t)((float)sum / (float)n_nodes), min, max. */
+typedef struct {
+  uint64_t left[2];         /* L: the unitig's first k-mer on the strand it is printed on (its rep) */
+  uint64_t right[2];        /* R: its last k-mer on that strand                                */
+  uint64_t sum;             /* sum of the self counts of its k-mers (compute_freq_map, :783-816) */
+  uint32_t n_nodes;         /* its k-mers                                                      */
+  uint16_t in[4];           /* L's in-edge counts A,C,G,T on L's strand                        */
+  uint16_t out[4];          /* R's out-edge counts A,C,G,T on R's strand                       */
+  uint16_t min, max;        /* smallest / largest self count                                   */
+} bruno_chain_edge;
+
 const char* bruno_status_string(bruno_status s);
 
 /* Replaces the construction of CountDBGType idx / ChainGraphType chainmap
@@ -221,6 +235,15 @@ bruno_status bruno_get_chain_nodes(bruno_job* job, bruno_chain_node* buf, uint64
 /* .chain.fasta / .branch.fasta bytes (after compact). */
 bruno_status bruno_get_unitigs(bruno_job* job, char* buf, uint64_t* n);
 bruno_status bruno_get_branch_fasta(bruno_job* job, char* buf, uint64_t* n);
+/* Non-benchmark outputs (after compact; not after a compact that recycles
+ * the table, which returns BRUNO_STATE).  Branch nodes sorted by k-mer with
+ * their counts: the .branch.edges lines (print_branch_edge_frequencies,
+ * compact_dbg_stats.hpp:219-262, print_graph_node order: in A C G T, out
+ * A C G T, self).  Per-unitig end edges and self-count summary: the
+ * .chain.edges lines (compute_freq_map + print_chain_frequencies,
+ * compact_dbg_freq_construct_clean.cpp:759-829). */
+bruno_status bruno_get_branch_nodes(bruno_job* job, bruno_node* buf, uint64_t* n);
+bruno_status bruno_get_chain_edges(bruno_job* job, bruno_chain_edge* buf, uint64_t* n);
 
 /* ---- Batches of independent jobs ---------------------------------------
  * A batch runs n_jobs independent assembly jobs (same k and thresholds,
@@ -261,6 +284,8 @@ bruno_status bruno_batch_get_branches(bruno_job* job, uint32_t idx, uint64_t* km
 bruno_status bruno_batch_get_chain_nodes(bruno_job* job, uint32_t idx, bruno_chain_node* buf, uint64_t* n);
 bruno_status bruno_batch_get_unitigs(bruno_job* job, uint32_t idx, char* buf, uint64_t* n);
 bruno_status bruno_batch_get_branch_fasta(bruno_job* job, uint32_t idx, char* buf, uint64_t* n);
+bruno_status bruno_batch_get_branch_nodes(bruno_job* job, uint32_t idx, bruno_node* buf, uint64_t* n);
+bruno_status bruno_batch_get_chain_edges(bruno_job* job, uint32_t idx, bruno_chain_edge* buf, uint64_t* n);
 
 /* Debug entry for the P0 parity object: parse only, write each canonical
  * (k-mer, biedge) record in read order (device pointers; out_kmers holds

@@ -153,3 +153,5 @@ def test_driver_edge_flag(tmp_path):
     assert open(prefix + ".chain.fasta", "rb").read() == o.chain_fasta()
     assert open(prefix + ".branch.fasta", "rb").read() == o.branch_fasta()
     assert open(prefix + ".chain.components", "rb").read() == o.chain_components()
+    assert open(prefix + ".branch.edges", "rb").read() == o.branch_edges()
+    assert open(prefix + ".chain.edges", "rb").read() == o.chain_edges()
