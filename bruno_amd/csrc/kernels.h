@@ -246,21 +246,18 @@ constexpr int SKETCH_SAMPLE_BITS = 6;
 constexpr int SKETCH_INDEX_BITS = 12;
 constexpr int SKETCH_M = 1 << SKETCH_INDEX_BITS;
 
-/* The canonical (k-mer, biedge) instances of one read of L >= K + 2 bases
- * at s, parsed by one wave: lane l takes instances p0 + l of a sliding
- * window of four packed words covering bases [p0, p0 + 128).  f(i, km, e)
- * runs for each instance i < L - K + 1. */
-template <typename KT, typename F>
-__device__ __forceinline__ void for_each_instance(const uint8_t* s, uint64_t L, int K, int lane, const kops<KT>& ops,
-                                                  F&& f) {
+/* The canonical (k-mer, biedge) instances of one read of L >= K + 2 bases,
+ * parsed by one wave: lane l takes instances p0 + l of a sliding window of
+ * four packed words covering bases [p0, p0 + 128).  code_of(q) gives the
+ * lane's base code in chunk q (base 64 q + l; 0 past the read).  f(i, km,
+ * e) runs for each instance i < L - K + 1. */
+template <typename KT, typename G, typename F>
+__device__ __forceinline__ void for_each_instance_from(uint64_t L, int K, int lane, const kops<KT>& ops, G&& code_of,
+                                                       F&& f) {
   const uint64_t nk = L - K + 1;
   uint64_t A, B, C, D, prevB = 0;
-  {
-    int c0 = (lane < L) ? base_code(s[lane]) : 0;
-    pack64(c0, A, B);
-    int c1 = (64 + lane < L) ? base_code(s[64 + lane]) : 0;
-    pack64(c1, C, D);
-  }
+  pack64(code_of(0), A, B);
+  pack64(code_of(1), C, D);
   for (uint64_t p0 = 0; p0 < nk; p0 += 64) {
     const uint64_t i = p0 + lane;
     if (i < nk) {
@@ -284,10 +281,18 @@ __device__ __forceinline__ void for_each_instance(const uint8_t* s, uint64_t L, 
     prevB = B;
     A = C;
     B = D;
-    uint64_t nxt = p0 + 128 + lane;
-    int cn = (nxt < L) ? base_code(s[nxt]) : 0;
-    pack64(cn, C, D);
+    pack64(code_of(p0 / 64 + 2), C, D);
   }
+}
+
+/* The same with the bases read from s. */
+template <typename KT, typename F>
+__device__ __forceinline__ void for_each_instance(const uint8_t* s, uint64_t L, int K, int lane, const kops<KT>& ops,
+                                                  F&& f) {
+  for_each_instance_from<KT>(L, K, lane, ops, [&](uint64_t q) -> int {
+    const uint64_t i = 64 * q + (uint64_t)lane;
+    return i < L ? base_code(s[i]) : 0;
+  }, f);
 }
 
 /* MODE 0: count records per read; MODE 1: write records; MODE 2: count into

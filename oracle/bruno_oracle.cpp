@@ -933,7 +933,7 @@ struct oracle_job {
   void make_edge_outputs() {
     branch_edges.clear();
     for (u128 b : branches) {
-      const Node* n = node_of(b);
+      const Node* n = node_of(b);  /* every branch is a node */
       std::string line = ops.to_string(b);
       for (int q = 4; q < 8; ++q) line += "\t" + std::to_string(n->c[q]);
       for (int q = 0; q < 4; ++q) line += "\t" + std::to_string(n->c[q]);
@@ -946,7 +946,8 @@ struct oracle_job {
     };
     std::map<u128, Freq> freq;
     for (auto& x : ranked) {
-      const uint16_t self = node_of(canon(lo_hi(x.k_lo, x.k_hi)))->c[8];
+      const Node* nk = node_of(canon(lo_hi(x.k_lo, x.k_hi)));
+      const uint16_t self = nk ? nk->c[8] : 0;
       Freq& f = freq[canon(lo_hi(x.rep_lo, x.rep_hi))];
       f.n += 1;
       f.sum += self;
@@ -984,9 +985,13 @@ struct oracle_job {
       const u128 cL = canon(l.L), cR = canon(l.R);
       const Node* nl = node_of(cL);
       const Node* nr = node_of(cR);
+      /* an end that is not in the graph (k-mer-only thresholds keep edges
+       * into removed k-mers, and the ranking can carry such a k-mer as a
+       * chain end): the reference asserts; the counts here read 0 */
+      if (!nl || !nr) ++chain_edge_misses;
       for (int c = 0; c < 4; ++c) {
-        l.v[c] = cL == l.L ? nl->c[4 + c] : nl->c[3 - c];
-        l.v[4 + c] = cR == l.R ? nr->c[c] : nr->c[4 + 3 - c];
+        l.v[c] = !nl ? 0 : cL == l.L ? nl->c[4 + c] : nl->c[3 - c];
+        l.v[4 + c] = !nr ? 0 : cR == l.R ? nr->c[c] : nr->c[4 + 3 - c];
       }
       auto f = freq.find(cL);
       if (f == freq.end()) {

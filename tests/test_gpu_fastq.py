@@ -126,3 +126,31 @@ def test_in_place_parse_equals_split_reads(crlf):
     o = run_oracle(31, th, b, off, 1)
     assert got[0] == o.chain_fasta() and got[1] == o.branch_fasta()
     assert got[2] == o.stat("n_instances")
+
+
+@pytest.mark.parametrize("crlf", [False, True])
+def test_in_place_parse_by_regions(crlf):
+    """The region count reads the sequence lines in place too (read bounds
+    from the start / end arrays), reads with N included."""
+    from bruno_amd.autotune import get_tuning, set_tuning
+    _, reads = random_reads(14, 40000, 6000, (40, 300), 0.01)
+    reads[7] = reads[7][:20] + "N" + reads[7][21:]
+    reads[9] = reads[9][:-1] + "N"
+    data = to_fastq(reads, crlf=crlf)
+    th = thresholds_L(2)
+    old = get_tuning()
+    set_tuning(count_regions=2)
+    try:
+        job = bruno_amd.GraphJob(31, th, capacity=1 << 22)
+        job.parse_count_fastq(data)
+        assert job.stats()["count_by_regions"] == 1
+        job.run(1)
+        got = (job.unitigs(), job.branch_fasta(), job.stats()["n_instances"], job.stats()["n_reads_dropped"])
+        job.close()
+    finally:
+        set_tuning(**old)
+    b, off = pack(split(data))
+    o = run_oracle(31, th, b, off, 1)
+    assert got[0] == o.chain_fasta() and got[1] == o.branch_fasta()
+    assert got[2] == o.stat("n_instances")
+    assert got[3] == 2

@@ -99,13 +99,17 @@ def test_skewed_input_falls_back_to_atomic_count():
 
 
 def test_edge_reads():
-    """Empty, short, N-containing reads and one 20 kbp read among 3000 short
-    ones (the long read's records overflow the workgroup's staging and are
-    appended one by one)."""
+    """Empty, short, N-containing reads (the N within the first, second and
+    later 64-base chunks) and one 20 kbp read among 3000 short ones (the
+    long read's records overflow the workgroup's staging and are appended
+    one by one)."""
     long_read = "".join(np.random.default_rng(3).choice(list("ACGT"), 20000))
     _, short = random_reads(17, 40000, 3000, 120, 0.01)
+    with_n = [long_read[100:500], long_read[600:1000], long_read[1100:1300]]
+    with_n = [with_n[0][:300] + "N" + with_n[0][301:], with_n[1][:150] + "N" + with_n[1][151:],
+              with_n[2][:191] + "N" + with_n[2][192:]]
     reads = ["", "ACGT" * 5, "A" * 33, "ACGTN" * 10, "C" * 31 + "A", long_read, long_read[5000:9000],
-             long_read[12000:15000][::-1].translate(str.maketrans("ACGT", "TGCA"))] + short
+             long_read[12000:15000][::-1].translate(str.maketrans("ACGT", "TGCA"))] + with_n + short
     b, off = pack(reads)
     for th in (None, thresholds_L(1)):
         g, _ = compare_all(31, th, b, off, capacity=REGION_CAP)

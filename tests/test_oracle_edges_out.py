@@ -97,3 +97,18 @@ def test_fixture_edge_outputs(name, tl):
     assert o.stat("chain_edge_misses") == 0
     assert o.branch_edges().decode() == nd.branch_edges(g)
     assert o.chain_edges().decode() == nd.chain_edges(g, 31)
+
+
+@pytest.mark.parametrize("case", CASES[:6])
+def test_kmer_only_thresholds_do_not_break_edge_outputs(case):
+    """-L 3 0 0 keeps edges into removed k-mers, so a ranked chain can carry
+    a removed k-mer as its far end (the reference asserts there).  The
+    oracle still finishes; ends it cannot find count as misses with zero
+    edge counts, and the branch lines are unaffected."""
+    seed, G, n, L, err, k = case
+    _, reads = random_reads(seed, G, n, L, err)
+    o = run(reads, k, [3, 65536, 0, 65536, 0, 65536])
+    g = nd.filtered(reads, k, [3, 65536, 0, 65536, 0, 65536])
+    assert o.branch_edges().decode() == nd.branch_edges(g)
+    lines = o.chain_edges().decode().splitlines()
+    assert len(lines) == len(o.chain_fasta().decode().split(">")) - 1
