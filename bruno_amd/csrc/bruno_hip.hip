@@ -345,6 +345,55 @@ __device__ uint32_t resolve(const Slot<KT>* T, uint64_t tbase, uint64_t tcap, ui
   return end | (uint32_t)(n_chain + v);
 }
 
+/* resolve() of a chain node's in neighbour (its OUT end faces the node when
+ * canonical) and out neighbour (its IN end) at once: the two probe
+ * sequences advance in lockstep, so their table reads are in flight
+ * together.  Each pointer is written only when its neighbour exists. */
+template <typename KT>
+__device__ void resolve_pair(const Slot<KT>* T, uint64_t tbase, uint64_t tcap, uint32_t rbits, const kops<KT>& ops,
+                             bool has_in, KT nin, bool has_out, KT nout, uint64_t n_chain, KT* vkmer, uint64_t vcap,
+                             Counters* ctr, uint32_t& pin, uint32_t& pout) {
+  const KT ri = ops.rc(nin), ro = ops.rc(nout);
+  const bool fi = !(nin < ri), fo = !(nout < ro);
+  const KT cin = fi ? ri : nin, cout = fo ? ro : nout;
+  const uint32_t ein = fi ? 0u : END_OUT, eout = fo ? END_OUT : 0u;
+  const Slot<KT>* B = T + tbase;
+  Probe qi(kmer_hash(cin), tcap, rbits), qo(kmer_hash(cout), tcap, rbits);
+  const uint64_t lim = tcap < MAX_PROBE ? tcap : MAX_PROBE;
+  bool di = !has_in, dout = !has_out;
+  uint64_t si = ~0ULL, so = ~0ULL;
+  for (uint64_t probe = 0; probe < lim && !(di && dout); ++probe) {
+    const int a = SlotOps<KT>::probe_state(B, qi.s, cin, true);
+    const int b = SlotOps<KT>::probe_state(B, qo.s, cout, true);
+    if (!di) {
+      if (a == 1) si = qi.s;
+      if (a) di = true;
+      else qi.next();
+    }
+    if (!dout) {
+      if (b == 1) so = qo.s;
+      if (b) dout = true;
+      else qo.next();
+    }
+  }
+  const NodeRec* ni = (const NodeRec*)&B[si == ~0ULL ? 0 : si].sub[0];
+  const NodeRec* no = (const NodeRec*)&B[so == ~0ULL ? 0 : so].sub[0];
+  const uint8_t cls_i = ni->cls, cls_o = no->cls;
+  const uint32_t idx_i = ni->idx, idx_o = no->idx;
+  auto finish = [&](bool found, uint8_t cls, uint32_t idx, KT cn, uint32_t end) -> uint32_t {
+    if (found && cls == CLS_CHAIN) return end | idx;
+    unsigned long long v = atomicAdd(&ctr->n_virt, 1ULL);
+    if (v >= vcap) {
+      atomicOr(&ctr->err, ERR_LIST_FULL);
+      return end | IDX_MASK;
+    }
+    vkmer[v] = cn;
+    return end | (uint32_t)(n_chain + v);
+  };
+  if (has_in) pin = finish(si != ~0ULL, cls_i, idx_i, cin, ein);
+  if (has_out) pout = finish(so != ~0ULL, cls_o, idx_o, cout, eout);
+}
+
 /* to_simple_biedge (debruijn_chain_node.hpp:84-141): one neighbour per side
  * at dist 1, else term_mask. */
 template <typename KT>
@@ -369,18 +418,18 @@ __global__ __launch_bounds__(BLOCK) void k_chain_init(const Slot<KT>* T, uint64_
     st.p[0] = st.p[1] = IDX_MASK;
     st.d[0] = st.d[1] = TERM_MASK;
     st.mail[0] = st.mail[1] = 0ULL;
-    for (int c = 0; c < 4; ++c)
-      if (nr->c[4 + c] > 0) {
-        /* in neighbour c.k[0..K-2]: when canonical its OUT end faces k */
-        st.p[0] = resolve<KT>(T, tbase, tcap, rbits, ops, ops.in_nbr(k, c), END_OUT, n_chain, vkmer, vcap, ctr);
-        st.d[0] = 1;
-      }
-    for (int c = 0; c < 4; ++c)
-      if (nr->c[c] > 0) {
-        /* out neighbour k[1..].c: when canonical its IN end faces k */
-        st.p[1] = resolve<KT>(T, tbase, tcap, rbits, ops, ops.out_nbr(k, c), 0u, n_chain, vkmer, vcap, ctr);
-        st.d[1] = 1;
-      }
+    /* a chain node has at most one in and one out edge */
+    int ci = -1, co = -1;
+    for (int c = 0; c < 4; ++c) {
+      if (nr->c[4 + c] > 0) ci = c;
+      if (nr->c[c] > 0) co = c;
+    }
+    /* in neighbour c.k[0..K-2]: when canonical its OUT end faces k; out
+     * neighbour k[1..].c: when canonical its IN end faces k */
+    resolve_pair<KT>(T, tbase, tcap, rbits, ops, ci >= 0, ops.in_nbr(k, ci < 0 ? 0 : ci), co >= 0,
+                     ops.out_nbr(k, co < 0 ? 0 : co), n_chain, vkmer, vcap, ctr, st.p[0], st.p[1]);
+    if (ci >= 0) st.d[0] = 1;
+    if (co >= 0) st.d[1] = 1;
     ckmer[i] = k;
     lr[i] = st;
   }

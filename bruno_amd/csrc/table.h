@@ -146,6 +146,15 @@ template <> struct SlotOps<uint64_t> {
     atomicOr(err, ERR_TABLE_FULL);
     return ~0ULL;
   }
+  /* One probe of a lookup: 1 = v is at slot s, 2 = v is absent (an empty
+   * slot, or present is required and v did not survive the filter), 0 =
+   * probe on. */
+  static __device__ __forceinline__ int probe_state(const S* T, uint64_t s, uint64_t v, bool require_present) {
+    const uint64_t k0 = T[s].key;
+    if (k0 == EMPTY64) return 2;
+    if ((k0 & KEY_BITS) != v) return 0;
+    return (require_present && !(k0 & F_PRESENT)) ? 2 : 1;
+  }
   /* Lookup only (separate kernels after counting); ~0 when absent.
    * require_present: only nodes that survived the filter. */
   static __device__ uint64_t find(const S* T, uint64_t span, uint32_t rbits, uint64_t v, bool require_present) {
@@ -211,6 +220,12 @@ template <> struct SlotOps<u128> {
     }
     atomicOr(err, ERR_TABLE_FULL);
     return ~0ULL;
+  }
+  static __device__ __forceinline__ int probe_state(const S* T, uint64_t s, u128 v, bool require_present) {
+    const uint64_t m = T[s].meta;
+    if (m == EMPTY64) return 2;
+    if (T[s].lo != (uint64_t)v || T[s].hi != (uint64_t)(v >> 64)) return 0;
+    return (require_present && !(m & F_PRESENT)) ? 2 : 1;
   }
   static __device__ uint64_t find(const S* T, uint64_t span, uint32_t rbits, u128 v, bool require_present) {
     uint64_t lo = (uint64_t)v, hi = (uint64_t)(v >> 64);
