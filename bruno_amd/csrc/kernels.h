@@ -134,23 +134,18 @@ __device__ __forceinline__ int base_code(uint8_t c) {
   }
 }
 
-__device__ __forceinline__ uint64_t spread32(uint32_t x) {
-  uint64_t v = x;
-  v = (v | (v << 16)) & 0x0000FFFF0000FFFFULL;
-  v = (v | (v << 8)) & 0x00FF00FF00FF00FFULL;
-  v = (v | (v << 4)) & 0x0F0F0F0F0F0F0F0FULL;
-  v = (v | (v << 2)) & 0x3333333333333333ULL;
-  v = (v | (v << 1)) & 0x5555555555555555ULL;
-  return v;
-}
-
 /* Pack 64 bases (one per lane, code 0..3) into two 32-base words with the
- * first base in the top bits, using two wavefront ballots. */
+ * first base in the top bits: bit p of w0 is bit (p & 1) of base 31 - p / 2
+ * (w1: base 63 - p / 2), so lane p fetches that base's code from its lane
+ * and one ballot per word gives the packed word.  The whole wave must be
+ * active. */
 __device__ __forceinline__ void pack64(int code, uint64_t& w0, uint64_t& w1) {
-  uint64_t m0 = __ballot(code & 1);
-  uint64_t m1 = __ballot((code >> 1) & 1);
-  w0 = spread32(__brev((uint32_t)m0)) | (spread32(__brev((uint32_t)m1)) << 1);
-  w1 = spread32(__brev((uint32_t)(m0 >> 32))) | (spread32(__brev((uint32_t)(m1 >> 32))) << 1);
+  const int lane = (int)__lane_id();
+  const int bit = lane & 1;
+  const int c0 = __shfl(code, 31 - (lane >> 1), 64);
+  const int c1 = __shfl(code, 63 - (lane >> 1), 64);
+  w0 = __ballot((c0 >> bit) & 1);
+  w1 = __ballot((c1 >> bit) & 1);
 }
 
 __device__ __forceinline__ uint64_t sel4(int q, uint64_t a, uint64_t b, uint64_t c, uint64_t d) {
