@@ -124,14 +124,15 @@ __global__ void k_fill64(uint64_t* p, uint64_t n, uint64_t v) {
 /* ------------------------------------------------------------------ */
 /* parse (+ count)                                                      */
 /* ------------------------------------------------------------------ */
-__device__ __forceinline__ int base_code(uint8_t c) {
-  switch (c) {
-    case 'A': case 'a': return 0;
-    case 'C': case 'c': return 1;
-    case 'G': case 'g': return 2;
-    case 'T': case 't': return 3;
-    default: return -1;
-  }
+/* A/a 0, C/c 1, G/g 2, T/t 3, any other byte -1, without branches: the
+ * byte is lower-cased (only these eight bytes map onto a, c, g, t), tested
+ * against the bit set {a, c, g, t} - 'a', and coded as ((c >> 1) ^ (c >> 2))
+ * & 3 (a 0x61 -> 0, c 0x63 -> 1, g 0x67 -> 2, t 0x74 -> 3). */
+__device__ __forceinline__ int base_code(uint8_t b) {
+  const uint32_t c = (uint32_t)b | 0x20u;
+  const uint32_t d = c - 0x61u;
+  const bool ok = d < 20u && ((0x80045u >> d) & 1u);
+  return ok ? (int)(((c >> 1) ^ (c >> 2)) & 3u) : -1;
 }
 
 /* Pack 64 bases (one per lane, code 0..3) into two 32-base words with the
