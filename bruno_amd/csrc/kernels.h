@@ -149,11 +149,14 @@ __device__ __forceinline__ void pack64(int code, uint64_t& w0, uint64_t& w1) {
   w1 = __ballot((c1 >> bit) & 1);
 }
 
+/* word q (0..3) of [a b c d] as a two-level select on the bits of q (a
+ * per-lane q then costs selects, not branches) */
 __device__ __forceinline__ uint64_t sel4(int q, uint64_t a, uint64_t b, uint64_t c, uint64_t d) {
-  return q == 0 ? a : q == 1 ? b : q == 2 ? c : d;
+  const uint64_t lo = (q & 1) ? b : a, hi = (q & 1) ? d : c;
+  return (q & 2) ? hi : lo;
 }
 /* 64 bits of the 256-bit window [a b c d] starting at base position pos
- * (pos <= 96). */
+ * (pos <= 96; at pos 96 the word after d is not read). */
 __device__ __forceinline__ uint64_t window64(int pos, uint64_t a, uint64_t b, uint64_t c, uint64_t d) {
   int q = pos >> 5, r = pos & 31;
   uint64_t hi = sel4(q, a, b, c, d), lo = sel4(q + 1, a, b, c, d);
