@@ -626,7 +626,7 @@ constexpr int RULER_LDS = 2048; /* rulers staged per block before an append */
 /* ruler_mask + 1 = sampling period of the rulers (a power of two).  Marks
  * the rulers (chain ends and sampled nodes) and lists them (LDS-staged
  * appends), and counts ends that break well-formedness. */
-__global__ __launch_bounds__(BLOCK) void k_lr_check(const LRNode* lr, uint64_t n_chain, uint8_t* ruler,
+__global__ __launch_bounds__(BLOCK) void k_lr_check(const LRNode* lr, uint64_t n_chain, uint64_t* ruler,
                                                     uint64_t ruler_mask, uint32_t* rlist, uint64_t rlist_cap,
                                                     Counters* ctr) {
   __shared__ LdsList<RULER_LDS, uint32_t> Lr;
@@ -658,8 +658,10 @@ __global__ __launch_bounds__(BLOCK) void k_lr_check(const LRNode* lr, uint64_t n
       }
       bool sampled = (fmix64(t * 0x9e3779b97f4a7c15ULL + 0x1234567ULL) & ruler_mask) == 0;
       is_ruler = head || sampled;
-      ruler[t] = is_ruler ? 1 : 0;
     }
+    /* the ruler bitmap, one word per wave (a wave covers 64 aligned nodes) */
+    const uint64_t word = __ballot(is_ruler);
+    if ((threadIdx.x & 63) == 0 && t < n_chain) ruler[t >> 6] = word;
     if (is_ruler) Lr.v[atomicAdd(&Lr.n, 1u)] = (uint32_t)t;
     /* all threads read the count between two barriers, then keep room for
      * one more tile */
@@ -680,7 +682,7 @@ struct WalkRec {
 /* From every ruler, along each non-terminal end, to the next ruler.  Each
  * visited node records, at the end it was entered through, the ruler and
  * the ruler's end the walk left from, and the distance. */
-__global__ __launch_bounds__(BLOCK) void k_lr_walk_nodes(const LRNode* lr, const uint8_t* ruler, const uint32_t* rlist,
+__global__ __launch_bounds__(BLOCK) void k_lr_walk_nodes(const LRNode* lr, const uint64_t* ruler, const uint32_t* rlist,
                                                          uint64_t n_rulers, WalkRec* rec, WalkRec* link, Counters* ctr) {
   const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
   for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < 2 * n_rulers; i += stride) {
@@ -695,7 +697,7 @@ __global__ __launch_bounds__(BLOCK) void k_lr_walk_nodes(const LRNode* lr, const
       uint32_t j = ptr & IDX_MASK;
       int e = (ptr & END_OUT) ? 1 : 0;
       ++dist;
-      if (ruler[j]) {
+      if ((ruler[j >> 6] >> (j & 63)) & 1) {
         link[2 * (uint64_t)r + E] = WalkRec{j, end_bit(e) | dist};
         break;
       }
@@ -773,7 +775,7 @@ __device__ __forceinline__ uint32_t ceil_log2(uint32_t L) { return L <= 1 ? 0 : 
  * (written in a second pass once T is known). */
 /* jT[job]: largest ceil(log2 L) over the job's paths; jcyc[job] = 1 when
  * the job has a cycle. */
-__global__ __launch_bounds__(BLOCK) void k_lr_finish(LRNode* lr, uint64_t n_chain, const uint8_t* ruler,
+__global__ __launch_bounds__(BLOCK) void k_lr_finish(LRNode* lr, uint64_t n_chain, const uint64_t* ruler,
                                                      const WalkRec* rec, const WalkRec* info, uint8_t* cyc,
                                                      const uint32_t* cjob, uint32_t n_jobs, uint32_t* jT,
                                                      uint32_t* jcyc, Counters* ctr) {
@@ -788,7 +790,7 @@ __global__ __launch_bounds__(BLOCK) void k_lr_finish(LRNode* lr, uint64_t n_chai
       nd[E] = st.d[E];
       if (is_terminal(st.d[E])) continue;
       uint32_t Z, f, D;
-      if (ruler[t]) {
+      if ((ruler[t >> 6] >> (t & 63)) & 1) {
         WalkRec w = info[2 * t + E];
         if (w.who == NONE32) {
           is_cycle = true;
@@ -2138,11 +2140,14 @@ bruno_status list_rank_fast(bruno_job* j, bool* used) {
   *used = false;
   uint64_t nc = j->n_chain;
   if (nc == 0) return BRUNO_OK;
-  uint8_t* ruler = nullptr;
+  /* ruler flags as a bitmap (n_chain / 8 bytes: small enough to stay in
+   * the device caches for the walks' random tests) */
+  uint64_t* ruler = nullptr;
+  const uint64_t ruler_bytes = (nc + 63) / 64 * 8;
   uint32_t* rlist = nullptr;
   uint32_t* heads = nullptr;
   WalkRec *rec = nullptr, *link = nullptr, *info = nullptr;
-  TRY(dev_alloc(j, (void**)&ruler, nc));
+  TRY(dev_alloc(j, (void**)&ruler, ruler_bytes));
   j->hctr->lr_conflict = 0;
   j->hctr->lr_next = 0;
   j->hctr->lr_count = 0;
@@ -2156,7 +2161,7 @@ bruno_status list_rank_fast(bruno_job* j, bool* used) {
   bool well_formed = j->hctr->lr_conflict == 0;
   if (!well_formed) {
     void* a = ruler;
-    dev_free(j, a, nc);
+    dev_free(j, a, ruler_bytes);
     a = rlist;
     dev_free(j, a, nc * 4);
     return BRUNO_OK;
