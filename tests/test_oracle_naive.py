@@ -21,6 +21,39 @@ def oracle_graph(o, k, nodes):
     return {kstr(n["kmer_lo"], n["kmer_hi"], k): list(map(int, n["counts"])) for n in nodes}
 
 
+def naive_components(g, k):
+    """.chain.components from the walked unitigs: node d of a unitig S (on
+    the strand it is printed on) is K = S[d:d+k], printed as canon(K), the
+    rep S[0:k], d and whether K is canonical, lines sorted by K
+    (print_chain_node, debruijn_chain_operations.hpp:427-442;
+    print_chain_nodes, compact_dbg_io.hpp:463-498)."""
+    rows = []
+    for rep, seq in nd.unitigs(g)[0]:
+        assert seq[:k] == rep
+        for d in range(len(seq) - k + 1):
+            K = seq[d:d + k]
+            rows.append((K, "%s\t%s\t%d\t%d\n" % (nd.canon(K), rep, d, int(K == nd.canon(K)))))
+    rows.sort()
+    return "".join(r for _, r in rows)
+
+
+def naive_k2(reads, k):
+    """The k2 multiset: distinct canonical (k-mer, biedge byte) pairs and
+    their instance counts, from the string-level records."""
+    from collections import Counter
+    return Counter((x, (nd.DNA16[a] << 4) | nd.DNA16[b]) for x, a, b in nd.records(reads, k))
+
+
+def oracle_k2(o, k):
+    lo, hi, e, c = o.k2()
+    got = {}
+    for i in range(len(e)):
+        key = (kstr(lo[i], hi[i], k), int(e[i]))
+        assert key not in got
+        got[key] = int(c[i])
+    return got
+
+
 def run(reads, k, th=None, variant=1, nthreads=1):
     o = Oracle(k, th, nthreads)
     b, off = pack(reads)
@@ -61,6 +94,26 @@ def test_oracle_matches_naive(case, tl):
     assert o.branch_fasta().decode() == nd.branch_fasta(exp)
     # P5 (compare as sets of (header, seq) and exact order when consistent)
     assert o.chain_fasta().decode() == nd.chain_fasta(exp)
+    # P4: every chain node's (canonical k-mer, rep, dist, strand), derived
+    # from the walked unitig strings
+    assert o.chain_components().decode() == naive_components(exp, k)
+    # the k2 counter before any filter
+    assert oracle_k2(o, k) == dict(naive_k2(reads, k))
+
+
+def test_planted_component_mistakes_are_caught():
+    """A strand flag or distance taken from the wrong end would change the
+    derived components: the derivation is not a restatement of the oracle."""
+    _, reads = random_reads(3, 2000, 300, 50, 0.01)
+    k = 11
+    o = run(reads, k, thresholds_L(2))
+    exp = nd.filtered(reads, k, thresholds_L(2))
+    good = naive_components(exp, k)
+    assert o.chain_components().decode() == good
+    lines = good.splitlines()
+    assert any(l.split("\t")[3] == "0" for l in lines) and any(l.split("\t")[3] == "1" for l in lines)
+    flipped = "".join("\t".join(l.split("\t")[:3] + [str(1 - int(l.split("\t")[3]))]) + "\n" for l in lines)
+    assert o.chain_components().decode() != flipped
 
 
 def test_multithreaded_oracle_is_identical():
