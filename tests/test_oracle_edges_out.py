@@ -112,3 +112,28 @@ def test_kmer_only_thresholds_do_not_break_edge_outputs(case):
     assert o.branch_edges().decode() == nd.branch_edges(g)
     lines = o.chain_edges().decode().splitlines()
     assert len(lines) == len(o.chain_fasta().decode().split(">")) - 1
+
+
+@pytest.mark.parametrize("case", CASES)
+@pytest.mark.parametrize("lo,hi", [(2, 65536), (3, 40)])
+def test_edge_filter_mode_outputs_match_recount(case, lo, hi):
+    """-E (block-construct driver): its .branch.edges / .chain.edges counts
+    come from recounting the reads' records after the k+1-mer filter
+    (count_edges / count_kmers, compact_dbg_stats.hpp:144-170, :645-689:
+    parse_and_filter_nodes, then saturating 16-bit counters for the selected
+    k-mers).  The string-level builder's edge-filtered graph holds exactly
+    those counts (kept records, kept edges, saturating), computed from the
+    read strings, so the oracle's edge-mode outputs are pinned by it."""
+    from oracle.oracle import Oracle
+    from helpers import pack
+    seed, G, n, L, err, k = case
+    _, reads = random_reads(seed, G, n, L, err)
+    th = [0, 65536, lo, hi, 0, 65536]
+    o = Oracle(k, np.array(th, np.uint64), 1, edge_filter=True)
+    b, off = pack(reads)
+    o.add_reads(b, off)
+    o.run(1)
+    g = nd.edge_filtered(reads, k, lo, hi)
+    assert o.stat("chain_edge_misses") == 0
+    assert o.branch_edges().decode() == nd.branch_edges(g)
+    assert o.chain_edges().decode() == nd.chain_edges(g, k)
