@@ -104,7 +104,7 @@ EXPORTS = (
     "bruno_server_get_stats", "bruno_server_destroy", "bruno_set_tuning", "bruno_get_tuning",
     "bruno_fastq_split", "bruno_parse_count_fastq", "bruno_parse_count_fastq_host", "bruno_estimate_distinct",
     "bruno_job_set_recycle", "bruno_get_branch_nodes", "bruno_get_chain_edges", "bruno_batch_get_branch_nodes",
-    "bruno_batch_get_chain_edges",
+    "bruno_batch_get_chain_edges", "bruno_fastq_split_noalloc",
 )
 
 _LIB = None
@@ -170,6 +170,7 @@ def lib():
         L.bruno_server_destroy.argtypes = [vp]
         L.bruno_set_tuning.argtypes = [ctypes.POINTER(Tuning)]
         L.bruno_fastq_split.argtypes = [vp, u64, vp, vp, u64p, u64p, vp]
+        L.bruno_fastq_split_noalloc.argtypes = [vp, u64, vp, vp, u64p, u64p, vp]
         L.bruno_parse_count_fastq.argtypes = [vp, vp, u64]
         L.bruno_parse_count_fastq_host.argtypes = [vp, vp, u64]
         L.bruno_job_set_recycle.argtypes = [vp, i]

@@ -26,6 +26,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "../../include/bruno_hip.h"
@@ -176,28 +177,26 @@ __global__ __launch_bounds__(256) void k_fq_copy(const uint8_t* s, const uint64_
   }
 }
 
-hipError_t scan_u64(const uint64_t* in, uint64_t* out_plus1, uint64_t n, hipStream_t st) {
-  /* out[0] = 0, out[i+1] = in[0] + ... + in[i] */
-  br::DevicePool& pool = br::DevicePool::current();
+/* out[0] = 0, out[i+1] = in[0] + ... + in[i], with the scan's temporary
+ * storage from mem */
+template <typename Mem>
+hipError_t scan_u64(const uint64_t* in, uint64_t* out_plus1, uint64_t n, hipStream_t st, Mem& mem) {
   hipError_t e = hipMemsetAsync(out_plus1, 0, 8, st);
   if (e != hipSuccess || n == 0) return e;
   size_t tmp = 0;
   e = hipcub::DeviceScan::InclusiveSum(nullptr, tmp, in, out_plus1 + 1, (int)n, st);
   if (e != hipSuccess) return e;
-  void* t = nullptr;
-  e = pool.alloc(&t, tmp + 16);
-  if (e != hipSuccess) return e;
-  e = hipcub::DeviceScan::InclusiveSum(t, tmp, in, out_plus1 + 1, (int)n, st);
-  hipError_t e2 = hipStreamSynchronize(st);
-  pool.release(t);
-  return e != hipSuccess ? e : e2;
+  uint8_t* t = nullptr;
+  if (!mem.get(&t, tmp + 16)) return hipErrorOutOfMemory;
+  return hipcub::DeviceScan::InclusiveSum(t, tmp, in, out_plus1 + 1, (int)n, st);
 }
 
-/* scratch blocks of one call, released when the call returns (after its
- * stream has finished with them) */
+/* Scratch of one call from the memory pool, released when the call returns
+ * (after its stream has finished with it). */
 struct Blocks {
   std::vector<void*> v;
   hipStream_t st;
+  static constexpr bruno_status SHORT = BRUNO_OOM;
   explicit Blocks(hipStream_t s) : st(s) {}
   ~Blocks() {
     if (!v.empty()) (void)hipStreamSynchronize(st);
@@ -211,13 +210,38 @@ struct Blocks {
     *p = (T*)q;
     return true;
   }
+  bool reads_fit(uint64_t) const { return true; }
 };
 
+/* Scratch of the allocation-free split: consecutive 256-byte-aligned pieces
+ * of one fixed device array (the caller's device's copy of `scratch`).
+ * Nothing is allocated or released. */
+constexpr uint64_t FQ_NOALLOC_MAX_BYTES = 1ULL << 30;   /* FASTQ bytes per call */
+constexpr uint64_t FQ_NOALLOC_MAX_READS = 1ULL << 21;   /* records per call     */
+constexpr uint64_t FQ_SCRATCH_BYTES = 80ULL << 20;
+static_assert(FQ_SCRATCH_BYTES >= 2 * (FQ_NOALLOC_MAX_BYTES / FQ_CHUNK + 2) * 8 + 256 + 4 * (FQ_NOALLOC_MAX_READS + 1) * 8 +
+                                      (8ULL << 20),
+              "scratch too small for its stated limits");
+struct Bump {
+  uint8_t* base;
+  uint64_t cap, used = 0;
+  static constexpr bruno_status SHORT = BRUNO_CAPACITY;
+  Bump(void* b, uint64_t c) : base((uint8_t*)b), cap(c) {}
+  template <typename T>
+  bool get(T** p, uint64_t bytes) {
+    const uint64_t need = (std::max<uint64_t>(bytes, 16) + 255) & ~255ULL;
+    if (need > cap - used) return false;
+    *p = (T*)(base + used);
+    used += need;
+    return true;
+  }
+  bool reads_fit(uint64_t R) const { return R <= FQ_NOALLOC_MAX_READS; }
+};
 
 /* One pass.  out_bases / out_offsets are either given (with capacities) or
  * obtained from alloc(bytes) once the sizes are known. */
-template <typename Alloc>
-bruno_status fastq_split_impl(const uint8_t* s, uint64_t bytes, char** out_bases, uint64_t** out_offs,
+template <typename Mem, typename Alloc>
+bruno_status fastq_split_impl(Mem& mem, const uint8_t* s, uint64_t bytes, char** out_bases, uint64_t** out_offs,
                               uint64_t cap_reads, uint64_t cap_bases, bool have_out, bool copy, Alloc alloc,
                               uint64_t* n_reads, uint64_t* n_bases, hipStream_t st,
                               uint64_t** out_start = nullptr, uint64_t** out_end = nullptr) {
@@ -230,12 +254,14 @@ bruno_status fastq_split_impl(const uint8_t* s, uint64_t bytes, char** out_bases
     return hipStreamSynchronize(st) == hipSuccess ? BRUNO_OK : BRUNO_INTERNAL;
   }
   const uint64_t nchunks = (bytes + FQ_CHUNK - 1) / FQ_CHUNK;
-  Blocks mem(st);
   unsigned long long *nl = nullptr, *base = nullptr;
   uint32_t* err = nullptr;
-  if (!mem.get(&nl, nchunks * 8) || !mem.get(&base, (nchunks + 1) * 8) || !mem.get(&err, 4)) return BRUNO_OOM;
+  if (!mem.get(&nl, nchunks * 8) || !mem.get(&base, (nchunks + 1) * 8) || !mem.get(&err, 4)) return Mem::SHORT;
   hipLaunchKernelGGL(k_fq_count, dim3((unsigned)nchunks), dim3(FQ_BLOCK), 0, st, s, bytes, nl);
-  if (scan_u64((const uint64_t*)nl, (uint64_t*)base, nchunks, st) != hipSuccess) return BRUNO_INTERNAL;
+  if (scan_u64((const uint64_t*)nl, (uint64_t*)base, nchunks, st, mem) != hipSuccess) {
+    (void)hipGetLastError();
+    return BRUNO_INTERNAL;
+  }
   unsigned long long total_nl = 0;
   uint8_t last = 0;
   if (hipMemcpyAsync(&total_nl, base + nchunks, 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
@@ -246,6 +272,7 @@ bruno_status fastq_split_impl(const uint8_t* s, uint64_t bytes, char** out_bases
   }
   const uint64_t T = total_nl + (last != '\n' ? 1 : 0);
   const uint64_t R = T / 4;
+  if (!mem.reads_fit(R)) return Mem::SHORT;
   FqOut o{};
   o.n_records = R;
   o.n_lines = T;
@@ -259,9 +286,9 @@ bruno_status fastq_split_impl(const uint8_t* s, uint64_t bytes, char** out_bases
     *out_start = o.seq_start;
     *out_end = o.seq_end;
   } else if (!mem.get(&o.seq_start, R * 8) || !mem.get(&o.seq_end, R * 8)) {
-    return BRUNO_OOM;
+    return Mem::SHORT;
   }
-  if (!mem.get(&lens, R * 8)) return BRUNO_OOM;
+  if (!mem.get(&lens, R * 8)) return Mem::SHORT;
   uint64_t* offs = nullptr;
   if (have_out) {
     if (R > cap_reads) return BRUNO_CAPACITY;
@@ -270,13 +297,16 @@ bruno_status fastq_split_impl(const uint8_t* s, uint64_t bytes, char** out_bases
     if (!(offs = (uint64_t*)alloc((R + 1) * 8))) return BRUNO_OOM;
     *out_offs = offs;
   } else if (!mem.get(&offs, (R + 1) * 8)) {
-    return BRUNO_OOM;
+    return Mem::SHORT;
   }
   hipLaunchKernelGGL(k_fq_mark, dim3((unsigned)nchunks), dim3(FQ_BLOCK), 0, st, s, bytes,
                      (const unsigned long long*)base, o);
   if (R)
     hipLaunchKernelGGL(k_fq_len, dim3((unsigned)((R + 255) / 256)), dim3(256), 0, st, o.seq_start, o.seq_end, R, lens);
-  if (scan_u64(lens, offs, R, st) != hipSuccess) return BRUNO_INTERNAL;
+  if (scan_u64(lens, offs, R, st, mem) != hipSuccess) {
+    (void)hipGetLastError();
+    return BRUNO_INTERNAL;
+  }
   uint32_t herr = 0;
   uint64_t nb = 0;
   if (hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
@@ -320,8 +350,64 @@ extern "C" bruno_status bruno_fastq_split(const char* d_fastq, uint64_t bytes, c
   char* ob = d_out_bases;
   uint64_t* oo = d_out_offsets;
   const uint64_t cr = *n_reads, cb = *n_bases;
-  return fastq_split_impl((const uint8_t*)d_fastq, bytes, &ob, &oo, cr, cb, have, have,
+  Blocks mem((hipStream_t)stream);
+  return fastq_split_impl(mem, (const uint8_t*)d_fastq, bytes, &ob, &oo, cr, cb, have, have,
                           [](uint64_t) -> void* { return nullptr; }, n_reads, n_bases, (hipStream_t)stream);
+}
+
+/* The device's copy of the allocation-free split's scratch (every device
+ * that loads this library holds one until the process ends). */
+static __device__ __align__(256) uint8_t scratch[FQ_SCRATCH_BYTES];
+
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[64] = {};
+
+/* Address of the current device's scratch, looked up once per device with
+ * that device current (a host-side use of the array's name is not a device
+ * address). */
+bruno_status scratch_of_current_device(void** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+    (void)hipGetLastError();
+    return BRUNO_NO_DEVICE;
+  }
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return BRUNO_INTERNAL;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return BRUNO_OK;
+}
+}  // namespace
+
+extern "C" bruno_status bruno_fastq_split_noalloc(const char* d_fastq, uint64_t bytes, char* d_out_bases,
+                                                  uint64_t* d_out_offsets, uint64_t* n_reads, uint64_t* n_bases,
+                                                  void* stream) {
+  if (!n_reads || !n_bases || (bytes && !d_fastq)) return BRUNO_BAD_INPUT;
+  if ((d_out_bases == nullptr) != (d_out_offsets == nullptr)) return BRUNO_BAD_INPUT;
+  if (bytes > FQ_NOALLOC_MAX_BYTES) return BRUNO_CAPACITY;
+  void* sp = nullptr;
+  bruno_status s = scratch_of_current_device(&sp);
+  if (s != BRUNO_OK) return s;
+  const bool have = d_out_bases != nullptr;
+  char* ob = d_out_bases;
+  uint64_t* oo = d_out_offsets;
+  const uint64_t cr = *n_reads, cb = *n_bases;
+  Bump mem(sp, FQ_SCRATCH_BYTES);
+  s = fastq_split_impl(mem, (const uint8_t*)d_fastq, bytes, &ob, &oo, cr, cb, have, have,
+                       [](uint64_t) -> void* { return nullptr; }, n_reads, n_bases, (hipStream_t)stream);
+  /* the next call may reuse the scratch at once, from any stream */
+  if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess) {
+    (void)hipGetLastError();
+    if (s == BRUNO_OK) s = BRUNO_INTERNAL;
+  }
+  return s;
 }
 
 /* Record ranges only: start / end byte of every sequence line in the file,
@@ -338,7 +424,8 @@ bruno_status bruno_fastq_ranges(const char* d_fastq, uint64_t bytes, void* (*all
     *n_reads = *n_bases = 0;
     return BRUNO_OK;
   }
-  return fastq_split_impl((const uint8_t*)d_fastq, bytes, &ob, &oo, 0, 0, false, false,
+  Blocks mem((hipStream_t)stream);
+  return fastq_split_impl(mem, (const uint8_t*)d_fastq, bytes, &ob, &oo, 0, 0, false, false,
                           [&](uint64_t b) { return alloc(ctx, b); }, n_reads, n_bases, (hipStream_t)stream, starts,
                           ends);
 }
@@ -351,6 +438,7 @@ bruno_status bruno_fastq_split_alloc(const char* d_fastq, uint64_t bytes, void* 
   if (!n_reads || !n_bases || !out_bases || !out_offsets || !alloc || (bytes && !d_fastq)) return BRUNO_BAD_INPUT;
   *out_bases = nullptr;
   *out_offsets = nullptr;
-  return fastq_split_impl((const uint8_t*)d_fastq, bytes, out_bases, out_offsets, 0, 0, false, true,
+  Blocks mem((hipStream_t)stream);
+  return fastq_split_impl(mem, (const uint8_t*)d_fastq, bytes, out_bases, out_offsets, 0, 0, false, true,
                           [&](uint64_t b) { return alloc(ctx, b); }, n_reads, n_bases, (hipStream_t)stream);
 }

@@ -372,19 +372,20 @@ def job_first_read(cfgs):
     return f
 
 
-def fastq_split_device(fastq_ptr, nbytes, stream=None):
+def fastq_split_device(fastq_ptr, nbytes, stream=None, noalloc=False):
     """Split FASTQ bytes in device memory into (bases, offsets) host arrays
-    (through device buffers; for tests)."""
+    (through device buffers; for tests).  noalloc: through
+    bruno_fastq_split_noalloc (the output buffers are still the caller's)."""
     from .device import DeviceBuffer
     L = lib()
+    fn, name = ((L.bruno_fastq_split_noalloc, "bruno_fastq_split_noalloc") if noalloc
+                else (L.bruno_fastq_split, "bruno_fastq_split"))
     nr, nb = ctypes.c_uint64(0), ctypes.c_uint64(0)
-    check(L.bruno_fastq_split(ctypes.c_void_p(fastq_ptr), int(nbytes), None, None, ctypes.byref(nr), ctypes.byref(nb),
-                              stream), "bruno_fastq_split")
+    check(fn(ctypes.c_void_p(fastq_ptr), int(nbytes), None, None, ctypes.byref(nr), ctypes.byref(nb), stream), name)
     db, do = DeviceBuffer(nb.value + 1), DeviceBuffer((nr.value + 1) * 8)
     try:
-        check(L.bruno_fastq_split(ctypes.c_void_p(fastq_ptr), int(nbytes), ctypes.c_void_p(db.ptr),
-                                  ctypes.c_void_p(do.ptr), ctypes.byref(nr), ctypes.byref(nb), stream),
-              "bruno_fastq_split")
+        check(fn(ctypes.c_void_p(fastq_ptr), int(nbytes), ctypes.c_void_p(db.ptr), ctypes.c_void_p(do.ptr),
+                 ctypes.byref(nr), ctypes.byref(nb), stream), name)
         return db.to_numpy(np.uint8, nb.value), do.to_numpy(np.uint64, nr.value + 1)
     finally:
         db.free()

@@ -186,6 +186,20 @@ bruno_status bruno_estimate_distinct(int k, const char* d_bases, const uint64_t*
  * with a non-ACGT base stay in the output; bruno_parse_count drops them. */
 bruno_status bruno_fastq_split(const char* d_fastq, uint64_t bytes, char* d_out_bases, uint64_t* d_out_offsets,
                                uint64_t* n_reads, uint64_t* n_bases, void* stream);
+/* bruno_fastq_split without any device allocation: same arguments, results
+ * and errors, but its scratch (per-chunk line counts, sequence-line
+ * bounds, prefix-sum storage) is one fixed 80 MiB device array that the
+ * library holds on every GPU it runs on until the process ends, instead of
+ * blocks from the memory pool.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Limits: at most 1 GiB of FASTQ bytes and 2^21
+ * records per call; larger inputs return BRUNO_CAPACITY.  Every call on a
+ * GPU shares that GPU's scratch: calls that overlap on one GPU (from
+ * different host threads or streams) race, so the caller serialises them.
+ * The call returns after its work on `stream` has finished.  No
+ * counterpart in the reference. */
+bruno_status bruno_fastq_split_noalloc(const char* d_fastq, uint64_t bytes, char* d_out_bases,
+                                       uint64_t* d_out_offsets, uint64_t* n_reads, uint64_t* n_bases, void* stream);
 /* parse_file_data + compute_biedge_freqencies on a whole FASTQ file:
  * device bytes, or host bytes copied first. */
 bruno_status bruno_parse_count_fastq(bruno_job* job, const char* d_fastq, uint64_t bytes);
