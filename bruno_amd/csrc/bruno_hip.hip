@@ -1181,6 +1181,47 @@ __device__ __forceinline__ void put_kmer(bruno_node& o, KT k) {
 }
 
 /* P1 counters (before filter) of slots [s0, s1); out == nullptr counts only */
+/* Moving a table into a larger one (a job sized from a sketch of its first
+ * batches whose later batches hold more distinct k-mers): every k-mer is
+ * inserted into the new table with its k2 entries and their counts, so its
+ * probe window follows the new size.  remap[old slot] = new slot. */
+template <typename KT>
+__global__ __launch_bounds__(BLOCK) void k_move_slots(const Slot<KT>* O, uint64_t ocap, Slot<KT>* T, uint64_t span,
+                                                     uint32_t rbits, Spill* sp, uint64_t spcap, uint64_t* remap,
+                                                     Counters* ctr) {
+  for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < ocap; i += (uint64_t)gridDim.x * BLOCK) {
+    const Slot<KT>* p = O + i;
+    if (!SlotOps<KT>::occupied(p)) continue;
+    int claimed = 0;
+    const uint64_t s = SlotOps<KT>::find_or_insert(T, span, rbits, SlotOps<KT>::kmer(p), &ctr->err, &claimed);
+    remap[i] = s;
+    if (s == ~0ULL) continue;
+#pragma unroll
+    for (int q = 0; q < Slot<KT>::NSUB; ++q) {
+      const uint64_t v = p->sub[q];
+      if (v != EMPTY64)
+        add_k2<KT>(T, s, (uint8_t)((v >> 48) - 1), sp, spcap, &ctr->err, v & 0xFFFFFFFFFFFFULL);
+    }
+  }
+}
+
+/* the old side table's entries (keyed by old slot) into the new table */
+template <typename KT>
+__global__ __launch_bounds__(BLOCK) void k_move_spill(const Spill* O, uint64_t ocap, const uint64_t* remap,
+                                                     Slot<KT>* T, Spill* sp, uint64_t spcap, Counters* ctr) {
+  for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < ocap; i += (uint64_t)gridDim.x * BLOCK) {
+    const uint64_t key = O[i].key;
+    if (key == EMPTY64) continue;
+    const uint64_t s = remap[key >> 2];
+    if (s == ~0ULL) continue;
+    for (int q = 0; q < SPILL_SUB; ++q) {
+      const uint64_t v = O[i].sub[q];
+      if (v != EMPTY64)
+        add_k2<KT>(T, s, (uint8_t)((v >> 48) - 1), sp, spcap, &ctr->err, v & 0xFFFFFFFFFFFFULL);
+    }
+  }
+}
+
 template <typename KT>
 __global__ __launch_bounds__(BLOCK) void k_get_counts(const Slot<KT>* T, uint64_t s0, uint64_t s1, const Spill* sp,
                                                       uint64_t spcap, bruno_node* out, uint64_t out_cap, Counters* ctr) {
@@ -1470,6 +1511,12 @@ struct bruno_job {
   uint64_t* cbits = nullptr;
   uint64_t* sbits = nullptr;
   Spill* spill = nullptr;
+  /* capacity 0 at create: the table is sized from a HyperLogLog sketch of
+   * every batch counted since the reset (SKETCH_M registers, merged), and
+   * moved into a larger table when a later batch needs more slots */
+  bool auto_size = false;
+  uint32_t* sketch = nullptr;
+  uint64_t auto_bases = 0;      /* bases of the batches sketched since the reset */
   Counters* ctr = nullptr;      /* device */
   Counters* hctr = nullptr;     /* pinned host mirror */
   uint64_t device_bytes = 0;
@@ -1529,7 +1576,6 @@ struct bruno_job {
   bool recycle = false;
   hipStream_t side = nullptr;
   hipEvent_t ev_chains = nullptr, ev_cleared = nullptr;
-  hipEvent_t ev_region[5] = {nullptr, nullptr, nullptr, nullptr, nullptr}; /* count by regions: two streams */
   bool clear_pending = false;
   bruno_phase_ms ms{};
 
@@ -1624,7 +1670,7 @@ void clear_table(bruno_job* j, hipStream_t st) {
     fill64(j, j->table, j->cap * slot_bytes(j) / 8, EMPTY64, st);
     j->table_valid = true;
   }
-  fill64(j, j->spill, j->spcap * sizeof(Spill) / 8, 0, st);
+  fill64(j, j->spill, j->spcap * sizeof(Spill) / 8, EMPTY64, st);
   fill64(j, j->d_jstat, 2 * j->n_jobs, 0, st);
 }
 
@@ -1660,10 +1706,10 @@ bruno_status alloc_table(bruno_job* j, const uint64_t* caps) {
   j->rbits = 0;
   j->rspan = 0;
   const uint32_t cr = tuning().count_regions;
-  if (nj == 1 && j->region_ok && j->words == 1 && cr &&
-      j->seg[1] >= (cr >= 2 ? REGION_MIN_CAP_ANY : REGION_MIN_CAP)) {
+  if (nj == 1 && j->region_ok && cr && j->seg[1] >= (cr >= 2 ? REGION_MIN_CAP_ANY : REGION_MIN_CAP)) {
     uint32_t b = REC_BUCKET_BITS;
-    const uint64_t most = tuning().region_slots;
+    /* the knob counts 32-byte slots: a 64-byte-slot region holds half */
+    const uint64_t most = j->words == 1 ? tuning().region_slots : tuning().region_slots / 2;
     while ((j->seg[1] + (1ULL << b) - 1) >> b > most) ++b;
     j->rbits = b;
     j->rspan = (j->seg[1] + (1ULL << b) - 1) >> b;
@@ -1672,7 +1718,9 @@ bruno_status alloc_table(bruno_job* j, const uint64_t* caps) {
   const uint64_t cap = j->seg[nj];
   j->cap = cap;
   TRY(dev_alloc(j, &j->table, cap * slot_bytes(j)));
-  j->spcap = std::max<uint64_t>(1024, cap / 8);
+  /* side-table records: one per 32 slots (the same bytes as a quarter of
+   * the slots' sub-entries) */
+  j->spcap = std::max<uint64_t>(1024, cap / 32);
   TRY(dev_alloc(j, (void**)&j->spill, j->spcap * sizeof(Spill)));
   TRY(dev_alloc(j, (void**)&j->d_seg, (nj + 1) * sizeof(uint64_t)));
   TRY(dev_alloc(j, (void**)&j->d_jstat, 2 * nj * sizeof(unsigned long long)));
@@ -1682,7 +1730,7 @@ bruno_status alloc_table(bruno_job* j, const uint64_t* caps) {
     fill64(j, j->table, cap * slot_bytes(j) / 8, EMPTY64);
     j->table_valid = true;
   }
-  fill64(j, j->spill, j->spcap * sizeof(Spill) / 8, 0);
+  fill64(j, j->spill, j->spcap * sizeof(Spill) / 8, EMPTY64);
   fill64(j, j->d_jstat, 2 * nj, 0);
   return BRUNO_OK;
 }
@@ -1709,12 +1757,13 @@ void ensure_table_valid(bruno_job* j) {
 }
 
 bool persistent(const bruno_job* j, const void* p) {
-  return p == j->table || p == j->spill || p == j->d_seg || p == j->d_jstat;
+  return p == j->table || p == j->spill || p == j->d_seg || p == j->d_jstat || (p && p == j->sketch);
 }
 uint64_t persistent_bytes(const bruno_job* j) {
-  if (!j->table) return 0;
+  const uint64_t sk = j->sketch ? SKETCH_M * 4 : 0;
+  if (!j->table) return sk;
   return j->cap * slot_bytes(j) + j->spcap * sizeof(Spill) + (j->n_jobs + 1) * sizeof(uint64_t) +
-         2 * j->n_jobs * sizeof(unsigned long long);
+         2 * j->n_jobs * sizeof(unsigned long long) + sk;
 }
 
 /* HyperLogLog estimate from SKETCH_M registers of a 1-in-64 hash sample,
@@ -1784,46 +1833,145 @@ bruno_status estimate_distinct(int K, const char* d_bases, const uint64_t* d_off
   return s;
 }
 
+/* Move a plain job's counted table into one of `want` slots (more than it
+ * has): a new table (region layout if it is large enough), every k-mer and
+ * k2 entry inserted again with its count, then the old table released.
+ * Only between counts: the filter has not run. */
+bruno_status grow_table(bruno_job* j, uint64_t want) {
+  HIP_CHECK(hipStreamSynchronize(j->stream));
+  void* otab = j->table;
+  void* osp = j->spill;
+  void* oseg = j->d_seg;
+  void* ojs = j->d_jstat;
+  const uint64_t ocap = j->cap, ospcap = j->spcap;
+  const uint32_t orbits = j->rbits;
+  const uint64_t orspan = j->rspan;
+  const std::vector<uint64_t> oseg_h = j->seg;
+  j->table = j->spill = nullptr;
+  j->d_seg = nullptr;
+  j->d_jstat = nullptr;
+  bruno_status st = alloc_table(j, &want);
+  uint64_t* remap = nullptr;
+  if (st == BRUNO_OK) st = dev_alloc(j, (void**)&remap, ocap * 8);
+  if (st != BRUNO_OK) {
+    /* keep the old table */
+    void* p;
+    p = j->table; dev_free(j, p, j->cap * slot_bytes(j));
+    p = j->spill; dev_free(j, p, j->spcap * sizeof(Spill));
+    p = j->d_seg; dev_free(j, p, (j->n_jobs + 1) * 8);
+    p = j->d_jstat; dev_free(j, p, 2 * j->n_jobs * 8);
+    j->table = otab;
+    j->spill = (Spill*)osp;
+    j->d_seg = (uint64_t*)oseg;
+    j->d_jstat = (unsigned long long*)ojs;
+    j->cap = ocap;
+    j->spcap = ospcap;
+    j->rbits = orbits;
+    j->rspan = orspan;
+    j->seg = oseg_h;
+    j->table_valid = true;
+    return st;
+  }
+  ensure_table_valid(j);
+  const unsigned g = grid_for(ocap, BLOCK, 1u << 15);
+  if (j->words == 1) {
+    hipLaunchKernelGGL((k_move_slots<uint64_t>), dim3(g), dim3(BLOCK), 0, j->stream, (const Slot<uint64_t>*)otab,
+                       ocap, (Slot<uint64_t>*)j->table, probe_span(j), j->rbits, j->spill, j->spcap, remap, j->ctr);
+    hipLaunchKernelGGL((k_move_spill<uint64_t>), dim3(grid_for(ospcap, BLOCK, 1u << 15)), dim3(BLOCK), 0, j->stream,
+                       (const Spill*)osp, ospcap, remap, (Slot<uint64_t>*)j->table, j->spill, j->spcap, j->ctr);
+  } else {
+    hipLaunchKernelGGL((k_move_slots<u128>), dim3(g), dim3(BLOCK), 0, j->stream, (const Slot<u128>*)otab, ocap,
+                       (Slot<u128>*)j->table, probe_span(j), j->rbits, j->spill, j->spcap, remap, j->ctr);
+    hipLaunchKernelGGL((k_move_spill<u128>), dim3(grid_for(ospcap, BLOCK, 1u << 15)), dim3(BLOCK), 0, j->stream,
+                       (const Spill*)osp, ospcap, remap, (Slot<u128>*)j->table, j->spill, j->spcap, j->ctr);
+  }
+  HIP_CHECK(hipMemcpyAsync(j->d_jstat, ojs, 2 * j->n_jobs * 8, hipMemcpyDeviceToDevice, j->stream));
+  bruno_status s2 = sync_counters(j);
+  void* p = remap;
+  dev_free(j, p, ocap * 8);
+  dev_free(j, otab, ocap * slot_bytes(j));
+  dev_free(j, osp, ospcap * sizeof(Spill));
+  dev_free(j, oseg, (j->n_jobs + 1) * 8);
+  dev_free(j, ojs, 2 * j->n_jobs * 8);
+  j->bits_fresh = false;
+  j->by_regions = false;
+  return s2;
+}
+
+/* Size (capacity-0 jobs) or grow the table for one more batch: its
+ * canonical k-mers are sketched into the job's registers, merged with
+ * every batch since the reset, and the union's estimate gives the slots
+ * needed (instances bounded by the bases so far).  A table that is short
+ * grows to at least half again its size, so a run of batches moves it few
+ * times.  The reference's hash maps grow as they fill instead. */
+bruno_status autosize(bruno_job* j, const char* d_bases, const uint64_t* d_offsets, uint64_t n_reads,
+                      uint64_t n_bases, const uint64_t* d_starts, const uint64_t* d_ends) {
+  if (!j->sketch) {
+    TRY(dev_alloc(j, (void**)&j->sketch, SKETCH_M * 4));
+    HIP_CHECK(hipMemsetAsync(j->sketch, 0, SKETCH_M * 4, j->stream));
+  }
+  j->auto_size = true;
+  if (j->K <= 31) TRY(sketch_t<uint64_t>(j->K, d_bases, d_offsets, n_reads, j->sketch, nullptr, j->stream, d_starts, d_ends));
+  else TRY(sketch_t<u128>(j->K, d_bases, d_offsets, n_reads, j->sketch, nullptr, j->stream, d_starts, d_ends));
+  std::vector<uint32_t> h(SKETCH_M);
+  HIP_CHECK(hipMemcpyAsync(h.data(), j->sketch, SKETCH_M * 4, hipMemcpyDeviceToHost, j->stream));
+  HIP_CHECK(hipStreamSynchronize(j->stream));
+  j->auto_bases += n_bases;
+  const uint64_t want = capacity_from_estimate(hll_estimate(h.data()), j->auto_bases, j->load_factor);
+  if (j->cap == 0) return alloc_table(j, &want);
+  if (want > j->cap) return grow_table(j, std::max<uint64_t>(want, j->cap + j->cap / 2));
+  return BRUNO_OK;
+}
+
 /* Count one batch of reads into a region-layout table (regions.h): bucketed
- * instance records, a sort of each bucket by region, one on-chip build per
- * region.  *done = false (and nothing counted) when the record buffers do
- * not fit or a bucket overflowed its share; the caller then counts with
- * atomics. */
+ * instance records, two multisplit passes that group each bucket's records
+ * by region (partition.h), one on-chip build per region.  *done = false
+ * (and nothing counted) when the record buffers do not fit, a bucket
+ * overflowed its share, or the table has more regions than two passes
+ * split; the caller then counts with atomics. */
+/* record staging per bucket in the parser's LDS: 48 KB of records */
+template <typename KT> constexpr int rec_stage() { return sizeof(typename RecOf<KT>::T) == 8 ? REC_STAGE : REC_STAGE / 2; }
+
+template <typename KT>
 bruno_status count_by_regions(bruno_job* j, const ParseArgs& a, bool* done) {
+  typedef typename RecOf<KT>::T R;
+  constexpr int STAGE = rec_stage<KT>();
+  constexpr uint64_t RB = sizeof(R);
   *done = false;
   const uint64_t n_reads = a.n_reads;
-  /* instance bound -> per-bucket capacity (hash buckets are even up to
-   * sampling noise; a quarter more plus a constant leaves a wide margin,
-   * plus the padding of one open chunk per workgroup) */
-  constexpr int NS = 2 * REC_BUCKETS + 3;
+  const int rb = (int)j->rbits - REC_BUCKET_BITS; /* region bits inside a bucket */
+  const int b1 = std::min(std::max(rb - 8, 0), SPLIT1_MAX_BITS), b2 = rb - b1;
+  if (rb < 0 || b2 > SPLIT2_MAX_BITS) return BRUNO_OK;
+  const uint64_t n_sb = (uint64_t)REC_BUCKETS << b1, nreg = 1ULL << j->rbits;
+  /* scratch: front / back per bucket, instances and dropped reads, the
+   * instance bound, the (bucket, digit) counts and the first pass's cursors */
+  const uint64_t NS = 2 * REC_BUCKETS + 3 + 2 * n_sb;
   unsigned long long* scratch = nullptr;
   TRY(dev_alloc(j, (void**)&scratch, NS * 8));
   unsigned long long *front = scratch, *back = scratch + REC_BUCKETS, *stat = scratch + 2 * REC_BUCKETS,
-                     *bound = scratch + 2 * REC_BUCKETS + 2;
+                     *bound = stat + 2, *hist = bound + 1, *cursor = hist + n_sb;
+  void* sp = scratch;
   HIP_CHECK(hipMemsetAsync(scratch, 0, NS * 8, j->stream));
   hipLaunchKernelGGL(k_count_instances, dim3(grid_for(n_reads)), dim3(BLOCK), 0, j->stream, a, bound);
   unsigned long long n_bound = 0;
   HIP_CHECK(hipMemcpyAsync(&n_bound, bound, 8, hipMemcpyDeviceToHost, j->stream));
   HIP_CHECK(hipStreamSynchronize(j->stream));
-  /* one workgroup per CU slot (persistent); chunks of up to 2048 records,
-   * smaller for small inputs so padding stays under ~1/8 of the records */
-  constexpr int rbs = REC_BS;
   /* reads per wave between appends: a round adds ~WPB x (instances per
    * read) / 32 records to each bucket, which must fit the room left after
    * an append (a quarter again for spread); reads too long for one per round go
    * to the atomic count (the stage would overflow into per-record atomics) */
+  constexpr int rbs = REC_BS;
   const double per_read = (double)n_bound / (double)std::max<uint64_t>(1, n_reads);
   const double per_round = (REC_BS / 64) * per_read / REC_BUCKETS * 1.25;
-  const uint32_t rpw = (uint32_t)std::min<double>(REC_RPW, (REC_STAGE - REC_FLUSH) / std::max(1.0, per_round));
+  const uint32_t rpw = (uint32_t)std::min<double>(REC_RPW, (STAGE - REC_FLUSH) / std::max(1.0, per_round));
   if (rpw == 0) {
-    void* sp0 = scratch;
-    dev_free(j, sp0, NS * 8);
+    dev_free(j, sp, NS * 8);
     return BRUNO_OK;
   }
   /* persistent: exactly the workgroups that fit on the device at once */
   static int per_cu = 0;
   if (per_cu == 0) {
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_parse_records<REC_BS, REC_RPW, REC_STAGE, REC_FLUSH>,
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_parse_records<KT, REC_BS, REC_RPW, STAGE, REC_FLUSH>,
                                                      rbs, 0) != hipSuccess ||
         per_cu <= 0) {
       (void)hipGetLastError();
@@ -1832,79 +1980,83 @@ bruno_status count_by_regions(bruno_job* j, const ParseArgs& a, bool* done) {
   }
   const uint64_t groups = std::max<uint64_t>(
       1, std::min<uint64_t>((n_reads + rbs / 64 * rpw - 1) / (rbs / 64 * rpw), (uint64_t)device_cus() * per_cu));
+  /* chunks of up to 2048 records, smaller for small inputs so padding
+   * stays under ~1/8 of the records; a flush appends at most REC_STAGE
+   * records, so a chunk holds at least that many */
   uint32_t chunk = 2048;
-  /* a flush appends at most REC_STAGE records, so a chunk holds at least that many */
-  while (chunk / 2 >= (uint32_t)REC_STAGE && (uint64_t)chunk * groups * REC_BUCKETS * 8 > n_bound) chunk >>= 1;
+  while (chunk / 2 >= (uint32_t)STAGE && (uint64_t)chunk * groups * REC_BUCKETS * 8 > n_bound) chunk >>= 1;
   const uint64_t bcap = n_bound / REC_BUCKETS + n_bound / (4 * REC_BUCKETS) + 65536 + groups * chunk;
-  void* sp = scratch;
-  uint64_t* recs = nullptr;
-  if (dev_alloc(j, (void**)&recs, REC_BUCKETS * bcap * 8) != BRUNO_OK) {
+  R* recs = nullptr;
+  if (dev_alloc(j, (void**)&recs, REC_BUCKETS * bcap * RB) != BRUNO_OK) {
     dev_free(j, sp, NS * 8);
     return BRUNO_OK;
   }
   void* rp = recs;
-  const uint32_t rb = j->rbits - REC_BUCKET_BITS; /* region bits inside a bucket */
-  RecArgs ra{recs, bcap, front, back, stat, chunk, (int)rb, rpw};
-  hipLaunchKernelGGL((k_parse_records<REC_BS, REC_RPW, REC_STAGE, REC_FLUSH>), dim3((unsigned)groups), dim3(rbs), 0,
+  RecArgs ra{recs, bcap, front, back, stat, hist, chunk, b1, rpw};
+  hipLaunchKernelGGL((k_parse_records<KT, REC_BS, REC_RPW, STAGE, REC_FLUSH>), dim3((unsigned)groups), dim3(rbs), 0,
                      j->stream, a, ra);
-  unsigned long long hcur[2 * REC_BUCKETS];
-  HIP_CHECK(hipMemcpyAsync(hcur, front, 2 * REC_BUCKETS * 8, hipMemcpyDeviceToHost, j->stream));
+  std::vector<unsigned long long> hcur(2 * REC_BUCKETS + n_sb);
+  HIP_CHECK(hipMemcpyAsync(hcur.data(), front, 2 * REC_BUCKETS * 8, hipMemcpyDeviceToHost, j->stream));
+  HIP_CHECK(hipMemcpyAsync(hcur.data() + 2 * REC_BUCKETS, hist, n_sb * 8, hipMemcpyDeviceToHost, j->stream));
   HIP_CHECK(hipStreamSynchronize(j->stream));
-  uint64_t nmax = 0;
   bool over = false;
-  for (int b = 0; b < REC_BUCKETS; ++b) {
-    over |= hcur[b] + hcur[REC_BUCKETS + b] > bcap;
-    nmax = std::max<uint64_t>(nmax, hcur[b] + hcur[REC_BUCKETS + b]);
-  }
-  if (over) {
-    dev_free(j, rp, REC_BUCKETS * bcap * 8);
+  for (int b = 0; b < REC_BUCKETS; ++b) over |= hcur[b] + hcur[REC_BUCKETS + b] > bcap;
+  /* sub-bucket starts: prefix of the (bucket, digit) counts */
+  std::vector<uint64_t> sb(n_sb + 1, 0);
+  for (uint64_t q = 0; q < n_sb; ++q) sb[q + 1] = sb[q] + hcur[2 * REC_BUCKETS + q];
+  const uint64_t N = sb[n_sb];
+  R* recs2 = nullptr;
+  uint64_t* d_sb = nullptr;
+  uint64_t* rstart = nullptr;
+  if (over || dev_alloc(j, (void**)&recs2, std::max<uint64_t>(1, N) * RB) != BRUNO_OK ||
+      dev_alloc(j, (void**)&d_sb, (n_sb + 1) * 8) != BRUNO_OK ||
+      dev_alloc(j, (void**)&rstart, (nreg + 1) * 8) != BRUNO_OK) {
+    void* p = rstart;
+    dev_free(j, p, (nreg + 1) * 8);
+    p = d_sb;
+    dev_free(j, p, (n_sb + 1) * 8);
+    p = recs2;
+    dev_free(j, p, std::max<uint64_t>(1, N) * RB);
+    dev_free(j, rp, REC_BUCKETS * bcap * RB);
     dev_free(j, sp, NS * 8);
     return BRUNO_OK;
   }
-  /* records appended from the back join the chunks */
+  HIP_CHECK(hipMemcpyAsync(d_sb, sb.data(), (n_sb + 1) * 8, hipMemcpyHostToDevice, j->stream));
+  HIP_CHECK(hipMemcpyAsync(cursor, sb.data(), n_sb * 8, hipMemcpyHostToDevice, j->stream));
+  HIP_CHECK(hipMemcpyAsync(rstart + nreg, &sb[n_sb], 8, hipMemcpyHostToDevice, j->stream));
+  /* pass 1: the chunked front of every bucket and the records appended
+   * from its back, by the top b1 region bits */
+  Split1Args s1{};
+  s1.in = recs;
+  s1.b1 = b1;
+  s1.cursor = cursor;
+  s1.out = recs2;
+  s1.out_n = N;
+  s1.err = &j->ctr->err;
+  uint64_t tiles = 0;
   for (int b = 0; b < REC_BUCKETS; ++b) {
-    const uint64_t nf = hcur[b], nb = hcur[REC_BUCKETS + b];
-    if (!nb) continue;
-    uint64_t* tmpb = nullptr;
-    TRY(dev_alloc(j, (void**)&tmpb, nb * 8));
-    uint64_t* area = recs + (uint64_t)b * bcap;
-    HIP_CHECK(hipMemcpyAsync(tmpb, area + bcap - nb, nb * 8, hipMemcpyDeviceToDevice, j->stream));
-    HIP_CHECK(hipMemcpyAsync(area + nf, tmpb, nb * 8, hipMemcpyDeviceToDevice, j->stream));
-    void* tp = tmpb;
-    dev_free(j, tp, nb * 8);
-    hcur[b] = nf + nb;
+    const uint64_t ranges[2][2] = {{(uint64_t)b * bcap, (uint64_t)b * bcap + hcur[b]},
+                                   {(uint64_t)(b + 1) * bcap - hcur[REC_BUCKETS + b], (uint64_t)(b + 1) * bcap}};
+    for (auto& r : ranges) {
+      if (r[1] <= r[0]) continue;
+      s1.seg[s1.n_seg++] = SplitSeg{r[0], r[1], tiles, (uint32_t)b, 0};
+      tiles += (r[1] - r[0] + SPLIT_TILE - 1) / SPLIT_TILE;
+    }
   }
-  /* sort each bucket on its region bits (the low rb bits of a record) on
-   * the job's stream and build its regions on the side stream, so the sort
-   * of bucket b + 1 overlaps the build of bucket b (two sort buffers) */
-  if (!j->side) j->side = shared_side_stream();
-  if (!j->ev_region[0]) {
-    for (int i = 0; i < 5; ++i)
-      if (hipEventCreateWithFlags(&j->ev_region[i], hipEventDisableTiming) != hipSuccess) {
-        (void)hipGetLastError();
-        j->ev_region[i] = nullptr;
-      }
-  }
-  bool two = j->side != nullptr;
-  for (int i = 0; i < 5; ++i) two = two && j->ev_region[i];
-  hipStream_t bst = two ? j->side : j->stream;
-  const uint64_t nreg = 1ULL << rb;
-  uint64_t* alt[2] = {nullptr, nullptr};
-  const uint64_t altn = std::max<uint64_t>(1, nmax);
-  TRY(dev_alloc(j, (void**)&alt[0], altn * 8));
-  if (two) TRY(dev_alloc(j, (void**)&alt[1], altn * 8));
-  else alt[1] = alt[0];
-  size_t tmp = 0;
-  {
-    hipcub::DoubleBuffer<uint64_t> db(recs, alt[0]);
-    HIP_CHECK(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp, db, (int)altn, 0, (int)rb, j->stream));
-  }
-  void* t = nullptr;
-  TRY(dev_alloc(j, &t, tmp));
+  s1.n_tiles = tiles;
+  const unsigned split_grid = device_cus() * 4;
+  if (tiles)
+    hipLaunchKernelGGL(k_split1<R>, dim3((unsigned)std::min<uint64_t>(tiles, split_grid)), dim3(SPLIT_BS), 0,
+                       j->stream, s1);
+  /* pass 2: each sub-bucket by the remaining b2 region bits, back into the
+   * parser's buffer, with every region's first record */
+  Split2Args s2{recs2, d_sb, n_sb, 64 - b1 - b2, b2, recs, rstart};
+  hipLaunchKernelGGL(k_split2<R>, dim3((unsigned)std::min<uint64_t>(n_sb, split_grid)), dim3(SPLIT_BS), 0, j->stream,
+                     s2);
   const int valid = j->table_valid ? 1 : 0;
   /* a first count builds every region: it also leaves the filter scan's
    * bits (stale once another count follows) */
-  const uint64_t stride = (j->rspan + 63) / 64, nbits = ((uint64_t)1 << j->rbits) * stride;
+  const uint64_t stride = (j->rspan + 63) / 64, nbits = nreg * stride;
   uint64_t *cbits = nullptr, *sbits = nullptr;
   if (!valid && dev_alloc(j, (void**)&cbits, nbits * 8) == BRUNO_OK &&
       dev_alloc(j, (void**)&sbits, nbits * 8) != BRUNO_OK) {
@@ -1913,55 +2065,33 @@ bruno_status count_by_regions(bruno_job* j, const ParseArgs& a, bool* done) {
     cbits = nullptr;
   }
   if (!sbits) cbits = nullptr;
-  hipEvent_t* ev_sorted = j->ev_region, *ev_built = j->ev_region + 2;
-  if (two) {
-    /* the build stream starts after everything before on the job's stream */
-    HIP_CHECK(hipEventRecord(j->ev_region[4], j->stream));
-    HIP_CHECK(hipStreamWaitEvent(bst, j->ev_region[4], 0));
-  }
-  for (int b = 0; b < REC_BUCKETS; ++b) {
-    const int sl = b & 1;
-    const uint64_t n = hcur[b];
-    if (n == 0 && valid) continue;
-    if (two && b >= 2) HIP_CHECK(hipStreamWaitEvent(j->stream, ev_built[sl], 0)); /* alt[sl] free again */
-    const uint64_t* sorted = recs + (uint64_t)b * bcap;
-    if (n > 1) {
-      hipcub::DoubleBuffer<uint64_t> db(recs + (uint64_t)b * bcap, alt[sl]);
-      size_t tb = tmp;
-      HIP_CHECK(hipcub::DeviceRadixSort::SortKeys(t, tb, db, (int)n, 0, (int)rb, j->stream));
-      sorted = db.Current();
-    }
-    if (two) {
-      HIP_CHECK(hipEventRecord(ev_sorted[sl], j->stream));
-      HIP_CHECK(hipStreamWaitEvent(bst, ev_sorted[sl], 0));
-    }
-    RegionArgs g{(Slot<uint64_t>*)j->table, j->rspan, j->rbits, (uint32_t)b, nreg, sorted, n, valid,
-                 j->spill, j->spcap, j->ctr, cbits, sbits, stride, j->thresholding, scan_lo(j)};
+  RegionArgs g{j->table, j->rspan, j->rbits, nreg, recs, rstart, valid, j->spill, j->spcap,
+               j->ctr, cbits, sbits, stride, j->thresholding, scan_lo(j)};
+  const dim3 g4((unsigned)std::min<uint64_t>(nreg, 4 * REGION_BUILD_CU_SLOTS)),
+      g2((unsigned)std::min<uint64_t>(nreg, 2 * REGION_BUILD_CU_SLOTS));
+  if (RB == 8) {
     if (j->rspan <= REGION_SMALL_SLOTS)
-      hipLaunchKernelGGL((k_region_build<REGION_SMALL_SLOTS, 256>),
-                         dim3((unsigned)std::min<uint64_t>(nreg, 4 * REGION_BUILD_CU_SLOTS)), dim3(256), 0, bst, g);
+      hipLaunchKernelGGL((k_region_build<uint64_t, REGION_SMALL_SLOTS, 256>), g4, dim3(256), 0, j->stream, g);
     else
-      hipLaunchKernelGGL((k_region_build<REGION_MAX_SLOTS, 512>),
-                         dim3((unsigned)std::min<uint64_t>(nreg, 2 * REGION_BUILD_CU_SLOTS)), dim3(512), 0, bst, g);
-    if (two) HIP_CHECK(hipEventRecord(ev_built[sl], bst));
-  }
-  if (two) {
-    HIP_CHECK(hipEventRecord(j->ev_region[4], bst));
-    HIP_CHECK(hipStreamWaitEvent(j->stream, j->ev_region[4], 0));
+      hipLaunchKernelGGL((k_region_build<uint64_t, REGION_MAX_SLOTS, 512>), g2, dim3(512), 0, j->stream, g);
+  } else {
+    if (j->rspan <= REGION_WIDE_SMALL_SLOTS)
+      hipLaunchKernelGGL((k_region_build<u128, REGION_WIDE_SMALL_SLOTS, 256>), g4, dim3(256), 0, j->stream, g);
+    else
+      hipLaunchKernelGGL((k_region_build<u128, REGION_WIDE_SLOTS, 512>), g2, dim3(512), 0, j->stream, g);
   }
   /* instances and dropped reads into the job's counters */
   hipLaunchKernelGGL(k_add_u64, dim3(1), dim3(64), 0, j->stream, (unsigned long long*)&j->ctr->n_instances, stat, 1);
   hipLaunchKernelGGL(k_add_u64, dim3(1), dim3(64), 0, j->stream, (unsigned long long*)&j->ctr->n_dropped, stat + 1,
                      1);
   HIP_CHECK(hipStreamSynchronize(j->stream));
-  dev_free(j, t, tmp);
-  void* p = alt[0];
-  dev_free(j, p, altn * 8);
-  if (two) {
-    p = alt[1];
-    dev_free(j, p, altn * 8);
-  }
-  dev_free(j, rp, REC_BUCKETS * bcap * 8);
+  void* p = rstart;
+  dev_free(j, p, (nreg + 1) * 8);
+  p = d_sb;
+  dev_free(j, p, (n_sb + 1) * 8);
+  p = recs2;
+  dev_free(j, p, std::max<uint64_t>(1, N) * RB);
+  dev_free(j, rp, REC_BUCKETS * bcap * RB);
   dev_free(j, sp, NS * 8);
   j->table_valid = true;
   j->cbits = cbits;
@@ -1996,9 +2126,9 @@ bruno_status parse_count_t(bruno_job* j, const char* d_bases, const uint64_t* d_
   const bruno_tuning t = tuning();
   j->by_regions = false;
   j->bits_fresh = false;
-  if (std::is_same<KT, uint64_t>::value && j->rbits && t.count_regions) {
+  if (j->rbits && t.count_regions) {
     bool done = false;
-    TRY(count_by_regions(j, a, &done));
+    TRY(count_by_regions<KT>(j, a, &done));
     if (done) {
       j->by_regions = true;
       return BRUNO_OK;
@@ -2057,22 +2187,27 @@ bruno_status filter_t(bruno_job* j) {
     hipLaunchKernelGGL((k_filter_cand<KT>), dim3(grid_for(n_cand)), dim3(BLOCK), 0, j->stream, (Slot<KT>*)j->table,
                        cand, n_cand, j->K, j->thresholding, j->th, nl, j->ctr);
   if (n_spilled) {
-    /* slot order: neighbouring threads then read neighbouring slots */
-    uint64_t* sorted = nullptr;
-    TRY(dev_alloc(j, (void**)&sorted, n_spilled * sizeof(uint64_t)));
-    int bits = 1;
-    while (bits < 64 && (1ULL << bits) < j->cap) ++bits;
-    size_t tmp = 0;
-    HIP_CHECK(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp, spilled, sorted, (int)n_spilled, 0, bits, j->stream));
+    /* slot order (the bits give it; the scan's lists are put in it), so
+     * neighbouring threads read neighbouring slots */
+    uint64_t* sorted = spilled;
     void* t = nullptr;
-    TRY(dev_alloc(j, &t, tmp));
-    HIP_CHECK(hipcub::DeviceRadixSort::SortKeys(t, tmp, spilled, sorted, (int)n_spilled, 0, bits, j->stream));
+    size_t tmp = 0;
+    if (!j->bits_fresh) {
+      TRY(dev_alloc(j, (void**)&sorted, n_spilled * sizeof(uint64_t)));
+      int bits = 1;
+      while (bits < 64 && (1ULL << bits) < j->cap) ++bits;
+      HIP_CHECK(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp, spilled, sorted, (int)n_spilled, 0, bits, j->stream));
+      TRY(dev_alloc(j, &t, tmp));
+      HIP_CHECK(hipcub::DeviceRadixSort::SortKeys(t, tmp, spilled, sorted, (int)n_spilled, 0, bits, j->stream));
+    }
     hipLaunchKernelGGL((k_filter_spilled<KT>), dim3(grid_for(n_spilled)), dim3(BLOCK), 0, j->stream,
                        (Slot<KT>*)j->table, (const Spill*)j->spill, j->spcap, j->K, j->thresholding, j->th, sorted,
                        n_spilled, nl, j->ctr);
-    dev_free(j, t, tmp);
-    void* a = sorted;
-    dev_free(j, a, n_spilled * sizeof(uint64_t));
+    if (sorted != spilled) {
+      dev_free(j, t, tmp);
+      void* a = sorted;
+      dev_free(j, a, n_spilled * sizeof(uint64_t));
+    }
   }
   TRY(sync_counters(j));
   void* fa = cand;
@@ -2774,8 +2909,6 @@ bruno_status bruno_job_destroy(bruno_job* j) {
    * ev_cleared) */
   if (j->clear_pending && j->ev_cleared) (void)hipEventSynchronize(j->ev_cleared);
   if (j->ev_chains) (void)hipEventDestroy(j->ev_chains);
-  for (int i = 0; i < 5; ++i)
-    if (j->ev_region[i]) (void)hipEventDestroy(j->ev_region[i]);
   if (j->ev_cleared) (void)hipEventDestroy(j->ev_cleared);
   for (void* p : j->allocs) pool_free(p);
   j->allocs.clear();
@@ -2824,6 +2957,8 @@ bruno_status bruno_job_reset(bruno_job* j) {
   j->ms = bruno_phase_ms{};
   memset(j->hctr, 0, sizeof(Counters));
   HIP_CHECK(hipMemsetAsync(j->ctr, 0, sizeof(Counters), j->stream));
+  j->auto_bases = 0;
+  if (j->sketch) HIP_CHECK(hipMemsetAsync(j->sketch, 0, SKETCH_M * 4, j->stream));
   if (j->table) {
     ev_record(j, 10);
     if (j->clear_pending) {
@@ -2850,17 +2985,9 @@ static bruno_status parse_count_common(bruno_job* j, const char* d_bases, const 
   TRY(sync_counters(j));
   j->ms.parse_count_ms += ev_ms(j, 0, 1);
   j->n_reads += n_reads;
-  /* k2 counts are uint32 in the reference (debruijn_graph_map.hpp:1124):
-   * a job must stay below 2^32 instances */
-  if (j->n_jobs == 1) {
-    if (j->hctr->n_instances >= (1ULL << 32)) return BRUNO_CAPACITY;
-  } else {
-    std::vector<unsigned long long> js(j->n_jobs);
-    HIP_CHECK(hipMemcpyAsync(js.data(), j->d_jstat, j->n_jobs * 8, hipMemcpyDeviceToHost, j->stream));
-    HIP_CHECK(hipStreamSynchronize(j->stream));
-    for (auto v : js)
-      if (v >= (1ULL << 32)) return BRUNO_CAPACITY;
-  }
+  /* No limit on a job's instances: sub-entries count to 2^48, and every
+   * reader takes a k2 count modulo 2^32, as the reference's uint32 k2
+   * counter (K2FreqType, debruijn_graph_map.hpp:160) holds it. */
   j->stage = ST_COUNTED;
   return BRUNO_OK;
 }
@@ -2874,17 +3001,14 @@ bruno_status bruno_parse_count(bruno_job* j, const char* d_bases, const uint64_t
     return BRUNO_OK;
   }
   if (!d_bases || !d_offsets) return BRUNO_BAD_INPUT;
-  if (j->cap == 0) {
-    /* size the table from a sketch of this batch's distinct k-mers
-     * (instances <= bases bounds it from above) */
+  if (j->cap == 0 || j->auto_size) {
+    /* size (or grow) the table from a sketch of the distinct k-mers of
+     * every batch so far (instances <= bases bounds them from above) */
     uint64_t ends[2];
     HIP_CHECK(hipMemcpyAsync(&ends[0], d_offsets, 8, hipMemcpyDeviceToHost, j->stream));
     HIP_CHECK(hipMemcpyAsync(&ends[1], d_offsets + n_reads, 8, hipMemcpyDeviceToHost, j->stream));
     HIP_CHECK(hipStreamSynchronize(j->stream));
-    double est = 0;
-    TRY(estimate_distinct(j->K, d_bases, d_offsets, n_reads, &est, j->stream));
-    uint64_t want = capacity_from_estimate(est, ends[1] - ends[0], j->load_factor);
-    TRY(alloc_table(j, &want));
+    TRY(autosize(j, d_bases, d_offsets, n_reads, ends[1] - ends[0], nullptr, nullptr));
   }
   j->job_reads[0] += n_reads;
   return parse_count_common(j, d_bases, d_offsets, n_reads, nullptr);
@@ -2960,14 +3084,7 @@ bruno_status bruno_parse_count_fastq(bruno_job* j, const char* d_fastq, uint64_t
    * byte ranges are materialised */
   bruno_status s = bruno_fastq_ranges(d_fastq, bytes, job_alloc, j, &st, &en, &nr, &nb, j->stream);
   if (s == BRUNO_OK && nr) {
-    if (j->cap == 0) {
-      double est = 0;
-      s = estimate_distinct(j->K, d_fastq, nullptr, nr, &est, j->stream, st, en);
-      if (s == BRUNO_OK) {
-        const uint64_t want = capacity_from_estimate(est, nb, j->load_factor);
-        s = alloc_table(j, &want);
-      }
-    }
+    if (j->cap == 0 || j->auto_size) s = autosize(j, d_fastq, nullptr, nr, nb, st, en);
     if (s == BRUNO_OK) {
       j->job_reads[0] += nr;
       s = parse_count_common(j, d_fastq, nullptr, nr, nullptr, st, en);

@@ -429,18 +429,13 @@ template <typename KT>
 __device__ __forceinline__ void gather_k2(const Slot<KT>* p, uint64_t s, const Spill* sp, uint64_t spcap, K2List& L) {
   sub_entries<KT, 25>(p, L);
   if (SlotOps<KT>::flags(p) & F_SPILL) {
-    for (int t = 0; t < 25 && L.n < 25; ++t) {
-      uint8_t e = BIEDGE25[t];
-      bool in_sub = false;
-      for (int j = 0; j < L.n; ++j) in_sub |= (L.e[j] == e);
-      if (in_sub) continue;
-      uint64_t c = spill_find(sp, spcap, (s << 9) | ((uint64_t)e + 1));
-      if (c) {
-        L.e[L.n] = e;
-        L.c[L.n] = (uint32_t)c;
+    spill_each(sp, spcap, s, [&](uint64_t code, uint64_t cnt) {
+      if (L.n < 25) {
+        L.e[L.n] = (uint8_t)(code - 1);
+        L.c[L.n] = (uint32_t)cnt;
         ++L.n;
       }
-    }
+    });
   }
 }
 

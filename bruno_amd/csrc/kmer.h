@@ -39,18 +39,27 @@ BR_HD uint64_t fmix64(uint64_t x) {
 }
 
 BR_HD uint64_t kmer_hash(uint64_t v) { return fmix64(v * 0x9e3779b97f4a7c15ULL + 0x632be59bd9b4e019ULL); }
-/* kmer_hash is a bijection of 64-bit words (odd multipliers, xor-shifts by
- * >= 32 bits undo themselves); its inverse recovers a k-mer from its hash */
-BR_HD uint64_t kmer_unhash(uint64_t h) {
+/* fmix64 is a bijection (odd multipliers, xor-shifts by >= 32 bits undo
+ * themselves); its inverse: */
+BR_HD uint64_t fmix64_inv(uint64_t h) {
   h ^= h >> 33;
   h *= 0x9cb4b2f8129337dbULL; /* 0xc4ceb9fe1a85ec53^-1 mod 2^64 */
   h ^= h >> 33;
   h *= 0x4f74430c22a54005ULL; /* 0xff51afd7ed558ccd^-1 */
   h ^= h >> 33;
-  return (h - 0x632be59bd9b4e019ULL) * 0xf1de83e19937733dULL; /* 0x9e3779b97f4a7c15^-1 */
+  return h;
+}
+/* so kmer_hash of a one-word k-mer is a bijection too; its inverse recovers
+ * the k-mer from its hash */
+BR_HD uint64_t kmer_unhash(uint64_t h) {
+  return (fmix64_inv(h) - 0x632be59bd9b4e019ULL) * 0xf1de83e19937733dULL; /* 0x9e3779b97f4a7c15^-1 */
 }
 BR_HD uint64_t kmer_hash(u128 v) {
   return fmix64((uint64_t)v ^ fmix64((uint64_t)(v >> 64) + 0x9e3779b97f4a7c15ULL));
+}
+/* a two-word k-mer from its hash and its high word */
+BR_HD u128 kmer_unhash(uint64_t h, uint64_t hi) {
+  return ((u128)hi << 64) | (fmix64_inv(h) ^ fmix64(hi + 0x9e3779b97f4a7c15ULL));
 }
 
 /* Reverse complement of a 64-bit word holding 32 bases: complement is the
@@ -60,11 +69,7 @@ BR_HD uint64_t rc_word(uint64_t x) {
   x = ~x;
   x = ((x >> 2) & 0x3333333333333333ULL) | ((x & 0x3333333333333333ULL) << 2);
   x = ((x >> 4) & 0x0F0F0F0F0F0F0F0FULL) | ((x & 0x0F0F0F0F0F0F0F0FULL) << 4);
-#if defined(__HIP_DEVICE_COMPILE__)
   return __builtin_bswap64(x);
-#else
-  return __builtin_bswap64(x);
-#endif
 }
 
 template <typename KT>

@@ -7,18 +7,18 @@
  *
  *   1. k_parse_records: the parser (same instances as k_parse) writes one
  *      8-byte record per instance, (hash bits 0..58) << 5 | biedge index,
- *      rotated left so the region bits below the bucket bits come lowest,
- *      into one of 32 buckets picked by hash bits 59..63.  A workgroup stages
- *      records per bucket in LDS and appends runs of >= 128 with one atomic.
- *   2. a radix sort of each bucket on its low (region) bits.
- *      (The region bits are put lowest because the library radix sort
- *      returned wrong orders for bit ranges starting above bit 0 on more
- *      than a block of keys.)
- *   3. k_region_build: a workgroup walks the sorted records of a run of
- *      regions; per region it loads the slots into LDS (or starts from EMPTY
- *      when the table holds nothing yet), adds the region's records with LDS
- *      atomics, and writes the region back with coalesced stores.  Entries beyond a slot's NSUB sub-entries go to
- *      the HBM side table exactly as in the atomic count.
+ *      into one of 32 buckets picked by hash bits 59..63, so a record's top
+ *      bits are its region inside the bucket.  A workgroup stages records
+ *      per bucket in LDS and appends runs of >= 32 with one atomic, and
+ *      counts its records per (bucket, top region bits) for the first
+ *      multisplit pass.
+ *   2. two multisplit passes (partition.h) group each bucket's records by
+ *      region and note where every region's records start.
+ *   3. k_region_build: a workgroup takes regions one by one; per region it
+ *      loads the slots into LDS (or starts from EMPTY when the table holds
+ *      nothing yet), adds the region's records with LDS atomics, and writes
+ *      the region back with coalesced stores.  Entries beyond a slot's NSUB
+ *      sub-entries go to the HBM side table exactly as in the atomic count.
  *
  * The table contents equal those of the atomic count up to the order of
  * keys inside a probe run, which no result depends on.
@@ -27,6 +27,7 @@
 #define BRUNO_REGIONS_H_
 
 #include "kernels.h"
+#include "partition.h"
 
 namespace br {
 
@@ -37,6 +38,9 @@ constexpr int REC_BUCKETS = 1 << REC_BUCKET_BITS;
  * one when a table is allocated. */
 constexpr uint32_t REGION_MAX_SLOTS = 2560;
 constexpr uint32_t REGION_SMALL_SLOTS = 1280;
+/* the same LDS for 64-byte slots (k > 31) */
+constexpr uint32_t REGION_WIDE_SLOTS = 1280;
+constexpr uint32_t REGION_WIDE_SMALL_SLOTS = 640;
 constexpr uint64_t REGION_BUILD_CU_SLOTS = 512; /* build workgroups resident on the device (x2 for small) */
 /* record parser: workgroup size, most reads per wave between appends (the
  * count picks fewer for longer reads, so a round's records fit the stage),
@@ -57,17 +61,16 @@ __device__ __forceinline__ uint32_t biedge_index(uint8_t e) {
  * find their workgroup's stage full (reads longer than the stage) are
  * appended one by one from the back. */
 struct RecArgs {
-  uint64_t* recs;            /* REC_BUCKETS x bcap records */
+  void* recs;                /* REC_BUCKETS x bcap records (RecOf<KT>::T) */
   uint64_t bcap;
   unsigned long long* front; /* per bucket: records in chunks taken (> bcap - back: overflow) */
   unsigned long long* back;  /* per bucket: records appended from the back */
   unsigned long long* stat;  /* [0] instances, [1] dropped reads */
+  unsigned long long* hist;  /* [REC_BUCKETS << b1]: records per (bucket, top b1 region bits) */
   uint32_t chunk;            /* power of two, >= REC_STAGE */
-  int rot;                   /* region bits inside a bucket (1..59): the record rotation */
+  int b1;                    /* region bits of the first multisplit pass (0..SPLIT1_MAX_BITS) */
   uint32_t rpw;              /* reads per wave between appends (1..REC_RPW) */
 };
-
-__device__ __forceinline__ uint64_t rotl64(uint64_t x, int r) { return (x << r) | (x >> (64 - r)); }
 
 /* Upper bound of the instances of a batch of reads (reads with non-ACGT
  * bases included). */
@@ -81,37 +84,52 @@ __global__ __launch_bounds__(BLOCK) void k_count_instances(ParseArgs a, unsigned
   block_add(n, out);
 }
 
-__device__ __forceinline__ void rec_put(const RecArgs& ra, uint32_t b, uint64_t at, uint64_t rec) {
-  if (at < ra.bcap) ra.recs[(uint64_t)b * ra.bcap + at] = rec;
+/* A k-mer instance's record: the key word (hash << 5) | biedge index, and
+ * for k > 31 the k-mer's high word (with the hash it gives the low word
+ * back, see kmer_unhash_lo). */
+template <typename KT> __device__ __forceinline__ typename RecOf<KT>::T make_rec(uint64_t key, KT km);
+template <> __device__ __forceinline__ uint64_t make_rec<uint64_t>(uint64_t key, uint64_t) { return key; }
+template <> __device__ __forceinline__ ulonglong2 make_rec<u128>(uint64_t key, u128 km) {
+  return make_ulonglong2(key, (uint64_t)(km >> 64));
 }
 
-/* a record the build skips (code 31), its region bits spread by position */
-__device__ __forceinline__ uint64_t rec_filler(const RecArgs& ra, uint64_t at) {
-  return rotl64((fmix64(at) << REC_BUCKET_BITS) | 31u, ra.rot);
+template <typename R>
+__device__ __forceinline__ void rec_put(const RecArgs& ra, uint32_t b, uint64_t at, const R& rec) {
+  if (at < ra.bcap) ((R*)ra.recs)[(uint64_t)b * ra.bcap + at] = rec;
 }
 
-/* Parse (k <= 31) into bucketed instance records.  The read loop is
+/* chunk padding: a record the first multisplit pass drops (code 31) */
+template <typename R> __device__ __forceinline__ R rec_filler();
+template <> __device__ __forceinline__ uint64_t rec_filler<uint64_t>() { return REC_CODE_FILL; }
+template <> __device__ __forceinline__ ulonglong2 rec_filler<ulonglong2>() { return make_ulonglong2(REC_CODE_FILL, 0); }
+
+/* Parse into bucketed instance records.  The read loop is
  * workgroup-uniform (wave w of a workgroup takes read r0 + w) so staged
  * buckets can be appended between reads. */
-template <int BS, int RPW, int STAGE, int FLUSH>
+template <typename KT, int BS, int RPW, int STAGE, int FLUSH>
 __global__ __launch_bounds__(BS) void k_parse_records(ParseArgs a, RecArgs ra) {
-  __shared__ uint64_t stg[REC_BUCKETS][STAGE];
+  typedef typename RecOf<KT>::T R;
+  __shared__ R stg[REC_BUCKETS][STAGE];
   __shared__ uint32_t cnt[REC_BUCKETS];
   __shared__ unsigned long long ch_base[REC_BUCKETS]; /* open chunk (~0: none) */
   __shared__ uint32_t ch_fill[REC_BUCKETS];
   __shared__ uint32_t fl_n1[REC_BUCKETS], fl_n2[REC_BUCKETS];
   __shared__ unsigned long long fl_d1[REC_BUCKETS], fl_d2[REC_BUCKETS];
+  __shared__ uint32_t hist[REC_BUCKETS << SPLIT1_MAX_BITS];
   constexpr int WPB = BS / 64;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int K = a.K;
   const uint32_t C = ra.chunk;
-  kops<uint64_t> ops(K);
+  const int b1 = ra.b1;
+  const uint32_t nh = REC_BUCKETS << b1;
+  kops<KT> ops(K);
   unsigned long long my_inst = 0, my_drop = 0;
   if (threadIdx.x < REC_BUCKETS) {
     cnt[threadIdx.x] = 0;
     ch_base[threadIdx.x] = ~0ULL;
     ch_fill[threadIdx.x] = 0;
   }
+  for (uint32_t i = threadIdx.x; i < nh; i += BS) hist[i] = 0;
   __syncthreads();
   /* append every bucket holding >= at staged records to its open chunk
    * (the rest to a new chunk), then empty them */
@@ -160,10 +178,12 @@ __global__ __launch_bounds__(BS) void k_parse_records(ParseArgs a, RecArgs ra) {
       if (__any(bad)) {
         if (lane == 0) ++my_drop;
       } else if (L >= (uint64_t)K + 2) {
-        for_each_instance<uint64_t>(s, L, K, lane, ops, [&](uint64_t, uint64_t km, uint8_t e) {
+        for_each_instance<KT>(s, L, K, lane, ops, [&](uint64_t, KT km, uint8_t e) {
           const uint64_t h = kmer_hash(km);
           const uint32_t b = (uint32_t)(h >> (64 - REC_BUCKET_BITS));
-          const uint64_t rec = rotl64((h << REC_BUCKET_BITS) | biedge_index(e), ra.rot);
+          const uint64_t key = (h << REC_BUCKET_BITS) | biedge_index(e);
+          const R rec = make_rec<KT>(key, km);
+          atomicAdd(&hist[(b << b1) | (b1 ? (uint32_t)(key >> (64 - b1)) : 0u)], 1u);
           const uint32_t pos = atomicAdd(&cnt[b], 1u);
           if (pos < (uint32_t)STAGE) {
             stg[b][pos] = rec;
@@ -185,8 +205,10 @@ __global__ __launch_bounds__(BS) void k_parse_records(ParseArgs a, RecArgs ra) {
   for (int b = w; b < REC_BUCKETS; b += WPB) {
     const unsigned long long base = ch_base[b];
     if (base == ~0ULL) continue;
-    for (uint32_t i = ch_fill[b] + lane; i < C; i += 64) rec_put(ra, b, base + i, rec_filler(ra, base + i));
+    for (uint32_t i = ch_fill[b] + lane; i < C; i += 64) rec_put(ra, b, base + i, rec_filler<R>());
   }
+  for (uint32_t i = threadIdx.x; i < nh; i += BS)
+    if (hist[i]) atomicAdd(&ra.hist[i], (unsigned long long)hist[i]);
   const unsigned long long si = wave_sum(my_inst), sd = wave_sum(my_drop);
   if (lane == 0) {
     if (si) atomicAdd(&ra.stat[0], si);
@@ -195,13 +217,12 @@ __global__ __launch_bounds__(BS) void k_parse_records(ParseArgs a, RecArgs ra) {
 }
 
 struct RegionArgs {
-  Slot<uint64_t>* table;
+  void* table;               /* Slot<KT> */
   uint64_t span;             /* slots per region */
-  uint32_t rbits;
-  uint32_t bucket;
-  uint64_t n_regions;        /* regions per bucket */
-  const uint64_t* recs;      /* the bucket's records sorted by region */
-  uint64_t n;                /* records of the bucket */
+  uint32_t rbits;            /* region bits (the bucket's 5 and rb below them) */
+  uint64_t n_regions;        /* 2^rbits */
+  const void* recs;          /* records (RecOf<KT>::T) grouped by region */
+  const uint64_t* region_start; /* [n_regions + 1]: region q holds recs[region_start[q], region_start[q + 1]) */
   int valid;                 /* the table holds earlier counts (else start from EMPTY) */
   Spill* spill;
   uint64_t spcap;
@@ -220,8 +241,8 @@ __device__ __forceinline__ uint64_t lds_load(const uint64_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-/* Find or claim k-mer v in the LDS region L of span slots from home slot s;
- * returns the slot, or ~0u when the region is full.  *claimed as in
+/* Find or claim k-mer km in the LDS region L of span slots from home slot
+ * s; returns the slot, or ~0u when the region is full.  *claimed as in
  * SlotOps::find_or_insert. */
 __device__ __forceinline__ uint32_t lds_claim(Slot<uint64_t>* L, uint32_t span, uint32_t s, uint64_t v,
                                               int* claimed) {
@@ -240,13 +261,41 @@ __device__ __forceinline__ uint32_t lds_claim(Slot<uint64_t>* L, uint32_t span, 
   }
   return ~0u;
 }
+/* Two-word keys: meta is claimed by CAS (EMPTY -> LOCK), the words are
+ * stored and meta published with a release store, all in one block ahead
+ * of any wait, so a lane of the claiming wave never waits for its own
+ * wave; readers wait (bounded) while meta is LOCK. */
+__device__ __forceinline__ uint32_t lds_claim(Slot<u128>* L, uint32_t span, uint32_t s, u128 v, int* claimed) {
+  const uint64_t lo = (uint64_t)v, hi = (uint64_t)(v >> 64);
+  for (uint32_t probe = 0; probe < span; ++probe) {
+    uint64_t m = __hip_atomic_load(&L[s].meta, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (m == EMPTY64) {
+      const uint64_t old = atomicCAS((unsigned long long*)&L[s].meta, EMPTY64, (unsigned long long)LOCK64);
+      if (old == EMPTY64) {
+        L[s].lo = lo;
+        L[s].hi = hi;
+        __hip_atomic_store(&L[s].meta, (uint64_t)0, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        *claimed = 1;
+        return s;
+      }
+      m = old;
+    }
+    for (uint32_t spin = 0; m == LOCK64 && spin < MAX_SPIN; ++spin)
+      m = __hip_atomic_load(&L[s].meta, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (m == LOCK64) return ~0u;
+    if (L[s].lo == lo && L[s].hi == hi) return s;
+    s = (s + 1 == span) ? 0 : s + 1;
+  }
+  return ~0u;
+}
 
 /* add_k2 on an LDS slot; gs is the slot's table index (side-table key) */
-__device__ __forceinline__ int lds_add_k2(Slot<uint64_t>* p, uint8_t e, uint64_t gs, Spill* sp, uint64_t spcap,
+template <typename KT>
+__device__ __forceinline__ int lds_add_k2(Slot<KT>* p, uint8_t e, uint64_t gs, Spill* sp, uint64_t spcap,
                                           uint32_t* err) {
   const uint64_t code = (uint64_t)e + 1;
 #pragma unroll
-  for (int j = 0; j < Slot<uint64_t>::NSUB; ++j) {
+  for (int j = 0; j < Slot<KT>::NSUB; ++j) {
     uint64_t v = lds_load(&p->sub[j]);
     if (v == EMPTY64) {
       const uint64_t old = atomicCAS((unsigned long long*)&p->sub[j], EMPTY64, (unsigned long long)((code << 48) | 1));
@@ -258,19 +307,28 @@ __device__ __forceinline__ int lds_add_k2(Slot<uint64_t>* p, uint8_t e, uint64_t
       return 0;
     }
   }
-  if (!(lds_load(&p->key) & F_SPILL)) atomicOr((unsigned long long*)&p->key, (unsigned long long)F_SPILL);
-  return spill_add(sp, spcap, (gs << 9) | code, err);
+  uint64_t* fw = SlotOps<KT>::flag_word(p);
+  if (!(lds_load(fw) & F_SPILL)) atomicOr((unsigned long long*)fw, (unsigned long long)F_SPILL);
+  return spill_add(sp, spcap, gs, code, err);
 }
 
-/* Add one record to the LDS region. */
-__device__ __forceinline__ void region_add(const RegionArgs& g, Slot<uint64_t>* L, uint32_t span, int rot,
-                                           uint64_t base, uint64_t rrec, unsigned long long& my_new,
+/* the k-mer of a record of region q */
+__device__ __forceinline__ uint64_t rec_hash(const RegionArgs& g, uint64_t q, uint64_t key) {
+  /* the bucket bits (the region's top 5 bits), then the record's */
+  return ((q >> (g.rbits - REC_BUCKET_BITS)) << (64 - REC_BUCKET_BITS)) | (key >> REC_BUCKET_BITS);
+}
+__device__ __forceinline__ uint64_t rec_kmer(uint64_t h, uint64_t) { return kmer_unhash(h); }
+__device__ __forceinline__ u128 rec_kmer(uint64_t h, const ulonglong2& r) { return kmer_unhash(h, r.y); }
+
+/* Add one record of region q (table slots from base) to the LDS region. */
+template <typename KT, typename R>
+__device__ __forceinline__ void region_add(const RegionArgs& g, Slot<KT>* L, uint32_t span, uint64_t q,
+                                           uint64_t base, const R& rec, unsigned long long& my_new,
                                            unsigned long long& my_k2) {
-  const uint64_t rec = rotl64(rrec, 64 - rot);
-  const uint32_t code = (uint32_t)(rec & 31);
-  if (code >= 25) return;
-  const uint64_t h = ((uint64_t)g.bucket << (64 - REC_BUCKET_BITS)) | (rec >> REC_BUCKET_BITS);
-  const uint64_t km = kmer_unhash(h);
+  const uint64_t key = rec_key(rec);
+  const uint32_t code = (uint32_t)(key & 31);
+  const uint64_t h = rec_hash(g, q, key);
+  const KT km = rec_kmer(h, rec);
   int claimed = 0;
   const uint32_t s = lds_claim(L, span, (uint32_t)__umul64hi(h << g.rbits, g.span), km, &claimed);
   if (s == ~0u) {
@@ -278,154 +336,86 @@ __device__ __forceinline__ void region_add(const RegionArgs& g, Slot<uint64_t>* 
     return;
   }
   my_new += claimed;
-  my_k2 += lds_add_k2(L + s, BIEDGE25[code], base + s, g.spill, g.spcap, &g.ctr->err);
+  my_k2 += lds_add_k2<KT>(L + s, BIEDGE25[code], base + s, g.spill, g.spcap, &g.ctr->err);
 }
 
-/* Workgroup-wide count of pred, through one LDS word the caller is not
- * using (the build's slot array between regions: two workgroups of 80 KB
- * fill a CU's LDS, so no other LDS is available). */
-__device__ __forceinline__ uint32_t block_count(bool pred, uint32_t* word) {
-  if (threadIdx.x == 0) *word = 0;
-  lds_barrier();
-  const uint64_t bal = __ballot(pred);
-  if ((threadIdx.x & 63) == 0 && bal) atomicAdd(word, (uint32_t)__popcll(bal));
-  lds_barrier();
-  const uint32_t c = *word;
-  lds_barrier();
-  return c;
-}
-
-/* First index of the sorted records whose region (rec & m) is >= target,
- * found by the whole workgroup: each round samples one record per thread,
- * evenly
- * spaced records and keeps the gap holding the boundary. */
-template <int BS>
-__device__ __forceinline__ uint64_t block_lower_bound(const uint64_t* recs, uint64_t n, uint64_t m, uint64_t target,
-                                                      uint32_t* word) {
-  uint64_t lo = 0, hi = n;
-  while (lo < hi) {
-    const uint64_t step = (hi - lo + BS - 1) / BS;
-    const uint64_t pt = lo + (uint64_t)threadIdx.x * step;
-    const uint32_t c = block_count(pt < hi && (recs[pt] & m) < target, word);
-    if (c == 0) return lo;
-    const uint64_t last_less = lo + (uint64_t)(c - 1) * step, first_ge = lo + (uint64_t)c * step;
-    lo = last_less + 1;
-    if (first_ge < hi) hi = first_ge;
-  }
-  return lo;
-}
-
-/* Build regions [qa, qb) of one bucket in LDS, workgroup i taking an even
- * share of the bucket's regions.  Its records are contiguous in the sorted
- * bucket: it finds the first with a binary search, then walks tiles of
- * BS * REGION_ITEMS records (all loads of a tile in flight at once, the
- * next tile loading while one is processed),
- * closing a region when the records move past it.  Regions without
- * records are written EMPTY when the table holds nothing yet and are
- * skipped otherwise.  Barriers order LDS only, so the write-back of one
- * region overlaps the next. */
-template <uint32_t SLOTS, int BS>
+/* Build regions [qa, qb), workgroup i taking an even share of the regions.
+ * Per region: set up the slots in LDS (EMPTY, or the table's slots when an
+ * earlier count wrote them; a region without records is then left as it
+ * is), add the region's records (REGION_ITEMS per thread at a time, all
+ * loads in flight together), write the filter pre-pass bits when asked, and
+ * write the region back with coalesced 16-byte stores.  Barriers order LDS
+ * only, so one region's write-back overlaps the next region's set-up. */
+template <typename KT, uint32_t SLOTS, int BS>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) void k_region_build(RegionArgs g) {
-  __shared__ Slot<uint64_t> L[SLOTS];
+  typedef typename RecOf<KT>::T R;
+  __shared__ Slot<KT> L[SLOTS];
   const uint32_t span = (uint32_t)g.span;
-  const uint32_t words = span * 4;
-  const int rot = (int)(g.rbits - REC_BUCKET_BITS);
-  const uint64_t nreg = g.n_regions, m = nreg - 1, n = g.n;
+  const uint32_t pairs = span * (uint32_t)(sizeof(Slot<KT>) / 16); /* 16-byte words of a region */
+  const uint64_t nreg = g.n_regions;
   const uint64_t qa = (uint64_t)blockIdx.x * nreg / gridDim.x, qb = ((uint64_t)blockIdx.x + 1) * nreg / gridDim.x;
+  const R* recs = (const R*)g.recs;
   unsigned long long my_new = 0, my_k2 = 0;
-  uint64_t* Lw = (uint64_t*)L;
-  uint64_t q = qa;
-  uint64_t p = qa < qb ? block_lower_bound<BS>(g.recs, n, m, qa, (uint32_t*)L) : n;
-  bool open = false; /* LDS holds region q */
-  /* tile t is processed while tile t + 1 loads */
+  ulonglong2* L2 = (ulonglong2*)L;
   constexpr uint64_t TILE = (uint64_t)REGION_ITEMS * BS;
-  uint64_t rr[REGION_ITEMS];
-#pragma unroll
-  for (int k = 0; k < REGION_ITEMS; ++k) {
-    const uint64_t i = p + (uint64_t)k * BS + threadIdx.x;
-    rr[k] = i < n ? g.recs[i] : ~0ULL;
-  }
-  uint64_t tile_end = p + TILE < n ? p + TILE : n;
-  /* region of the tile's last record; past the bucket's end: none */
-  uint64_t last = tile_end < n ? (g.recs[tile_end - 1] & m) : nreg;
-  while (q < qb) {
-    uint64_t rn[REGION_ITEMS];
-#pragma unroll
-    for (int k = 0; k < REGION_ITEMS; ++k) {
-      const uint64_t i = tile_end + (uint64_t)k * BS + threadIdx.x;
-      rn[k] = i < n ? g.recs[i] : ~0ULL;
+  for (uint64_t q = qa; q < qb; ++q) {
+    const uint64_t lo = g.region_start[q], hi = g.region_start[q + 1];
+    if (g.valid && lo == hi) continue; /* no records: the region stays as it is */
+    const uint64_t base = q * g.span;
+    ulonglong2* T2 = (ulonglong2*)((Slot<KT>*)g.table + base);
+    if (g.valid) {
+      for (uint32_t i = threadIdx.x; i < pairs; i += BS) L2[i] = T2[i];
+    } else {
+      const ulonglong2 e2 = make_ulonglong2(EMPTY64, EMPTY64);
+      for (uint32_t i = threadIdx.x; i < pairs; i += BS) L2[i] = e2;
     }
-    const uint64_t next_end = tile_end + TILE < n ? tile_end + TILE : n;
-    const uint64_t next_last = next_end < n ? (g.recs[next_end - 1] & m) : nreg;
-    while (q < qb && q <= last) {
-      if (!open) {
-        if (g.valid) {
-          bool has = false;
+    lds_barrier();
+    for (uint64_t i0 = lo; i0 < hi; i0 += TILE) {
+      R rr[REGION_ITEMS];
 #pragma unroll
-          for (int k = 0; k < REGION_ITEMS; ++k) has |= rr[k] != ~0ULL && (rr[k] & m) == q;
-          if (!block_count(has, (uint32_t*)L)) { /* no records: the region stays as it is */
-            ++q;
-            continue;
-          }
-        }
-        ulonglong2* T2 = (ulonglong2*)(g.table + (((uint64_t)g.bucket << rot) | q) * g.span);
-        if (g.valid) {
-          for (uint32_t i = threadIdx.x; i < words / 2; i += BS) ((ulonglong2*)Lw)[i] = T2[i];
-        } else {
-          for (uint32_t i = threadIdx.x; i < words; i += BS) Lw[i] = EMPTY64;
-        }
-        lds_barrier();
-        open = true;
+      for (int k = 0; k < REGION_ITEMS; ++k) {
+        const uint64_t i = i0 + (uint64_t)k * BS + threadIdx.x;
+        rr[k] = i < hi ? recs[i] : rec_zero<R>();
       }
-      const uint64_t base = (((uint64_t)g.bucket << rot) | q) * g.span;
 #pragma unroll
       for (int k = 0; k < REGION_ITEMS; ++k)
-        if (rr[k] != ~0ULL && (rr[k] & m) == q) region_add(g, L, span, rot, base, rr[k], my_new, my_k2);
-      if (q == last) break; /* the region continues in the next tile */
-      lds_barrier();
-      if (g.cbits) {
-        /* the filter scan's choice for every slot of the region, one
-         * ballot per 64 slots */
-        const uint64_t region = ((uint64_t)g.bucket << rot) | q;
-        for (uint32_t i0 = 0; i0 < span; i0 += BS) {
-          const uint32_t i = i0 + threadIdx.x;
-          bool c = false, sp = false;
-          if (i < span) {
-            const uint64_t key = L[i].key;
-            if (key != EMPTY64) {
-              if (key & F_SPILL) {
-                sp = true;
-              } else if (!g.thresholding) {
-                c = true;
-              } else {
-                uint64_t tot = 0;
+        if (i0 + (uint64_t)k * BS + threadIdx.x < hi) region_add<KT, R>(g, L, span, q, base, rr[k], my_new, my_k2);
+    }
+    lds_barrier();
+    if (g.cbits) {
+      /* the filter scan's choice for every slot of the region, one
+       * ballot per 64 slots */
+      for (uint32_t i0 = 0; i0 < span; i0 += BS) {
+        const uint32_t i = i0 + threadIdx.x;
+        bool c = false, sp = false;
+        if (i < span) {
+          const uint64_t fw = *SlotOps<KT>::flag_word(L + i);
+          if (fw != EMPTY64) {
+            if (fw & F_SPILL) {
+              sp = true;
+            } else if (!g.thresholding) {
+              c = true;
+            } else {
+              uint64_t tot = 0;
 #pragma unroll
-                for (int j = 0; j < Slot<uint64_t>::NSUB; ++j) {
-                  const uint64_t v = L[i].sub[j];
-                  tot += v != EMPTY64 ? (v & 0xFFFFFFFFFFFFULL) : 0;
-                }
-                c = tot >= g.t1lo;
+              for (int j = 0; j < Slot<KT>::NSUB; ++j) {
+                const uint64_t v = L[i].sub[j];
+                tot += v != EMPTY64 ? (v & 0xFFFFFFFFFFFFULL) : 0;
               }
+              c = tot >= g.t1lo;
             }
           }
-          const uint64_t bc = __ballot(c), bsp = __ballot(sp);
-          const uint32_t w0 = i0 + (threadIdx.x & ~63u);
-          if ((threadIdx.x & 63) == 0 && w0 < span) {
-            g.cbits[region * g.stride + w0 / 64] = bc;
-            g.sbits[region * g.stride + w0 / 64] = bsp;
-          }
+        }
+        const uint64_t bc = __ballot(c), bsp = __ballot(sp);
+        const uint32_t w0 = i0 + (threadIdx.x & ~63u);
+        if ((threadIdx.x & 63) == 0 && w0 < span) {
+          g.cbits[q * g.stride + w0 / 64] = bc;
+          g.sbits[q * g.stride + w0 / 64] = bsp;
         }
       }
-      ulonglong2* T2 = (ulonglong2*)(g.table + base);
-      for (uint32_t i = threadIdx.x; i < words / 2; i += BS) T2[i] = ((const ulonglong2*)Lw)[i];
-      lds_barrier();
-      open = false;
-      ++q;
     }
-#pragma unroll
-    for (int k = 0; k < REGION_ITEMS; ++k) rr[k] = rn[k];
-    tile_end = next_end;
-    last = next_last;
+    for (uint32_t i = threadIdx.x; i < pairs; i += BS) T2[i] = L2[i];
+    lds_barrier();
   }
   const unsigned long long sn = wave_sum(my_new), sk = wave_sum(my_k2);
   if ((threadIdx.x & 63) == 0) {

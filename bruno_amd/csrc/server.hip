@@ -100,7 +100,7 @@ namespace {
  * reads themselves and a fixed allowance for small buffers. */
 uint64_t footprint(const bruno_job_desc& j, uint64_t cap, uint64_t n_reads, uint64_t n_bases) {
   const uint64_t slot = j.k <= 31 ? 32 : 64;
-  const uint64_t table = cap * slot + std::max<uint64_t>(1024, cap / 8) * 16;
+  const uint64_t table = cap * slot + std::max<uint64_t>(1024, cap / 32) * 64;
   return table + table / 4 + cap * 8 + n_bases + 8 * (n_reads + 1) + (16ULL << 20);
 }
 

@@ -56,9 +56,17 @@ struct NodeRec {
 constexpr uint8_t CLS_BRANCH = 1;
 constexpr uint8_t CLS_CHAIN = 2;
 
+/* Side table of k2 entries beyond a slot's NSUB sub-entries: records of
+ * SPILL_SUB more entries, keyed by (slot << 2) | part.  Part p + 1 of a
+ * slot is opened only once part p is full (entries are never removed), so
+ * a reader stops at the first part with an empty entry; one record holds
+ * every entry of almost every such slot (a k-mer has at most 25 biedges).
+ * Empty record / entry = all ones, so one fill clears the table. */
+constexpr int SPILL_SUB = 7;
+constexpr int SPILL_PARTS = 4; /* NSUB 3 + 4 x 7 >= 25 */
 struct Spill {
-  uint64_t key; /* (slot << 9) | code, 0 = empty */
-  uint64_t cnt;
+  uint64_t key;
+  uint64_t sub[SPILL_SUB];      /* (code << 48) | count, as a slot's sub-entries */
 };
 
 template <typename KT> struct Slot;
@@ -76,6 +84,7 @@ template <> struct Slot<u128> {
 static_assert(sizeof(Slot<uint64_t>) == 32, "slot size");
 static_assert(sizeof(Slot<u128>) == 64, "slot size");
 static_assert(sizeof(NodeRec) == 24, "node size");
+static_assert(sizeof(Spill) == 64, "side-table record size");
 
 __device__ __forceinline__ uint64_t ld_relaxed(const uint64_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -248,67 +257,92 @@ template <> struct SlotOps<u128> {
   static __device__ __forceinline__ uint64_t flags(const S* p) { return p->meta; }
 };
 
-/* Count one instance of side-table key (slot << 9) | code.  Returns 1 if
- * the entry is new. */
-__device__ __forceinline__ int spill_add(Spill* sp, uint64_t spcap, uint64_t key, uint32_t* err) {
+/* The record of part `part` of slot s: found, or claimed when claim is
+ * set; nullptr when absent (or the table is full, flagged in err). */
+__device__ __forceinline__ Spill* spill_record(Spill* sp, uint64_t spcap, uint64_t s, int part, bool claim,
+                                               uint32_t* err) {
+  const uint64_t key = (s << 2) | (uint64_t)part;
   uint64_t h = home_slot(fmix64(key), spcap);
   const uint64_t lim = spcap < MAX_PROBE ? spcap : MAX_PROBE;
   for (uint64_t probe = 0; probe < lim; ++probe) {
     Spill* q = sp + h;
     uint64_t k0 = ld_relaxed(&q->key);
-    if (k0 == 0) {
-      uint64_t old = cas64(&q->key, 0, key);
-      if (old == 0) {
-        add64(&q->cnt, 1);
-        return 1;
-      }
+    if (k0 == EMPTY64) {
+      if (!claim) return nullptr;
+      const uint64_t old = cas64(&q->key, EMPTY64, key);
+      if (old == EMPTY64) return q;
       k0 = old;
     }
-    if (k0 == key) {
-      add64(&q->cnt, 1);
-      return 0;
-    }
+    if (k0 == key) return q;
     h = (h + 1 == spcap) ? 0 : h + 1;
+  }
+  if (claim) atomicOr(err, ERR_SPILL_FULL);
+  return nullptr;
+}
+
+/* Count n instances (one unless a table is being moved) of entry code
+ * (biedge + 1) of slot s in the side table.  Returns 1 if the entry is new. */
+__device__ __forceinline__ int spill_add(Spill* sp, uint64_t spcap, uint64_t s, uint64_t code, uint32_t* err,
+                                         uint64_t n = 1) {
+  for (int part = 0; part < SPILL_PARTS; ++part) {
+    Spill* q = spill_record(sp, spcap, s, part, true, err);
+    if (!q) return 0;
+#pragma unroll
+    for (int j = 0; j < SPILL_SUB; ++j) {
+      uint64_t v = ld_relaxed(&q->sub[j]);
+      if (v == EMPTY64) {
+        const uint64_t old = cas64(&q->sub[j], EMPTY64, (code << 48) | n);
+        if (old == EMPTY64) return 1;
+        v = old;
+      }
+      if ((v >> 48) == code) {
+        add64(&q->sub[j], n);
+        return 0;
+      }
+    }
   }
   atomicOr(err, ERR_SPILL_FULL);
   return 0;
 }
 
-/* Add one instance of biedge e to slot s.  Returns 1 if a new k2 entry was
- * created. */
+/* f(code, count) for every side-table entry of slot s (after counting). */
+template <typename F>
+__device__ __forceinline__ void spill_each(const Spill* sp, uint64_t spcap, uint64_t s, F&& f) {
+  for (int part = 0; part < SPILL_PARTS; ++part) {
+    const Spill* q = spill_record(const_cast<Spill*>(sp), spcap, s, part, false, nullptr);
+    if (!q) return;
+    for (int j = 0; j < SPILL_SUB; ++j) {
+      const uint64_t v = q->sub[j];
+      if (v == EMPTY64) return;
+      f(v >> 48, v & 0xFFFFFFFFFFFFULL);
+    }
+  }
+}
+
+/* Add n instances (one unless a table is being moved) of biedge e to slot
+ * s.  Returns 1 if a new k2 entry was created. */
 template <typename KT>
-__device__ int add_k2(Slot<KT>* T, uint64_t s, uint8_t e, Spill* sp, uint64_t spcap, uint32_t* err) {
+__device__ int add_k2(Slot<KT>* T, uint64_t s, uint8_t e, Spill* sp, uint64_t spcap, uint32_t* err,
+                      uint64_t n = 1) {
   Slot<KT>* p = T + s;
   const uint64_t code = (uint64_t)e + 1;
 #pragma unroll
   for (int j = 0; j < Slot<KT>::NSUB; ++j) {
     uint64_t v = ld_relaxed(&p->sub[j]);
     if (v == EMPTY64) {
-      uint64_t old = cas64(&p->sub[j], EMPTY64, (code << 48) | 1);
+      uint64_t old = cas64(&p->sub[j], EMPTY64, (code << 48) | n);
       if (old == EMPTY64) return 1;
       v = old;
     }
     if ((v >> 48) == code) {
-      add64(&p->sub[j], 1);
+      add64(&p->sub[j], n);
       return 0;
     }
   }
   /* spill: mark the slot, then count in the side table */
   uint64_t* fw = SlotOps<KT>::flag_word(p);
   if (!(ld_relaxed(fw) & F_SPILL)) or64(fw, F_SPILL);
-  return spill_add(sp, spcap, (s << 9) | code, err);
-}
-
-__device__ __forceinline__ uint64_t spill_find(const Spill* sp, uint64_t spcap, uint64_t key) {
-  uint64_t h = home_slot(fmix64(key), spcap);
-  const uint64_t lim = spcap < MAX_PROBE ? spcap : MAX_PROBE;
-  for (uint64_t probe = 0; probe < lim; ++probe) {
-    uint64_t k0 = sp[h].key;
-    if (k0 == 0) return 0;
-    if (k0 == key) return sp[h].cnt;
-    h = (h + 1 == spcap) ? 0 : h + 1;
-  }
-  return 0;
+  return spill_add(sp, spcap, s, code, err, n);
 }
 
 }  // namespace br

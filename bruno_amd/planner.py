@@ -28,11 +28,11 @@ def capacity(s, k, load_factor=0.5):
 
 
 def footprint(s, k, load_factor=0.5, cap=None):
-    """Estimated peak HBM bytes of one job: table + side table (1/8 of the
-    slots, 16 B each) + present list (8 B per distinct k-mer) + reads +
+    """Estimated peak HBM bytes of one job: table + side table (one 64-byte
+    record per 32 slots) + present list (8 B per distinct k-mer) + reads +
     ~300 B per genome k-mer for the chain, list-ranking and output arrays."""
     cap = capacity(s, k, load_factor) if cap is None else cap
-    table = cap * SLOT_BYTES[words(k)] + (cap // 8) * 16
+    table = cap * SLOT_BYTES[words(k)] + max(1024, cap // 32) * 64
     present = 8 * est_distinct(s, k)
     reads = s.n_bases + 8 * (s.n_reads + 1)
     chains = 300 * s.genome_len * words(k)

@@ -1,5 +1,6 @@
 """Counting by table region (bucketed instance records, per-region builds in
-LDS) against the oracle, and against the atomic count on the same inputs.
+LDS) against the oracle, and against the atomic count on the same inputs,
+for one-word (k <= 31) and two-word (k <= 63) keys.
 count_regions 2 gives tables of >= 2^22 slots of plain k <= 31 jobs the
 region layout (the default bar, 2^27 slots, is for speed only)."""
 import numpy as np
@@ -21,6 +22,10 @@ CASES = [
     (7, 600, 120, (40, 300), 0.01, 31),
     (11, 5000, 500, (60, 200), 0.02, 25),
     (13, 30000, 3000, 150, 0.01, 31),
+    # two-word keys: 64-byte slots, 16-byte records carrying the k-mer's high word
+    (21, 6000, 500, (80, 250), 0.01, 63),
+    (22, 30000, 2000, 250, 0.01, 47),
+    (23, 4000, 400, 120, 0.02, 32),
 ]
 
 
@@ -140,7 +145,7 @@ def test_empty_and_reused_job():
     job.close()
 
 
-@pytest.mark.parametrize("k", [15, 31])
+@pytest.mark.parametrize("k", [15, 31, 45, 63])
 def test_region_vs_atomic_tables_agree(k, direct_count):
     """Counts, k2 entries and every output of the region build equal the
     atomic count's on a bigger random genome (side-table spills included)."""
@@ -172,13 +177,14 @@ def test_long_reads_count_with_atomics():
     assert g["stats"]["count_by_regions"] == 0
 
 
-def test_spilled_entries_in_regions():
+@pytest.mark.parametrize("k", [31, 63])
+def test_spilled_entries_in_regions(k):
     """k-mers with more distinct biedges than a slot holds: the region build
     sends the extra entries to the side table, keyed by table slot."""
     rng = np.random.default_rng(5)
     reads = []
     for _ in range(40):
-        core = "".join(rng.choice(list("ACGT"), 31))
+        core = "".join(rng.choice(list("ACGT"), k))
         for a in "ACGT":
             for c in "ACGT":
                 r = "".join(rng.choice(list("ACGT"), 10)) + a + core + c + "".join(rng.choice(list("ACGT"), 10))
@@ -187,6 +193,6 @@ def test_spilled_entries_in_regions():
     reads = [reads[i] for i in order]
     b, off = pack(reads)
     for th in (None, thresholds_L(2)):
-        g, _ = compare_all(31, th, b, off, capacity=REGION_CAP)
+        g, _ = compare_all(k, th, b, off, capacity=REGION_CAP)
         assert g["stats"]["count_by_regions"] == 1
         assert g["stats"]["n_overflow"] >= 40
