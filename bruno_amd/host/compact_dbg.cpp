@@ -10,6 +10,8 @@
  * lower <= count < upper times over the reads.
  *
  * Writes <prefix>.branch.fasta and <prefix>.chain.fasta, and unless -B also
+ * <prefix>.graph.all (print_graph_edge_frequencies, compact_dbg_stats.hpp
+ * :348-370; the reference writes it in debug builds, :509-526),
  * <prefix>.chain.components (print_chain_nodes, compact_dbg_io.hpp:463-498),
  * <prefix>.branch.edges (print_branch_edge_frequencies,
  * compact_dbg_stats.hpp:219-262) and <prefix>.chain.edges (compute_freq_map +
@@ -109,6 +111,8 @@ int main(int argc, char** argv) {
       idx.get_map().compute_biedge_freqencies_fastq(bytes);
     }
     idx.get_map().local_insert_by_freqencies();
+    /* the filtered graph, before chain extraction (:509-526) */
+    if (!benchmark) write_file(prefix + ".graph.all", idx.graph_all());
     chainmap.extract_chains(idx);
     size_t iters = lr_opt ? chainmap.list_rank_min_update() : chainmap.list_rank();
     write_file(prefix + ".branch.fasta", chainmap.branch_fasta());

@@ -58,6 +58,19 @@ struct kmer_value {
   }
 };
 
+/* One print_graph_node line (debruijn_graph_operations.hpp:58-78): the
+ * k-mer, then in A C G T, out A C G T and self, tab separated. */
+inline std::string node_line(const bruno_node& n, int k) {
+  kmer_value km;
+  km.w[0] = n.kmer[0];
+  km.w[1] = n.kmer[1];
+  std::string out = km.to_string(k);
+  static const int order[9] = {4, 5, 6, 7, 0, 1, 2, 3, 8};
+  for (int q : order) out += "\t" + std::to_string(n.counts[q]);
+  out += "\n";
+  return out;
+}
+
 /* Shared device job (one per graph + chain graph pair). */
 class job {
  public:
@@ -138,6 +151,19 @@ class debruijn_graph {
     return v;
   }
   job& get_job() { return j_; }
+  /* the .graph.all bytes of print_graph_edge_frequencies -> print_graph_nodes
+   * (compact_dbg_stats.hpp:266-303, :348-370): every node of the filtered
+   * graph sorted by k-mer, print_graph_node (debruijn_graph_operations.hpp
+   * :58-78) lines: k-mer, in A C G T, out A C G T, self */
+  std::string graph_all() const {
+    std::vector<bruno_node> v = nodes();
+    std::sort(v.begin(), v.end(), [](const bruno_node& a, const bruno_node& b) {
+      return a.kmer[1] != b.kmer[1] ? a.kmer[1] < b.kmer[1] : a.kmer[0] < b.kmer[0];
+    });
+    std::string out;
+    for (auto& n : v) out += node_line(n, j_.k());
+    return out;
+  }
 
  private:
   job& j_;
@@ -197,15 +223,7 @@ class debruijn_chain_graph {
     std::vector<bruno_node> v(n);
     if (n) check(bruno_get_branch_nodes(j_.get(), v.data(), &n), "branch_edges");
     std::string out;
-    for (auto& b : v) {
-      kmer_value km;
-      km.w[0] = b.kmer[0];
-      km.w[1] = b.kmer[1];
-      out += km.to_string(j_.k());
-      static const int order[9] = {4, 5, 6, 7, 0, 1, 2, 3, 8};
-      for (int q : order) out += "\t" + std::to_string(b.counts[q]);
-      out += "\n";
-    }
+    for (auto& b : v) out += node_line(b, j_.k());
     return out;
   }
   /* the .chain.edges bytes of compute_freq_map + print_chain_frequencies
