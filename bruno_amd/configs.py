@@ -8,6 +8,8 @@ config3: 20 Mbp (same repeat layer), 250 bp reads at 40x, 0.5% substitutions
 config4: 256 independent jobs: random genomes log-uniform in 50 kbp - 20 Mbp,
          read length uniform in {100, 150, 250}, coverage uniform in 20-60x,
          1% substitutions, k=31 (run as batches)
+config5: config4-distribution jobs with k = 31 or 63 (80/20) arriving as a
+         Poisson process (served)
 All use the -T -L 3 thresholds.
 """
 import math
@@ -54,6 +56,17 @@ def config4_jobs(seed=4, n_jobs=256, scale=1.0, k=31):
         jobs.append(SynthConfig(seed=seed * 100003 + i, genome_len=G, n_reads=max(1, int(round(cov * G / L))),
                                 read_len=L, sub_rate=0.01))
     return jobs
+
+
+def config5_jobs(seed=5, n_jobs=256, rate=40.0, scale=1.0):
+    """The served workload: jobs from the config4 distribution, k = 31 or 63
+    (80/20), arriving as a Poisson process at `rate` jobs per second.
+    Returns (job list, k per job, arrival times in seconds)."""
+    cfgs = config4_jobs(seed=seed, n_jobs=n_jobs, scale=scale)
+    rng = np.random.default_rng(seed + 1)
+    ks = [31 if x < 0.8 else 63 for x in rng.random(n_jobs)]
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_jobs))
+    return cfgs, ks, arrivals
 
 
 def config4(seed=4, scale=1.0, n_jobs=256):
