@@ -201,6 +201,8 @@ def main():
     ap.add_argument("--autotune", action="store_true",
                     help="tune load factor and launch knobs on this workload first (untimed)")
     ap.add_argument("--tuning", default="", help="apply a tuning saved by bruno_amd.autotune")
+    ap.add_argument("--knob", action="append", default=[],
+                    help="set one tuning knob before the run, name=value (bruno_set_tuning)")
     ap.add_argument("--fastq", action="store_true",
                     help="feed the reads as FASTQ bytes in HBM (split on the device inside each step)")
     ap.add_argument("--edge-filter", action="store_true",
@@ -245,6 +247,9 @@ def main():
 
     lf = args.load_factor
     tuned = None
+    if args.knob:
+        from bruno_amd.autotune import set_tuning
+        set_tuning(**{kv.split("=")[0]: int(kv.split("=")[1]) for kv in args.knob})
     if args.tuning:
         from bruno_amd.autotune import apply
         tuned = apply(args.tuning)

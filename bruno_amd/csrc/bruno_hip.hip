@@ -348,11 +348,20 @@ __device__ uint32_t resolve(const Slot<KT>* T, uint64_t tbase, uint64_t tcap, ui
 /* resolve() of a chain node's in neighbour (its OUT end faces the node when
  * canonical) and out neighbour (its IN end) at once: the two probe
  * sequences advance in lockstep, so their table reads are in flight
- * together.  Each pointer is written only when its neighbour exists. */
+ * together.  Each pointer is written only when its neighbour exists.
+ *
+ * It also tests, from the neighbours' records it has read, that the chain
+ * graph is well formed for list ranking: every edge of the node (k, chain
+ * index self) must be answered by the neighbour's edge back, so that the
+ * neighbour's facing end points back at k (a chain neighbour) or gets
+ * marked terminal by k_termini (a branch neighbour).  Each end that fails
+ * (neighbour absent, no edge back, or an end pointing at itself) adds one
+ * to ill. */
 template <typename KT>
 __device__ void resolve_pair(const Slot<KT>* T, uint64_t tbase, uint64_t tcap, uint32_t rbits, const kops<KT>& ops,
-                             bool has_in, KT nin, bool has_out, KT nout, uint64_t n_chain, KT* vkmer, uint64_t vcap,
-                             Counters* ctr, uint32_t& pin, uint32_t& pout) {
+                             KT k, uint32_t self, bool has_in, KT nin, bool has_out, KT nout, uint64_t n_chain,
+                             KT* vkmer, uint64_t vcap, Counters* ctr, uint32_t& pin, uint32_t& pout,
+                             unsigned long long& ill) {
   const KT ri = ops.rc(nin), ro = ops.rc(nout);
   const bool fi = !(nin < ri), fo = !(nout < ro);
   const KT cin = fi ? ri : nin, cout = fo ? ro : nout;
@@ -380,6 +389,21 @@ __device__ void resolve_pair(const Slot<KT>* T, uint64_t tbase, uint64_t tcap, u
   const NodeRec* no = (const NodeRec*)&B[so == ~0ULL ? 0 : so].sub[0];
   const uint8_t cls_i = ni->cls, cls_o = no->cls;
   const uint32_t idx_i = ni->idx, idx_o = no->idx;
+  /* the edge back: the in neighbour c.k[..K-2] reaches k with base last(k)
+   * out of its OUT end (canonical) or comp(last(k)) into its IN end
+   * (flipped); the out neighbour k[1..].c reaches it with first(k) into its
+   * IN end or comp(first(k)) out of its OUT end */
+  const int lb = ops.last(k), fb = ops.first(k);
+  if (has_in) {
+    const bool back = si != ~0ULL && (fi ? ni->c[4 + 3 - lb] : ni->c[lb]) > 0;
+    const bool loop = si != ~0ULL && cls_i == CLS_CHAIN && idx_i == self && ein == 0u;
+    ill += (!back || loop) ? 1 : 0;
+  }
+  if (has_out) {
+    const bool back = so != ~0ULL && (fo ? no->c[3 - fb] : no->c[4 + fb]) > 0;
+    const bool loop = so != ~0ULL && cls_o == CLS_CHAIN && idx_o == self && eout == END_OUT;
+    ill += (!back || loop) ? 1 : 0;
+  }
   auto finish = [&](bool found, uint8_t cls, uint32_t idx, KT cn, uint32_t end) -> uint32_t {
     if (found && cls == CLS_CHAIN) return end | idx;
     unsigned long long v = atomicAdd(&ctr->n_virt, 1ULL);
@@ -404,6 +428,7 @@ __global__ __launch_bounds__(BLOCK) void k_chain_init(const Slot<KT>* T, uint64_
                                                       Counters* ctr) {
   kops<KT> ops(K);
   const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+  unsigned long long ill = 0;
   for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < n_chain; i += stride) {
     const Slot<KT>* p = T + chain_slot[i];
     uint64_t tbase = 0, tcap = cap;
@@ -426,13 +451,14 @@ __global__ __launch_bounds__(BLOCK) void k_chain_init(const Slot<KT>* T, uint64_
     }
     /* in neighbour c.k[0..K-2]: when canonical its OUT end faces k; out
      * neighbour k[1..].c: when canonical its IN end faces k */
-    resolve_pair<KT>(T, tbase, tcap, rbits, ops, ci >= 0, ops.in_nbr(k, ci < 0 ? 0 : ci), co >= 0,
-                     ops.out_nbr(k, co < 0 ? 0 : co), n_chain, vkmer, vcap, ctr, st.p[0], st.p[1]);
+    resolve_pair<KT>(T, tbase, tcap, rbits, ops, k, (uint32_t)i, ci >= 0, ops.in_nbr(k, ci < 0 ? 0 : ci), co >= 0,
+                     ops.out_nbr(k, co < 0 ? 0 : co), n_chain, vkmer, vcap, ctr, st.p[0], st.p[1], ill);
     if (ci >= 0) st.d[0] = 1;
     if (co >= 0) st.d[1] = 1;
     ckmer[i] = k;
     lr[i] = st;
   }
+  block_add(ill, &ctr->lr_ill);
 }
 
 /* setup_chain_termini: each branch node marks the facing end of its chain
@@ -625,38 +651,22 @@ constexpr int RULER_LDS = 2048; /* rulers staged per block before an append */
 
 /* ruler_mask + 1 = sampling period of the rulers (a power of two).  Marks
  * the rulers (chain ends and sampled nodes) and lists them (LDS-staged
- * appends), and counts ends that break well-formedness. */
-__global__ __launch_bounds__(BLOCK) void k_lr_check(const LRNode* lr, uint64_t n_chain, uint64_t* ruler,
-                                                    uint64_t ruler_mask, uint32_t* rlist, uint64_t rlist_cap,
-                                                    Counters* ctr) {
+ * appends).  (Whether the chain graph is well formed was counted by
+ * k_chain_init, which read every neighbour's record anyway.) */
+__global__ __launch_bounds__(BLOCK) void k_lr_rulers(const LRNode* lr, uint64_t n_chain, uint64_t* ruler,
+                                                     uint64_t ruler_mask, uint32_t* rlist, uint64_t rlist_cap,
+                                                     Counters* ctr) {
   __shared__ LdsList<RULER_LDS, uint32_t> Lr;
   if (threadIdx.x == 0) Lr.n = 0;
   __syncthreads();
   const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
-  unsigned long long bad = 0;
   for (uint64_t base = (uint64_t)blockIdx.x * BLOCK; base < n_chain; base += stride) {
     const uint64_t t = base + threadIdx.x;
     bool is_ruler = false;
     if (t < n_chain) {
       const LRNode st = lr[t];
-      bool head = false;
-      for (int E = 0; E < 2; ++E) {
-        uint32_t d = st.d[E];
-        if (is_terminal(d)) {
-          head = true;
-          continue;
-        }
-        /* initial state: a non-terminal end has dist 1 and a resolved pointer */
-        uint32_t j = st.p[E] & IDX_MASK;
-        int e = (st.p[E] & END_OUT) ? 1 : 0;
-        if (d != 1u || j >= n_chain || (j == t && e == E)) {
-          ++bad;
-          continue;
-        }
-        const LRNode o = lr[j];
-        if (o.d[e] != 1u || o.p[e] != ((uint32_t)t | end_bit(E))) ++bad;
-      }
-      bool sampled = (fmix64(t * 0x9e3779b97f4a7c15ULL + 0x1234567ULL) & ruler_mask) == 0;
+      const bool head = is_terminal(st.d[0]) || is_terminal(st.d[1]);
+      const bool sampled = (fmix64(t * 0x9e3779b97f4a7c15ULL + 0x1234567ULL) & ruler_mask) == 0;
       is_ruler = head || sampled;
     }
     /* the ruler bitmap, one word per wave (a wave covers 64 aligned nodes) */
@@ -671,7 +681,6 @@ __global__ __launch_bounds__(BLOCK) void k_lr_check(const LRNode* lr, uint64_t n
     if (nr > RULER_LDS - BLOCK) lds_flush(Lr, rlist, rlist_cap, &ctr->lr_next, &ctr->err);
   }
   lds_flush(Lr, rlist, rlist_cap, &ctr->lr_next, &ctr->err);
-  block_add(bad, &ctr->lr_conflict);
 }
 
 struct WalkRec {
@@ -1930,12 +1939,19 @@ bruno_status autosize(bruno_job* j, const char* d_bases, const uint64_t* d_offse
  * overflowed its share, or the table has more regions than two passes
  * split; the caller then counts with atomics. */
 /* record staging per bucket in the parser's LDS: 48 KB of records */
-template <typename KT> constexpr int rec_stage() { return sizeof(typename RecOf<KT>::T) == 8 ? REC_STAGE : REC_STAGE / 2; }
+/* the record parser's shape: 16-byte records (k > 31) stage half as many
+ * per bucket in the same LDS, so the workgroup appends four times as often
+ * (a quarter of the records per append, under the halved threshold) */
+template <typename KT> struct RecShape {
+  static constexpr bool wide = sizeof(typename RecOf<KT>::T) == 16;
+  static constexpr int BS = REC_BS, PF = wide ? REC_PF / 4 : REC_PF;
+  static constexpr int STAGE = wide ? REC_STAGE / 2 : REC_STAGE, FLUSH = wide ? REC_FLUSH / 2 : REC_FLUSH;
+};
 
 template <typename KT>
 bruno_status count_by_regions(bruno_job* j, const ParseArgs& a, bool* done) {
   typedef typename RecOf<KT>::T R;
-  constexpr int STAGE = rec_stage<KT>();
+  constexpr int STAGE = RecShape<KT>::STAGE;
   constexpr uint64_t RB = sizeof(R);
   *done = false;
   const uint64_t n_reads = a.n_reads;
@@ -1944,113 +1960,147 @@ bruno_status count_by_regions(bruno_job* j, const ParseArgs& a, bool* done) {
   if (rb < 0 || b2 > SPLIT2_MAX_BITS) return BRUNO_OK;
   const uint64_t n_sb = (uint64_t)REC_BUCKETS << b1, nreg = 1ULL << j->rbits;
   /* scratch: front / back per bucket, instances and dropped reads, the
-   * instance bound, the (bucket, digit) counts and the first pass's cursors */
-  const uint64_t NS = 2 * REC_BUCKETS + 3 + 2 * n_sb;
+   * instance bound, the first pass's overflow flag, its cursors and exact
+   * counts (when the first try overflows) */
+  const uint64_t NS = 2 * REC_BUCKETS + 4 + 2 * n_sb;
   unsigned long long* scratch = nullptr;
   TRY(dev_alloc(j, (void**)&scratch, NS * 8));
   unsigned long long *front = scratch, *back = scratch + REC_BUCKETS, *stat = scratch + 2 * REC_BUCKETS,
-                     *bound = stat + 2, *hist = bound + 1, *cursor = hist + n_sb;
+                     *bound = stat + 2, *overflow = bound + 1, *cursor = overflow + 1, *counts = cursor + n_sb;
   void* sp = scratch;
   HIP_CHECK(hipMemsetAsync(scratch, 0, NS * 8, j->stream));
   hipLaunchKernelGGL(k_count_instances, dim3(grid_for(n_reads)), dim3(BLOCK), 0, j->stream, a, bound);
   unsigned long long n_bound = 0;
   HIP_CHECK(hipMemcpyAsync(&n_bound, bound, 8, hipMemcpyDeviceToHost, j->stream));
   HIP_CHECK(hipStreamSynchronize(j->stream));
-  /* reads per wave between appends: a round adds ~WPB x (instances per
-   * read) / 32 records to each bucket, which must fit the room left after
-   * an append (a quarter again for spread); reads too long for one per round go
-   * to the atomic count (the stage would overflow into per-record atomics) */
-  constexpr int rbs = REC_BS;
-  const double per_read = (double)n_bound / (double)std::max<uint64_t>(1, n_reads);
-  const double per_round = (REC_BS / 64) * per_read / REC_BUCKETS * 1.25;
-  const uint32_t rpw = (uint32_t)std::min<double>(REC_RPW, (STAGE - REC_FLUSH) / std::max(1.0, per_round));
-  if (rpw == 0) {
-    dev_free(j, sp, NS * 8);
-    return BRUNO_OK;
-  }
+  typedef RecShape<KT> RS;
+  auto parse_kernel = k_parse_records<KT, RS::BS, RS::PF, RS::STAGE, RS::FLUSH>;
   /* persistent: exactly the workgroups that fit on the device at once */
   static int per_cu = 0;
   if (per_cu == 0) {
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_parse_records<KT, REC_BS, REC_RPW, STAGE, REC_FLUSH>,
-                                                     rbs, 0) != hipSuccess ||
-        per_cu <= 0) {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, parse_kernel, RS::BS, 0) != hipSuccess || per_cu <= 0) {
       (void)hipGetLastError();
       per_cu = 1;
     }
   }
-  const uint64_t groups = std::max<uint64_t>(
-      1, std::min<uint64_t>((n_reads + rbs / 64 * rpw - 1) / (rbs / 64 * rpw), (uint64_t)device_cus() * per_cu));
+  const uint64_t groups =
+      std::max<uint64_t>(1, std::min<uint64_t>((n_reads + RS::BS - 1) / RS::BS, (uint64_t)device_cus() * per_cu));
   /* chunks of up to 2048 records, smaller for small inputs so padding
    * stays under ~1/8 of the records; a flush appends at most REC_STAGE
    * records, so a chunk holds at least that many */
   uint32_t chunk = 2048;
   while (chunk / 2 >= (uint32_t)STAGE && (uint64_t)chunk * groups * REC_BUCKETS * 8 > n_bound) chunk >>= 1;
   const uint64_t bcap = n_bound / REC_BUCKETS + n_bound / (4 * REC_BUCKETS) + 65536 + groups * chunk;
+  /* each sub-bucket's room in the first pass's output: its share of the
+   * bucket plus 3% and a constant (a sub-bucket's count strays from its
+   * share by about its square root) */
+  auto room = [&](uint64_t n_b) { return (n_b >> b1) + (n_b >> b1) / 32 + 256; };
+  const uint64_t out_cap = n_sb * room(bcap);
   R* recs = nullptr;
-  if (dev_alloc(j, (void**)&recs, REC_BUCKETS * bcap * RB) != BRUNO_OK) {
+  /* the parser's buffer also takes the second pass's output (<= the first's) */
+  const uint64_t recs_n = std::max<uint64_t>(REC_BUCKETS * bcap, out_cap);
+  if (dev_alloc(j, (void**)&recs, recs_n * RB) != BRUNO_OK) {
     dev_free(j, sp, NS * 8);
     return BRUNO_OK;
   }
   void* rp = recs;
-  RecArgs ra{recs, bcap, front, back, stat, hist, chunk, b1, rpw};
-  hipLaunchKernelGGL((k_parse_records<KT, REC_BS, REC_RPW, STAGE, REC_FLUSH>), dim3((unsigned)groups), dim3(rbs), 0,
+  RecArgs ra{recs, bcap, front, back, stat, chunk};
+  hipLaunchKernelGGL(parse_kernel, dim3((unsigned)groups), dim3(RS::BS), 0,
                      j->stream, a, ra);
-  std::vector<unsigned long long> hcur(2 * REC_BUCKETS + n_sb);
+  std::vector<unsigned long long> hcur(2 * REC_BUCKETS);
   HIP_CHECK(hipMemcpyAsync(hcur.data(), front, 2 * REC_BUCKETS * 8, hipMemcpyDeviceToHost, j->stream));
-  HIP_CHECK(hipMemcpyAsync(hcur.data() + 2 * REC_BUCKETS, hist, n_sb * 8, hipMemcpyDeviceToHost, j->stream));
   HIP_CHECK(hipStreamSynchronize(j->stream));
   bool over = false;
   for (int b = 0; b < REC_BUCKETS; ++b) over |= hcur[b] + hcur[REC_BUCKETS + b] > bcap;
-  /* sub-bucket starts: prefix of the (bucket, digit) counts */
-  std::vector<uint64_t> sb(n_sb + 1, 0);
-  for (uint64_t q = 0; q < n_sb; ++q) sb[q + 1] = sb[q] + hcur[2 * REC_BUCKETS + q];
-  const uint64_t N = sb[n_sb];
+  /* sub-bucket rooms from the bucket sizes (records plus chunk padding) */
+  std::vector<uint64_t> base(n_sb + 1, 0);
+  for (uint64_t q = 0; q < n_sb; ++q) base[q + 1] = base[q] + room(hcur[q >> b1] + hcur[REC_BUCKETS + (q >> b1)]);
   R* recs2 = nullptr;
-  uint64_t* d_sb = nullptr;
-  uint64_t* rstart = nullptr;
-  if (over || dev_alloc(j, (void**)&recs2, std::max<uint64_t>(1, N) * RB) != BRUNO_OK ||
-      dev_alloc(j, (void**)&d_sb, (n_sb + 1) * 8) != BRUNO_OK ||
-      dev_alloc(j, (void**)&rstart, (nreg + 1) * 8) != BRUNO_OK) {
+  uint64_t *d_base = nullptr, *d_out = nullptr, *rstart = nullptr;
+  const uint64_t C = base[n_sb];
+  auto give_up = [&]() {
     void* p = rstart;
     dev_free(j, p, (nreg + 1) * 8);
-    p = d_sb;
+    p = d_out;
+    dev_free(j, p, (n_sb + 1) * 8);
+    p = d_base;
     dev_free(j, p, (n_sb + 1) * 8);
     p = recs2;
-    dev_free(j, p, std::max<uint64_t>(1, N) * RB);
-    dev_free(j, rp, REC_BUCKETS * bcap * RB);
+    dev_free(j, p, std::max<uint64_t>(1, C) * RB);
+    dev_free(j, rp, recs_n * RB);
     dev_free(j, sp, NS * 8);
+  };
+  if (over || dev_alloc(j, (void**)&recs2, std::max<uint64_t>(1, C) * RB) != BRUNO_OK ||
+      dev_alloc(j, (void**)&d_base, (n_sb + 1) * 8) != BRUNO_OK ||
+      dev_alloc(j, (void**)&d_out, (n_sb + 1) * 8) != BRUNO_OK ||
+      dev_alloc(j, (void**)&rstart, (nreg + 1) * 8) != BRUNO_OK) {
+    give_up();
     return BRUNO_OK;
   }
-  HIP_CHECK(hipMemcpyAsync(d_sb, sb.data(), (n_sb + 1) * 8, hipMemcpyHostToDevice, j->stream));
-  HIP_CHECK(hipMemcpyAsync(cursor, sb.data(), n_sb * 8, hipMemcpyHostToDevice, j->stream));
-  HIP_CHECK(hipMemcpyAsync(rstart + nreg, &sb[n_sb], 8, hipMemcpyHostToDevice, j->stream));
   /* pass 1: the chunked front of every bucket and the records appended
    * from its back, by the top b1 region bits */
   Split1Args s1{};
   s1.in = recs;
   s1.b1 = b1;
   s1.cursor = cursor;
+  s1.limit = d_base + 1;
   s1.out = recs2;
-  s1.out_n = N;
-  s1.err = &j->ctr->err;
+  s1.overflow = overflow;
+  /* segments (tiles numbered through them) grouped by the XCD that splits
+   * their bucket, bucket % SPLIT_XCDS */
   uint64_t tiles = 0;
-  for (int b = 0; b < REC_BUCKETS; ++b) {
-    const uint64_t ranges[2][2] = {{(uint64_t)b * bcap, (uint64_t)b * bcap + hcur[b]},
-                                   {(uint64_t)(b + 1) * bcap - hcur[REC_BUCKETS + b], (uint64_t)(b + 1) * bcap}};
-    for (auto& r : ranges) {
-      if (r[1] <= r[0]) continue;
-      s1.seg[s1.n_seg++] = SplitSeg{r[0], r[1], tiles, (uint32_t)b, 0};
-      tiles += (r[1] - r[0] + SPLIT_TILE - 1) / SPLIT_TILE;
+  for (int x = 0; x < SPLIT_XCDS; ++x) {
+    s1.xcd_tiles[x] = tiles;
+    for (int b = x; b < REC_BUCKETS; b += SPLIT_XCDS) {
+      const uint64_t ranges[2][2] = {{(uint64_t)b * bcap, (uint64_t)b * bcap + hcur[b]},
+                                     {(uint64_t)(b + 1) * bcap - hcur[REC_BUCKETS + b], (uint64_t)(b + 1) * bcap}};
+      for (auto& r : ranges) {
+        if (r[1] <= r[0]) continue;
+        s1.seg[s1.n_seg++] = SplitSeg{r[0], r[1], tiles, (uint32_t)b, 0};
+        tiles += (r[1] - r[0] + SPLIT_TILE - 1) / SPLIT_TILE;
+      }
     }
   }
+  s1.xcd_tiles[SPLIT_XCDS] = tiles;
   s1.n_tiles = tiles;
   const unsigned split_grid = device_cus() * 4;
-  if (tiles)
-    hipLaunchKernelGGL(k_split1<R>, dim3((unsigned)std::min<uint64_t>(tiles, split_grid)), dim3(SPLIT_BS), 0,
-                       j->stream, s1);
-  /* pass 2: each sub-bucket by the remaining b2 region bits, back into the
-   * parser's buffer, with every region's first record */
-  Split2Args s2{recs2, d_sb, n_sb, 64 - b1 - b2, b2, recs, rstart};
+  /* k_split1: a multiple of SPLIT_XCDS workgroups */
+  const dim3 g1((unsigned)(SPLIT_XCDS * std::max<uint64_t>(1, std::min<uint64_t>(tiles, split_grid) / SPLIT_XCDS)));
+  std::vector<unsigned long long> cur(n_sb + 1);
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    HIP_CHECK(hipMemcpyAsync(d_base, base.data(), (n_sb + 1) * 8, hipMemcpyHostToDevice, j->stream));
+    HIP_CHECK(hipMemcpyAsync(cursor, base.data(), n_sb * 8, hipMemcpyHostToDevice, j->stream));
+    if (tiles) hipLaunchKernelGGL(k_split1<R>, g1, dim3(SPLIT_BS), 0, j->stream, s1);
+    HIP_CHECK(hipMemcpyAsync(cur.data(), cursor, n_sb * 8, hipMemcpyDeviceToHost, j->stream));
+    HIP_CHECK(hipMemcpyAsync(&cur[n_sb], overflow, 8, hipMemcpyDeviceToHost, j->stream));
+    HIP_CHECK(hipStreamSynchronize(j->stream));
+    if (!cur[n_sb]) break;
+    if (attempt == 1) {
+      give_up();
+      return BRUNO_INTERNAL; /* exact room cannot overflow */
+    }
+    /* a sub-bucket ran out of room: exact counts, then the pass again */
+    HIP_CHECK(hipMemsetAsync(counts, 0, n_sb * 8, j->stream));
+    HIP_CHECK(hipMemsetAsync(overflow, 0, 8, j->stream));
+    hipLaunchKernelGGL(k_split_count<R>, g1, dim3(SPLIT_BS), 0, j->stream, s1, counts);
+    std::vector<unsigned long long> hc(n_sb);
+    HIP_CHECK(hipMemcpyAsync(hc.data(), counts, n_sb * 8, hipMemcpyDeviceToHost, j->stream));
+    HIP_CHECK(hipStreamSynchronize(j->stream));
+    for (uint64_t q = 0; q < n_sb; ++q) base[q + 1] = base[q] + hc[q];
+    if (base[n_sb] > C) {
+      give_up();
+      return BRUNO_INTERNAL;
+    }
+  }
+  /* pass 2: each sub-bucket by the remaining b2 region bits, into the
+   * parser's buffer without gaps, with every region's first record */
+  std::vector<uint64_t> out(n_sb + 1, 0);
+  for (uint64_t q = 0; q < n_sb; ++q) out[q + 1] = out[q] + (cur[q] - base[q]);
+  const uint64_t N = out[n_sb];
+  HIP_CHECK(hipMemcpyAsync(d_out, out.data(), (n_sb + 1) * 8, hipMemcpyHostToDevice, j->stream));
+  HIP_CHECK(hipMemcpyAsync(d_base, base.data(), n_sb * 8, hipMemcpyHostToDevice, j->stream));
+  HIP_CHECK(hipMemcpyAsync(rstart + nreg, &out[n_sb], 8, hipMemcpyHostToDevice, j->stream));
+  Split2Args s2{recs2, d_base, d_out, n_sb, 64 - b1 - b2, b2, recs, rstart};
   hipLaunchKernelGGL(k_split2<R>, dim3((unsigned)std::min<uint64_t>(n_sb, split_grid)), dim3(SPLIT_BS), 0, j->stream,
                      s2);
   const int valid = j->table_valid ? 1 : 0;
@@ -2085,14 +2135,8 @@ bruno_status count_by_regions(bruno_job* j, const ParseArgs& a, bool* done) {
   hipLaunchKernelGGL(k_add_u64, dim3(1), dim3(64), 0, j->stream, (unsigned long long*)&j->ctr->n_dropped, stat + 1,
                      1);
   HIP_CHECK(hipStreamSynchronize(j->stream));
-  void* p = rstart;
-  dev_free(j, p, (nreg + 1) * 8);
-  p = d_sb;
-  dev_free(j, p, (n_sb + 1) * 8);
-  p = recs2;
-  dev_free(j, p, std::max<uint64_t>(1, N) * RB);
-  dev_free(j, rp, REC_BUCKETS * bcap * RB);
-  dev_free(j, sp, NS * 8);
+  (void)N;
+  give_up(); /* every buffer of the count back to the pool */
   j->table_valid = true;
   j->cbits = cbits;
   j->sbits = sbits;
@@ -2228,6 +2272,7 @@ bruno_status chains_t(bruno_job* j) {
   TRY(dev_alloc(j, &j->vkmer, j->vcap * sizeof(KT)));
   TRY(dev_alloc(j, &j->bkmer, std::max<uint64_t>(1, j->n_branch) * sizeof(KT)));
   TRY(dev_alloc(j, (void**)&j->lr, std::max<uint64_t>(1, nc) * sizeof(LRNode)));
+  HIP_CHECK(hipMemsetAsync(&j->ctr->lr_ill, 0, sizeof(unsigned long long), j->stream));
   hipLaunchKernelGGL((k_chain_init<KT>), dim3(grid_for(nc)), dim3(BLOCK), 0, j->stream, (const Slot<KT>*)j->table,
                      probe_span(j), j->rbits, j->K, j->chain_slot, j->chain_job, j->d_seg, j->n_jobs, nc,
                      (KT*)j->ckmer, j->lr, (KT*)j->vkmer, j->vcap, j->ctr);
@@ -2282,6 +2327,9 @@ bruno_status list_rank_fast(bruno_job* j, bool* used) {
   uint32_t* rlist = nullptr;
   uint32_t* heads = nullptr;
   WalkRec *rec = nullptr, *link = nullptr, *info = nullptr;
+  /* k_chain_init counted the ill-formed ends */
+  const bool well_formed = j->hctr->lr_ill == 0;
+  if (!well_formed) return BRUNO_OK;
   TRY(dev_alloc(j, (void**)&ruler, ruler_bytes));
   j->hctr->lr_conflict = 0;
   j->hctr->lr_next = 0;
@@ -2290,17 +2338,9 @@ bruno_status list_rank_fast(bruno_job* j, bool* used) {
   HIP_CHECK(hipMemcpyAsync(&j->ctr->lr_count, &j->hctr->lr_count, 4 * sizeof(unsigned long long),
                            hipMemcpyHostToDevice, j->stream));
   TRY(dev_alloc(j, (void**)&rlist, nc * 4));
-  hipLaunchKernelGGL(k_lr_check, dim3(grid_for(nc)), dim3(BLOCK), 0, j->stream, j->lr, nc, ruler,
+  hipLaunchKernelGGL(k_lr_rulers, dim3(grid_for(nc)), dim3(BLOCK), 0, j->stream, j->lr, nc, ruler,
                      (uint64_t)tuning().ruler_sample - 1, rlist, nc, j->ctr);
   TRY(sync_counters(j));
-  bool well_formed = j->hctr->lr_conflict == 0;
-  if (!well_formed) {
-    void* a = ruler;
-    dev_free(j, a, ruler_bytes);
-    a = rlist;
-    dev_free(j, a, nc * 4);
-    return BRUNO_OK;
-  }
   uint64_t nr = j->hctr->lr_next;
   TRY(dev_alloc(j, (void**)&rec, nc * 2 * sizeof(WalkRec)));
   TRY(dev_alloc(j, (void**)&link, nc * 2 * sizeof(WalkRec)));

@@ -69,6 +69,7 @@ struct Counters {
   unsigned long long lr_count, lr_next, lr_cycle, lr_conflict;
   unsigned long long n_cycle, n_isolated, n_out, n_spill;
   unsigned long long n_cand;
+  unsigned long long lr_ill;    /* chain ends that break well-formedness (k_chain_init) */
   uint32_t err;
   uint32_t pad;
 };

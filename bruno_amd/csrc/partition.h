@@ -10,15 +10,22 @@
  * contiguous run:
  *
  *   k_split1   all buckets at once, by the top b1 region bits (up to 128
- *              sub-buckets per bucket).  The parser counted every
- *              (bucket, digit) pair, so each sub-bucket's place is known;
+ *              sub-buckets per bucket).  Each sub-bucket gets room for its
+ *              share of the bucket plus a margin (hash bits are uniform, so a
+ *              sub-bucket holds 1/2^b1 of its bucket up to sampling noise);
  *              a workgroup sorts a tile of records by digit in LDS and
- *              reserves each digit's run with one atomic per tile.  The
- *              chunk padding the parser left behind (code 31) is dropped.
+ *              reserves each digit's run with one atomic per tile.  A
+ *              sub-bucket that would overflow its room (a k-mer in a large
+ *              share of all instances) is reported, and the pass runs again
+ *              with room from exact counts (k_split_count).  The chunk
+ *              padding the parser left behind (code 31) is dropped.
  *   k_split2   one workgroup per sub-bucket (persistent), by the remaining
  *              b2 <= 9 region bits: a count pass over the sub-bucket, then
- *              the same LDS tile sort with workgroup-local cursors.  It
- *              also writes the first record of every region.
+ *              the same LDS tile sort with workgroup-local cursors, into a
+ *              compact output.  It also writes the first record of every
+ *              region.
+ *
+ * Both passes load the next tile while sorting and writing the current one.
  *
  * Neither pass keeps the order of records inside a digit, which nothing
  * downstream depends on (a region's slots come out the same up to the
@@ -58,16 +65,21 @@ struct SplitSeg {
 };
 constexpr int SPLIT_MAX_SEGS = 64;
 
+/* Workgroups go to the device's XCDs (each with its own L2) in turn:
+ * workgroup w runs on XCD w % SPLIT_XCDS. */
+constexpr int SPLIT_XCDS = 8;
+
 struct Split1Args {
   const void* in;                 /* records (RecOf<KT>::T) */
-  SplitSeg seg[SPLIT_MAX_SEGS];
+  SplitSeg seg[SPLIT_MAX_SEGS];   /* grouped by bucket % SPLIT_XCDS */
   int n_seg;
   uint64_t n_tiles;
+  uint64_t xcd_tiles[SPLIT_XCDS + 1]; /* tiles of XCD x's buckets: [xcd_tiles[x], xcd_tiles[x + 1]) */
   int b1;                         /* digit bits (0: one sub-bucket per bucket) */
   unsigned long long* cursor;     /* [32 << b1]: next free position of each sub-bucket */
+  const uint64_t* limit;          /* [32 << b1]: end of each sub-bucket's room */
   void* out;
-  uint64_t out_n;                 /* records the parser counted: out's size */
-  uint32_t* err;
+  unsigned long long* overflow;   /* set when a sub-bucket ran out of room */
 };
 
 /* Exclusive prefix of cnt[0..n) into start[0..n) (n <= 512), computed by
@@ -135,49 +147,103 @@ __device__ __forceinline__ void tile_split(const R (&r)[SPLIT_ITEMS], const bool
   __syncthreads();
 }
 
+/* the records of tile t of the split segments (ok: a real record) */
+template <typename R>
+__device__ __forceinline__ void seg_tile_load(const Split1Args& a, uint64_t t, R (&r)[SPLIT_ITEMS],
+                                              bool (&ok)[SPLIT_ITEMS], uint32_t& bucket) {
+  int g = 0;
+  while (g + 1 < a.n_seg && a.seg[g + 1].first_tile <= t) ++g;
+  const SplitSeg& sg = a.seg[g];
+  const uint64_t lo = sg.begin + (t - sg.first_tile) * SPLIT_TILE;
+  const R* in = (const R*)a.in;
+  bucket = sg.bucket;
+#pragma unroll
+  for (int k = 0; k < SPLIT_ITEMS; ++k) {
+    const uint64_t i = lo + (uint64_t)k * SPLIT_BS + threadIdx.x;
+    r[k] = i < sg.end ? in[i] : rec_zero<R>();
+    ok[k] = i < sg.end && (uint32_t)(rec_key(r[k]) & 31) != REC_CODE_FILL;
+  }
+}
+
+/* The buckets of one XCD are split by that XCD's workgroups only, so the
+ * runs that neighbouring tiles append next to each other (sharing cache
+ * lines at their ends) meet in one L2 instead of being written back in
+ * parts by two.  The grid is a multiple of SPLIT_XCDS. */
 template <typename R>
 __global__ __launch_bounds__(SPLIT_BS) void k_split1(Split1Args a) {
   __shared__ R buf[SPLIT_TILE];
-  const R* in = (const R*)a.in;
-  R* out = (R*)a.out;
   __shared__ uint32_t cnt[1 << SPLIT1_MAX_BITS], start[1 << SPLIT1_MAX_BITS];
-  __shared__ unsigned long long pos[1 << SPLIT1_MAX_BITS];
+  __shared__ unsigned long long pos[1 << SPLIT1_MAX_BITS], lim[1 << SPLIT1_MAX_BITS];
+  R* out = (R*)a.out;
   const int b1 = a.b1;
   const int nb = 1 << b1;
-  for (uint64_t t = blockIdx.x; t < a.n_tiles; t += gridDim.x) {
-    int g = 0;
-    while (g + 1 < a.n_seg && a.seg[g + 1].first_tile <= t) ++g;
-    const SplitSeg sg = a.seg[g];
-    const uint64_t lo = sg.begin + (t - sg.first_tile) * SPLIT_TILE;
+  const uint32_t x = blockIdx.x % SPLIT_XCDS, per = gridDim.x / SPLIT_XCDS;
+  const uint64_t t0 = a.xcd_tiles[x] + blockIdx.x / SPLIT_XCDS, t1 = a.xcd_tiles[x + 1];
+  R rn[SPLIT_ITEMS];
+  bool okn[SPLIT_ITEMS];
+  uint32_t bn = 0;
+  if (t0 < t1) seg_tile_load(a, t0, rn, okn, bn);
+  for (uint64_t t = t0; t < t1; t += per) {
     R r[SPLIT_ITEMS];
     bool ok[SPLIT_ITEMS];
 #pragma unroll
     for (int k = 0; k < SPLIT_ITEMS; ++k) {
-      const uint64_t i = lo + (uint64_t)k * SPLIT_BS + threadIdx.x;
-      r[k] = i < sg.end ? in[i] : rec_zero<R>();
-      ok[k] = i < sg.end && (uint32_t)(rec_key(r[k]) & 31) != REC_CODE_FILL;
+      r[k] = rn[k];
+      ok[k] = okn[k];
     }
-    unsigned long long* cur = a.cursor + ((uint64_t)sg.bucket << b1);
+    const uint32_t bucket = bn;
+    /* the next tile's loads go out before this tile is sorted and written */
+    if (t + per < t1) seg_tile_load(a, t + per, rn, okn, bn);
+    unsigned long long* cur = a.cursor + ((uint64_t)bucket << b1);
+    const uint64_t* limit = a.limit + ((uint64_t)bucket << b1);
     tile_split(
         r, ok, nb, buf, cnt, start, [&](uint64_t x) -> uint32_t { return b1 ? (uint32_t)(x >> (64 - b1)) : 0u; },
-        [&](int i, uint32_t c) { pos[i] = c ? atomicAdd(&cur[i], (unsigned long long)c) : 0ULL; },
+        [&](int i, uint32_t c) {
+          pos[i] = c ? atomicAdd(&cur[i], (unsigned long long)c) : 0ULL;
+          lim[i] = limit[i];
+        },
         [&](uint32_t dx, uint32_t j, const R& x) {
-          /* the parser's counts bound every sub-bucket; a record beyond
-           * them would mean a miscount, reported rather than written */
+          /* a sub-bucket past its room: reported, and the pass runs again
+           * with exact room */
           const uint64_t at = pos[dx] + j;
-          if (at < a.out_n) out[at] = x;
-          else atomicOr(a.err, ERR_LIST_FULL);
+          if (at < lim[dx]) out[at] = x;
+          else atomicOr(a.overflow, 1ULL);
         });
+  }
+}
+
+/* Exact records per (bucket, top b1 region bits): the room k_split1 takes
+ * when the estimate ran short. */
+template <typename R>
+__global__ __launch_bounds__(SPLIT_BS) void k_split_count(Split1Args a, unsigned long long* counts) {
+  __shared__ uint32_t cnt[1 << SPLIT1_MAX_BITS];
+  const int b1 = a.b1;
+  const int nb = 1 << b1;
+  for (uint64_t t = blockIdx.x; t < a.n_tiles; t += gridDim.x) {
+    R r[SPLIT_ITEMS];
+    bool ok[SPLIT_ITEMS];
+    uint32_t bucket = 0;
+    seg_tile_load(a, t, r, ok, bucket);
+    for (int i = threadIdx.x; i < nb; i += SPLIT_BS) cnt[i] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < SPLIT_ITEMS; ++k)
+      if (ok[k]) atomicAdd(&cnt[b1 ? (uint32_t)(rec_key(r[k]) >> (64 - b1)) : 0u], 1u);
+    __syncthreads();
+    for (int i = threadIdx.x; i < nb; i += SPLIT_BS)
+      if (cnt[i]) atomicAdd(&counts[((uint64_t)bucket << b1) + i], (unsigned long long)cnt[i]);
+    __syncthreads();
   }
 }
 
 struct Split2Args {
   const void* in;                 /* records (RecOf<KT>::T) */
-  const uint64_t* sb_start;       /* [n_sb + 1]: sub-bucket s holds in[sb_start[s], sb_start[s + 1]) */
+  const uint64_t* in_start;       /* [n_sb]: sub-bucket s holds in[in_start[s], + its count) */
+  const uint64_t* out_start;      /* [n_sb + 1]: and goes to out[out_start[s], out_start[s + 1]) */
   uint64_t n_sb;
   int shift;                      /* digit = (rec >> shift) & (2^b2 - 1) */
   int b2;
-  void* out;                      /* same positions, grouped by region */
+  void* out;                      /* grouped by region */
   uint64_t* region_start;         /* [n_sb << b2 (+1)]: first record of every region */
 };
 
@@ -192,38 +258,53 @@ __global__ __launch_bounds__(SPLIT_BS) void k_split2(Split2Args a) {
   const uint32_t nb = 1u << b2, m = nb - 1;
   auto digit = [&](uint64_t x) -> uint32_t { return b2 ? (uint32_t)(x >> shift) & m : 0u; };
   for (uint64_t s = blockIdx.x; s < a.n_sb; s += gridDim.x) {
-    const uint64_t lo = a.sb_start[s], hi = a.sb_start[s + 1];
+    const uint64_t o0 = a.out_start[s], n = a.out_start[s + 1] - o0;
+    const uint64_t lo = a.in_start[s], hi = lo + n;
     /* count pass: every digit's records in the sub-bucket */
     for (uint32_t i = threadIdx.x; i < nb; i += SPLIT_BS) cnt[i] = 0;
     __syncthreads();
-    for (uint64_t i0 = lo; i0 < hi; i0 += SPLIT_TILE) {
-      uint64_t r[SPLIT_ITEMS];
+    for (uint64_t i0 = lo; i0 < hi; i0 += 2 * SPLIT_TILE) {
+      uint64_t r[2 * SPLIT_ITEMS];
 #pragma unroll
-      for (int k = 0; k < SPLIT_ITEMS; ++k) {
+      for (int k = 0; k < 2 * SPLIT_ITEMS; ++k) {
         const uint64_t i = i0 + (uint64_t)k * SPLIT_BS + threadIdx.x;
         r[k] = i < hi ? rec_key(in[i]) : 0;
       }
 #pragma unroll
-      for (int k = 0; k < SPLIT_ITEMS; ++k)
+      for (int k = 0; k < 2 * SPLIT_ITEMS; ++k)
         if (i0 + (uint64_t)k * SPLIT_BS + threadIdx.x < hi) atomicAdd(&cnt[digit(r[k])], 1u);
     }
     __syncthreads();
     wave_exclusive_scan(cnt, start, (int)nb);
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < nb; i += SPLIT_BS) {
-      next[i] = lo + start[i];
-      a.region_start[(s << b2) + i] = lo + start[i];
+      next[i] = o0 + start[i];
+      a.region_start[(s << b2) + i] = o0 + start[i];
     }
     __syncthreads();
-    /* scatter pass: tile by tile, each digit's run after the previous tile's */
+    /* scatter pass: tile by tile, each digit's run after the previous
+     * tile's; the next tile loads while one is sorted and written */
+    R rn[SPLIT_ITEMS];
+    bool okn[SPLIT_ITEMS];
+#pragma unroll
+    for (int k = 0; k < SPLIT_ITEMS; ++k) {
+      const uint64_t i = lo + (uint64_t)k * SPLIT_BS + threadIdx.x;
+      okn[k] = i < hi;
+      rn[k] = okn[k] ? in[i] : rec_zero<R>();
+    }
     for (uint64_t i0 = lo; i0 < hi; i0 += SPLIT_TILE) {
       R r[SPLIT_ITEMS];
       bool ok[SPLIT_ITEMS];
 #pragma unroll
       for (int k = 0; k < SPLIT_ITEMS; ++k) {
-        const uint64_t i = i0 + (uint64_t)k * SPLIT_BS + threadIdx.x;
-        ok[k] = i < hi;
-        r[k] = ok[k] ? in[i] : rec_zero<R>();
+        r[k] = rn[k];
+        ok[k] = okn[k];
+      }
+#pragma unroll
+      for (int k = 0; k < SPLIT_ITEMS; ++k) {
+        const uint64_t i = i0 + SPLIT_TILE + (uint64_t)k * SPLIT_BS + threadIdx.x;
+        okn[k] = i < hi;
+        rn[k] = okn[k] ? in[i] : rec_zero<R>();
       }
       tile_split(
           r, ok, (int)nb, buf, cnt, start, digit,

@@ -1,17 +1,18 @@
 /*
- * regions.h -- counting by table region (k <= 31, plain jobs).
+ * regions.h -- counting by table region (plain jobs, k <= 63).
  *
- * The table is 2^rbits regions of rspan <= REGION_MAX_SLOTS slots; a k-mer's
- * probe window never leaves the region its hash's top rbits bits pick (see
- * Probe in table.h).  Counting then needs no HBM atomics on the table:
+ * The table is 2^rbits regions of rspan slots (one region fills a build
+ * workgroup's LDS); a k-mer's probe window never leaves the region its
+ * hash's top rbits bits pick (see Probe in table.h).  Counting then needs no
+ * HBM atomics on the table:
  *
- *   1. k_parse_records: the parser (same instances as k_parse) writes one
- *      8-byte record per instance, (hash bits 0..58) << 5 | biedge index,
- *      into one of 32 buckets picked by hash bits 59..63, so a record's top
- *      bits are its region inside the bucket.  A workgroup stages records
- *      per bucket in LDS and appends runs of >= 32 with one atomic, and
- *      counts its records per (bucket, top region bits) for the first
- *      multisplit pass.
+ *   1. k_parse_records: one read per lane, k-mer and reverse complement
+ *      rolled along the read; each instance becomes a record (8 bytes,
+ *      (hash bits 0..58) << 5 | biedge index; k > 31 adds the k-mer's high
+ *      word) in one of 32 buckets picked by hash bits 59..63, so a record's
+ *      top bits are its region inside the bucket.  A workgroup stages
+ *      records per bucket in LDS and appends runs of >= FLUSH with one
+ *      atomic per chunk.
  *   2. two multisplit passes (partition.h) group each bucket's records by
  *      region and note where every region's records start.
  *   3. k_region_build: a workgroup takes regions one by one; per region it
@@ -42,10 +43,10 @@ constexpr uint32_t REGION_SMALL_SLOTS = 1280;
 constexpr uint32_t REGION_WIDE_SLOTS = 1280;
 constexpr uint32_t REGION_WIDE_SMALL_SLOTS = 640;
 constexpr uint64_t REGION_BUILD_CU_SLOTS = 512; /* build workgroups resident on the device (x2 for small) */
-/* record parser: workgroup size, most reads per wave between appends (the
- * count picks fewer for longer reads, so a round's records fit the stage),
- * staged records per bucket, append threshold */
-constexpr int REC_BS = 512, REC_RPW = 4, REC_STAGE = 192, REC_FLUSH = 32;
+/* record parser (k <= 31): workgroup size, positions between appends,
+ * staged records per bucket, append threshold (16-byte records, k > 31:
+ * a quarter of the positions, half the stage and half the threshold) */
+constexpr int REC_BS = 512, REC_PF = 8, REC_STAGE = 192, REC_FLUSH = 32;
 constexpr int REGION_ITEMS = 8;              /* records per thread loaded at once by the build */
 
 /* biedge byte (nibbles 0, 1, 2, 4, 8 per side) -> index into BIEDGE25 */
@@ -66,10 +67,7 @@ struct RecArgs {
   unsigned long long* front; /* per bucket: records in chunks taken (> bcap - back: overflow) */
   unsigned long long* back;  /* per bucket: records appended from the back */
   unsigned long long* stat;  /* [0] instances, [1] dropped reads */
-  unsigned long long* hist;  /* [REC_BUCKETS << b1]: records per (bucket, top b1 region bits) */
   uint32_t chunk;            /* power of two, >= REC_STAGE */
-  int b1;                    /* region bits of the first multisplit pass (0..SPLIT1_MAX_BITS) */
-  uint32_t rpw;              /* reads per wave between appends (1..REC_RPW) */
 };
 
 /* Upper bound of the instances of a batch of reads (reads with non-ACGT
@@ -103,10 +101,19 @@ template <typename R> __device__ __forceinline__ R rec_filler();
 template <> __device__ __forceinline__ uint64_t rec_filler<uint64_t>() { return REC_CODE_FILL; }
 template <> __device__ __forceinline__ ulonglong2 rec_filler<ulonglong2>() { return make_ulonglong2(REC_CODE_FILL, 0); }
 
-/* Parse into bucketed instance records.  The read loop is
- * workgroup-uniform (wave w of a workgroup takes read r0 + w) so staged
- * buckets can be appended between reads. */
-template <typename KT, int BS, int RPW, int STAGE, int FLUSH>
+/* Parse into bucketed instance records, one read per lane.  A lane rolls
+ * its read's k-mer and reverse complement along the read one base at a
+ * time (a shift and an or each), so an instance costs its hash and its
+ * record, not a window extraction and a reverse complement.  The lanes
+ * step through their reads in aligned 16-byte blocks, all lanes of the
+ * workgroup at the same block index (bases before a read's start or past
+ * its end are masked off), loading the next block while the current one
+ * is parsed; every PF positions the workgroup appends its staged records
+ * (about BS x PF / 32 per bucket), so the loop runs to the longest read of
+ * the workgroup's BS reads.  A read with a non-ACGT base is dropped whole
+ * (NFilterSequencesIterator, compact_dbg_freq_construct_clean.cpp:210-213),
+ * found by a first pass over its blocks. */
+template <typename KT, int BS, int PF, int STAGE, int FLUSH>
 __global__ __launch_bounds__(BS) void k_parse_records(ParseArgs a, RecArgs ra) {
   typedef typename RecOf<KT>::T R;
   __shared__ R stg[REC_BUCKETS][STAGE];
@@ -115,13 +122,12 @@ __global__ __launch_bounds__(BS) void k_parse_records(ParseArgs a, RecArgs ra) {
   __shared__ uint32_t ch_fill[REC_BUCKETS];
   __shared__ uint32_t fl_n1[REC_BUCKETS], fl_n2[REC_BUCKETS];
   __shared__ unsigned long long fl_d1[REC_BUCKETS], fl_d2[REC_BUCKETS];
-  __shared__ uint32_t hist[REC_BUCKETS << SPLIT1_MAX_BITS];
+  __shared__ uint32_t blocks_max;
   constexpr int WPB = BS / 64;
+  static_assert(16 % PF == 0, "flush points split a block evenly");
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int K = a.K;
   const uint32_t C = ra.chunk;
-  const int b1 = ra.b1;
-  const uint32_t nh = REC_BUCKETS << b1;
   kops<KT> ops(K);
   unsigned long long my_inst = 0, my_drop = 0;
   if (threadIdx.x < REC_BUCKETS) {
@@ -129,7 +135,6 @@ __global__ __launch_bounds__(BS) void k_parse_records(ParseArgs a, RecArgs ra) {
     ch_base[threadIdx.x] = ~0ULL;
     ch_fill[threadIdx.x] = 0;
   }
-  for (uint32_t i = threadIdx.x; i < nh; i += BS) hist[i] = 0;
   __syncthreads();
   /* append every bucket holding >= at staged records to its open chunk
    * (the rest to a new chunk), then empty them */
@@ -164,41 +169,98 @@ __global__ __launch_bounds__(BS) void k_parse_records(ParseArgs a, RecArgs ra) {
     }
     lds_barrier();
   };
-  const uint32_t rpw = ra.rpw;
-  for (uint64_t r0 = (uint64_t)blockIdx.x * WPB * rpw; r0 < a.n_reads; r0 += (uint64_t)gridDim.x * WPB * rpw) {
-    for (uint32_t jr = 0; jr < rpw; ++jr) {
-    const uint64_t r = r0 + (uint64_t)jr * WPB + w;
+  const int sh = 2 * (K - 1);
+  const KT kmask = ops.mask;
+  for (uint64_t r0 = (uint64_t)blockIdx.x * BS; r0 < a.n_reads; r0 += (uint64_t)gridDim.x * BS) {
+    const uint64_t r = r0 + threadIdx.x;
+    uint64_t lo = 0, L = 0;
     if (r < a.n_reads) {
-      const uint64_t off = a.starts ? a.starts[r] : a.offsets[r];
-      const uint64_t L = (a.starts ? a.ends[r] : a.offsets[r + 1]) - off;
-      const uint8_t* s = a.bases + off;
-      /* NFilterSequencesIterator (compact_dbg_freq_construct_clean.cpp:210-213) */
-      int bad = 0;
-      for (uint64_t i = lane; i < L; i += 64) bad |= (base_code(s[i]) < 0);
-      if (__any(bad)) {
-        if (lane == 0) ++my_drop;
-      } else if (L >= (uint64_t)K + 2) {
-        for_each_instance<KT>(s, L, K, lane, ops, [&](uint64_t, KT km, uint8_t e) {
-          const uint64_t h = kmer_hash(km);
-          const uint32_t b = (uint32_t)(h >> (64 - REC_BUCKET_BITS));
-          const uint64_t key = (h << REC_BUCKET_BITS) | biedge_index(e);
-          const R rec = make_rec<KT>(key, km);
-          atomicAdd(&hist[(b << b1) | (b1 ? (uint32_t)(key >> (64 - b1)) : 0u)], 1u);
-          const uint32_t pos = atomicAdd(&cnt[b], 1u);
-          if (pos < (uint32_t)STAGE) {
-            stg[b][pos] = rec;
-          } else {
-            /* stage full (a long read): one record from the back */
-            const unsigned long long k = atomicAdd(&ra.back[b], 1ULL);
-            if (k < ra.bcap) rec_put(ra, b, ra.bcap - 1 - k, rec);
-          }
-          ++my_inst;
-        });
+      lo = a.starts ? a.starts[r] : a.offsets[r];
+      L = (a.starts ? a.ends[r] : a.offsets[r + 1]) - lo;
+    }
+    const uint8_t* s = a.bases + lo;
+    const uint4* blk = (const uint4*)((uintptr_t)s & ~(uintptr_t)15);
+    const int64_t mis = (int64_t)((uintptr_t)s & 15);
+    /* the blocks holding the read; an aligned block that holds a byte of
+     * the read never crosses a page the read does not touch */
+    const uint64_t nblk = L ? (uint64_t)(mis + (int64_t)L + 15) / 16 : 0;
+    bool bad = false;
+    for (uint64_t t = 0; t < nblk; ++t) {
+      const uint4 v = blk[t];
+      const uint32_t wv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int64_t j = (int64_t)(16 * t) + q - mis;
+        if (j >= 0 && j < (int64_t)L) bad |= base_code((uint8_t)(wv[q >> 2] >> (8 * (q & 3)))) < 0;
       }
     }
+    if (bad) ++my_drop;
+    /* reads shorter than the parser window k + 2 give nothing
+     * (debruijn_biedge_loader.hpp:464-465) */
+    const uint64_t Lp = (bad || L < (uint64_t)K + 2) ? 0 : L;
+    const uint64_t nb = Lp ? (uint64_t)(mis + (int64_t)Lp + 16) / 16 : 0; /* through position Lp */
+    /* every thread has read the last round's maximum before it is reset */
+    __syncthreads();
+    if (threadIdx.x == 0) blocks_max = 0;
+    __syncthreads();
+    if (nb) atomicMax(&blocks_max, (uint32_t)nb);
+    __syncthreads();
+    const uint32_t nbw = blocks_max;
+    /* fw: the last k bases read, rv its reverse complement; gone: the base
+     * that left fw at the previous step.  At position j >= k the instance
+     * starting at j - k is complete: its out base is base j (none at
+     * j == L) and its in base the one that left at step j - 1 (none for
+     * the first instance). */
+    KT fw = 0, rv = 0;
+    uint32_t gone = 0;
+    uint4 cur = nb ? blk[0] : make_uint4(0, 0, 0, 0);
+    for (uint32_t t = 0; t < nbw; ++t) {
+      const uint4 nxt = (t + 1 < nb && t + 1 < nblk) ? blk[t + 1] : make_uint4(0, 0, 0, 0);
+      const uint32_t wv[4] = {cur.x, cur.y, cur.z, cur.w};
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int64_t j = (int64_t)(16 * t) + q - mis;
+        if (j >= 0 && j <= (int64_t)Lp) {
+          const uint32_t x = ((wv[q >> 2] >> (8 * (q & 3))) & 0xFFu) | 0x20u;
+          const uint32_t c = ((x >> 1) ^ (x >> 2)) & 3u; /* A/C/G/T, either case */
+          const bool in_read = j < (int64_t)Lp;
+          if (j >= K) {
+            const bool flip = !(fw < rv);
+            const KT km = flip ? rv : fw;
+            const uint32_t has_in = j > K, has_out = in_read;
+            const uint32_t il = !has_in ? 0u : flip ? 0u : 1u + gone;
+            const uint32_t ir = !has_out ? 0u : flip ? 0u : 1u + c;
+            /* biedge_rc: the in side becomes the complement of the out
+             * base and the other way round */
+            const uint32_t il2 = flip && has_out ? 4u - c : il;
+            const uint32_t ir2 = flip && has_in ? 4u - gone : ir;
+            const uint64_t h = kmer_hash(km);
+            const uint32_t b = (uint32_t)(h >> (64 - REC_BUCKET_BITS));
+            const uint64_t key = (h << REC_BUCKET_BITS) | (il2 * 5u + ir2);
+            const R rec = make_rec<KT>(key, km);
+            const uint32_t pos = atomicAdd(&cnt[b], 1u);
+            if (pos < (uint32_t)STAGE) {
+              stg[b][pos] = rec;
+            } else {
+              /* stage full: one record from the back */
+              const unsigned long long k = atomicAdd(&ra.back[b], 1ULL);
+              if (k < ra.bcap) rec_put(ra, b, ra.bcap - 1 - k, rec);
+            }
+            ++my_inst;
+          }
+          if (in_read) {
+            gone = (uint32_t)(fw >> sh) & 3u;
+            fw = ((fw << 2) | (KT)c) & kmask;
+            rv = (rv >> 2) | ((KT)(3u - c) << sh);
+          }
+        }
+        if (q % PF == PF - 1) {
+          lds_barrier();
+          flush(FLUSH);
+        }
+      }
+      cur = nxt;
     }
-    lds_barrier();
-    flush(FLUSH);
   }
   flush(1);
   /* pad the open chunks */
@@ -207,8 +269,6 @@ __global__ __launch_bounds__(BS) void k_parse_records(ParseArgs a, RecArgs ra) {
     if (base == ~0ULL) continue;
     for (uint32_t i = ch_fill[b] + lane; i < C; i += 64) rec_put(ra, b, base + i, rec_filler<R>());
   }
-  for (uint32_t i = threadIdx.x; i < nh; i += BS)
-    if (hist[i]) atomicAdd(&ra.hist[i], (unsigned long long)hist[i]);
   const unsigned long long si = wave_sum(my_inst), sd = wave_sum(my_drop);
   if (lane == 0) {
     if (si) atomicAdd(&ra.stat[0], si);
@@ -241,15 +301,64 @@ __device__ __forceinline__ uint64_t lds_load(const uint64_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-/* Find or claim k-mer km in the LDS region L of span slots from home slot
+/* A region in LDS, one row per slot word (row 0 the key / meta word, then
+ * the two-word key's words, then the sub-entries): a wave's lanes touch
+ * random slots, and with 32- or 64-byte slots side by side every key word
+ * would sit in one of a few banks; in rows, consecutive slots' words are
+ * consecutive and random slots spread over all banks.  Set-up and
+ * write-back move the region's bytes in table order, 16 per lane, so every
+ * load or store instruction covers whole cache lines (lane i takes piece
+ * i % P of slot i / P, P = 16-byte pieces per slot). */
+template <typename KT, uint32_t SLOTS>
+struct LdsRegion {
+  static constexpr int W = sizeof(Slot<KT>) / 8;
+  static constexpr int P = W / 2;
+  static constexpr int SUB0 = sizeof(KT) == 8 ? 1 : 3; /* first sub-entry row */
+  uint64_t w[W][SLOTS];
+  __device__ __forceinline__ uint64_t* flag(uint32_t s) { return &w[0][s]; }
+  __device__ __forceinline__ uint64_t* sub(int j, uint32_t s) { return &w[SUB0 + j][s]; }
+  /* slots [0, span) from / to their table copy t, BS threads */
+  template <int BS>
+  __device__ __forceinline__ void load(uint32_t span, const Slot<KT>* t) {
+    const ulonglong2* v = (const ulonglong2*)t;
+    for (uint32_t i = threadIdx.x; i < span * P; i += BS) {
+      const ulonglong2 x = v[i];
+      const uint32_t s = i / P, q = i % P;
+      w[2 * q][s] = x.x;
+      w[2 * q + 1][s] = x.y;
+    }
+  }
+  template <int BS>
+  __device__ __forceinline__ void clear(uint32_t span) {
+    for (uint32_t s = threadIdx.x; s < span; s += BS)
+#pragma unroll
+      for (int q = 0; q < W; ++q) w[q][s] = EMPTY64;
+  }
+  template <int BS>
+  __device__ __forceinline__ void store(uint32_t span, Slot<KT>* t) const {
+    typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+    u64x2* v = (u64x2*)t;
+    for (uint32_t i = threadIdx.x; i < span * P; i += BS) {
+      const uint32_t s = i / P, q = i % P;
+      u64x2 x;
+      x.x = w[2 * q][s];
+      x.y = w[2 * q + 1][s];
+      /* streaming: the region is not read again during the count */
+      __builtin_nontemporal_store(x, v + i);
+    }
+  }
+};
+
+/* Find or claim k-mer v in the LDS region L of span slots from home slot
  * s; returns the slot, or ~0u when the region is full.  *claimed as in
  * SlotOps::find_or_insert. */
-__device__ __forceinline__ uint32_t lds_claim(Slot<uint64_t>* L, uint32_t span, uint32_t s, uint64_t v,
+template <uint32_t SLOTS>
+__device__ __forceinline__ uint32_t lds_claim(LdsRegion<uint64_t, SLOTS>& L, uint32_t span, uint32_t s, uint64_t v,
                                               int* claimed) {
   for (uint32_t probe = 0; probe < span; ++probe) {
-    uint64_t k0 = lds_load(&L[s].key);
+    uint64_t k0 = lds_load(L.flag(s));
     if (k0 == EMPTY64) {
-      const uint64_t old = atomicCAS((unsigned long long*)&L[s].key, EMPTY64, (unsigned long long)v);
+      const uint64_t old = atomicCAS((unsigned long long*)L.flag(s), EMPTY64, (unsigned long long)v);
       if (old == EMPTY64) {
         *claimed = 1;
         return s;
@@ -265,49 +374,52 @@ __device__ __forceinline__ uint32_t lds_claim(Slot<uint64_t>* L, uint32_t span, 
  * stored and meta published with a release store, all in one block ahead
  * of any wait, so a lane of the claiming wave never waits for its own
  * wave; readers wait (bounded) while meta is LOCK. */
-__device__ __forceinline__ uint32_t lds_claim(Slot<u128>* L, uint32_t span, uint32_t s, u128 v, int* claimed) {
+template <uint32_t SLOTS>
+__device__ __forceinline__ uint32_t lds_claim(LdsRegion<u128, SLOTS>& L, uint32_t span, uint32_t s, u128 v,
+                                              int* claimed) {
   const uint64_t lo = (uint64_t)v, hi = (uint64_t)(v >> 64);
   for (uint32_t probe = 0; probe < span; ++probe) {
-    uint64_t m = __hip_atomic_load(&L[s].meta, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    uint64_t m = __hip_atomic_load(L.flag(s), __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
     if (m == EMPTY64) {
-      const uint64_t old = atomicCAS((unsigned long long*)&L[s].meta, EMPTY64, (unsigned long long)LOCK64);
+      const uint64_t old = atomicCAS((unsigned long long*)L.flag(s), EMPTY64, (unsigned long long)LOCK64);
       if (old == EMPTY64) {
-        L[s].lo = lo;
-        L[s].hi = hi;
-        __hip_atomic_store(&L[s].meta, (uint64_t)0, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        L.w[1][s] = lo;
+        L.w[2][s] = hi;
+        __hip_atomic_store(L.flag(s), (uint64_t)0, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
         *claimed = 1;
         return s;
       }
       m = old;
     }
     for (uint32_t spin = 0; m == LOCK64 && spin < MAX_SPIN; ++spin)
-      m = __hip_atomic_load(&L[s].meta, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+      m = __hip_atomic_load(L.flag(s), __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
     if (m == LOCK64) return ~0u;
-    if (L[s].lo == lo && L[s].hi == hi) return s;
+    if (L.w[1][s] == lo && L.w[2][s] == hi) return s;
     s = (s + 1 == span) ? 0 : s + 1;
   }
   return ~0u;
 }
 
-/* add_k2 on an LDS slot; gs is the slot's table index (side-table key) */
-template <typename KT>
-__device__ __forceinline__ int lds_add_k2(Slot<KT>* p, uint8_t e, uint64_t gs, Spill* sp, uint64_t spcap,
-                                          uint32_t* err) {
+/* add_k2 on LDS slot s; gs is the slot's table index (side-table key) */
+template <typename KT, uint32_t SLOTS>
+__device__ __forceinline__ int lds_add_k2(LdsRegion<KT, SLOTS>& L, uint32_t s, uint8_t e, uint64_t gs, Spill* sp,
+                                          uint64_t spcap, uint32_t* err) {
   const uint64_t code = (uint64_t)e + 1;
 #pragma unroll
   for (int j = 0; j < Slot<KT>::NSUB; ++j) {
-    uint64_t v = lds_load(&p->sub[j]);
+    uint64_t* q = L.sub(j, s);
+    uint64_t v = lds_load(q);
     if (v == EMPTY64) {
-      const uint64_t old = atomicCAS((unsigned long long*)&p->sub[j], EMPTY64, (unsigned long long)((code << 48) | 1));
+      const uint64_t old = atomicCAS((unsigned long long*)q, EMPTY64, (unsigned long long)((code << 48) | 1));
       if (old == EMPTY64) return 1;
       v = old;
     }
     if ((v >> 48) == code) {
-      atomicAdd((unsigned long long*)&p->sub[j], 1ULL);
+      atomicAdd((unsigned long long*)q, 1ULL);
       return 0;
     }
   }
-  uint64_t* fw = SlotOps<KT>::flag_word(p);
+  uint64_t* fw = L.flag(s);
   if (!(lds_load(fw) & F_SPILL)) atomicOr((unsigned long long*)fw, (unsigned long long)F_SPILL);
   return spill_add(sp, spcap, gs, code, err);
 }
@@ -321,8 +433,8 @@ __device__ __forceinline__ uint64_t rec_kmer(uint64_t h, uint64_t) { return kmer
 __device__ __forceinline__ u128 rec_kmer(uint64_t h, const ulonglong2& r) { return kmer_unhash(h, r.y); }
 
 /* Add one record of region q (table slots from base) to the LDS region. */
-template <typename KT, typename R>
-__device__ __forceinline__ void region_add(const RegionArgs& g, Slot<KT>* L, uint32_t span, uint64_t q,
+template <typename KT, uint32_t SLOTS, typename R>
+__device__ __forceinline__ void region_add(const RegionArgs& g, LdsRegion<KT, SLOTS>& L, uint32_t span, uint64_t q,
                                            uint64_t base, const R& rec, unsigned long long& my_new,
                                            unsigned long long& my_k2) {
   const uint64_t key = rec_key(rec);
@@ -336,50 +448,76 @@ __device__ __forceinline__ void region_add(const RegionArgs& g, Slot<KT>* L, uin
     return;
   }
   my_new += claimed;
-  my_k2 += lds_add_k2<KT>(L + s, BIEDGE25[code], base + s, g.spill, g.spcap, &g.ctr->err);
+  my_k2 += lds_add_k2<KT, SLOTS>(L, s, BIEDGE25[code], base + s, g.spill, g.spcap, &g.ctr->err);
 }
 
 /* Build regions [qa, qb), workgroup i taking an even share of the regions.
  * Per region: set up the slots in LDS (EMPTY, or the table's slots when an
  * earlier count wrote them; a region without records is then left as it
- * is), add the region's records (REGION_ITEMS per thread at a time, all
- * loads in flight together), write the filter pre-pass bits when asked, and
- * write the region back with coalesced 16-byte stores.  Barriers order LDS
- * only, so one region's write-back overlaps the next region's set-up. */
+ * is), add the region's records (REGION_ITEMS per thread at a time), write
+ * the filter pre-pass bits when asked, and write the region back (16-byte
+ * stores, a slot per lane).  The next region's first records are loaded
+ * before this region's records are added, so their latency hides behind
+ * the LDS work; barriers order LDS only, so one region's write-back
+ * overlaps the next region's set-up. */
 template <typename KT, uint32_t SLOTS, int BS>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) void k_region_build(RegionArgs g) {
   typedef typename RecOf<KT>::T R;
-  __shared__ Slot<KT> L[SLOTS];
+  /* records per thread per tile: a 64-byte-slot region holds half the
+   * slots, so half as many (and the registers stay without spills) */
+  constexpr int REGION_ITEMS = sizeof(R) == 8 ? br::REGION_ITEMS : br::REGION_ITEMS / 2;
+  __shared__ LdsRegion<KT, SLOTS> L;
   const uint32_t span = (uint32_t)g.span;
-  const uint32_t pairs = span * (uint32_t)(sizeof(Slot<KT>) / 16); /* 16-byte words of a region */
   const uint64_t nreg = g.n_regions;
   const uint64_t qa = (uint64_t)blockIdx.x * nreg / gridDim.x, qb = ((uint64_t)blockIdx.x + 1) * nreg / gridDim.x;
   const R* recs = (const R*)g.recs;
   unsigned long long my_new = 0, my_k2 = 0;
-  ulonglong2* L2 = (ulonglong2*)L;
   constexpr uint64_t TILE = (uint64_t)REGION_ITEMS * BS;
-  for (uint64_t q = qa; q < qb; ++q) {
-    const uint64_t lo = g.region_start[q], hi = g.region_start[q + 1];
-    if (g.valid && lo == hi) continue; /* no records: the region stays as it is */
-    const uint64_t base = q * g.span;
-    ulonglong2* T2 = (ulonglong2*)((Slot<KT>*)g.table + base);
-    if (g.valid) {
-      for (uint32_t i = threadIdx.x; i < pairs; i += BS) L2[i] = T2[i];
-    } else {
-      const ulonglong2 e2 = make_ulonglong2(EMPTY64, EMPTY64);
-      for (uint32_t i = threadIdx.x; i < pairs; i += BS) L2[i] = e2;
+  /* first tile of the next region with records, loaded ahead */
+  R rn[REGION_ITEMS];
+  uint64_t nq = qa, nlo = 0, nhi = 0;
+  auto ahead = [&](uint64_t from) {
+    nq = from;
+    for (; nq < qb; ++nq) {
+      nlo = g.region_start[nq];
+      nhi = g.region_start[nq + 1];
+      if (!(g.valid && nlo == nhi)) break; /* no records: the region stays as it is */
     }
-    lds_barrier();
-    for (uint64_t i0 = lo; i0 < hi; i0 += TILE) {
-      R rr[REGION_ITEMS];
+    if (nq < qb) {
 #pragma unroll
       for (int k = 0; k < REGION_ITEMS; ++k) {
-        const uint64_t i = i0 + (uint64_t)k * BS + threadIdx.x;
-        rr[k] = i < hi ? recs[i] : rec_zero<R>();
+        const uint64_t i = nlo + (uint64_t)k * BS + threadIdx.x;
+        rn[k] = i < nhi ? recs[i] : rec_zero<R>();
+      }
+    }
+  };
+  ahead(qa);
+  while (nq < qb) {
+    const uint64_t q = nq, lo = nlo, hi = nhi;
+    R rr[REGION_ITEMS];
+#pragma unroll
+    for (int k = 0; k < REGION_ITEMS; ++k) rr[k] = rn[k];
+    const uint64_t base = q * g.span;
+    Slot<KT>* T = (Slot<KT>*)g.table + base;
+    if (g.valid) {
+      L.template load<BS>(span, T);
+    } else {
+      L.template clear<BS>(span);
+    }
+    ahead(q + 1);
+    lds_barrier();
+    for (uint64_t i0 = lo; i0 < hi; i0 += TILE) {
+      if (i0 != lo) {
+#pragma unroll
+        for (int k = 0; k < REGION_ITEMS; ++k) {
+          const uint64_t i = i0 + (uint64_t)k * BS + threadIdx.x;
+          rr[k] = i < hi ? recs[i] : rec_zero<R>();
+        }
       }
 #pragma unroll
       for (int k = 0; k < REGION_ITEMS; ++k)
-        if (i0 + (uint64_t)k * BS + threadIdx.x < hi) region_add<KT, R>(g, L, span, q, base, rr[k], my_new, my_k2);
+        if (i0 + (uint64_t)k * BS + threadIdx.x < hi)
+          region_add<KT, SLOTS, R>(g, L, span, q, base, rr[k], my_new, my_k2);
     }
     lds_barrier();
     if (g.cbits) {
@@ -389,7 +527,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) void k_
         const uint32_t i = i0 + threadIdx.x;
         bool c = false, sp = false;
         if (i < span) {
-          const uint64_t fw = *SlotOps<KT>::flag_word(L + i);
+          const uint64_t fw = *L.flag(i);
           if (fw != EMPTY64) {
             if (fw & F_SPILL) {
               sp = true;
@@ -399,7 +537,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) void k_
               uint64_t tot = 0;
 #pragma unroll
               for (int j = 0; j < Slot<KT>::NSUB; ++j) {
-                const uint64_t v = L[i].sub[j];
+                const uint64_t v = *L.sub(j, i);
                 tot += v != EMPTY64 ? (v & 0xFFFFFFFFFFFFULL) : 0;
               }
               c = tot >= g.t1lo;
@@ -414,7 +552,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) void k_
         }
       }
     }
-    for (uint32_t i = threadIdx.x; i < pairs; i += BS) T2[i] = L2[i];
+    L.template store<BS>(span, T);
     lds_barrier();
   }
   const unsigned long long sn = wave_sum(my_new), sk = wave_sum(my_k2);

@@ -168,13 +168,14 @@ def test_region_vs_atomic_tables_agree(k, direct_count):
     assert outs[0] == outs[1]
 
 
-def test_long_reads_count_with_atomics():
-    """Reads too long for one per wave between appends go to the atomic
-    count, with the same results."""
+def test_long_reads_count_by_regions():
+    """Reads of 60 to 3000 bases in one workgroup: the parser walks to the
+    longest read of its workgroup, appending every few positions, with the
+    same results."""
     _, reads = random_reads(19, 30000, 60, 3000, 0.01)
     b, off = pack(reads)
     g, _ = compare_all(31, thresholds_L(2), b, off, capacity=REGION_CAP)
-    assert g["stats"]["count_by_regions"] == 0
+    assert g["stats"]["count_by_regions"] == 1
 
 
 @pytest.mark.parametrize("k", [31, 63])
