@@ -269,6 +269,7 @@ struct oracle_job {
   std::vector<u128> branches;      // P3 (sorted)
   ChainMap chain;
   uint64_t n_cycle = 0, n_isolated = 0, iterations = 0;
+  std::vector<u128> cycle_kmers;         // the chain nodes separate_cycles removed
   std::vector<oracle_chain_node> ranked;  // P4 (sorted by chain_rep_less)
   std::string branch_fasta, chain_fasta, chain_components;
   std::string branch_edges, chain_edges;  /* non-benchmark outputs */
@@ -814,11 +815,13 @@ struct oracle_job {
     for (size_t i = 0; i < chain.keys.size(); ++i)
       if (chain.alive[i] && is_unfinished(chain.vals[i])) any_unf = true;
     n_cycle = 0;
+    cycle_kmers.clear();
     if (any_unf) {
       uint32_t maxd = 1u << iters;
       for (size_t i = 0; i < chain.keys.size(); ++i)
         if (chain.alive[i] && is_cycle(chain.vals[i], maxd)) {
           chain.alive[i] = 0;
+          cycle_kmers.push_back(chain.keys[i]);
           ++n_cycle;
         }
     }
@@ -1014,6 +1017,320 @@ struct oracle_job {
     }
   }
 
+  /* ---------------- graph cleaning ------------------------------------
+   * clean_deadend_and_bubbles_naive (test/common/compact_dbg_clean.hpp
+   * :560-850), run by do_work when cleaning is on
+   * (compact_dbg_freq_construct_clean.cpp:633-701) after the cycle and
+   * isolated removal (:606-610).  Each round summarises every chain from its
+   * termini, picks deadends and bubbles, erases the edges that tie them to
+   * their branch points when those edges are weaker than k1_lo + 1, drops
+   * nodes left without edges, and compacts the graph again from scratch. */
+
+  /* summarized_chain (debruijn_chain_node.hpp:165-174): branch before the
+   * 5' end, 5' terminus (the chain representative's strand), 3' terminus,
+   * branch after the 3' end, distance between the termini, and the total
+   * frequency of the edges entering the 5' end / leaving the 3' end. */
+  struct Summary {
+    u128 b5, t5, t3, b3;
+    uint32_t len;
+    uint16_t f5, f6;
+  };
+  Summary rc_summary(const Summary& x) const {  // transform::reverse_complement (:259-268)
+    return Summary{ops.rc(x.b3), ops.rc(x.t3), ops.rc(x.t5), ops.rc(x.b5), x.len, x.f6, x.f5};
+  }
+  Biedge rc_biedge(const Biedge& b) const { return Biedge{ops.rc(b.out), ops.rc(b.in), b.dout, b.din}; }
+
+  static uint16_t in_total(const Counts& c) {  // get_in_edge_total_frequency (debruijn_graph_node.hpp:307-313)
+    uint16_t t = 0;
+    for (int i = 4; i < 8; ++i) t = sat_add16(t, c[i]);
+    return t;
+  }
+  static uint16_t out_total(const Counts& c) {  // get_out_edge_total_frequency (:299-305)
+    uint16_t t = 0;
+    for (int i = 0; i < 4; ++i) t = sat_add16(t, c[i]);
+    return t;
+  }
+  /* make_summarized_chain (debruijn_chain_node.hpp:201-240) */
+  Summary summary_both(u128 ff, const Biedge& fs, u128 sf, const Biedge& ss) const {
+    return Summary{fs.in, ff, sf, ss.out, get_chain_dist(fs.din) + get_chain_dist(fs.dout),
+                   (uint16_t)(points_to_self(fs.din) ? 0 : 1), (uint16_t)(points_to_self(ss.dout) ? 0 : 1)};
+  }
+  Summary summary_5p(u128 ff, const Biedge& fs) const {
+    return Summary{fs.in, ff, fs.out, 0, get_chain_dist(fs.din) + get_chain_dist(fs.dout),
+                   (uint16_t)(points_to_self(fs.din) ? 0 : 1), 0};
+  }
+  Summary summary_3p(u128 sf, const Biedge& ss) const {
+    return Summary{0, ss.in, sf, ss.out, get_chain_dist(ss.din) + get_chain_dist(ss.dout), 0,
+                   (uint16_t)(points_to_self(ss.dout) ? 0 : 1)};
+  }
+
+  /* to_summarized_chains(dbg) (debruijn_chain_graph.hpp:2044-2180): one
+   * half summary per chain terminus (IsTerminusOrIsolated, :418-420) with
+   * the terminus node's edge frequency, then the halves of a chain (same
+   * 5' terminus) merged by merge_summarized_chains (debruijn_chain_node.hpp
+   * :177-198). */
+  std::vector<Summary> summarize() const {
+    std::vector<Summary> half;
+    for (size_t i = 0; i < chain.keys.size(); ++i) {
+      if (!chain.alive[i]) continue;
+      const u128 k = chain.keys[i];
+      const Biedge& b = chain.vals[i];
+      const bool ti = is_chain_terminal(b.din), to = is_chain_terminal(b.dout);
+      if (!ti && !to) continue;
+      const Node* n = node_of(k);
+      const uint16_t fin = n ? in_total(n->c) : 0, fout = n ? out_total(n->c) : 0;
+      Summary s;
+      if (ti && to) {
+        s = summary_both(k, b, k, b);
+        s.f5 = fin;
+        s.f6 = fout;
+      } else if (ti) {
+        const u128 r = ops.rc(b.out);
+        if (k < r) {
+          s = summary_5p(k, b);
+          s.f5 = fin;
+        } else if (k > r) {
+          s = summary_3p(ops.rc(k), rc_biedge(b));
+          s.f6 = fin;
+        } else {
+          continue;  // the reference throws: a chain cannot end in its own reverse complement
+        }
+      } else {
+        const u128 r = ops.rc(k);
+        if (b.in < r) {
+          s = summary_3p(k, b);
+          s.f6 = fout;
+        } else if (b.in > r) {
+          s = summary_5p(r, rc_biedge(b));
+          s.f5 = fout;
+        } else {
+          continue;
+        }
+      }
+      half.push_back(s);
+    }
+    /* sorted by 5' terminus, then by the 3' branch; a total order here (the
+     * reference's distributed sort leaves ties unordered, and merging is
+     * symmetric in them) */
+    std::sort(half.begin(), half.end(), [](const Summary& a, const Summary& b) {
+      if (a.t5 != b.t5) return a.t5 < b.t5;
+      if (a.b3 != b.b3) return a.b3 < b.b3;
+      if (a.b5 != b.b5) return a.b5 < b.b5;
+      if (a.f5 != b.f5) return a.f5 < b.f5;
+      return a.f6 < b.f6;
+    });
+    std::vector<Summary> out;
+    Summary last{};
+    bool have_last = false;
+    for (const Summary& x : half) {
+      if (x.len == 0) {
+        out.push_back(x);  // unit chain: both termini in one record
+      } else if (have_last && x.t5 == last.t5) {
+        Summary m = x;  // merge_summarized_chains(x, last)
+        if (m.f5 < last.f5) {
+          m.b5 = last.b5;
+          m.f5 = last.f5;
+        }
+        if (m.f6 < last.f6) {
+          m.b3 = last.b3;
+          m.f6 = last.f6;
+        }
+        out.push_back(m);
+      }
+      last = x;
+      have_last = true;
+    }
+    return out;
+  }
+
+  /* find_deadends (debruijn_topo_operations.hpp:152-177) with
+   * deadend_filter (compact_dbg_clean.hpp:61-76): exactly one open end,
+   * shorter than 2k - 1, and both end frequencies below min_freq. */
+  std::vector<Summary> find_deadends(const std::vector<Summary>& all, uint64_t min_freq) const {
+    std::vector<Summary> out;
+    const uint32_t max_len = 2 * (uint32_t)K - 1;
+    for (const Summary& x : all)
+      if (x.len < max_len && x.f5 < min_freq && x.f6 < min_freq && ((x.f5 == 0) ^ (x.f6 == 0))) out.push_back(x);
+    return out;
+  }
+
+  /* find_bubbles (debruijn_topo_operations.hpp:290-409) over
+   * find_normal_chains (:256-283): chains with both ends on branch points,
+   * put on the strand whose 5' branch is the smaller, sorted by (5' branch,
+   * 3' branch, length); runs of chains between the same two branch points
+   * where each neighbour pair differs in length by at most one and has an
+   * end frequency below min_freq (bubble_filter, compact_dbg_clean.hpp
+   * :79-101) are bubbles when the run holds two or more chains. */
+  std::vector<Summary> find_bubbles(const std::vector<Summary>& all, uint64_t min_freq) const {
+    std::vector<Summary> v;
+    for (const Summary& x : all)
+      if (x.len > 0 && x.f5 > 0 && x.f6 > 0) v.push_back(x.b5 > ops.rc(x.b3) ? rc_summary(x) : x);
+    std::sort(v.begin(), v.end(), [](const Summary& a, const Summary& b) {
+      if (a.b5 != b.b5) return a.b5 < b.b5;
+      if (a.b3 != b.b3) return a.b3 < b.b3;
+      if (a.len != b.len) return a.len < b.len;
+      if (a.t5 != b.t5) return a.t5 < b.t5;
+      return a.t3 < b.t3;
+    });
+    auto pair_ok = [&](const Summary& l, const Summary& r) {
+      const uint32_t d = l.len > r.len ? l.len - r.len : r.len - l.len;
+      return d <= 1 && (l.f5 < min_freq || l.f6 < min_freq || r.f5 < min_freq || r.f6 < min_freq);
+    };
+    std::vector<Summary> out;
+    size_t first = 0;
+    for (size_t i = 1; i <= v.size(); ++i) {
+      if (i < v.size() && v[i - 1].b5 == v[i].b5 && v[i - 1].b3 == v[i].b3 && pair_ok(v[i - 1], v[i])) continue;
+      if (i - first > 1)
+        for (size_t q = first; q < i; ++q) out.push_back(v[q]);
+      first = i;
+    }
+    return out;
+  }
+
+  /* the node record of k-mer x (any strand) and the counter of its edge on
+   * side `out` (else in) with base c, on x's strand */
+  uint16_t* edge_counter(u128 x, bool out, int c) {
+    const u128 y = canon(x);
+    auto it = std::lower_bound(nodes.begin(), nodes.end(), y, [](const Node& n, u128 v) { return n.kmer < v; });
+    if (it == nodes.end() || it->kmer != y) return nullptr;
+    if (y != x) {  // the edge on the other strand: sides swap, bases complement
+      out = !out;
+      c = 3 - c;
+    }
+    return &it->c[(out ? 0 : 4) + c];
+  }
+  /* erase_edges(query, edge_freq_filter) (debruijn_graph_map.hpp:769-773):
+   * find_edges with the predicate (:616-666) keeps the queried edges whose
+   * count is below min_freq (and not 0), and erase_edges_internal
+   * (:737-747) clears them on both nodes (get_remote_edges :466-503); node
+   * self counts are not touched. */
+  size_t erase_weak_edges(const std::vector<std::pair<u128, std::pair<bool, int>>>& q, uint64_t min_freq) {
+    std::vector<std::pair<u128, std::pair<bool, int>>> hit;
+    for (auto& e : q) {
+      const uint16_t* p = edge_counter(e.first, e.second.first, e.second.second);
+      if (p && *p > 0 && *p < min_freq) hit.push_back(e);
+    }
+    for (auto& e : hit) {
+      const u128 x = e.first;
+      const bool out = e.second.first;
+      const int c = e.second.second;
+      if (uint16_t* p = edge_counter(x, out, c)) *p = 0;
+      const u128 y = out ? ops.next_from_char(x, c) : ops.next_reverse_from_char(x, c);
+      /* y's edge back to x: an in edge with x's first base, or an out edge
+       * with x's last base */
+      if (uint16_t* p = edge_counter(y, !out, out ? ops.base(x, 0) : ops.base(x, K - 1))) *p = 0;
+    }
+    return hit.size();
+  }
+  /* erase_if(IsIsolated) (debruijn_graph_filters.hpp:163-172) */
+  void erase_isolated_nodes() {
+    nodes.erase(std::remove_if(nodes.begin(), nodes.end(),
+                               [](const Node& n) { return in_edge_count(n.c) == 0 && out_edge_count(n.c) == 0; }),
+                nodes.end());
+  }
+  /* erase_nodes (debruijn_graph_map.hpp:822-856): the nodes, and every
+   * neighbour's edge back to them */
+  void erase_nodes(const std::vector<u128>& ks) {
+    for (u128 k : ks) {
+      const Node* n = node_of(k);
+      if (!n) continue;
+      const Counts c = n->c;
+      for (int b = 0; b < 4; ++b) {
+        if (c[b] > 0)
+          if (uint16_t* p = edge_counter(ops.next_from_char(k, b), false, ops.base(k, 0))) *p = 0;
+        if (c[4 + b] > 0)
+          if (uint16_t* p = edge_counter(ops.next_reverse_from_char(k, b), true, ops.base(k, K - 1))) *p = 0;
+      }
+    }
+    std::vector<u128> sorted(ks);
+    std::sort(sorted.begin(), sorted.end());
+    nodes.erase(std::remove_if(nodes.begin(), nodes.end(),
+                               [&](const Node& n) { return std::binary_search(sorted.begin(), sorted.end(), n.kmer); }),
+                nodes.end());
+  }
+
+  uint64_t clean_rounds = 0;
+  uint64_t clean_deadends = 0, clean_bubbles = 0;
+
+  void clean(int variant) {
+    /* do_work: cycles and edgeless nodes leave the graph first */
+    erase_nodes(cycle_kmers);
+    erase_isolated_nodes();
+    const uint64_t min_freq = th[2] + 1;  // edge_freq_filter(threshes[2] + 1)
+    std::vector<Summary> all = summarize();
+    std::vector<Summary> dead = find_deadends(all, min_freq), bub = find_bubbles(all, min_freq);
+    clean_rounds = clean_deadends = clean_bubbles = 0;
+    while (!dead.empty() || !bub.empty()) {
+      clean_deadends += dead.size();
+      clean_bubbles += bub.size();
+      /* a deadend's edge to its branch point: the branch's out edge to the
+       * 5' terminus (that terminus's last base) or its in edge from the 3'
+       * terminus (that terminus's first base) */
+      std::vector<std::pair<u128, std::pair<bool, int>>> q;
+      for (const Summary& x : dead) {
+        if (x.f5 > 0) q.push_back({x.b5, {true, ops.base(x.t5, K - 1)}});
+        if (x.f6 > 0) q.push_back({x.b3, {false, ops.base(x.t3, 0)}});
+      }
+      size_t erased = erase_weak_edges(q, min_freq);
+      erase_isolated_nodes();
+      q.clear();
+      for (const Summary& x : bub) {
+        q.push_back({x.b5, {true, ops.base(x.t5, K - 1)}});
+        q.push_back({x.b3, {false, ops.base(x.t3, 0)}});
+      }
+      erased += erase_weak_edges(q, min_freq);
+      erase_isolated_nodes();
+      /* a round that erased nothing would find the same chains again (the
+       * reference loops for ever there: a branch-side edge count at or
+       * above min_freq whose terminus-side count is below it) */
+      if (erased == 0) break;
+      extract_chains();
+      list_rank(variant);
+      erase_nodes(cycle_kmers);
+      erase_isolated_nodes();
+      all = summarize();
+      dead = find_deadends(all, min_freq);
+      bub = find_bubbles(all, min_freq);
+      ++clean_rounds;
+    }
+    /* the branch points printed after cleaning are those of the cleaned
+     * graph (print_branch_fasta(idx), :680-686) */
+    branches.clear();
+    for (auto& n : nodes)
+      if (is_branch(n)) branches.push_back(n.kmer);
+  }
+
+  std::string clean_branch_fasta, clean_chain_fasta, clean_chain_components, clean_branch_edges, clean_chain_edges;
+  std::vector<oracle_chain_node> clean_ranked;
+
+  /* cleaning after run(): the .clean.* outputs of do_work; the first
+   * compaction's outputs are kept */
+  int run_clean(int variant) {
+    const std::string bf = branch_fasta, cf = chain_fasta, cc = chain_components, be = branch_edges,
+                      ce = chain_edges;
+    const uint64_t it0 = iterations, cy0 = n_cycle, is0 = n_isolated;
+    const std::vector<oracle_chain_node> rk0 = ranked;
+    clean(variant);
+    make_output();
+    make_edge_outputs();
+    clean_ranked.swap(ranked);
+    ranked = rk0;
+    iterations = it0;
+    n_cycle = cy0;
+    n_isolated = is0;
+    clean_branch_fasta.swap(branch_fasta);
+    clean_chain_fasta.swap(chain_fasta);
+    clean_chain_components.swap(chain_components);
+    clean_branch_edges.swap(branch_edges);
+    clean_chain_edges.swap(chain_edges);
+    branch_fasta = bf;
+    chain_fasta = cf;
+    chain_components = cc;
+    branch_edges = be;
+    chain_edges = ce;
+    return 0;
+  }
+
   int run(int variant) {
     double t0 = now_s();
     parse_all();
@@ -1124,6 +1441,19 @@ uint64_t oracle_get_chain_nodes(oracle_job* j, oracle_chain_node* out, uint64_t 
   for (uint64_t i = 0; i < j->ranked.size() && i < cap; ++i) out[i] = j->ranked[i];
   return j->ranked.size();
 }
+int oracle_run_clean(oracle_job* j, int lr_variant) { return j->run_clean(lr_variant); }
+uint64_t oracle_clean_rounds(oracle_job* j) { return j->clean_rounds; }
+uint64_t oracle_clean_deadends(oracle_job* j) { return j->clean_deadends; }
+uint64_t oracle_clean_bubbles(oracle_job* j) { return j->clean_bubbles; }
+uint64_t oracle_get_clean_output(oracle_job* j, int which, char* out, uint64_t cap) {
+  const std::string* s[5] = {&j->clean_chain_fasta, &j->clean_branch_fasta, &j->clean_chain_components,
+                             &j->clean_branch_edges, &j->clean_chain_edges};
+  if (which < 0 || which > 4) return 0;
+  return copy_str(*s[which], out, cap);
+}
+uint64_t oracle_n_clean_chain_nodes(oracle_job* j) { return j->clean_ranked.size(); }
+uint64_t oracle_get_clean_nodes(oracle_job* j, oracle_node* out, uint64_t cap) { return copy_nodes(j->nodes, out, cap); }
+
 void oracle_phase_seconds(oracle_job* j, double* out6) {
   for (int i = 0; i < 6; ++i) out6[i] = j->phase[i];
 }

@@ -100,6 +100,23 @@ uint64_t oracle_get_chain_edges(oracle_job* j, char* out, uint64_t cap);
 uint64_t oracle_chain_edge_misses(oracle_job* j);
 uint64_t oracle_get_chain_nodes(oracle_job* j, oracle_chain_node* out, uint64_t cap);
 
+/* Graph cleaning after oracle_run (clean_deadend_and_bubbles_naive,
+ * test/common/compact_dbg_clean.hpp:560-850, do_work with cleaning on,
+ * compact_dbg_freq_construct_clean.cpp:606-701): erases the weak edges
+ * that tie short deadends and bubbles to their branch points, round by
+ * round, recompacting after each.  The first compaction's outputs stay as
+ * they are; the cleaned graph's outputs (.clean.*) come from
+ * oracle_get_clean_output: which = 0 chain FASTA, 1 branch FASTA, 2 chain
+ * components, 3 branch edges, 4 chain edges.  The graph nodes afterwards
+ * are the cleaned graph's (oracle_get_clean_nodes). */
+int oracle_run_clean(oracle_job* j, int lr_variant);
+uint64_t oracle_clean_rounds(oracle_job* j);
+uint64_t oracle_clean_deadends(oracle_job* j);
+uint64_t oracle_clean_bubbles(oracle_job* j);
+uint64_t oracle_get_clean_output(oracle_job* j, int which, char* out, uint64_t cap);
+uint64_t oracle_n_clean_chain_nodes(oracle_job* j);
+uint64_t oracle_get_clean_nodes(oracle_job* j, oracle_node* out, uint64_t cap);
+
 /* Wall-clock seconds of the last oracle_run's phases:
  * [0]=parse [1]=count [2]=filter [3]=chains [4]=list rank [5]=output. */
 void oracle_phase_seconds(oracle_job* j, double* out6);

@@ -65,6 +65,15 @@ def lib():
             f.argtypes = [vp, vp, u64]
         L.oracle_phase_seconds.argtypes = [vp, vp]
         L.oracle_set_order_seed.argtypes = [vp, u64]
+        L.oracle_run_clean.argtypes = [vp, ctypes.c_int]
+        for name in ["clean_rounds", "clean_deadends", "clean_bubbles", "n_clean_chain_nodes"]:
+            f = getattr(L, "oracle_" + name)
+            f.restype = u64
+            f.argtypes = [vp]
+        L.oracle_get_clean_output.restype = u64
+        L.oracle_get_clean_output.argtypes = [vp, ctypes.c_int, vp, u64]
+        L.oracle_get_clean_nodes.restype = u64
+        L.oracle_get_clean_nodes.argtypes = [vp, vp, u64]
         _LIB = L
     return _LIB
 
@@ -164,6 +173,25 @@ class Oracle:
 
     def chain_edges(self):
         return self._str("oracle_get_chain_edges")
+
+    def run_clean(self, lr_variant=1):
+        """Graph cleaning after run(): the .clean.* outputs."""
+        rc = lib().oracle_run_clean(self.h, lr_variant)
+        if rc != 0:
+            raise RuntimeError("oracle_run_clean failed %d" % rc)
+
+    CLEAN_OUTPUTS = ("chain_fasta", "branch_fasta", "chain_components", "branch_edges", "chain_edges")
+
+    def clean_output(self, name):
+        which = self.CLEAN_OUTPUTS.index(name)
+        f = lib().oracle_get_clean_output
+        n = f(self.h, which, None, 0)
+        buf = ctypes.create_string_buffer(max(n, 1))
+        f(self.h, which, buf, n)
+        return buf.raw[:n]
+
+    def clean_nodes(self):
+        return self._nodes("oracle_get_clean_nodes")
 
     def phase_seconds(self):
         out = np.zeros(6, np.float64)

@@ -200,8 +200,10 @@ def classify(g):
     return br, ch
 
 
-def unitigs(g):
-    """Walk chains of non-branch nodes.  Returns (sorted [(rep, seq)], n_cycle_nodes).
+def walk(g):
+    """Walk chains of non-branch nodes.  Returns (paths, cycle_nodes): every
+    chain as the list of its oriented k-mers in walking order, and the set of
+    canonical k-mers on closed walks.
 
     Works on oriented k-mers: the successor of oriented k-mer u is the unique
     out neighbour of u (in u's orientation) if u has exactly one, it is a
@@ -210,18 +212,8 @@ def unitigs(g):
     cycles and dropped."""
     br, ch = classify(g)
 
-    def out_edges(u):
-        x = canon(u)
-        n = g[x]
-        if u == x:
-            return [u[1:] + c for c in "ACGT" if n[IDX[c]] > 0]
-        return [u[1:] + COMP[c] for c in "ACGT" if n[4 + IDX[c]] > 0]
-
-    def in_edges(u):
-        return [rc(v) for v in out_edges(rc(u))]
-
     def succ(u):
-        o = out_edges(u)
+        o = out_neighbours(g, u)
         if len(o) != 1:
             return None
         v = o[0]
@@ -230,8 +222,8 @@ def unitigs(g):
         return v
 
     seen = set()
-    result = []
-    cycle_nodes = 0
+    paths = []
+    cycles = set()
     for x in sorted(ch):
         if x in seen:
             continue
@@ -261,7 +253,7 @@ def unitigs(g):
                 comp.add(canon(v))
                 v = succ(v)
             seen |= comp
-            cycle_nodes += len(comp)
+            cycles |= comp
             continue
         path = [u]
         while True:
@@ -271,12 +263,33 @@ def unitigs(g):
             path.append(v)
         for p in path:
             seen.add(canon(p))
+        paths.append(path)
+    return paths, cycles
+
+
+def out_neighbours(g, u):
+    """The out neighbours of oriented k-mer u (on u's strand)."""
+    x = canon(u)
+    n = g[x]
+    if u == x:
+        return [u[1:] + c for c in "ACGT" if n[IDX[c]] > 0]
+    return [u[1:] + COMP[c] for c in "ACGT" if n[4 + IDX[c]] > 0]
+
+
+def unitigs(g):
+    """(sorted [(rep, seq)], n_cycle_nodes) of the walked chains: each chain
+    in the orientation that starts with its representative (the smaller of
+    its first k-mer and its last k-mer's reverse complement)."""
+    paths, cycles = walk(g)
+    result = []
+    for path in paths:
+        k = len(path[0])
         s = path[0] + "".join(p[-1] for p in path[1:])
-        rep = min(s[:len(x)], rc(s)[:len(x)])
-        seq = s if s[:len(x)] == rep else rc(s)
+        rep = min(s[:k], rc(s)[:k])
+        seq = s if s[:k] == rep else rc(s)
         result.append((rep, seq))
     result.sort()
-    return result, cycle_nodes
+    return result, len(cycles)
 
 
 def chain_fasta(g):
@@ -323,3 +336,149 @@ def chain_edges(g, k):
         vals = in_counts(L) + out_counts(R) + [mean, min(selfs), max(selfs)]
         lines.append((L, L + "\t" + R + "\t" + "".join(str(v) + "\t" for v in vals) + "\n"))
     return "".join(t for _, t in sorted(lines))
+
+
+def _edge_index(u, out, c):
+    """(canonical k-mer, counter index) of oriented u's edge on side out
+    (else in) with base c."""
+    x = canon(u)
+    if u == x:
+        return x, (IDX[c] if out else 4 + IDX[c])
+    return x, (4 + IDX[COMP[c]] if out else IDX[COMP[c]])
+
+
+def _erase_edge_both(g, u, out, c):
+    x, i = _edge_index(u, out, c)
+    if x in g:
+        g[x][i] = 0
+    if out:
+        y, j = _edge_index(u[1:] + c, False, u[0])
+    else:
+        y, j = _edge_index(c + u[:-1], True, u[-1])
+    if y in g:
+        g[y][j] = 0
+
+
+def _drop_isolated(g):
+    for x in [x for x, n in g.items() if sum(n[:8]) == 0]:
+        del g[x]
+
+
+def _drop_nodes(g, xs):
+    for x in xs:
+        n = g.get(x)
+        if n is None:
+            continue
+        for c in "ACGT":
+            if n[IDX[c]] > 0:
+                _erase_edge_both(g, x, True, c)
+            if n[4 + IDX[c]] > 0:
+                _erase_edge_both(g, x, False, c)
+        del g[x]
+
+
+def _total(g, u, out):
+    x = canon(u)
+    n = g[x]
+    idx = range(0, 4) if (out == (u == x)) else range(4, 8)
+    return min(65535, sum(n[i] for i in idx))
+
+
+def chain_summaries(g):
+    """One summary per walked chain: (branch before the 5' end or all-A,
+    5' k-mer, 3' k-mer, branch after the 3' end or all-A, length in k-mers
+    minus one, total count of edges into the 5' end, out of the 3' end),
+    oriented so the 5' k-mer is the chain's representative."""
+    paths, _ = walk(g)
+    out = []
+    for path in paths:
+        k = len(path[0])
+        s = path[0] + "".join(p[-1] for p in path[1:])
+        if s[:k] != min(s[:k], rc(s)[:k]):
+            path = [rc(u) for u in reversed(path)]
+        t5, t3 = path[0], path[-1]
+        ins = [rc(v) for v in out_neighbours(g, rc(t5))]
+        outs = out_neighbours(g, t3)
+        b5 = ins[0] if ins else "A" * k
+        b3 = outs[0] if outs else "A" * k
+        out.append((b5, t5, t3, b3, len(path) - 1, _total(g, t5, False), _total(g, t3, True)))
+    return out
+
+
+def clean(g, k, th2):
+    """The graph after bruno's deadend / bubble cleaning loop, restated on
+    strings: chains summarised by walking, a deadend is a chain with exactly
+    one open end, shorter than 2k - 1, whose end counts are both below
+    th2 + 1; bubbles are runs of chains between the same two branch points
+    (on the strand whose 5' branch is smaller, ordered by length) whose
+    neighbours differ in length by at most one and have an end count below
+    th2 + 1, when a run holds two or more chains.  The edges tying them to
+    their branch points are erased (both directions) when weaker than
+    th2 + 1, edgeless nodes are dropped, cycles are dropped, and the loop
+    repeats until nothing is found (or a round erases nothing).  Returns
+    (graph, rounds)."""
+    g = {x: list(n) for x, n in g.items()}
+    min_freq = th2 + 1
+    _, cyc = walk(g)
+    _drop_nodes(g, cyc)
+    _drop_isolated(g)
+    rounds = 0
+
+    def find(g):
+        summ = chain_summaries(g)
+        dead = [x for x in summ if x[4] < 2 * k - 1 and x[5] < min_freq and x[6] < min_freq
+                and ((x[5] == 0) != (x[6] == 0))]
+        norm = []
+        for x in summ:
+            if x[4] > 0 and x[5] > 0 and x[6] > 0:
+                if x[0] > rc(x[3]):
+                    x = (rc(x[3]), rc(x[2]), rc(x[1]), rc(x[0]), x[4], x[6], x[5])
+                norm.append(x)
+        norm.sort(key=lambda x: (x[0], x[3], x[4], x[1], x[2]))
+        bub = []
+        first = 0
+        for i in range(1, len(norm) + 1):
+            if i < len(norm):
+                a, b = norm[i - 1], norm[i]
+                if a[0] == b[0] and a[3] == b[3] and abs(a[4] - b[4]) <= 1 and \
+                        min(a[5], a[6], b[5], b[6]) < min_freq:
+                    continue
+            if i - first > 1:
+                bub += norm[first:i]
+            first = i
+        return dead, bub
+
+    def erase_weak(g, q):
+        hit = []
+        for u, out, c in q:
+            x, i = _edge_index(u, out, c)
+            if x in g and 0 < g[x][i] < min_freq:
+                hit.append((u, out, c))
+        for u, out, c in hit:
+            _erase_edge_both(g, u, out, c)
+        return len(hit)
+
+    dead, bub = find(g)
+    while dead or bub:
+        q = []
+        for b5, t5, t3, b3, _, f5, f6 in dead:
+            if f5 > 0:
+                q.append((b5, True, t5[-1]))
+            if f6 > 0:
+                q.append((b3, False, t3[0]))
+        erased = erase_weak(g, q)
+        _drop_isolated(g)
+        q = []
+        for b5, t5, t3, b3, _, _, _ in bub:
+            q.append((b5, True, t5[-1]))
+            q.append((b3, False, t3[0]))
+        erased += erase_weak(g, q)
+        _drop_isolated(g)
+        if erased == 0:
+            break  # nothing changed: the same chains would be found again
+        _, cyc = walk(g)
+        _drop_nodes(g, cyc)
+        _drop_isolated(g)
+        dead, bub = find(g)
+        rounds += 1
+    return g, rounds
