@@ -35,6 +35,11 @@ class Stats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in STAT_FIELDS]
 
 
+class CleanStats(ctypes.Structure):
+    _fields_ = [("rounds", ctypes.c_uint32), ("deadends", ctypes.c_uint64), ("bubble_chains", ctypes.c_uint64),
+                ("edges_erased", ctypes.c_uint64)]
+
+
 class SynthDesc(ctypes.Structure):
     _fields_ = [("seed", ctypes.c_uint64), ("genome_len", ctypes.c_uint64),
                 ("n_families", ctypes.c_uint32), ("family_min_len", ctypes.c_uint32),
@@ -104,7 +109,7 @@ EXPORTS = (
     "bruno_server_get_stats", "bruno_server_destroy", "bruno_set_tuning", "bruno_get_tuning",
     "bruno_fastq_split", "bruno_parse_count_fastq", "bruno_parse_count_fastq_host", "bruno_estimate_distinct",
     "bruno_job_set_recycle", "bruno_get_branch_nodes", "bruno_get_chain_edges", "bruno_batch_get_branch_nodes",
-    "bruno_batch_get_chain_edges", "bruno_fastq_split_noalloc",
+    "bruno_batch_get_chain_edges", "bruno_fastq_split_noalloc", "bruno_clean",
 )
 
 _LIB = None
@@ -135,6 +140,7 @@ def lib():
         for n in ("bruno_filter", "bruno_build_chains", "bruno_compact"):
             getattr(L, n).argtypes = [vp]
         L.bruno_run_pipeline.argtypes = [vp, i]
+        L.bruno_clean.argtypes = [vp, i, u64, ctypes.POINTER(CleanStats)]
         L.bruno_list_rank.argtypes = [vp, i, ctypes.POINTER(ctypes.c_uint32)]
         L.bruno_get_stats.argtypes = [vp, ctypes.POINTER(Stats)]
         L.bruno_get_phase_ms.argtypes = [vp, ctypes.POINTER(PhaseMs)]

@@ -1438,6 +1438,21 @@ __global__ void k_head_flags(const uint32_t* dist_sorted, uint64_t n, uint8_t* f
   if (i < n) flags[i] = dist_sorted[i] == 0;
 }
 
+#include "clean.h"
+
+/* the sort keys of the bubble candidates, one array per field */
+template <typename KT>
+__global__ void k_summary_keys(const ChainSummary<KT>* S, uint64_t n, KT* b5, KT* t5, KT* t3, KT* b3, uint32_t* len) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    b5[i] = S[i].b5;
+    t5[i] = S[i].t5;
+    t3[i] = S[i].t3;
+    b3[i] = S[i].b3;
+    len[i] = S[i].len;
+  }
+}
+
 /* reads of one synthetic job at bases, offsets[r] = obase + r * L */
 __global__ void k_synth(br::synth_params p, char* bases, uint64_t* offsets, uint64_t obase) {
   uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1550,6 +1565,7 @@ struct bruno_job {
   uint64_t n_present = 0, n_branch = 0, n_chain = 0, n_virt = 0;
   uint64_t* chain_slot = nullptr;
   uint64_t* branch_slot = nullptr;
+  uint64_t list_cap = 0; /* entries each list has room for */
 
   /* chains */
   void* ckmer = nullptr;
@@ -2222,6 +2238,7 @@ bruno_status filter_t(bruno_job* j) {
   const uint64_t lcap = std::max<uint64_t>(1, n_cand + n_spilled);
   TRY(dev_alloc(j, (void**)&j->chain_slot, lcap * sizeof(uint64_t)));
   TRY(dev_alloc(j, (void**)&j->branch_slot, lcap * sizeof(uint64_t)));
+  j->list_cap = lcap;
   if (j->n_jobs > 1) {
     TRY(dev_alloc(j, (void**)&j->chain_job, lcap * sizeof(uint32_t)));
     TRY(dev_alloc(j, (void**)&j->branch_job, lcap * sizeof(uint32_t)));
@@ -3223,6 +3240,277 @@ bruno_status bruno_compact(bruno_job* j) {
   /* fasta_len = heads*per_head + (n - heads)  =>  heads = (len - n)/(per_head-1) */
   j->n_unitigs = (j->fasta_len >= n_body && per_head > 1) ? (j->fasta_len - n_body) / (per_head - 1) : 0;
   j->stage = ST_COMPACTED;
+  return BRUNO_OK;
+}
+
+/* ================================================================== */
+/* graph cleaning (host side)                                           */
+/* ================================================================== */
+extern "C++" {
+/* Everything allocated after the node lists (chains, ranking, output) goes
+ * back to the pool; the table and the node lists stay. */
+static void free_after_lists(bruno_job* j) {
+  (void)hipStreamSynchronize(j->stream);
+  std::vector<void*> keep;
+  for (void* p : j->allocs) {
+    if (persistent(j, p) || p == j->chain_slot || p == j->branch_slot) keep.push_back(p);
+    else pool_free(p);
+  }
+  j->allocs = keep;
+  j->device_bytes = persistent_bytes(j) + 2 * j->list_cap * sizeof(uint64_t);
+  j->ckmer = j->vkmer = j->bkmer = j->out_k = j->out_rep = j->bsorted = nullptr;
+  j->lr = nullptr;
+  j->list[0] = j->list[1] = nullptr;
+  j->alive = nullptr;
+  j->out_dist = j->perm = j->dist_sorted = nullptr;
+  j->out_job = nullptr;
+  j->fasta = j->bfasta = nullptr;
+  j->cbits = j->sbits = nullptr;
+  j->bits_fresh = false;
+  j->lr_contracted = j->out_by_chains = false;
+  j->n_out = j->n_unitigs = j->fasta_len = j->bfasta_len = 0;
+  j->chain_start.clear();
+  j->fasta_off.clear();
+  j->branch_start.clear();
+}
+
+/* new chain / branch lists from the nodes still in the graph, classified
+ * by their current edges */
+template <typename KT>
+static bruno_status relist_t(bruno_job* j) {
+  const uint64_t nc = j->n_chain, nb = j->n_branch, lcap = std::max<uint64_t>(1, nc + nb);
+  uint64_t *cs = nullptr, *bs = nullptr;
+  TRY(dev_alloc(j, (void**)&cs, lcap * sizeof(uint64_t)));
+  TRY(dev_alloc(j, (void**)&bs, lcap * sizeof(uint64_t)));
+  HIP_CHECK(hipMemsetAsync(&j->ctr->n_branch, 0, 2 * sizeof(unsigned long long), j->stream));
+  NodeLists nl{cs, bs, nullptr, nullptr, lcap, j->d_seg, 1};
+  if (nc + nb)
+    hipLaunchKernelGGL((k_relist<KT>), dim3(grid_for(nc + nb)), dim3(BLOCK), 0, j->stream, (Slot<KT>*)j->table, j->K,
+                       j->chain_slot, nc, j->branch_slot, nb, nl, j->ctr);
+  TRY(sync_counters(j));
+  void* a = j->chain_slot;
+  dev_free(j, a, j->list_cap * sizeof(uint64_t));
+  a = j->branch_slot;
+  dev_free(j, a, j->list_cap * sizeof(uint64_t));
+  j->chain_slot = cs;
+  j->branch_slot = bs;
+  j->list_cap = lcap;
+  j->n_chain = j->hctr->n_chain;
+  j->n_branch = j->hctr->n_branch;
+  j->n_present = j->n_chain + j->n_branch;
+  return BRUNO_OK;
+}
+
+/* erase_nodes of the separated chain nodes (cycles and edgeless ones) */
+template <typename KT>
+static bruno_status drop_separated_t(bruno_job* j) {
+  const uint64_t nc = j->n_chain;
+  if (!nc || !j->alive) return BRUNO_OK;
+  const uint64_t tcap = probe_span(j); /* plain jobs: one probe window per k-mer over the table */
+  uint8_t* mark = nullptr;
+  TRY(dev_alloc(j, (void**)&mark, nc));
+  hipLaunchKernelGGL((k_drop_separated<KT>), dim3(grid_for(nc)), dim3(BLOCK), 0, j->stream, (Slot<KT>*)j->table, tcap,
+                     j->rbits, j->K, j->chain_slot, j->alive, nc, mark);
+  hipLaunchKernelGGL((k_drop_separated_edges<KT>), dim3(grid_for(nc)), dim3(BLOCK), 0, j->stream,
+                     (Slot<KT>*)j->table, tcap, j->rbits, j->K, j->chain_slot, mark, nc);
+  void* a = mark;
+  dev_free(j, a, nc);
+  return BRUNO_OK;
+}
+
+/* erase_if(IsIsolated) over both node lists */
+template <typename KT>
+static bruno_status drop_edgeless_t(bruno_job* j) {
+  if (j->n_chain)
+    hipLaunchKernelGGL((k_drop_edgeless<KT>), dim3(grid_for(j->n_chain)), dim3(BLOCK), 0, j->stream,
+                       (Slot<KT>*)j->table, j->chain_slot, j->n_chain);
+  if (j->n_branch)
+    hipLaunchKernelGGL((k_drop_edgeless<KT>), dim3(grid_for(j->n_branch)), dim3(BLOCK), 0, j->stream,
+                       (Slot<KT>*)j->table, j->branch_slot, j->n_branch);
+  HIP_CHECK(hipGetLastError());
+  return BRUNO_OK;
+}
+
+/* the compacted graph again, from the node lists on */
+template <typename KT>
+static bruno_status recompact_t(bruno_job* j, int variant) {
+  TRY(relist_t<KT>(j));
+  free_after_lists(j);
+  HIP_CHECK(hipMemsetAsync(&j->ctr->n_virt, 0, sizeof(unsigned long long), j->stream));
+  j->stage = ST_FILTERED;
+  TRY(bruno_build_chains(j));
+  TRY(bruno_list_rank(j, variant, nullptr));
+  return bruno_compact(j);
+}
+
+/* Deadend and bubble edges of the compacted graph: dq one query per deadend,
+ * bq two per bubble chain (device arrays the caller frees). */
+template <typename KT>
+static bruno_status pick_t(bruno_job* j, uint64_t min_freq, EdgeQuery<KT>** dq, uint64_t* ndq, EdgeQuery<KT>** bq,
+                           uint64_t* nbq, uint64_t* dq_bytes, uint64_t* bq_bytes) {
+  *dq = *bq = nullptr;
+  *ndq = *nbq = *dq_bytes = *bq_bytes = 0;
+  const uint64_t n = j->n_out, nu = j->n_unitigs;
+  if (nu == 0) return BRUNO_OK;
+  const uint64_t tcap = probe_span(j); /* plain jobs: one probe window per k-mer over the table */
+  uint8_t* flags = nullptr;
+  uint32_t* starts = nullptr;
+  ChainSummary<KT>*S = nullptr, *N = nullptr;
+  unsigned long long* cnt = nullptr;
+  void* t = nullptr;
+  size_t tmp = 0;
+  hipcub::CountingInputIterator<uint32_t> pos(0);
+  TRY(dev_alloc(j, (void**)&flags, n));
+  TRY(dev_alloc(j, (void**)&starts, (n + 1) * 4)); /* room for every node: checked against nu below */
+  TRY(dev_alloc(j, (void**)&cnt, 4 * sizeof(unsigned long long)));
+  TRY(dev_alloc(j, (void**)&S, nu * sizeof(ChainSummary<KT>)));
+  TRY(dev_alloc(j, (void**)&N, nu * sizeof(ChainSummary<KT>)));
+  TRY(dev_alloc(j, (void**)dq, nu * sizeof(EdgeQuery<KT>)));
+  *dq_bytes = nu * sizeof(EdgeQuery<KT>);
+  HIP_CHECK(hipMemsetAsync(cnt, 0, 4 * sizeof(unsigned long long), j->stream));
+  hipLaunchKernelGGL(k_head_flags, dim3(grid_for(n, BLOCK, 1u << 30)), dim3(BLOCK), 0, j->stream, j->dist_sorted, n,
+                     flags);
+  HIP_CHECK(hipcub::DeviceSelect::Flagged(nullptr, tmp, pos, flags, starts, (uint32_t*)(cnt + 3), (int)n, j->stream));
+  TRY(dev_alloc(j, &t, std::max<size_t>(tmp, 1)));
+  HIP_CHECK(hipcub::DeviceSelect::Flagged(t, tmp, pos, flags, starts, (uint32_t*)(cnt + 3), (int)n, j->stream));
+  uint32_t heads = 0;
+  HIP_CHECK(hipMemcpyAsync(&heads, cnt + 3, 4, hipMemcpyDeviceToHost, j->stream));
+  HIP_CHECK(hipStreamSynchronize(j->stream));
+  if (heads != nu) return BRUNO_INTERNAL; /* one dist-0 node per unitig */
+  const uint32_t n32 = (uint32_t)n;
+  HIP_CHECK(hipMemcpyAsync(starts + nu, &n32, 4, hipMemcpyHostToDevice, j->stream));
+  hipLaunchKernelGGL((k_chain_summaries<KT>), dim3(grid_for(nu)), dim3(BLOCK), 0, j->stream,
+                     (const Slot<KT>*)j->table, tcap, j->rbits, j->K, j->perm, (const KT*)j->out_k, j->dist_sorted,
+                     starts, nu, S, j->ctr);
+  hipLaunchKernelGGL((k_pick_chains<KT>), dim3(grid_for(nu)), dim3(BLOCK), 0, j->stream, S, nu, j->K, min_freq, *dq,
+                     cnt, nu, N, cnt + 1, j->ctr);
+  unsigned long long h[4];
+  HIP_CHECK(hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, j->stream));
+  TRY(sync_counters(j));
+  *ndq = h[0];
+  const uint64_t nn = h[1];
+  if (nn > 0) {
+    KT *b5 = nullptr, *t5 = nullptr, *t3 = nullptr, *b3 = nullptr;
+    uint32_t *len = nullptr, *order = nullptr;
+    TRY(dev_alloc(j, (void**)&b5, nn * sizeof(KT)));
+    TRY(dev_alloc(j, (void**)&t5, nn * sizeof(KT)));
+    TRY(dev_alloc(j, (void**)&t3, nn * sizeof(KT)));
+    TRY(dev_alloc(j, (void**)&b3, nn * sizeof(KT)));
+    TRY(dev_alloc(j, (void**)&len, nn * 4));
+    TRY(dev_alloc(j, (void**)&order, nn * 4));
+    TRY(dev_alloc(j, (void**)bq, 2 * nn * sizeof(EdgeQuery<KT>)));
+    *bq_bytes = 2 * nn * sizeof(EdgeQuery<KT>);
+    hipLaunchKernelGGL((k_summary_keys<KT>), dim3(grid_for(nn, BLOCK, 1u << 30)), dim3(BLOCK), 0, j->stream, N, nn,
+                       b5, t5, t3, b3, len);
+    /* by (5' branch, 3' branch, length, 5' end, 3' end): stable passes from
+     * the last key to the first */
+    TRY(sort_perm_kmer<KT>(j, t3, order, nn, j->K, false));
+    TRY(sort_perm_kmer<KT>(j, t5, order, nn, j->K, true));
+    TRY(sort_perm_u32(j, len, order, nn, 32));
+    TRY(sort_perm_kmer<KT>(j, b3, order, nn, j->K, true));
+    TRY(sort_perm_kmer<KT>(j, b5, order, nn, j->K, true));
+    hipLaunchKernelGGL((k_pick_bubbles<KT>), dim3(grid_for(nn)), dim3(BLOCK), 0, j->stream, N, order, nn, j->K,
+                       min_freq, *bq, cnt + 2, 2 * nn, j->ctr);
+    HIP_CHECK(hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, j->stream));
+    TRY(sync_counters(j));
+    *nbq = h[2];
+    void* a = b5;
+    dev_free(j, a, nn * sizeof(KT));
+    a = t5;
+    dev_free(j, a, nn * sizeof(KT));
+    a = t3;
+    dev_free(j, a, nn * sizeof(KT));
+    a = b3;
+    dev_free(j, a, nn * sizeof(KT));
+    a = len;
+    dev_free(j, a, nn * 4);
+    a = order;
+    dev_free(j, a, nn * 4);
+  }
+  if (j->hctr->err) return BRUNO_INTERNAL;
+  void* a = flags;
+  dev_free(j, a, n);
+  a = starts;
+  dev_free(j, a, (n + 1) * 4);
+  a = cnt;
+  dev_free(j, a, 4 * sizeof(unsigned long long));
+  a = S;
+  dev_free(j, a, nu * sizeof(ChainSummary<KT>));
+  a = N;
+  dev_free(j, a, nu * sizeof(ChainSummary<KT>));
+  dev_free(j, t, std::max<size_t>(tmp, 1));
+  return BRUNO_OK;
+}
+
+/* the queried edges below min_freq, erased on both nodes */
+template <typename KT>
+static bruno_status erase_t(bruno_job* j, const EdgeQuery<KT>* q, uint64_t n, uint64_t min_freq, uint64_t* erased) {
+  *erased = 0;
+  if (n == 0) return BRUNO_OK;
+  const uint64_t tcap = probe_span(j); /* plain jobs: one probe window per k-mer over the table */
+  uint8_t* hit = nullptr;
+  unsigned long long* cnt = nullptr;
+  TRY(dev_alloc(j, (void**)&hit, n));
+  TRY(dev_alloc(j, (void**)&cnt, sizeof(unsigned long long)));
+  HIP_CHECK(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), j->stream));
+  hipLaunchKernelGGL((k_edge_hits<KT>), dim3(grid_for(n)), dim3(BLOCK), 0, j->stream, (const Slot<KT>*)j->table, tcap,
+                     j->rbits, j->K, q, n, min_freq, hit);
+  hipLaunchKernelGGL((k_erase_edges<KT>), dim3(grid_for(n)), dim3(BLOCK), 0, j->stream, (Slot<KT>*)j->table, tcap,
+                     j->rbits, j->K, q, n, hit, cnt);
+  unsigned long long h = 0;
+  HIP_CHECK(hipMemcpyAsync(&h, cnt, sizeof(h), hipMemcpyDeviceToHost, j->stream));
+  HIP_CHECK(hipStreamSynchronize(j->stream));
+  *erased = h;
+  void* a = hit;
+  dev_free(j, a, n);
+  a = cnt;
+  dev_free(j, a, sizeof(unsigned long long));
+  return BRUNO_OK;
+}
+
+template <typename KT>
+static bruno_status clean_t(bruno_job* j, int variant, uint64_t k1_lo, bruno_clean_stats* st) {
+  const uint64_t min_freq = k1_lo + 1; /* edge_freq_filter(threshes[2] + 1) */
+  /* do_work drops cycles and edgeless nodes before cleaning */
+  TRY(drop_separated_t<KT>(j));
+  TRY(drop_edgeless_t<KT>(j));
+  for (;;) {
+    EdgeQuery<KT>*dq = nullptr, *bq = nullptr;
+    uint64_t ndq = 0, nbq = 0, e1 = 0, e2 = 0, dqb = 0, bqb = 0;
+    bruno_status s = pick_t<KT>(j, min_freq, &dq, &ndq, &bq, &nbq, &dqb, &bqb);
+    if (s == BRUNO_OK) s = erase_t<KT>(j, dq, ndq, min_freq, &e1);
+    if (s == BRUNO_OK) s = drop_edgeless_t<KT>(j);
+    if (s == BRUNO_OK) s = erase_t<KT>(j, bq, nbq, min_freq, &e2);
+    if (s == BRUNO_OK) s = drop_edgeless_t<KT>(j);
+    void* a = dq;
+    dev_free(j, a, dqb);
+    a = bq;
+    dev_free(j, a, bqb);
+    if (s != BRUNO_OK) return s;
+    st->deadends += ndq;
+    st->bubble_chains += nbq / 2;
+    st->edges_erased += e1 + e2;
+    /* nothing found, or nothing weak enough to erase: the same chains
+     * would be found again */
+    if (e1 + e2 == 0) break;
+    TRY(recompact_t<KT>(j, variant));
+    TRY(drop_separated_t<KT>(j));
+    TRY(drop_edgeless_t<KT>(j));
+    ++st->rounds;
+  }
+  /* the outputs of the cleaned graph (its branch points and unitigs) */
+  return recompact_t<KT>(j, variant);
+}
+}  // extern "C++"
+
+bruno_status bruno_clean(bruno_job* j, int variant, uint64_t k1_lo, bruno_clean_stats* out) {
+  if (!valid_job(j) || (variant != 0 && variant != 1)) return BRUNO_BAD_INPUT;
+  if (j->n_jobs != 1) return BRUNO_BAD_INPUT;
+  if (j->stage != ST_COMPACTED || j->recycle || j->clear_pending) return BRUNO_STATE;
+  bruno_clean_stats st{};
+  bruno_status s = j->words == 1 ? clean_t<uint64_t>(j, variant, k1_lo, &st) : clean_t<u128>(j, variant, k1_lo, &st);
+  if (s != BRUNO_OK) return s;
+  if (out) *out = st;
   return BRUNO_OK;
 }
 

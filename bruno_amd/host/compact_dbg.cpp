@@ -2,7 +2,16 @@
  * compact_dbg.cpp -- host driver with the reference driver's call order and
  * flags (test/test/compact_dbg_freq_construct_clean.cpp:431-840, :848-1030):
  *
- *   compact_dbg [-T] [-E] [-L t | -L k k1 k2] [-U ...] [-R] [-B] [-K k] -O prefix reads.fastq...
+ *   compact_dbg [-T] [-E] [-L t | -L k k1 k2] [-U ...] [-R] [-B] [--clean] [-K k] -O prefix reads.fastq...
+ *
+ * It mirrors the benchmarked binary, which is built with cleaning compiled
+ * out (-DNCLEAN; its -N switch then does nothing and is accepted here the
+ * same way).  --clean runs the cleaning that binary leaves out
+ * (compact_dbg_freq_construct_clean.cpp:633-701): after the files above it
+ * writes <prefix>.clean.branch.fasta and <prefix>.clean.chain.fasta, and
+ * unless -B <prefix>.clean.chain.components, .clean.branch.edges and
+ * .clean.chain.edges, from the graph cleaned of short weak deadends and
+ * weak bubbles.
  *
  * -E filters as the block-construct driver does
  * (compact_debruijn_graph_block_construct.cpp:415-480, :686-712): with -T
@@ -53,7 +62,7 @@ static std::string canon_string(const std::string& s) {
 
 int main(int argc, char** argv) {
   std::vector<size_t> threshes(6, 65536);
-  bool thresholding = false, benchmark = false, lr_opt = false, edge_filter = false;
+  bool thresholding = false, benchmark = false, lr_opt = false, edge_filter = false, clean = false;
   int k = 31;
   std::string prefix = "./output";
   std::vector<std::string> files;
@@ -67,12 +76,15 @@ int main(int argc, char** argv) {
     else if (a == "-B") benchmark = true;
     else if (a == "-E") edge_filter = true;
     else if (a == "-R") lr_opt = true;
+    else if (a == "--clean") clean = true;
+    else if (a == "-N") continue;
     else if (a == "-L") vals(lower);
     else if (a == "-U") vals(upper);
     else if (a == "-K" && i + 1 < argc) k = atoi(argv[++i]);
     else if (a == "-O" && i + 1 < argc) prefix = argv[++i];
     else if (a == "-h" || a == "--help") {
-      std::cout << "usage: compact_dbg [-T] [-E] [-L t|-L k k1 k2] [-U ...] [-R] [-B] [-K k] -O prefix reads.fastq...\n";
+      std::cout << "usage: compact_dbg [-T] [-E] [-L t|-L k k1 k2] [-U ...] [-R] [-B] [--clean] [-K k] -O prefix "
+                   "reads.fastq...\n";
       return 0;
     } else files.push_back(a);
   }
@@ -115,21 +127,31 @@ int main(int argc, char** argv) {
     if (!benchmark) write_file(prefix + ".graph.all", idx.graph_all());
     chainmap.extract_chains(idx);
     size_t iters = lr_opt ? chainmap.list_rank_min_update() : chainmap.list_rank();
-    write_file(prefix + ".branch.fasta", chainmap.branch_fasta());
-    write_file(prefix + ".chain.fasta", chainmap.chain_fasta());
-    if (!benchmark) {
-      /* print_chain_nodes: sorted by K, "canon(K)\trep\tdist\t(K==canon)" */
-      auto nodes = chainmap.to_ranked_chain_nodes();
-      std::sort(nodes.begin(), nodes.end(),
-                [](const listranked_chain_node& a, const listranked_chain_node& b) { return a.kmer < b.kmer; });
-      std::string out;
-      for (auto& n : nodes) {
-        std::string ks = n.kmer.to_string(k), c = canon_string(ks);
-        out += c + "\t" + n.rep.to_string(k) + "\t" + std::to_string(n.dist) + "\t" + (ks == c ? "1" : "0") + "\n";
+    auto outputs = [&](const std::string& p) {
+      write_file(p + ".branch.fasta", chainmap.branch_fasta());
+      write_file(p + ".chain.fasta", chainmap.chain_fasta());
+      if (!benchmark) {
+        /* print_chain_nodes: sorted by K, "canon(K)\trep\tdist\t(K==canon)" */
+        auto nodes = chainmap.to_ranked_chain_nodes();
+        std::sort(nodes.begin(), nodes.end(),
+                  [](const listranked_chain_node& a, const listranked_chain_node& b) { return a.kmer < b.kmer; });
+        std::string out;
+        for (auto& n : nodes) {
+          std::string ks = n.kmer.to_string(k), c = canon_string(ks);
+          out += c + "\t" + n.rep.to_string(k) + "\t" + std::to_string(n.dist) + "\t" + (ks == c ? "1" : "0") + "\n";
+        }
+        write_file(p + ".chain.components", out);
+        write_file(p + ".branch.edges", chainmap.branch_edges());
+        write_file(p + ".chain.edges", chainmap.chain_edges());
       }
-      write_file(prefix + ".chain.components", out);
-      write_file(prefix + ".branch.edges", chainmap.branch_edges());
-      write_file(prefix + ".chain.edges", chainmap.chain_edges());
+    };
+    outputs(prefix);
+    if (clean) {
+      const bruno_clean_stats cs = clean_deadend_and_bubbles(idx, chainmap, threshes, lr_opt);
+      std::printf("cleaning: %u rounds, %llu deadends, %llu bubble chains, %llu edges erased\n", cs.rounds,
+                  (unsigned long long)cs.deadends, (unsigned long long)cs.bubble_chains,
+                  (unsigned long long)cs.edges_erased);
+      outputs(prefix + ".clean");
     }
     bruno_stats st;
     check(bruno_get_stats(j.get(), &st), "stats");

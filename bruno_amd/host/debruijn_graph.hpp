@@ -272,7 +272,21 @@ class debruijn_chain_graph {
   }
   job& j_;
   bool compacted_ = false;
+  friend bruno_clean_stats clean_deadend_and_bubbles(debruijn_graph&, debruijn_chain_graph&,
+                                                     const std::vector<size_t>&, bool);
 };
+
+/* clean_deadend_and_bubbles_naive (test/common/compact_dbg_clean.hpp
+ * :560-850) with do_work's cycle / edgeless-node erasure before it
+ * (compact_dbg_freq_construct_clean.cpp:606-610): afterwards idx and
+ * chainmap hold the cleaned graph, whose outputs are the .clean.* files. */
+inline bruno_clean_stats clean_deadend_and_bubbles(debruijn_graph&, debruijn_chain_graph& chainmap,
+                                                   const std::vector<size_t>& threshes, bool lr_optimized) {
+  chainmap.compact();
+  bruno_clean_stats st{};
+  check(bruno_clean(chainmap.j_.get(), lr_optimized ? 1 : 0, threshes[2], &st), "clean_deadend_and_bubbles");
+  return st;
+}
 
 }  // namespace bruno_amd
 

@@ -220,6 +220,14 @@ class GraphJob(_Views):
     def run(self, variant=1):
         check(lib().bruno_run_pipeline(self.h, int(variant)), "bruno_run_pipeline")
 
+    def clean(self, variant=1, k1_lo=65536):
+        """Deadend / bubble cleaning after run(); the getters then return
+        the cleaned graph's outputs.  k1_lo: the driver's lower k+1-mer
+        threshold (65536 without -L)."""
+        st = _lib.CleanStats()
+        check(lib().bruno_clean(self.h, int(variant), int(k1_lo), ctypes.byref(st)), "bruno_clean")
+        return {f: int(getattr(st, f)) for f, _ in _lib.CleanStats._fields_}
+
     # ---- getters -----------------------------------------------------
     def _call(self, name, *args):
         fn = getattr(lib(), "bruno_get_" + name)

@@ -230,6 +230,27 @@ bruno_status bruno_compact(bruno_job* job);
 /* Convenience: filter + build_chains + list_rank + compact. */
 bruno_status bruno_run_pipeline(bruno_job* job, int variant);
 
+/* Graph cleaning after bruno_compact: replaces the cycle / edgeless-node
+ * erasure of do_work (compact_dbg_freq_construct_clean.cpp:606-610) and
+ * clean_deadend_and_bubbles_naive (test/common/compact_dbg_clean.hpp
+ * :560-850) with find_deadends / find_bubbles / erase_edges
+ * (debruijn_topo_operations.hpp:152-409, debruijn_graph_map.hpp:737-773).
+ * Round by round it erases, on both nodes, the edges that tie short weak
+ * deadends and weak bubbles to their branch points when their count is
+ * below k1_lo + 1 (k1_lo: the driver's threshes[2], 65536 without -L),
+ * drops nodes left without edges and compacts the graph again, until a
+ * round finds nothing.  Afterwards the job holds the cleaned graph: the
+ * getters return its unitigs, chain nodes, branch points and edge outputs
+ * (the .clean.* files).  Plain jobs only (BRUNO_BAD_INPUT for a batch);
+ * BRUNO_STATE before bruno_compact or with table recycling on. */
+typedef struct {
+  uint32_t rounds;         /* rounds that erased edges */
+  uint64_t deadends;       /* deadend chains found, summed over rounds */
+  uint64_t bubble_chains;  /* chains in bubbles found, summed over rounds */
+  uint64_t edges_erased;   /* queried edges below the threshold, erased */
+} bruno_clean_stats;
+bruno_status bruno_clean(bruno_job* job, int variant, uint64_t k1_lo, bruno_clean_stats* out);
+
 bruno_status bruno_get_stats(bruno_job* job, bruno_stats* out);
 bruno_status bruno_get_phase_ms(bruno_job* job, bruno_phase_ms* out);
 

@@ -1452,6 +1452,12 @@ uint64_t oracle_get_clean_output(oracle_job* j, int which, char* out, uint64_t c
   return copy_str(*s[which], out, cap);
 }
 uint64_t oracle_n_clean_chain_nodes(oracle_job* j) { return j->clean_ranked.size(); }
+uint64_t oracle_get_clean_chain_nodes(oracle_job* j, oracle_chain_node* out, uint64_t cap) {
+  const uint64_t n = j->clean_ranked.size();
+  if (out)
+    for (uint64_t i = 0; i < n && i < cap; ++i) out[i] = j->clean_ranked[i];
+  return n;
+}
 uint64_t oracle_get_clean_nodes(oracle_job* j, oracle_node* out, uint64_t cap) { return copy_nodes(j->nodes, out, cap); }
 
 void oracle_phase_seconds(oracle_job* j, double* out6) {

@@ -115,6 +115,7 @@ uint64_t oracle_clean_deadends(oracle_job* j);
 uint64_t oracle_clean_bubbles(oracle_job* j);
 uint64_t oracle_get_clean_output(oracle_job* j, int which, char* out, uint64_t cap);
 uint64_t oracle_n_clean_chain_nodes(oracle_job* j);
+uint64_t oracle_get_clean_chain_nodes(oracle_job* j, oracle_chain_node* out, uint64_t cap);
 uint64_t oracle_get_clean_nodes(oracle_job* j, oracle_node* out, uint64_t cap);
 
 /* Wall-clock seconds of the last oracle_run's phases:

@@ -74,6 +74,8 @@ def lib():
         L.oracle_get_clean_output.argtypes = [vp, ctypes.c_int, vp, u64]
         L.oracle_get_clean_nodes.restype = u64
         L.oracle_get_clean_nodes.argtypes = [vp, vp, u64]
+        L.oracle_get_clean_chain_nodes.restype = u64
+        L.oracle_get_clean_chain_nodes.argtypes = [vp, vp, u64]
         _LIB = L
     return _LIB
 
@@ -192,6 +194,13 @@ class Oracle:
 
     def clean_nodes(self):
         return self._nodes("oracle_get_clean_nodes")
+
+    def clean_chain_nodes(self):
+        f = lib().oracle_get_clean_chain_nodes
+        n = f(self.h, None, 0)
+        out = np.zeros(n, CHAIN_DTYPE)
+        f(self.h, out.ctypes.data, n)
+        return out
 
     def phase_seconds(self):
         out = np.zeros(6, np.float64)
