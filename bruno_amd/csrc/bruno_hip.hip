@@ -2131,10 +2131,20 @@ bruno_status count_by_regions(bruno_job* j, const ParseArgs& a, bool* done) {
     cbits = nullptr;
   }
   if (!sbits) cbits = nullptr;
-  RegionArgs g{j->table, j->rspan, j->rbits, nreg, recs, rstart, valid, j->spill, j->spcap,
-               j->ctr, cbits, sbits, stride, j->thresholding, scan_lo(j)};
   const dim3 g4((unsigned)std::min<uint64_t>(nreg, 4 * REGION_BUILD_CU_SLOTS)),
       g2((unsigned)std::min<uint64_t>(nreg, 2 * REGION_BUILD_CU_SLOTS));
+  const bool small = RB == 8 ? j->rspan <= REGION_SMALL_SLOTS : j->rspan <= REGION_WIDE_SMALL_SLOTS;
+  const uint64_t n_waves = small ? (uint64_t)g4.x * 4 : (uint64_t)g2.x * 8;
+  /* side-table queues: a wave's share of the instances / 128 (a k-mer
+   * spills only past its slot's sub-entries), at least 1024 entries; a
+   * full queue adds inline */
+  uint32_t spq_cap = (uint32_t)std::min<uint64_t>(65536, std::max<uint64_t>(1024, n_bound / 128 / n_waves));
+  uint64_t* spq = nullptr;
+  uint32_t* spq_n = nullptr;
+  TRY(dev_alloc(j, (void**)&spq_n, n_waves * 4));
+  if (dev_alloc(j, (void**)&spq, n_waves * spq_cap * 8) != BRUNO_OK) spq_cap = 0;
+  RegionArgs g{j->table, j->rspan, j->rbits, nreg, recs, rstart, valid, j->spill, j->spcap, spq, spq_n, spq_cap,
+               j->ctr, cbits, sbits, stride, j->thresholding, scan_lo(j)};
   if (RB == 8) {
     if (j->rspan <= REGION_SMALL_SLOTS)
       hipLaunchKernelGGL((k_region_build<uint64_t, REGION_SMALL_SLOTS, 256>), g4, dim3(256), 0, j->stream, g);
@@ -2146,12 +2156,19 @@ bruno_status count_by_regions(bruno_job* j, const ParseArgs& a, bool* done) {
     else
       hipLaunchKernelGGL((k_region_build<u128, REGION_WIDE_SLOTS, 512>), g2, dim3(512), 0, j->stream, g);
   }
+  if (spq_cap)
+    hipLaunchKernelGGL(k_spill_flush, dim3((unsigned)n_waves), dim3(BLOCK), 0, j->stream, spq, spq_n, spq_cap,
+                       j->spill, j->spcap, j->ctr);
   /* instances and dropped reads into the job's counters */
   hipLaunchKernelGGL(k_add_u64, dim3(1), dim3(64), 0, j->stream, (unsigned long long*)&j->ctr->n_instances, stat, 1);
   hipLaunchKernelGGL(k_add_u64, dim3(1), dim3(64), 0, j->stream, (unsigned long long*)&j->ctr->n_dropped, stat + 1,
                      1);
   HIP_CHECK(hipStreamSynchronize(j->stream));
   (void)N;
+  void* qp = spq;
+  dev_free(j, qp, n_waves * spq_cap * 8);
+  qp = spq_n;
+  dev_free(j, qp, n_waves * 4);
   give_up(); /* every buffer of the count back to the pool */
   j->table_valid = true;
   j->cbits = cbits;

@@ -295,6 +295,13 @@ __device__ __forceinline__ void for_each_instance(const uint8_t* s, uint64_t L, 
   }, f);
 }
 
+/* lane l's value of a 64-bit register (l wave-uniform) */
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, int l) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
+  return ((uint64_t)hi << 32) | lo;
+}
+
 /* MODE 0: count records per read; MODE 1: write records; MODE 2: count into
  * the table; MODE 3: distinct-k-mer sketch only.  BS is the workgroup size
  * (the work is per wave, so any multiple of 64 works; it is a tuning knob). */

@@ -286,6 +286,14 @@ struct RegionArgs {
   int valid;                 /* the table holds earlier counts (else start from EMPTY) */
   Spill* spill;
   uint64_t spcap;
+  /* side-table entries are queued, not added inline: wave w of the grid
+   * appends (slot << 5 | biedge index) to spq[w * spq_cap ...] and leaves
+   * its count in spq_n[w]; k_spill_flush adds them after the build (an
+   * inline add is a chain of dependent HBM atomics that the whole
+   * workgroup waits for at the next barrier) */
+  uint64_t* spq;
+  uint32_t* spq_n;
+  uint32_t spq_cap;
   Counters* ctr;
   /* filter pre-pass (optional): per region `stride` words of candidate
    * bits (occupied, no spilled entries, k-mer count >= t1lo when
@@ -400,10 +408,13 @@ __device__ __forceinline__ uint32_t lds_claim(LdsRegion<u128, SLOTS>& L, uint32_
   return ~0u;
 }
 
-/* add_k2 on LDS slot s; gs is the slot's table index (side-table key) */
+/* add_k2 on LDS slot s; gs is the slot's table index (side-table key).
+ * Returns 1 for a new k2 entry in the slot, 0 for a count added, and
+ * LDS_SPILL when the entry belongs in the side table (the slot is marked
+ * as having spilled entries; the caller queues the entry). */
+constexpr int LDS_SPILL = 2;
 template <typename KT, uint32_t SLOTS>
-__device__ __forceinline__ int lds_add_k2(LdsRegion<KT, SLOTS>& L, uint32_t s, uint8_t e, uint64_t gs, Spill* sp,
-                                          uint64_t spcap, uint32_t* err) {
+__device__ __forceinline__ int lds_add_k2(LdsRegion<KT, SLOTS>& L, uint32_t s, uint8_t e) {
   const uint64_t code = (uint64_t)e + 1;
 #pragma unroll
   for (int j = 0; j < Slot<KT>::NSUB; ++j) {
@@ -421,7 +432,14 @@ __device__ __forceinline__ int lds_add_k2(LdsRegion<KT, SLOTS>& L, uint32_t s, u
   }
   uint64_t* fw = L.flag(s);
   if (!(lds_load(fw) & F_SPILL)) atomicOr((unsigned long long*)fw, (unsigned long long)F_SPILL);
-  return spill_add(sp, spcap, gs, code, err);
+  return LDS_SPILL;
+}
+
+/* an empty use of a register: the compiler waits for its load here */
+__device__ __forceinline__ void consume(uint64_t v) { asm volatile("" ::"v"(v)); }
+__device__ __forceinline__ void consume(const ulonglong2& v) {
+  asm volatile("" ::"v"(v.x));
+  asm volatile("" ::"v"(v.y));
 }
 
 /* the k-mer of a record of region q */
@@ -432,11 +450,22 @@ __device__ __forceinline__ uint64_t rec_hash(const RegionArgs& g, uint64_t q, ui
 __device__ __forceinline__ uint64_t rec_kmer(uint64_t h, uint64_t) { return kmer_unhash(h); }
 __device__ __forceinline__ u128 rec_kmer(uint64_t h, const ulonglong2& r) { return kmer_unhash(h, r.y); }
 
-/* Add one record of region q (table slots from base) to the LDS region. */
+/* The biedge byte of biedge index i = il * 5 + ir (nibble 0 for side index
+ * 0, else 1 << (index - 1)), computed: a table lookup would be a global
+ * load per record on the build's critical path. */
+__device__ __forceinline__ uint32_t biedge_of_index(uint32_t i) {
+  const uint32_t il = (i * 52u) >> 8, ir = i - 5u * il; /* i / 5 for i < 25 */
+  return (((1u << il) >> 1) << 4) | ((1u << ir) >> 1);
+}
+
+/* Add one record of region q (table slots from base) to the LDS region.
+ * Returns the side-table entry to queue (slot << 5 | biedge index), or
+ * NO_SPILL. */
+constexpr uint64_t NO_SPILL = ~0ULL;
 template <typename KT, uint32_t SLOTS, typename R>
-__device__ __forceinline__ void region_add(const RegionArgs& g, LdsRegion<KT, SLOTS>& L, uint32_t span, uint64_t q,
-                                           uint64_t base, const R& rec, unsigned long long& my_new,
-                                           unsigned long long& my_k2) {
+__device__ __forceinline__ uint64_t region_add(const RegionArgs& g, LdsRegion<KT, SLOTS>& L, uint32_t span,
+                                               uint64_t q, uint64_t base, const R& rec, unsigned long long& my_new,
+                                               unsigned long long& my_k2) {
   const uint64_t key = rec_key(rec);
   const uint32_t code = (uint32_t)(key & 31);
   const uint64_t h = rec_hash(g, q, key);
@@ -445,10 +474,46 @@ __device__ __forceinline__ void region_add(const RegionArgs& g, LdsRegion<KT, SL
   const uint32_t s = lds_claim(L, span, (uint32_t)__umul64hi(h << g.rbits, g.span), km, &claimed);
   if (s == ~0u) {
     atomicOr(&g.ctr->err, ERR_TABLE_FULL);
-    return;
+    return NO_SPILL;
   }
   my_new += claimed;
-  my_k2 += lds_add_k2<KT, SLOTS>(L, s, BIEDGE25[code], base + s, g.spill, g.spcap, &g.ctr->err);
+  const int r = lds_add_k2<KT, SLOTS>(L, s, (uint8_t)biedge_of_index(code));
+  if (r == LDS_SPILL) return ((base + s) << 5) | code;
+  my_k2 += r;
+  return NO_SPILL;
+}
+
+/* Queue side-table entries from converged code: one ballot per record
+ * slot of the tile; qn (this wave's queue fill) stays wave-uniform.  A full
+ * queue adds the entry inline. */
+__device__ __forceinline__ void spill_queue(const RegionArgs& g, uint64_t* q, uint32_t& qn, uint64_t e,
+                                            unsigned long long& my_k2) {
+  const uint64_t m = __ballot(e != NO_SPILL);
+  if (!m) return;
+  const uint32_t lane = __lane_id();
+  const uint32_t at = qn + (uint32_t)__popcll(m & ((1ULL << lane) - 1));
+  if (e != NO_SPILL) {
+    if (at < g.spq_cap) {
+      q[at] = e;
+    } else {
+      const uint64_t bq = biedge_of_index((uint32_t)(e & 31)) + 1;
+      my_k2 += spill_add(g.spill, g.spcap, e >> 5, bq, &g.ctr->err);
+    }
+  }
+  qn += (uint32_t)__popcll(m);
+}
+
+/* Side-table entries the region builds queued, one workgroup per queue. */
+__global__ __launch_bounds__(BLOCK) void k_spill_flush(const uint64_t* spq, const uint32_t* spq_n, uint32_t spq_cap,
+                                                        Spill* spill, uint64_t spcap, Counters* ctr) {
+  const uint32_t n = min(spq_n[blockIdx.x], spq_cap);
+  const uint64_t* q = spq + (uint64_t)blockIdx.x * spq_cap;
+  unsigned long long mine = 0;
+  for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
+    const uint64_t e = q[i];
+    mine += spill_add(spill, spcap, e >> 5, biedge_of_index((uint32_t)(e & 31)) + 1, &ctr->err);
+  }
+  block_add(mine, &ctr->n_k2);
 }
 
 /* Build regions [qa, qb), workgroup i taking an even share of the regions.
@@ -473,14 +538,33 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) void k_
   const R* recs = (const R*)g.recs;
   unsigned long long my_new = 0, my_k2 = 0;
   constexpr uint64_t TILE = (uint64_t)REGION_ITEMS * BS;
+  /* region_start[q] for the workgroup's regions, 64 at a time: lane l of
+   * every wave holds entry rs_base + l and the next 64 are loaded ahead,
+   * so a region's bounds are two lane reads, not a dependent load */
+  const int lane = threadIdx.x & 63;
+  uint64_t rs_base = qa;
+  auto rs_load = [&](uint64_t b) -> uint64_t { return b + lane <= nreg ? g.region_start[b + lane] : 0; };
+  uint64_t rs = rs_load(qa), rs_nx = rs_load(qa + 64);
+  auto rstart = [&](uint64_t q) -> uint64_t { /* q never decreases */
+    while (q >= rs_base + 64) {
+      rs = rs_nx;
+      rs_base += 64;
+      rs_nx = rs_load(rs_base + 64);
+    }
+    return readlane64(rs, (int)(q - rs_base));
+  };
+  /* this wave's side-table queue */
+  const uint64_t wave_id = (uint64_t)blockIdx.x * (BS / 64) + (threadIdx.x >> 6);
+  uint64_t* spq = g.spq + wave_id * g.spq_cap;
+  uint32_t qn = 0;
   /* first tile of the next region with records, loaded ahead */
   R rn[REGION_ITEMS];
   uint64_t nq = qa, nlo = 0, nhi = 0;
   auto ahead = [&](uint64_t from) {
     nq = from;
     for (; nq < qb; ++nq) {
-      nlo = g.region_start[nq];
-      nhi = g.region_start[nq + 1];
+      nlo = rstart(nq);
+      nhi = rstart(nq + 1);
       if (!(g.valid && nlo == nhi)) break; /* no records: the region stays as it is */
     }
     if (nq < qb) {
@@ -506,19 +590,36 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) void k_
     }
     ahead(q + 1);
     lds_barrier();
-    for (uint64_t i0 = lo; i0 < hi; i0 += TILE) {
-      if (i0 != lo) {
+    /* the first tile, loaded ahead; a region longer than one tile loads
+     * the rest here (in registers of their own, so the first tile's adds
+     * wait for nothing) */
 #pragma unroll
-        for (int k = 0; k < REGION_ITEMS; ++k) {
-          const uint64_t i = i0 + (uint64_t)k * BS + threadIdx.x;
-          rr[k] = i < hi ? recs[i] : rec_zero<R>();
-        }
+    for (int k = 0; k < REGION_ITEMS; ++k) {
+      uint64_t e = NO_SPILL;
+      if (lo + (uint64_t)k * BS + threadIdx.x < hi) e = region_add<KT, SLOTS, R>(g, L, span, q, base, rr[k], my_new, my_k2);
+      spill_queue(g, spq, qn, e, my_k2);
+    }
+    for (uint64_t i0 = lo + TILE; i0 < hi; i0 += TILE) {
+      R rt[REGION_ITEMS];
+#pragma unroll
+      for (int k = 0; k < REGION_ITEMS; ++k) {
+        const uint64_t i = i0 + (uint64_t)k * BS + threadIdx.x;
+        rt[k] = i < hi ? recs[i] : rec_zero<R>();
       }
 #pragma unroll
-      for (int k = 0; k < REGION_ITEMS; ++k)
+      for (int k = 0; k < REGION_ITEMS; ++k) {
+        uint64_t e = NO_SPILL;
         if (i0 + (uint64_t)k * BS + threadIdx.x < hi)
-          region_add<KT, SLOTS, R>(g, L, span, q, base, rr[k], my_new, my_k2);
+          e = region_add<KT, SLOTS, R>(g, L, span, q, base, rt[k], my_new, my_k2);
+        spill_queue(g, spq, qn, e, my_k2);
+      }
     }
+    /* the next region's records (loaded since this region began) are
+     * waited for here, before the write-back's stores: stores and loads
+     * share one in-order counter, so waiting for them after the stores
+     * would wait for the stores too */
+#pragma unroll
+    for (int k = 0; k < REGION_ITEMS; ++k) consume(rn[k]);
     lds_barrier();
     if (g.cbits) {
       /* the filter scan's choice for every slot of the region, one
@@ -559,6 +660,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) void k_
   if ((threadIdx.x & 63) == 0) {
     if (sn) atomicAdd(&g.ctr->n_distinct, sn);
     if (sk) atomicAdd(&g.ctr->n_k2, sk);
+    g.spq_n[wave_id] = qn;
   }
 }
 
