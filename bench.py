@@ -211,6 +211,9 @@ def main():
     ap.add_argument("--edge-outputs", action="store_true",
                     help="after the timed steps, run one more step without table recycling and time the "
                          ".branch.edges / .chain.edges getters (non-benchmark outputs)")
+    ap.add_argument("--fuse-filter", action="store_true",
+                    help="run the filter inside the region build's write-back (bruno_job_set_fused_filter) "
+                         "instead of as a pass of its own after the count")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the timed steps, copy 4 GiB device-to-device once (a plain copy kernel of "
                          "known size, to scale FETCH_SIZE / WRITE_SIZE counters)")
@@ -264,6 +267,8 @@ def main():
     # steps repeat one job: its table is cleared beside list ranking and
     # output of each step, so the next step's reset has nothing left to clear
     job.set_recycle(True)
+    # optionally the filter fused into the count (the job's one batch)
+    job.set_fused_filter(args.fuse_filter)
     times, phases = [], []
     stats = None
     for it in range(args.warmup + args.steps):
@@ -284,6 +289,23 @@ def main():
         print("step %d: %.1f ms  %s" % (it, dt * 1e3, json.dumps({k_: round(v, 2) for k_, v in job.phase_ms().items()})),
               file=sys.stderr, flush=True)
     n_inst = stats["n_instances"]
+    fused = bool(stats["filter_fused"])
+    unfused = None
+    if fused:
+        # one more (untimed) step with the filter as a pass of its own, for
+        # the filter's own roofline beside the fused one
+        job.set_fused_filter(False)
+        job.reset()
+        if fq is not None:
+            job.parse_count_fastq_device(fq.ptr, fq.nbytes)
+        else:
+            job.parse_count_device(bases.ptr, offs.ptr, s.n_reads)
+        job.run(args.variant)
+        synchronize()
+        up = job.phase_ms()
+        unfused = {"parse_count_ms": round(up["parse_count_ms"], 3), "filter_ms": round(up["filter_ms"], 3),
+                   "filter_achieved": filter_bytes(stats, k) / (up["filter_ms"] * 1e-3) / 1e9,
+                   "filter_frac": filter_bytes(stats, k) / (up["filter_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS}
     total = sum(times)
     value = n_inst * len(times) / total
     pc_ms = float(np.mean([p["parse_count_ms"] for p in phases]))
@@ -321,8 +343,9 @@ def main():
                                            "n_unitigs", "n_cycle_nodes", "iterations", "capacity", "device_bytes",
                                            "peak_device_bytes")},
         "roofline": {
-            "kernel": ("count phase: k_parse_records + k_split1 + k_split2 + k_region_build (device events "
-                       "around the phase)") if stats["count_by_regions"] else
+            "kernel": ("count phase: k_parse_records + k_split1 + k_split2 + k_region_build%s + k_spill_flush "
+                       "(device events around the phase)" % (" (with the filter fused into its write-back)"
+                                                              if fused else "")) if stats["count_by_regions"] else
                       "k_parse (parse + atomic k2 count)",
             "bound": "hbm",
             "achieved": achieved,
@@ -350,6 +373,25 @@ def main():
             "bytes_per_unit": "16 B per distinct k2-mer read (24 B for k > 31) + 32 B per kept k-mer written (48 B)",
         },
     }
+    if fused:
+        # the filter runs inside the count's region build: its work is timed
+        # with the count, so the count + filter phases are measured together
+        both = count_bytes(s.n_bases, n_inst) + filter_bytes(stats, k)
+        both_ms = pc_ms + ph["filter_ms"]
+        out["roofline_filter"] = {
+            "kernel": "filter fused into k_region_build's write-back (local_insert_by_freqencies + classification "
+                      "in LDS), then bruno_filter's remainder (list holes, k_filter_bits, k_filter_spilled): "
+                      "count + filter phases together",
+            "bound": "hbm",
+            "achieved": both / (both_ms * 1e-3) / 1e9,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": both / (both_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "bytes_per_unit": "count (1 B per base + 128 B per k-mer instance) + filter (16 B per distinct k2-mer "
+                              "+ 32 B per node; 24 B and 48 B for k > 31), over the two phases' time",
+            "remainder_ms": ph["filter_ms"],
+            "unfused": unfused,
+        }
     if args.edge_outputs:
         out["edge_outputs"] = time_edge_outputs(job, bases, offs, s, fq)
     if tuned is not None:

@@ -28,7 +28,8 @@ class PhaseMs(ctypes.Structure):
 STAT_FIELDS = ("n_reads", "n_reads_dropped", "n_instances", "n_distinct", "n_k2", "n_overflow",
                "n_nodes", "n_branches", "n_chain_nodes", "n_cycle_nodes", "n_isolated",
                "n_unitig_nodes", "n_unitigs", "iterations", "n_conflicts", "capacity",
-               "device_bytes", "peak_device_bytes", "output_by_chains", "count_by_regions")
+               "device_bytes", "peak_device_bytes", "output_by_chains", "count_by_regions",
+               "filter_fused")
 
 
 class Stats(ctypes.Structure):
@@ -52,7 +53,8 @@ class SynthDesc(ctypes.Structure):
 class Tuning(ctypes.Structure):
     _fields_ = [("parse_block", ctypes.c_uint32), ("parse_grid", ctypes.c_uint32),
                 ("scan_grid", ctypes.c_uint32), ("ruler_sample", ctypes.c_uint32),
-                ("count_regions", ctypes.c_uint32), ("region_slots", ctypes.c_uint32)]
+                ("count_regions", ctypes.c_uint32), ("region_slots", ctypes.c_uint32),
+                ("fuse_list_cap", ctypes.c_uint32)]
 
 
 class ServerDesc(ctypes.Structure):
@@ -108,7 +110,7 @@ EXPORTS = (
     "bruno_server_get_unitigs", "bruno_server_get_branch_fasta", "bruno_server_release",
     "bruno_server_get_stats", "bruno_server_destroy", "bruno_set_tuning", "bruno_get_tuning",
     "bruno_fastq_split", "bruno_parse_count_fastq", "bruno_parse_count_fastq_host", "bruno_estimate_distinct",
-    "bruno_job_set_recycle", "bruno_get_branch_nodes", "bruno_get_chain_edges", "bruno_batch_get_branch_nodes",
+    "bruno_job_set_recycle", "bruno_job_set_fused_filter", "bruno_get_branch_nodes", "bruno_get_chain_edges", "bruno_batch_get_branch_nodes",
     "bruno_batch_get_chain_edges", "bruno_fastq_split_noalloc", "bruno_clean",
 )
 
@@ -180,6 +182,7 @@ def lib():
         L.bruno_parse_count_fastq.argtypes = [vp, vp, u64]
         L.bruno_parse_count_fastq_host.argtypes = [vp, vp, u64]
         L.bruno_job_set_recycle.argtypes = [vp, i]
+        L.bruno_job_set_fused_filter.argtypes = [vp, i]
         L.bruno_estimate_distinct.argtypes = [i, vp, vp, u64, ctypes.POINTER(ctypes.c_double), u64p,
                                               ctypes.c_double, vp]
         L.bruno_get_tuning.argtypes = [ctypes.POINTER(Tuning)]

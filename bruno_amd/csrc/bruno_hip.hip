@@ -13,6 +13,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <new>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/bruno_hip.h"
@@ -30,52 +31,6 @@ namespace {
 /* ================================================================== */
 /* filter + classify (one thread per table slot)                        */
 /* ================================================================== */
-/* Classify a surviving node (IsBranchPoint / IsChainNode,
- * debruijn_graph_filters.hpp:142-220). */
-template <typename KT>
-__device__ __forceinline__ NodeRec make_node(const kops<KT>& ops, KT k, const uint16_t (&c)[9]) {
-  bool pal = ops.is_rc_palindrome(k) || kplus1_pal<KT>(ops, k, c);
-  int ni = n_in_edges(c), no = n_out_edges(c);
-  NodeRec rec;
-#pragma unroll
-  for (int q = 0; q < 9; ++q) rec.c[q] = c[q];
-  rec.cls = (ni >= 2 || no >= 2 || pal) ? CLS_BRANCH : CLS_CHAIN;
-  rec.pad = 0;
-  rec.idx = 0xFFFFFFFFu;
-  return rec;
-}
-
-/* Node lists written by the filter: chain and branch slots (the index in
- * its list goes into the node record) and, in a batch, each node's job. */
-struct NodeLists {
-  uint64_t* chain_slot;
-  uint64_t* branch_slot;
-  uint32_t* chain_job;
-  uint32_t* branch_job;
-  uint64_t cap; /* capacity of each list */
-  const uint64_t* seg;
-  uint32_t n_jobs;
-};
-
-/* Write the record in place (sub area of the slot) with its list index,
- * mark the slot present and append it to its list at position pos. */
-template <typename KT>
-__device__ __forceinline__ void store_node(Slot<KT>* T, uint64_t s, NodeRec rec, unsigned long long pos,
-                                           const NodeLists& nl, Counters* ctr) {
-  if (pos >= nl.cap || pos > 0x7FFFFFFFULL) {
-    atomicOr(&ctr->err, ERR_LIST_FULL);
-    return;
-  }
-  rec.idx = (uint32_t)pos;
-  Slot<KT>* p = T + s;
-  *(NodeRec*)&p->sub[0] = rec;
-  uint64_t* fw = SlotOps<KT>::flag_word(p);
-  *fw = *fw | F_PRESENT;
-  const bool chain = rec.cls == CLS_CHAIN;
-  (chain ? nl.chain_slot : nl.branch_slot)[pos] = s;
-  if (nl.n_jobs > 1) (chain ? nl.chain_job : nl.branch_job)[pos] = job_of(nl.seg, nl.n_jobs, s);
-}
-
 constexpr int FILTER_SPT = 4; /* slots per thread per reservation */
 
 /* Flag word and k-mer count (sum of the in-slot k2 counts) of one slot,
@@ -235,6 +190,63 @@ __global__ __launch_bounds__(BLOCK) void k_filter_bits(const uint64_t* cbits, co
   }
 }
 
+/* Node lists after a count with the filter fused: positions [0, e) of a
+ * list hold nodes and the unused positions of the build's waves' last
+ * chunks (holes[2 w + l] = start << 7 | count for list l, marked
+ * LIST_HOLE).  The holes are closed by moving the nodes found above the
+ * final count into the holes below it; a moved node's record gets its new
+ * index.  k_hole_count: holes below e0 / e1, per list. */
+__global__ __launch_bounds__(BLOCK) void k_hole_count(const uint64_t* holes, uint64_t n_waves, uint64_t e0,
+                                                      uint64_t e1, unsigned long long* out) {
+  const uint64_t w = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+  unsigned long long c[2] = {0, 0};
+  if (w < n_waves) {
+#pragma unroll
+    for (int l = 0; l < 2; ++l) {
+      const uint64_t h = holes[2 * w + l], start = h >> 7, cnt = h & 127, e = l ? e1 : e0;
+      c[l] = start < e ? (cnt < e - start ? cnt : e - start) : 0;
+    }
+  }
+  const unsigned long long s0 = wave_sum(c[0]), s1 = wave_sum(c[1]);
+  if ((threadIdx.x & 63) == 0) {
+    if (s0) atomicAdd(out, s0);
+    if (s1) atomicAdd(out + 1, s1);
+  }
+}
+/* The nodes at [n, e) of a list, gathered (one atomic per wave). */
+__global__ __launch_bounds__(BLOCK) void k_list_tail(const uint64_t* list, uint64_t n, uint64_t e, uint64_t* tail,
+                                                     uint64_t tail_cap, unsigned long long* cnt) {
+  const uint32_t lane = threadIdx.x & 63;
+  for (uint64_t p0 = n + (uint64_t)blockIdx.x * BLOCK; p0 < e; p0 += (uint64_t)gridDim.x * BLOCK) {
+    const uint64_t p = p0 + threadIdx.x;
+    const bool v = p < e && list[p] != LIST_HOLE;
+    const uint64_t m = __ballot(v);
+    uint64_t b = 0;
+    if (lane == 0 && m) b = atomicAdd(cnt, (unsigned long long)__popcll(m));
+    b = readlane64(b, 0);
+    const uint64_t at = b + __popcll(m & ((1ULL << lane) - 1));
+    if (v && at < tail_cap) tail[at] = p;
+  }
+}
+/* Holes of list l below n filled from the gathered nodes. */
+template <typename KT>
+__global__ __launch_bounds__(BLOCK) void k_list_fill(const uint64_t* holes, uint64_t n_waves, int l, uint64_t* list,
+                                                     uint64_t n, const uint64_t* tail,
+                                                     const unsigned long long* n_tail, unsigned long long* cnt,
+                                                     Slot<KT>* T) {
+  const uint64_t w = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (w >= n_waves) return;
+  const uint64_t h = holes[2 * w + l], start = h >> 7, c = h & 127;
+  const unsigned long long nt = *n_tail;
+  for (uint64_t p = start; p < start + c && p < n; ++p) {
+    const unsigned long long k = atomicAdd(cnt, 1ULL);
+    if (k >= nt) continue; /* more holes than nodes above: counted, and reported by the host */
+    const uint64_t s = list[tail[k]];
+    list[p] = s;
+    ((NodeRec*)&T[s].sub[0])->idx = (uint32_t)p;
+  }
+}
+
 /* Pass 2: local_insert_by_freqencies on the candidate slots (all k2
  * entries in the slot), node classification, record written in place and
  * the node appended to the chain or branch list. */
@@ -257,16 +269,17 @@ __global__ __launch_bounds__(BLOCK) void k_filter_cand(Slot<KT>* T, const uint64
       K2L<Slot<KT>::NSUB> L;
       sub_entries<KT, Slot<KT>::NSUB>(p, L);
       KT k = SlotOps<KT>::kmer(p);
+      const KmerFacts f = kmer_facts(ops, k);
       uint16_t c[9];
       if (thresholding == 2) {
         kept = edge_counts<KT, Slot<KT>::NSUB>(ops, k, L, th, c);
       } else if (thresholding) {
-        kept = p2_counts<KT, Slot<KT>::NSUB>(ops, k, L, th, c);
+        kept = p2_counts_slot<KT, Slot<KT>::NSUB>(ops, f, L, th, c);
       } else {
         p1_counts(L, c);
         kept = true;
       }
-      if (kept) rec = make_node<KT>(ops, k, c);
+      if (kept) rec = make_node_facts<KT>(ops, f, c);
     }
     const bool ch = kept && rec.cls == CLS_CHAIN, br = kept && rec.cls == CLS_BRANCH;
     unsigned long long pc = block_reserve(ch, &ctr->n_chain, sm);
@@ -1534,6 +1547,13 @@ struct bruno_job {
   bool bits_fresh = false;
   uint64_t* cbits = nullptr;
   uint64_t* sbits = nullptr;
+  /* fuse_filter: a count that builds every region also filters (asked by
+   * bruno_job_set_fused_filter); fused: the last count did, leaving the
+   * node lists with its waves' unused positions (holes, 2 per wave) */
+  bool fuse_filter = false;
+  bool fused = false;
+  uint64_t* holes = nullptr;
+  uint64_t n_hole_waves = 0;
   Spill* spill = nullptr;
   /* capacity 0 at create: the table is sized from a HyperLogLog sketch of
    * every batch counted since the reset (SKETCH_M registers, merged), and
@@ -1622,7 +1642,7 @@ namespace {
 /* Process-wide tuning knobs (bruno_set_tuning). */
 std::mutex g_tuning_mu;
 /* defaults: the autotuner's picks on the 100 Mbp / 33 M-read workload */
-bruno_tuning g_tuning = {256, 65536, 32768, 64, 1, 2560};
+bruno_tuning g_tuning = {256, 65536, 32768, 64, 1, 2560, 0};
 bruno_tuning tuning() {
   std::lock_guard<std::mutex> g(g_tuning_mu);
   return g_tuning;
@@ -2112,7 +2132,6 @@ bruno_status count_by_regions(bruno_job* j, const ParseArgs& a, bool* done) {
    * parser's buffer without gaps, with every region's first record */
   std::vector<uint64_t> out(n_sb + 1, 0);
   for (uint64_t q = 0; q < n_sb; ++q) out[q + 1] = out[q] + (cur[q] - base[q]);
-  const uint64_t N = out[n_sb];
   HIP_CHECK(hipMemcpyAsync(d_out, out.data(), (n_sb + 1) * 8, hipMemcpyHostToDevice, j->stream));
   HIP_CHECK(hipMemcpyAsync(d_base, base.data(), n_sb * 8, hipMemcpyHostToDevice, j->stream));
   HIP_CHECK(hipMemcpyAsync(rstart + nreg, &out[n_sb], 8, hipMemcpyHostToDevice, j->stream));
@@ -2143,19 +2162,57 @@ bruno_status count_by_regions(bruno_job* j, const ParseArgs& a, bool* done) {
   uint32_t* spq_n = nullptr;
   TRY(dev_alloc(j, (void**)&spq_n, n_waves * 4));
   if (dev_alloc(j, (void**)&spq, n_waves * spq_cap * 8) != BRUNO_OK) spq_cap = 0;
-  RegionArgs g{j->table, j->rspan, j->rbits, nreg, recs, rstart, valid, j->spill, j->spcap, spq, spq_n, spq_cap,
-               j->ctr, cbits, sbits, stride, j->thresholding, scan_lo(j)};
-  if (RB == 8) {
-    if (j->rspan <= REGION_SMALL_SLOTS)
-      hipLaunchKernelGGL((k_region_build<uint64_t, REGION_SMALL_SLOTS, 256>), g4, dim3(256), 0, j->stream, g);
-    else
-      hipLaunchKernelGGL((k_region_build<uint64_t, REGION_MAX_SLOTS, 512>), g2, dim3(512), 0, j->stream, g);
-  } else {
-    if (j->rspan <= REGION_WIDE_SMALL_SLOTS)
-      hipLaunchKernelGGL((k_region_build<u128, REGION_WIDE_SMALL_SLOTS, 256>), g4, dim3(256), 0, j->stream, g);
-    else
-      hipLaunchKernelGGL((k_region_build<u128, REGION_WIDE_SLOTS, 512>), g2, dim3(512), 0, j->stream, g);
+  /* the filter fused into the build (a first count, by request): node
+   * lists with room for the nodes the table can hold (at most half its
+   * slots, and at most the instances / the count threshold) plus every
+   * wave's open chunks; a node that finds no room stays a candidate for
+   * the filter pass */
+  bool fuse = j->fuse_filter && cbits && j->n_jobs == 1;
+  NodeLists nl{};
+  uint64_t* holes = nullptr;
+  uint64_t lcap = 0;
+  if (fuse) {
+    const uint64_t lo = j->thresholding ? std::max<uint64_t>(1, scan_lo(j)) : 1;
+    lcap = std::min<uint64_t>(j->cap / 2, n_bound / lo) + 2 * FUSE_CHUNK * n_waves + 1024;
+    lcap = std::min<uint64_t>(lcap, 0x7FFFFFFFULL);
+    if (tuning().fuse_list_cap) lcap = std::min<uint64_t>(lcap, tuning().fuse_list_cap);
+    uint64_t *cs = nullptr, *bs = nullptr;
+    if (dev_alloc(j, (void**)&cs, lcap * 8) == BRUNO_OK && dev_alloc(j, (void**)&bs, lcap * 8) == BRUNO_OK &&
+        dev_alloc(j, (void**)&holes, 2 * n_waves * 8) == BRUNO_OK) {
+      j->chain_slot = cs;
+      j->branch_slot = bs;
+      j->list_cap = lcap;
+      nl = NodeLists{cs, bs, nullptr, nullptr, lcap, j->d_seg, 1};
+      HIP_CHECK(hipMemsetAsync(&j->ctr->n_present, 0, 3 * sizeof(unsigned long long), j->stream));
+    } else {
+      void* p = cs;
+      dev_free(j, p, lcap * 8);
+      p = bs;
+      dev_free(j, p, lcap * 8);
+      p = holes;
+      dev_free(j, p, 2 * n_waves * 8);
+      holes = nullptr;
+      fuse = false;
+    }
   }
+  RegionArgs g{j->table, j->rspan, j->rbits, nreg, recs, rstart, valid, j->spill, j->spcap, spq, spq_n, spq_cap,
+               j->ctr, cbits, sbits, stride, j->thresholding, scan_lo(j), fuse ? 1 : 0, j->K, j->th, nl, holes};
+  auto build = [&](auto fuse_tag) {
+    constexpr bool F = decltype(fuse_tag)::value;
+    if (RB == 8) {
+      if (j->rspan <= REGION_SMALL_SLOTS)
+        hipLaunchKernelGGL((k_region_build<uint64_t, REGION_SMALL_SLOTS, 256, F>), g4, dim3(256), 0, j->stream, g);
+      else
+        hipLaunchKernelGGL((k_region_build<uint64_t, REGION_MAX_SLOTS, 512, F>), g2, dim3(512), 0, j->stream, g);
+    } else {
+      if (j->rspan <= REGION_WIDE_SMALL_SLOTS)
+        hipLaunchKernelGGL((k_region_build<u128, REGION_WIDE_SMALL_SLOTS, 256, F>), g4, dim3(256), 0, j->stream, g);
+      else
+        hipLaunchKernelGGL((k_region_build<u128, REGION_WIDE_SLOTS, 512, F>), g2, dim3(512), 0, j->stream, g);
+    }
+  };
+  if (fuse) build(std::true_type{});
+  else build(std::false_type{});
   if (spq_cap)
     hipLaunchKernelGGL(k_spill_flush, dim3((unsigned)n_waves), dim3(BLOCK), 0, j->stream, spq, spq_n, spq_cap,
                        j->spill, j->spcap, j->ctr);
@@ -2164,7 +2221,6 @@ bruno_status count_by_regions(bruno_job* j, const ParseArgs& a, bool* done) {
   hipLaunchKernelGGL(k_add_u64, dim3(1), dim3(64), 0, j->stream, (unsigned long long*)&j->ctr->n_dropped, stat + 1,
                      1);
   HIP_CHECK(hipStreamSynchronize(j->stream));
-  (void)N;
   void* qp = spq;
   dev_free(j, qp, n_waves * spq_cap * 8);
   qp = spq_n;
@@ -2174,6 +2230,9 @@ bruno_status count_by_regions(bruno_job* j, const ParseArgs& a, bool* done) {
   j->cbits = cbits;
   j->sbits = sbits;
   j->bits_fresh = cbits != nullptr;
+  j->fused = fuse;
+  j->holes = holes;
+  j->n_hole_waves = fuse ? n_waves : 0;
   *done = true;
   return BRUNO_OK;
 }
@@ -2223,10 +2282,76 @@ bruno_status parse_count_t(bruno_job* j, const char* d_bases, const uint64_t* d_
   return BRUNO_OK;
 }
 
+/* After a count with the filter fused: the node lists without holes, the
+ * list counters at the nodes placed (returned in placed[2]). */
+template <typename KT>
+bruno_status close_list_holes(bruno_job* j, uint64_t* placed) {
+  TRY(sync_counters(j));
+  const uint64_t nw = j->n_hole_waves, cap = j->list_cap;
+  const uint64_t e[2] = {std::min<uint64_t>(j->hctr->n_chain, cap), std::min<uint64_t>(j->hctr->n_branch, cap)};
+  unsigned long long* tmp = nullptr;
+  uint64_t* tail = nullptr;
+  TRY(dev_alloc(j, (void**)&tmp, 4 * 8));
+  TRY(dev_alloc(j, (void**)&tail, std::max<uint64_t>(1, nw * FUSE_CHUNK) * 8));
+  HIP_CHECK(hipMemsetAsync(tmp, 0, 4 * 8, j->stream));
+  if (nw) hipLaunchKernelGGL(k_hole_count, dim3(grid_for(nw)), dim3(BLOCK), 0, j->stream, j->holes, nw, e[0], e[1], tmp);
+  unsigned long long h[4];
+  HIP_CHECK(hipMemcpyAsync(h, tmp, 2 * 8, hipMemcpyDeviceToHost, j->stream));
+  HIP_CHECK(hipStreamSynchronize(j->stream));
+  uint64_t* lists[2] = {j->chain_slot, j->branch_slot};
+  for (int l = 0; l < 2; ++l) {
+    if (h[l] > e[l]) return BRUNO_INTERNAL;
+    const uint64_t n = e[l] - h[l];
+    placed[l] = n;
+    if (n == e[l]) continue;
+    HIP_CHECK(hipMemsetAsync(tmp + 2, 0, 2 * 8, j->stream));
+    hipLaunchKernelGGL(k_list_tail, dim3(grid_for(e[l] - n)), dim3(BLOCK), 0, j->stream, lists[l], n, e[l], tail,
+                       std::max<uint64_t>(1, nw * FUSE_CHUNK), tmp + 2);
+    hipLaunchKernelGGL(k_list_fill<KT>, dim3(grid_for(nw)), dim3(BLOCK), 0, j->stream, j->holes, nw, l, lists[l], n,
+                       tail, tmp + 2, tmp + 3, (Slot<KT>*)j->table);
+    HIP_CHECK(hipMemcpyAsync(h + 2, tmp + 2, 2 * 8, hipMemcpyDeviceToHost, j->stream));
+    HIP_CHECK(hipStreamSynchronize(j->stream));
+    /* the nodes above n and the holes below it pair up exactly */
+    if (h[2] != h[3] || h[2] > nw * FUSE_CHUNK) return BRUNO_INTERNAL;
+  }
+  unsigned long long nc[2] = {placed[1], placed[0]}; /* n_branch, n_chain */
+  HIP_CHECK(hipMemcpyAsync(&j->ctr->n_branch, nc, 2 * 8, hipMemcpyHostToDevice, j->stream));
+  HIP_CHECK(hipStreamSynchronize(j->stream));
+  void* a = tmp;
+  dev_free(j, a, 4 * 8);
+  a = tail;
+  dev_free(j, a, std::max<uint64_t>(1, nw * FUSE_CHUNK) * 8);
+  a = j->holes;
+  dev_free(j, a, 2 * nw * 8);
+  j->holes = nullptr;
+  return BRUNO_OK;
+}
+
+/* The node lists with room for cap entries each, keeping the first nc /
+ * nb entries. */
+static bruno_status grow_lists(bruno_job* j, uint64_t cap, uint64_t nc, uint64_t nb) {
+  uint64_t *cs = nullptr, *bs = nullptr;
+  TRY(dev_alloc(j, (void**)&cs, cap * 8));
+  TRY(dev_alloc(j, (void**)&bs, cap * 8));
+  if (nc) HIP_CHECK(hipMemcpyAsync(cs, j->chain_slot, nc * 8, hipMemcpyDeviceToDevice, j->stream));
+  if (nb) HIP_CHECK(hipMemcpyAsync(bs, j->branch_slot, nb * 8, hipMemcpyDeviceToDevice, j->stream));
+  void* a = j->chain_slot;
+  dev_free(j, a, j->list_cap * 8);
+  a = j->branch_slot;
+  dev_free(j, a, j->list_cap * 8);
+  j->chain_slot = cs;
+  j->branch_slot = bs;
+  j->list_cap = cap;
+  return BRUNO_OK;
+}
+
 template <typename KT>
 bruno_status filter_t(bruno_job* j) {
   ensure_table_valid(j);
-  HIP_CHECK(hipMemsetAsync(&j->ctr->n_present, 0, 3 * sizeof(unsigned long long), j->stream));
+  /* nodes the count placed (filter fused into it) */
+  uint64_t placed[2] = {0, 0};
+  if (j->fused) TRY(close_list_holes<KT>(j, placed));
+  else HIP_CHECK(hipMemsetAsync(&j->ctr->n_present, 0, 3 * sizeof(unsigned long long), j->stream));
   HIP_CHECK(hipMemsetAsync(&j->ctr->n_spill, 0, sizeof(unsigned long long), j->stream));
   HIP_CHECK(hipMemsetAsync(&j->ctr->n_cand, 0, sizeof(unsigned long long), j->stream));
   /* candidates: at most the distinct k-mers; slots with spilled entries: at
@@ -2251,16 +2376,21 @@ bruno_status filter_t(bruno_job* j) {
   TRY(sync_counters(j));
   const uint64_t n_cand = j->hctr->n_cand, n_spilled = j->hctr->n_spill;
   if (n_cand > n_distinct || n_spilled > n_spill_cap) return BRUNO_CAPACITY;
-  /* nodes: at most the candidates plus the spilled slots, in either list */
-  const uint64_t lcap = std::max<uint64_t>(1, n_cand + n_spilled);
-  TRY(dev_alloc(j, (void**)&j->chain_slot, lcap * sizeof(uint64_t)));
-  TRY(dev_alloc(j, (void**)&j->branch_slot, lcap * sizeof(uint64_t)));
-  j->list_cap = lcap;
-  if (j->n_jobs > 1) {
-    TRY(dev_alloc(j, (void**)&j->chain_job, lcap * sizeof(uint32_t)));
-    TRY(dev_alloc(j, (void**)&j->branch_job, lcap * sizeof(uint32_t)));
+  /* nodes: at most the candidates plus the spilled slots, in either list,
+   * after the nodes already placed */
+  const uint64_t lcap = std::max<uint64_t>(1, std::max(placed[0], placed[1]) + n_cand + n_spilled);
+  if (!j->chain_slot) {
+    TRY(dev_alloc(j, (void**)&j->chain_slot, lcap * sizeof(uint64_t)));
+    TRY(dev_alloc(j, (void**)&j->branch_slot, lcap * sizeof(uint64_t)));
+    j->list_cap = lcap;
+  } else if (lcap > j->list_cap) {
+    TRY(grow_lists(j, lcap, placed[0], placed[1]));
   }
-  NodeLists nl{j->chain_slot, j->branch_slot, j->chain_job, j->branch_job, lcap, j->d_seg, j->n_jobs};
+  if (j->n_jobs > 1) {
+    TRY(dev_alloc(j, (void**)&j->chain_job, j->list_cap * sizeof(uint32_t)));
+    TRY(dev_alloc(j, (void**)&j->branch_job, j->list_cap * sizeof(uint32_t)));
+  }
+  NodeLists nl{j->chain_slot, j->branch_slot, j->chain_job, j->branch_job, j->list_cap, j->d_seg, j->n_jobs};
   if (n_cand)
     hipLaunchKernelGGL((k_filter_cand<KT>), dim3(grid_for(n_cand)), dim3(BLOCK), 0, j->stream, (Slot<KT>*)j->table,
                        cand, n_cand, j->K, j->thresholding, j->th, nl, j->ctr);
@@ -3018,6 +3148,9 @@ bruno_status bruno_job_reset(bruno_job* j) {
   j->n_cycle = j->n_isolated = j->conflicts = 0;
   j->lr_contracted = j->out_by_chains = false;
   j->bits_fresh = false;
+  j->fused = false;
+  j->holes = nullptr;
+  j->n_hole_waves = 0;
   j->cbits = j->sbits = nullptr;
   j->n_out = j->n_unitigs = j->fasta_len = j->bfasta_len = 0;
   j->n_reads = 0;
@@ -3052,6 +3185,7 @@ bruno_status bruno_job_reset(bruno_job* j) {
 static bruno_status parse_count_common(bruno_job* j, const char* d_bases, const uint64_t* d_offsets, uint64_t n_reads,
                                        const uint64_t* d_first, const uint64_t* d_starts = nullptr,
                                        const uint64_t* d_ends = nullptr) {
+  if (j->fused) return BRUNO_STATE; /* the filter already ran on the table */
   ev_record(j, 0);
   if (j->words == 1) TRY(parse_count_t<uint64_t>(j, d_bases, d_offsets, n_reads, d_first, d_starts, d_ends));
   else TRY(parse_count_t<u128>(j, d_bases, d_offsets, n_reads, d_first, d_starts, d_ends));
@@ -3069,7 +3203,7 @@ static bruno_status parse_count_common(bruno_job* j, const char* d_bases, const 
 bruno_status bruno_parse_count(bruno_job* j, const char* d_bases, const uint64_t* d_offsets, uint64_t n_reads) {
   if (!valid_job(j)) return BRUNO_BAD_INPUT;
   if (j->n_jobs != 1) return BRUNO_BAD_INPUT; /* a batch names each read's job */
-  if (j->stage > ST_COUNTED) return BRUNO_STATE;
+  if (j->stage > ST_COUNTED || j->fused) return BRUNO_STATE;
   if (n_reads == 0) {
     j->stage = ST_COUNTED;
     return BRUNO_OK;
@@ -3151,7 +3285,7 @@ static void* job_alloc(void* ctx, uint64_t bytes) {
 bruno_status bruno_parse_count_fastq(bruno_job* j, const char* d_fastq, uint64_t bytes) {
   if (!valid_job(j) || (bytes && !d_fastq)) return BRUNO_BAD_INPUT;
   if (j->n_jobs != 1) return BRUNO_BAD_INPUT;
-  if (j->stage > ST_COUNTED) return BRUNO_STATE;
+  if (j->stage > ST_COUNTED || j->fused) return BRUNO_STATE;
   uint64_t nr = 0, nb = 0;
   uint64_t *st = nullptr, *en = nullptr;
   /* the sequence lines are parsed where they are in the file: only their
@@ -3562,6 +3696,7 @@ bruno_status bruno_get_stats(bruno_job* j, bruno_stats* o) {
   o->peak_device_bytes = j->peak_bytes;
   o->output_by_chains = j->out_by_chains;
   o->count_by_regions = j->by_regions;
+  o->filter_fused = j->fused;
   return BRUNO_OK;
 }
 
@@ -3619,7 +3754,7 @@ bruno_status bruno_batch_get_stats(bruno_job* j, uint32_t idx, bruno_stats* o) {
 
 bruno_status bruno_batch_get_counts(bruno_job* j, uint32_t idx, bruno_node* buf, uint64_t* n) {
   if (!valid_idx(j, idx) || !n) return BRUNO_BAD_INPUT;
-  if (j->stage != ST_COUNTED) return BRUNO_STATE;
+  if (j->stage != ST_COUNTED || j->fused) return BRUNO_STATE; /* fused: the nodes replaced the k2 entries */
   ensure_table_valid(j);
   const uint64_t s0 = j->seg.empty() ? 0 : j->seg[idx], s1 = j->seg.empty() ? 0 : j->seg[idx + 1];
   const bool one = j->n_jobs == 1;
@@ -3676,7 +3811,7 @@ bruno_status bruno_batch_get_counts(bruno_job* j, uint32_t idx, bruno_node* buf,
 bruno_status bruno_batch_get_k2(bruno_job* j, uint32_t idx, uint64_t* kmers, uint8_t* biedges, uint32_t* counts,
                                 uint64_t* n) {
   if (!valid_idx(j, idx) || !n) return BRUNO_BAD_INPUT;
-  if (j->stage != ST_COUNTED) return BRUNO_STATE;
+  if (j->stage != ST_COUNTED || j->fused) return BRUNO_STATE; /* fused: the nodes replaced the k2 entries */
   ensure_table_valid(j);
   const uint64_t s0 = j->seg.empty() ? 0 : j->seg[idx], s1 = j->seg.empty() ? 0 : j->seg[idx + 1];
   const bool one = j->n_jobs == 1;
@@ -4127,6 +4262,12 @@ bruno_status bruno_dev_free(void* p) {
     return BRUNO_INTERNAL;
   }
   pool_free(p);
+  return BRUNO_OK;
+}
+
+bruno_status bruno_job_set_fused_filter(bruno_job* j, int on) {
+  if (!valid_job(j)) return BRUNO_BAD_INPUT;
+  j->fuse_filter = on != 0;
   return BRUNO_OK;
 }
 

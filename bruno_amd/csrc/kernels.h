@@ -547,6 +547,103 @@ __device__ bool p2_counts(const kops<KT>& ops, KT k, const K2L<N>& L, const Thre
   return have;
 }
 
+/* What the filter and the classification need of the k-mer itself. */
+struct KmerFacts {
+  bool kpal;            /* rc palindrome (even k) */
+  bool low, high;       /* k-1 prefix / suffix rc palindromes (odd k) */
+  uint32_t first, last; /* first and last base */
+};
+template <typename KT>
+__device__ __forceinline__ KmerFacts kmer_facts(const kops<KT>& ops, KT k) {
+  return KmerFacts{ops.is_rc_palindrome(k), ops.km1_low(k), ops.km1_high(k), (uint32_t)ops.first(k),
+                   (uint32_t)ops.last(k)};
+}
+
+/* p2_counts for the few k2 entries a slot holds, without branches: every
+ * entry is evaluated (an empty one, count 0 and biedge 0, adds nothing and
+ * is never kept).  Sums are taken in 32 bits over terms clamped at 65535,
+ * which gives the same saturated values as the 64-bit sums clamped after;
+ * thresholds are clamped at 65536, which no clamped value reaches. */
+template <typename KT, int N>
+__device__ __forceinline__ bool p2_counts_slot(const kops<KT>& ops, const KmerFacts& f, const K2L<N>& L,
+                                               const Thresh& t, uint16_t (&c)[9]) {
+  const uint32_t M = 65535;
+  auto th = [](uint64_t x) -> uint32_t { return x < 65536 ? (uint32_t)x : 65536u; };
+  const uint32_t lo0 = th(t.t0lo), hi0 = th(t.t0hi), lo1 = th(t.t1lo), hi1 = th(t.t1hi), lo2 = th(t.t2lo),
+                 hi2 = th(t.t2hi);
+  const bool odd = ops.K & 1;
+  /* the edge whose k+1-mer is an rc palindrome counts half (is_k1_rc_palindrome) */
+  const uint32_t split_in = (odd && f.high && !f.kpal) ? (1u << (3 - f.last)) : 0u;
+  const uint32_t split_out = (odd && f.low && !f.kpal) ? (1u << (3 - f.first)) : 0u;
+  uint32_t kf = 0, kin[4] = {0, 0, 0, 0}, kout[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    const uint32_t cu = j < L.n ? L.c[j] : 0u, e = j < L.n ? L.e[j] : 0u;
+    const uint32_t ni = e >> 4, no = e & 15, half = (cu >> 1) + (cu & 1);
+    const uint32_t vi = min(ni == split_in ? half : cu, M), vo = min(no == split_out ? half : cu, M);
+    kf += min(cu, M);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      kin[q] += ni == (1u << q) ? vi : 0u;
+      kout[q] += no == (1u << q) ? vo : 0u;
+    }
+  }
+  kf = min(kf, M);
+  const bool kf_bad = kf < lo0 || kf >= hi0;
+  uint32_t acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  bool have = false;
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    const uint32_t cu = j < L.n ? min((uint32_t)L.c[j], M) : 0u, e = j < L.n ? L.e[j] : 0u;
+    const uint32_t ni = e >> 4, no = e & 15;
+    /* an entry without an edge on a side takes t1hi there: flagged */
+    uint32_t k1i = 65536u, k1o = 65536u;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      k1i = ni == (1u << q) ? min(kin[q], M) : k1i;
+      k1o = no == (1u << q) ? min(kout[q], M) : k1o;
+    }
+    const bool f2 = k1i < lo1 || k1i >= hi1, f1 = k1o < lo1 || k1o >= hi1;
+    const bool keep = j < L.n && !(cu < lo2 || cu >= hi2) && !kf_bad && !(f1 && f2);
+    const uint32_t ki = keep && !f2 ? ni : 0u, ko = keep && !f1 ? no : 0u;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      acc[q] += ko == (1u << q) ? cu : 0u;
+      acc[4 + q] += ki == (1u << q) ? cu : 0u;
+    }
+    acc[8] += keep ? cu : 0u;
+    have |= keep;
+  }
+#pragma unroll
+  for (int q = 0; q < 9; ++q) c[q] = (uint16_t)min(acc[q], M);
+  return have;
+}
+
+/* make_node with the k-mer's facts at hand, without branches. */
+template <typename KT>
+__device__ __forceinline__ NodeRec make_node_facts(const kops<KT>& ops, const KmerFacts& f, const uint16_t (&c)[9]) {
+  uint32_t om = 0, im = 0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    om |= c[q] > 0 ? (1u << q) : 0u;
+    im |= c[4 + q] > 0 ? (1u << q) : 0u;
+  }
+  /* IsKplus1Palindrome (debruijn_graph_filters.hpp:57-95), odd k: the k-1
+   * prefix an rc palindrome and an out edge whose base is the first base
+   * or its complement, or the k-1 suffix one and an in edge whose base is
+   * the last base or its complement */
+  const bool odd = ops.K & 1;
+  const uint32_t fm = (1u << f.first) | (1u << (3 - f.first)), lm = (1u << f.last) | (1u << (3 - f.last));
+  const bool pal = f.kpal || (odd && ((f.low && (om & fm)) || (f.high && (im & lm))));
+  NodeRec rec;
+#pragma unroll
+  for (int q = 0; q < 9; ++q) rec.c[q] = c[q];
+  rec.cls = (__popc(im) >= 2 || __popc(om) >= 2 || pal) ? CLS_BRANCH : CLS_CHAIN;
+  rec.pad = 0;
+  rec.idx = 0xFFFFFFFFu;
+  return rec;
+}
+
 /* The k+1-mer edge-frequency filter of the block-construct driver
  * (compact_debruijn_graph_block_construct.cpp:415-480): an edge is kept when
  * the count of its canonical k+1-mer over the reads is in [t1lo, t1hi)
@@ -629,6 +726,52 @@ __device__ __forceinline__ bool kplus1_pal(const kops<KT>& ops, KT k, const uint
 
 __device__ __forceinline__ int n_out_edges(const uint16_t* c) { return (c[0] > 0) + (c[1] > 0) + (c[2] > 0) + (c[3] > 0); }
 __device__ __forceinline__ int n_in_edges(const uint16_t* c) { return (c[4] > 0) + (c[5] > 0) + (c[6] > 0) + (c[7] > 0); }
+
+/* Classify a surviving node (IsBranchPoint / IsChainNode,
+ * debruijn_graph_filters.hpp:142-220). */
+template <typename KT>
+__device__ __forceinline__ NodeRec make_node(const kops<KT>& ops, KT k, const uint16_t (&c)[9]) {
+  bool pal = ops.is_rc_palindrome(k) || kplus1_pal<KT>(ops, k, c);
+  int ni = n_in_edges(c), no = n_out_edges(c);
+  NodeRec rec;
+#pragma unroll
+  for (int q = 0; q < 9; ++q) rec.c[q] = c[q];
+  rec.cls = (ni >= 2 || no >= 2 || pal) ? CLS_BRANCH : CLS_CHAIN;
+  rec.pad = 0;
+  rec.idx = 0xFFFFFFFFu;
+  return rec;
+}
+
+/* Node lists written by the filter: chain and branch slots (the index in
+ * its list goes into the node record) and, in a batch, each node's job. */
+struct NodeLists {
+  uint64_t* chain_slot;
+  uint64_t* branch_slot;
+  uint32_t* chain_job;
+  uint32_t* branch_job;
+  uint64_t cap; /* capacity of each list */
+  const uint64_t* seg;
+  uint32_t n_jobs;
+};
+
+/* Write the record in place (sub area of the slot) with its list index,
+ * mark the slot present and append it to its list at position pos. */
+template <typename KT>
+__device__ __forceinline__ void store_node(Slot<KT>* T, uint64_t s, NodeRec rec, unsigned long long pos,
+                                           const NodeLists& nl, Counters* ctr) {
+  if (pos >= nl.cap || pos > 0x7FFFFFFFULL) {
+    atomicOr(&ctr->err, ERR_LIST_FULL);
+    return;
+  }
+  rec.idx = (uint32_t)pos;
+  Slot<KT>* p = T + s;
+  *(NodeRec*)&p->sub[0] = rec;
+  uint64_t* fw = SlotOps<KT>::flag_word(p);
+  *fw = *fw | F_PRESENT;
+  const bool chain = rec.cls == CLS_CHAIN;
+  (chain ? nl.chain_slot : nl.branch_slot)[pos] = s;
+  if (nl.n_jobs > 1) (chain ? nl.chain_job : nl.branch_job)[pos] = job_of(nl.seg, nl.n_jobs, s);
+}
 
 }  // namespace br
 

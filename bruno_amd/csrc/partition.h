@@ -260,7 +260,9 @@ __global__ __launch_bounds__(SPLIT_BS) void k_split2(Split2Args a) {
   for (uint64_t s = blockIdx.x; s < a.n_sb; s += gridDim.x) {
     const uint64_t o0 = a.out_start[s], n = a.out_start[s + 1] - o0;
     const uint64_t lo = a.in_start[s], hi = lo + n;
-    /* count pass: every digit's records in the sub-bucket */
+    /* count pass: every digit's records in the sub-bucket (a region's
+     * count cannot be estimated: a k-mer repeated thousands of times puts
+     * all its instances in one region) */
     for (uint32_t i = threadIdx.x; i < nb; i += SPLIT_BS) cnt[i] = 0;
     __syncthreads();
     for (uint64_t i0 = lo; i0 < hi; i0 += 2 * SPLIT_TILE) {

@@ -303,7 +303,22 @@ struct RegionArgs {
   uint64_t stride;
   int thresholding;
   uint64_t t1lo;
+  /* the filter fused into the build (fuse != 0, with the bits): every
+   * candidate slot gets local_insert_by_freqencies and its branch / chain
+   * class in LDS before the write-back, and its node record; the candidate
+   * bits then keep only what a wave could not place (a full node list).
+   * List positions come in chunks of FUSE_CHUNK per wave; wave w leaves
+   * its open chunks' unused positions marked LIST_HOLE and described in
+   * holes[2 w + list] = start << 7 | count. */
+  int fuse; /* the kernel's FUSE parameter */
+  int K;
+  Thresh th;
+  NodeLists nl;
+  uint64_t* holes;
 };
+
+constexpr uint32_t FUSE_CHUNK = 64;
+constexpr uint64_t LIST_HOLE = ~0ULL;
 
 __device__ __forceinline__ uint64_t lds_load(const uint64_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -503,6 +518,130 @@ __device__ __forceinline__ void spill_queue(const RegionArgs& g, uint64_t* q, ui
   qn += (uint32_t)__popcll(m);
 }
 
+/* ------------------------------------------------------------------ */
+/* the filter fused into the build                                      */
+/* ------------------------------------------------------------------ */
+template <uint32_t SLOTS>
+__device__ __forceinline__ uint64_t lds_kmer(LdsRegion<uint64_t, SLOTS>& L, uint32_t s) {
+  return *L.flag(s) & KEY_BITS;
+}
+template <uint32_t SLOTS>
+__device__ __forceinline__ u128 lds_kmer(LdsRegion<u128, SLOTS>& L, uint32_t s) {
+  return ((u128)L.w[2][s] << 64) | L.w[1][s];
+}
+
+/* The filter pre-pass's choice for LDS slot i: a candidate (occupied, no
+ * side-table entries, not yet a node, k-mer count >= t1lo when
+ * thresholding), or *spilled for a slot with side-table entries. */
+template <typename KT, uint32_t SLOTS>
+__device__ __forceinline__ bool lds_candidate(const RegionArgs& g, LdsRegion<KT, SLOTS>& L, uint32_t i,
+                                              bool& spilled) {
+  const uint64_t fw = *L.flag(i);
+  spilled = fw != EMPTY64 && (fw & F_SPILL);
+  if (fw == EMPTY64 || (fw & (F_SPILL | F_PRESENT))) return false;
+  if (!g.thresholding) return true;
+  uint64_t tot = 0;
+#pragma unroll
+  for (int j = 0; j < Slot<KT>::NSUB; ++j) {
+    const uint64_t v = *L.sub(j, i);
+    tot += v != EMPTY64 ? (v & 0xFFFFFFFFFFFFULL) : 0;
+  }
+  return tot >= g.t1lo;
+}
+
+/* A wave's open chunk of a node list: positions [base, base + size), of
+ * which the first `used` are given out.  Wave-uniform. */
+struct ListChunk {
+  uint64_t base;
+  uint32_t used;
+};
+
+/* Consecutive list positions for the lanes with take set, called by the
+ * whole wave: from the open chunk, then from a new chunk of `size` >= 64
+ * (one atomic) when it runs out. */
+__device__ __forceinline__ uint64_t chunk_take(ListChunk& ch, bool take, unsigned long long* counter,
+                                               uint32_t size = FUSE_CHUNK) {
+  const uint64_t m = __ballot(take);
+  const uint32_t n = (uint32_t)__popcll(m);
+  const uint32_t lane = __lane_id();
+  const uint32_t rank = (uint32_t)__popcll(m & ((1ULL << lane) - 1));
+  const uint32_t room = size - ch.used;
+  uint64_t pos = ch.base + ch.used + rank;
+  if (n > room) {
+    uint64_t nb = 0;
+    if (lane == 0) nb = atomicAdd(counter, (unsigned long long)size);
+    nb = readlane64(nb, 0);
+    if (rank >= room) pos = nb + (rank - room);
+    ch.base = nb;
+    ch.used = n - room;
+  } else {
+    ch.used += n;
+  }
+  return pos;
+}
+
+/* The fused filter for queued slots of the region whose table slots start
+ * at base (lane l holds LDS slot qs when active): the slot's k2 entries
+ * from LDS, local_insert_by_freqencies (debruijn_graph_map.hpp:1351-1609;
+ * the k+1-mer edge filter, or none, as the job asks), IsBranchPoint /
+ * IsChainNode (debruijn_graph_filters.hpp:142-220), then the node record
+ * with its list position and F_PRESENT written into the LDS slot and the
+ * slot appended to its list.  Called by the whole wave; true (for the
+ * wave) when a node found its list full: that slot stays a candidate for
+ * the filter pass. */
+template <typename KT, uint32_t SLOTS>
+__device__ __forceinline__ bool fuse_nodes(const RegionArgs& g, LdsRegion<KT, SLOTS>& L, uint64_t base, uint32_t qs,
+                                           bool active, ListChunk (&ch)[2]) {
+  constexpr int N = Slot<KT>::NSUB;
+  const kops<KT> ops(g.K);
+  bool kept = false;
+  NodeRec rec;
+  rec.cls = 0;
+  if (active) {
+    K2L<N> Lk;
+    Lk.n = 0;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      const uint64_t v = *L.sub(j, qs);
+      const bool ok = v != EMPTY64;
+      Lk.e[j] = ok ? (uint8_t)((v >> 48) - 1) : 0;
+      Lk.c[j] = ok ? (uint32_t)(v & 0xFFFFFFFFFFFFULL) : 0;
+      Lk.n += ok;
+    }
+    const KT k = lds_kmer(L, qs);
+    const KmerFacts f = kmer_facts(ops, k);
+    uint16_t c[9];
+    if (g.thresholding == 2) {
+      kept = edge_counts<KT, N>(ops, k, Lk, g.th, c);
+    } else if (g.thresholding) {
+      kept = p2_counts_slot<KT, N>(ops, f, Lk, g.th, c);
+    } else {
+      p1_counts(Lk, c);
+      kept = true;
+    }
+    if (kept) rec = make_node_facts<KT>(ops, f, c);
+  }
+  const bool chn = kept && rec.cls == CLS_CHAIN, brn = kept && rec.cls == CLS_BRANCH;
+  const uint64_t pc = chunk_take(ch[0], chn, &g.ctr->n_chain);
+  const uint64_t pb = chunk_take(ch[1], brn, &g.ctr->n_branch);
+  bool full = false;
+  if (kept) {
+    const uint64_t pos = chn ? pc : pb;
+    if (pos < g.nl.cap && pos <= 0x7FFFFFFFULL) {
+      rec.idx = (uint32_t)pos;
+      uint64_t w[3];
+      __builtin_memcpy(w, &rec, sizeof(rec));
+#pragma unroll
+      for (int j = 0; j < 3; ++j) *L.sub(j, qs) = w[j];
+      *L.flag(qs) |= F_PRESENT;
+      (chn ? g.nl.chain_slot : g.nl.branch_slot)[pos] = base + qs;
+    } else {
+      full = true;
+    }
+  }
+  return __ballot(full) != 0;
+}
+
 /* Side-table entries the region builds queued, one workgroup per queue. */
 __global__ __launch_bounds__(BLOCK) void k_spill_flush(const uint64_t* spq, const uint32_t* spq_n, uint32_t spq_cap,
                                                         Spill* spill, uint64_t spcap, Counters* ctr) {
@@ -525,7 +664,7 @@ __global__ __launch_bounds__(BLOCK) void k_spill_flush(const uint64_t* spq, cons
  * before this region's records are added, so their latency hides behind
  * the LDS work; barriers order LDS only, so one region's write-back
  * overlaps the next region's set-up. */
-template <typename KT, uint32_t SLOTS, int BS>
+template <typename KT, uint32_t SLOTS, int BS, bool FUSE>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) void k_region_build(RegionArgs g) {
   typedef typename RecOf<KT>::T R;
   /* records per thread per tile: a 64-byte-slot region holds half the
@@ -557,6 +696,8 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) void k_
   const uint64_t wave_id = (uint64_t)blockIdx.x * (BS / 64) + (threadIdx.x >> 6);
   uint64_t* spq = g.spq + wave_id * g.spq_cap;
   uint32_t qn = 0;
+  /* this wave's open chunks of the chain and branch lists (fused filter) */
+  ListChunk ch[2] = {{0, FUSE_CHUNK}, {0, FUSE_CHUNK}};
   /* first tile of the next region with records, loaded ahead */
   R rn[REGION_ITEMS];
   uint64_t nq = qa, nlo = 0, nhi = 0;
@@ -621,30 +762,46 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) void k_
 #pragma unroll
     for (int k = 0; k < REGION_ITEMS; ++k) consume(rn[k]);
     lds_barrier();
+    bool left = false; /* a candidate of this wave's slots found its list full */
+    if constexpr (FUSE) {
+      /* each wave queues the candidates of its slots (lane l holds the
+       * l-th) and filters them 64 at a time, so every lane of a pass has
+       * a slot */
+      const uint32_t lane = __lane_id();
+      uint32_t fs = 0, fn = 0; /* this lane's queued slot, the queue's fill */
+      for (uint32_t i0 = 0; i0 < span; i0 += BS) {
+        const uint32_t i = i0 + threadIdx.x;
+        bool sp = false;
+        const bool c = i < span && lds_candidate(g, L, i, sp);
+        const uint64_t m = __ballot(c);
+        const uint32_t n = (uint32_t)__popcll(m);
+        if (n == 0) continue;
+        const uint32_t rank = (uint32_t)__popcll(m & ((1ULL << lane) - 1));
+        /* a permutation of the lanes: the candidates to lanes fn + rank,
+         * the other lanes after them (mod 64) */
+        const uint32_t to = (c ? fn + rank : fn + n + lane - rank) & 63;
+        const uint32_t v = (uint32_t)__builtin_amdgcn_ds_permute((int)(to << 2), (int)i);
+        if (fn + n < 64) {
+          if (((lane - fn) & 63) < n) fs = v;
+          fn += n;
+        } else {
+          if (lane >= fn) fs = v;
+          left |= fuse_nodes<KT, SLOTS>(g, L, base, fs, true, ch);
+          fs = v; /* lanes below the new fill hold the rest */
+          fn = fn + n - 64;
+        }
+      }
+      if (fn) left |= fuse_nodes<KT, SLOTS>(g, L, base, fs, lane < fn, ch);
+    }
     if (g.cbits) {
       /* the filter scan's choice for every slot of the region, one
-       * ballot per 64 slots */
+       * ballot per 64 slots; with the filter fused, only what a full
+       * list left */
       for (uint32_t i0 = 0; i0 < span; i0 += BS) {
         const uint32_t i = i0 + threadIdx.x;
         bool c = false, sp = false;
-        if (i < span) {
-          const uint64_t fw = *L.flag(i);
-          if (fw != EMPTY64) {
-            if (fw & F_SPILL) {
-              sp = true;
-            } else if (!g.thresholding) {
-              c = true;
-            } else {
-              uint64_t tot = 0;
-#pragma unroll
-              for (int j = 0; j < Slot<KT>::NSUB; ++j) {
-                const uint64_t v = *L.sub(j, i);
-                tot += v != EMPTY64 ? (v & 0xFFFFFFFFFFFFULL) : 0;
-              }
-              c = tot >= g.t1lo;
-            }
-          }
-        }
+        if (i < span) c = lds_candidate(g, L, i, sp);
+        if (FUSE && !left) c = false;
         const uint64_t bc = __ballot(c), bsp = __ballot(sp);
         const uint32_t w0 = i0 + (threadIdx.x & ~63u);
         if ((threadIdx.x & 63) == 0 && w0 < span) {
@@ -653,6 +810,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) void k_
         }
       }
     }
+    if (FUSE) lds_barrier(); /* the node records are in LDS */
     L.template store<BS>(span, T);
     lds_barrier();
   }
@@ -661,6 +819,17 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) void k_
     if (sn) atomicAdd(&g.ctr->n_distinct, sn);
     if (sk) atomicAdd(&g.ctr->n_k2, sk);
     g.spq_n[wave_id] = qn;
+  }
+  if constexpr (FUSE) {
+    /* the open chunks' unused positions */
+    const uint32_t lane = __lane_id();
+#pragma unroll
+    for (int l = 0; l < 2; ++l) {
+      const uint64_t start = ch[l].base + ch[l].used;
+      const uint32_t cnt = FUSE_CHUNK - ch[l].used;
+      if (lane < cnt && start + lane < g.nl.cap) (l ? g.nl.branch_slot : g.nl.chain_slot)[start + lane] = LIST_HOLE;
+      if (lane == 0) g.holes[2 * wave_id + l] = (start << 7) | cnt;
+    }
   }
 }
 

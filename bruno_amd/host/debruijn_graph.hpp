@@ -118,6 +118,10 @@ class debruijn_graph_map {
   }
   /* insert (:871-921), the unthresholded build_index path */
   size_t insert(const read_batch& reads) { return compute_biedge_freqencies(reads); }
+  /* the next batch is the last before local_insert_by_freqencies: when it
+   * is the first count of a region table, the filter runs inside it
+   * (bruno_job_set_fused_filter); no counterpart in the reference */
+  void last_batch_next(bool on = true) { check(bruno_job_set_fused_filter(j_.get(), on ? 1 : 0), "fused filter"); }
   /* local_insert_by_freqencies (:1258-1623); thresholds come from the job */
   size_t local_insert_by_freqencies() {
     check(bruno_filter(j_.get()), "local_insert_by_freqencies");

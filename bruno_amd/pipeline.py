@@ -183,6 +183,12 @@ class GraphJob(_Views):
         build_chains)."""
         check(lib().bruno_job_set_recycle(self.h, 1 if on else 0), "bruno_job_set_recycle")
 
+    def set_fused_filter(self, on=True):
+        """Filter fused into the job's first count (bruno_job_set_fused_filter):
+        the region build also filters and classifies, bruno_filter finishes
+        the rest; the job then takes no further batch before a reset."""
+        check(lib().bruno_job_set_fused_filter(self.h, 1 if on else 0), "bruno_job_set_fused_filter")
+
     # ---- phases ------------------------------------------------------
     def parse_count(self, bases, offsets):
         """Host arrays (bytes / uint8 and uint64 offsets, n_reads+1)."""

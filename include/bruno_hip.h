@@ -99,6 +99,7 @@ typedef struct {
                                  contraction ranking; 0: full sort (same bytes)  */
   uint64_t count_by_regions;  /* 1: the last count built the table region by region
                                  from instance records; 0: atomic counting in HBM */
+  uint64_t filter_fused;      /* 1: that count also ran the filter (bruno_job_set_fused_filter) */
 } bruno_stats;
 
 /* Listranked chain node (debruijn_chain_node.hpp:156): K on the rep strand,
@@ -153,6 +154,23 @@ bruno_status bruno_job_reset(bruno_job* job);
  * filtered graph is gone after bruno_build_chains: bruno_get_nodes returns
  * BRUNO_STATE.  No counterpart in the reference. */
 bruno_status bruno_job_set_recycle(bruno_job* job, int on);
+
+/* Filter fused into the count.  With on != 0, a count that builds every
+ * region of the job's table (its first bruno_parse_count, counted by
+ * regions) also runs the filter: local_insert_by_freqencies
+ * (debruijn_graph_map.hpp:1351-1609; the k+1-mer edge filter or none, as
+ * the job's thresholding asks) and the branch / chain classification
+ * (debruijn_graph_filters.hpp:142-220) for every slot it writes back, in
+ * on-chip memory before the write-back, appending the nodes to the node
+ * lists.  bruno_filter then only finishes what remains: slots with more
+ * k2-mers than a slot holds, and any node the lists had no room for.
+ * Results equal the unfused path's.  After such a count the job takes no
+ * further batch (bruno_parse_count* returns BRUNO_STATE until
+ * bruno_job_reset) and the k2 / count getters return BRUNO_STATE, since
+ * node records replace the k2 entries.  Off by default; a count that falls
+ * back to atomics (small tables, batches) is not fused.  No counterpart in
+ * the reference, whose filter is a pass of its own over the count map. */
+bruno_status bruno_job_set_fused_filter(bruno_job* job, int on);
 
 /* Replaces parse_file_data<debruijn_kmer_simple_biedge_parser> +
  * compute_biedge_freqencies (compact_dbg_freq_construct_clean.cpp:318-328,
@@ -362,6 +380,9 @@ typedef struct {
                               count.                                                         */
   uint32_t region_slots;   /* most slots of a region when a region table is allocated:
                               2560 (80 KB of LDS per build workgroup) or 1280 (40 KB)       */
+  uint32_t fuse_list_cap;  /* a count with the filter fused gives each node list room for at
+                              most this many nodes (0: every node the table can hold); the
+                              nodes beyond it are left to bruno_filter                      */
 } bruno_tuning;
 bruno_status bruno_set_tuning(const bruno_tuning* t);
 bruno_status bruno_get_tuning(bruno_tuning* t);

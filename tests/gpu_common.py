@@ -11,11 +11,13 @@ def kv(words, k):
     return int(words[0]) | (int(words[1]) << 64 if len(words) > 1 else 0)
 
 
-def run_gpu(k, th, bases, offs, variant=1, capacity=0, stages=True, edge_filter=False):
+def run_gpu(k, th, bases, offs, variant=1, capacity=0, stages=True, edge_filter=False, fused=False):
     job = bruno_amd.GraphJob(k, th, capacity=capacity, edge_filter=edge_filter)
+    if fused:
+        job.set_fused_filter(True)
     job.parse_count(bases, offs)
     res = {}
-    if stages:
+    if stages and not fused:
         res["counts"] = job.counts()
         res["k2"] = job.k2()
     job.filter()
@@ -48,8 +50,9 @@ def oracle_node_table(arr):
     return {(int(r["kmer_lo"]) | (int(r["kmer_hi"]) << 64)): tuple(int(x) for x in r["counts"]) for r in arr}
 
 
-def compare_all(k, th, bases, offs, variant=1, nthreads=1, stages=True, capacity=0, edge_filter=False):
-    g = run_gpu(k, th, bases, offs, variant, capacity=capacity, stages=stages, edge_filter=edge_filter)
+def compare_all(k, th, bases, offs, variant=1, nthreads=1, stages=True, capacity=0, edge_filter=False, fused=False):
+    stages = stages and not fused  # a fused count leaves node records, not k2 entries
+    g = run_gpu(k, th, bases, offs, variant, capacity=capacity, stages=stages, edge_filter=edge_filter, fused=fused)
     o = run_oracle(k, th, bases, offs, variant, nthreads, edge_filter=edge_filter)
     compare_results(k, g, o, variant, stages)
     return g, o
