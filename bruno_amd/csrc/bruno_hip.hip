@@ -700,37 +700,67 @@ struct WalkRec {
   uint32_t who;   /* ruler (node walk) or chain end (ruler walk) */
   uint32_t edist; /* bit 31: end, low 31 bits: distance */
 };
+/* A node walk's record kept in the node's own mailbox word of the end it
+ * was entered through (the mailboxes serve only the round-by-round path,
+ * which finds them zero): (end | distance) << 32 | ruler number; zero = no
+ * walk reached it (a real walk has distance >= 1). */
+__device__ __forceinline__ unsigned long long walk_word(uint32_t rid, uint32_t edist) {
+  return ((unsigned long long)edist << 32) | rid;
+}
 
-/* From every ruler, along each non-terminal end, to the next ruler.  Each
- * visited node records, at the end it was entered through, the ruler and
- * the ruler's end the walk left from, and the distance. */
-__global__ __launch_bounds__(BLOCK) void k_lr_walk_nodes(const LRNode* lr, const uint64_t* ruler, const uint32_t* rlist,
-                                                         uint64_t n_rulers, WalkRec* rec, WalkRec* link, Counters* ctr) {
+/* From every ruler (number rid in the ruler list), along each non-terminal
+ * end, to the next ruler.  Each visited node records, in its mailbox of
+ * the end it was entered through, the ruler number, the ruler's end the
+ * walk left from and the distance: the node's next pointer is read first,
+ * so the record goes into a line the walk has just brought in. */
+__global__ __launch_bounds__(BLOCK) void k_lr_walk_nodes(LRNode* lr, const uint64_t* ruler, const uint32_t* rlist,
+                                                         uint64_t n_rulers, WalkRec* link, Counters* ctr) {
   const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
   for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < 2 * n_rulers; i += stride) {
-    const uint32_t r = rlist[i >> 1];
+    const uint32_t rid = (uint32_t)(i >> 1), r = rlist[rid];
     const int E = (int)(i & 1);
     if (is_terminal(lr[r].d[E])) continue;
-    uint32_t cur = r;
-    int leave = E;
+    uint32_t ptr = lr[r].p[E];
     uint32_t dist = 0;
     for (;;) {
-      uint32_t ptr = lr[cur].p[leave];
-      uint32_t j = ptr & IDX_MASK;
-      int e = (ptr & END_OUT) ? 1 : 0;
+      const uint32_t j = ptr & IDX_MASK;
+      const int e = (ptr & END_OUT) ? 1 : 0;
       ++dist;
       if ((ruler[j >> 6] >> (j & 63)) & 1) {
         link[2 * (uint64_t)r + E] = WalkRec{j, end_bit(e) | dist};
         break;
       }
-      rec[2 * (uint64_t)j + e] = WalkRec{r, end_bit(E) | dist};
+      ptr = lr[j].p[1 - e];
+      lr[j].mail[e] = walk_word(rid, end_bit(E) | dist);
       if (dist >= WALK_MAX) {
         atomicOr(&ctr->err, ERR_WALK);
         break;
       }
-      cur = j;
-      leave = 1 - e;
     }
+  }
+}
+
+/* Per ruler number and end Er: what lies past the ruler when a walk left
+ * it through Er, i.e. through its other end: the ruler itself when that
+ * end is terminal (distance 0 more), else the chain end the ruler walk
+ * found that way (NONE32: a cycle).  Dense by ruler number, so the finish
+ * pass reads one small table instead of two scattered records. */
+__global__ __launch_bounds__(BLOCK) void k_lr_ruler_ends(const LRNode* lr, const uint32_t* rlist, uint64_t n_rulers,
+                                                         const WalkRec* info, WalkRec* rend) {
+  const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= 2 * n_rulers) return;
+  const uint32_t r = rlist[i >> 1];
+  const int Er = (int)(i & 1);
+  rend[i] = is_terminal(lr[r].d[1 - Er]) ? WalkRec{r, end_bit(Er)} : info[2 * (uint64_t)r + (1 - Er)];
+}
+
+/* The mailboxes back to empty, for the round-by-round path after a walk
+ * that ran past its bound. */
+__global__ __launch_bounds__(BLOCK) void k_lr_zero_mail(LRNode* lr, uint64_t n_chain) {
+  const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+  for (uint64_t t = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; t < n_chain; t += stride) {
+    lr[t].mail[0] = 0ULL;
+    lr[t].mail[1] = 0ULL;
   }
 }
 
@@ -798,7 +828,7 @@ __device__ __forceinline__ uint32_t ceil_log2(uint32_t L) { return L <= 1 ? 0 : 
 /* jT[job]: largest ceil(log2 L) over the job's paths; jcyc[job] = 1 when
  * the job has a cycle. */
 __global__ __launch_bounds__(BLOCK) void k_lr_finish(LRNode* lr, uint64_t n_chain, const uint64_t* ruler,
-                                                     const WalkRec* rec, const WalkRec* info, uint8_t* cyc,
+                                                     const WalkRec* rend, const WalkRec* info, uint8_t* cyc,
                                                      const uint32_t* cjob, uint32_t n_jobs, uint32_t* jT,
                                                      uint32_t* jcyc, Counters* ctr) {
   const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
@@ -822,29 +852,22 @@ __global__ __launch_bounds__(BLOCK) void k_lr_finish(LRNode* lr, uint64_t n_chai
         f = w.edist & END_OUT;
         D = w.edist & DIST_MASK;
       } else {
-        WalkRec w = rec[2 * t + E];
-        if (w.who == NONE32) {
+        /* the walk that reached this end: ruler number, its end, distance */
+        const uint32_t rid = (uint32_t)st.mail[E], edist = (uint32_t)(st.mail[E] >> 32);
+        const uint32_t d = edist & DIST_MASK;
+        if (d == 0) {
           is_cycle = true;
           break;
         }
-        uint32_t r = w.who;
-        int Er = (w.edist & END_OUT) ? 1 : 0;
-        uint32_t d = w.edist & DIST_MASK;
-        /* continue past ruler r, leaving through its other end */
-        if (is_terminal(lr[r].d[1 - Er])) {
-          Z = r;
-          f = end_bit(Er);
-          D = d;
-        } else {
-          WalkRec v = info[2 * (uint64_t)r + (1 - Er)];
-          if (v.who == NONE32) {
-            is_cycle = true;
-            break;
-          }
-          Z = v.who;
-          f = v.edist & END_OUT;
-          D = d + (v.edist & DIST_MASK);
+        /* continue past that ruler, leaving through its other end */
+        const WalkRec v = rend[2 * (uint64_t)rid + ((edist & END_OUT) ? 1 : 0)];
+        if (v.who == NONE32) {
+          is_cycle = true;
+          break;
         }
+        Z = v.who;
+        f = v.edist & END_OUT;
+        D = d + (v.edist & DIST_MASK);
       }
       np[E] = Z | f;
       nd[E] = D | TERM_MASK;
@@ -2490,7 +2513,7 @@ bruno_status list_rank_fast(bruno_job* j, bool* used) {
   const uint64_t ruler_bytes = (nc + 63) / 64 * 8;
   uint32_t* rlist = nullptr;
   uint32_t* heads = nullptr;
-  WalkRec *rec = nullptr, *link = nullptr, *info = nullptr;
+  WalkRec *rend = nullptr, *link = nullptr, *info = nullptr;
   /* k_chain_init counted the ill-formed ends */
   const bool well_formed = j->hctr->lr_ill == 0;
   if (!well_formed) return BRUNO_OK;
@@ -2506,15 +2529,14 @@ bruno_status list_rank_fast(bruno_job* j, bool* used) {
                      (uint64_t)tuning().ruler_sample - 1, rlist, nc, j->ctr);
   TRY(sync_counters(j));
   uint64_t nr = j->hctr->lr_next;
-  TRY(dev_alloc(j, (void**)&rec, nc * 2 * sizeof(WalkRec)));
+  TRY(dev_alloc(j, (void**)&rend, std::max<uint64_t>(1, nr) * 2 * sizeof(WalkRec)));
   TRY(dev_alloc(j, (void**)&link, nc * 2 * sizeof(WalkRec)));
   TRY(dev_alloc(j, (void**)&info, nc * 2 * sizeof(WalkRec)));
-  fill64(j, rec, nc * 2, ~0ULL);
   fill64(j, link, nc * 2, ~0ULL);
   fill64(j, info, nc * 2, ~0ULL);
   if (nr)
     hipLaunchKernelGGL(k_lr_walk_nodes, dim3(grid_for(2 * nr, BLOCK, 1u << 20)), dim3(BLOCK), 0, j->stream, j->lr,
-                       ruler, rlist, nr, rec, link, j->ctr);
+                       ruler, rlist, nr, link, j->ctr);
   TRY(dev_alloc(j, (void**)&heads, std::max<uint64_t>(1, 2 * nr) * 4));
   j->hctr->lr_next = 0;
   HIP_CHECK(hipMemcpyAsync(&j->ctr->lr_next, &j->hctr->lr_next, sizeof(unsigned long long), hipMemcpyHostToDevice,
@@ -2529,6 +2551,7 @@ bruno_status list_rank_fast(bruno_job* j, bool* used) {
     if (j->hctr->err == ERR_WALK) {
       j->hctr->err = 0;
       HIP_CHECK(hipMemsetAsync(&j->ctr->err, 0, 4, j->stream));
+      hipLaunchKernelGGL(k_lr_zero_mail, dim3(grid_for(nc)), dim3(BLOCK), 0, j->stream, j->lr, nc);
       HIP_CHECK(hipStreamSynchronize(j->stream));
       *fell_back = true;
       return BRUNO_OK;
@@ -2556,7 +2579,10 @@ bruno_status list_rank_fast(bruno_job* j, bool* used) {
   TRY(dev_alloc(j, (void**)&cyc, nc));
   TRY(dev_alloc(j, (void**)&jbuf, 3 * nj * sizeof(uint32_t)));
   HIP_CHECK(hipMemsetAsync(jbuf, 0, 2 * nj * sizeof(uint32_t), j->stream));
-  hipLaunchKernelGGL(k_lr_finish, dim3(grid_for(nc)), dim3(BLOCK), 0, j->stream, j->lr, nc, ruler, rec, info, cyc,
+  if (nr)
+    hipLaunchKernelGGL(k_lr_ruler_ends, dim3(grid_for(2 * nr, BLOCK, 1u << 30)), dim3(BLOCK), 0, j->stream, j->lr,
+                       rlist, nr, info, rend);
+  hipLaunchKernelGGL(k_lr_finish, dim3(grid_for(nc)), dim3(BLOCK), 0, j->stream, j->lr, nc, ruler, rend, info, cyc,
                      j->chain_job, nj, jbuf, jbuf + nj, j->ctr);
   std::vector<uint32_t> hj(3 * nj, 0);
   HIP_CHECK(hipMemcpyAsync(hj.data(), jbuf, 2 * nj * sizeof(uint32_t), hipMemcpyDeviceToHost, j->stream));
