@@ -261,6 +261,7 @@ __global__ __launch_bounds__(BLOCK) void k_filter_cand(Slot<KT>* T, const uint64
     const uint64_t i = base + threadIdx.x;
     bool kept = false;
     uint64_t s = 0;
+    KT k = 0;
     NodeRec rec;
     rec.cls = 0;
     if (i < n_cand) {
@@ -268,7 +269,7 @@ __global__ __launch_bounds__(BLOCK) void k_filter_cand(Slot<KT>* T, const uint64
       Slot<KT>* p = T + s;
       K2L<Slot<KT>::NSUB> L;
       sub_entries<KT, Slot<KT>::NSUB>(p, L);
-      KT k = SlotOps<KT>::kmer(p);
+      k = SlotOps<KT>::kmer(p);
       const KmerFacts f = kmer_facts(ops, k);
       uint16_t c[9];
       if (thresholding == 2) {
@@ -284,7 +285,7 @@ __global__ __launch_bounds__(BLOCK) void k_filter_cand(Slot<KT>* T, const uint64
     const bool ch = kept && rec.cls == CLS_CHAIN, br = kept && rec.cls == CLS_BRANCH;
     unsigned long long pc = block_reserve(ch, &ctr->n_chain, sm);
     unsigned long long pb = block_reserve(br, &ctr->n_branch, sm2);
-    if (kept) store_node<KT>(T, s, rec, ch ? pc : pb, nl, ctr);
+    if (kept) store_node<KT>(T, s, rec, ch ? pc : pb, nl, ctr, k);
   }
 }
 
@@ -302,6 +303,7 @@ __global__ __launch_bounds__(BLOCK) void k_filter_spilled(Slot<KT>* T, const Spi
     const uint64_t i = base + threadIdx.x;
     bool kept = false;
     uint64_t s = 0;
+    KT k = 0;
     NodeRec rec;
     rec.cls = 0;
     if (i < n_spilled) {
@@ -309,7 +311,7 @@ __global__ __launch_bounds__(BLOCK) void k_filter_spilled(Slot<KT>* T, const Spi
       Slot<KT>* p = T + s;
       K2List L;
       gather_k2<KT>(p, s, sp, spcap, L);
-      KT k = SlotOps<KT>::kmer(p);
+      k = SlotOps<KT>::kmer(p);
       uint16_t c[9];
       if (thresholding == 2) {
         kept = edge_counts<KT, 25>(ops, k, L, th, c);
@@ -324,7 +326,7 @@ __global__ __launch_bounds__(BLOCK) void k_filter_spilled(Slot<KT>* T, const Spi
     const bool ch = kept && rec.cls == CLS_CHAIN, br = kept && rec.cls == CLS_BRANCH;
     unsigned long long pc = block_reserve(ch, &ctr->n_chain, sm);
     unsigned long long pb = block_reserve(br, &ctr->n_branch, sm2);
-    if (kept) store_node<KT>(T, s, rec, ch ? pc : pb, nl, ctr);
+    if (kept) store_node<KT>(T, s, rec, ch ? pc : pb, nl, ctr, k);
   }
 }
 
@@ -437,38 +439,47 @@ template <typename KT>
 __global__ __launch_bounds__(BLOCK) void k_chain_init(const Slot<KT>* T, uint64_t cap, uint32_t rbits, int K,
                                                       const uint64_t* chain_slot,
                                                       const uint32_t* chain_job, const uint64_t* seg, uint32_t n_jobs,
-                                                      uint64_t n_chain, KT* ckmer, LRNode* lr, KT* vkmer, uint64_t vcap,
-                                                      Counters* ctr) {
+                                                      uint64_t n_chain, KT* ckmer, const uint8_t* cedge, LRNode* lr,
+                                                      KT* vkmer, uint64_t vcap, Counters* ctr) {
   kops<KT> ops(K);
   const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
   unsigned long long ill = 0;
   for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < n_chain; i += stride) {
-    const Slot<KT>* p = T + chain_slot[i];
     uint64_t tbase = 0, tcap = cap;
     if (n_jobs > 1) {
       const uint32_t job = chain_job[i];
       tbase = seg[job];
       tcap = seg[job + 1] - tbase;
     }
-    KT k = SlotOps<KT>::kmer(p);
-    const NodeRec* nr = (const NodeRec*)&p->sub[0];
+    /* a chain node has at most one in and one out edge: from the filter's
+     * k-mer and edge byte when it left them, else from the node's slot */
+    KT k;
+    int ci = -1, co = -1;
+    if (cedge) {
+      k = ckmer[i];
+      const uint32_t eb = cedge[i];
+      ci = (int)(eb & 15) - 1;
+      co = (int)(eb >> 4) - 1;
+    } else {
+      const Slot<KT>* p = T + chain_slot[i];
+      k = SlotOps<KT>::kmer(p);
+      const NodeRec* nr = (const NodeRec*)&p->sub[0];
+      for (int c = 0; c < 4; ++c) {
+        if (nr->c[4 + c] > 0) ci = c;
+        if (nr->c[c] > 0) co = c;
+      }
+    }
     LRNode st;
     st.p[0] = st.p[1] = IDX_MASK;
     st.d[0] = st.d[1] = TERM_MASK;
     st.mail[0] = st.mail[1] = 0ULL;
-    /* a chain node has at most one in and one out edge */
-    int ci = -1, co = -1;
-    for (int c = 0; c < 4; ++c) {
-      if (nr->c[4 + c] > 0) ci = c;
-      if (nr->c[c] > 0) co = c;
-    }
     /* in neighbour c.k[0..K-2]: when canonical its OUT end faces k; out
      * neighbour k[1..].c: when canonical its IN end faces k */
     resolve_pair<KT>(T, tbase, tcap, rbits, ops, k, (uint32_t)i, ci >= 0, ops.in_nbr(k, ci < 0 ? 0 : ci), co >= 0,
                      ops.out_nbr(k, co < 0 ? 0 : co), n_chain, vkmer, vcap, ctr, st.p[0], st.p[1], ill);
     if (ci >= 0) st.d[0] = 1;
     if (co >= 0) st.d[1] = 1;
-    ckmer[i] = k;
+    if (!cedge) ckmer[i] = k;
     lr[i] = st;
   }
   block_add(ill, &ctr->lr_ill);
@@ -1610,7 +1621,9 @@ struct bruno_job {
   uint64_t* branch_slot = nullptr;
   uint64_t list_cap = 0; /* entries each list has room for */
 
-  /* chains */
+  /* chains (ckmer and cedge: written by the filter, by chain-list
+   * position, when it ran as its own pass) */
+  uint8_t* cedge = nullptr;
   void* ckmer = nullptr;
   void* vkmer = nullptr;
   void* bkmer = nullptr;
@@ -2413,7 +2426,14 @@ bruno_status filter_t(bruno_job* j) {
     TRY(dev_alloc(j, (void**)&j->chain_job, j->list_cap * sizeof(uint32_t)));
     TRY(dev_alloc(j, (void**)&j->branch_job, j->list_cap * sizeof(uint32_t)));
   }
-  NodeLists nl{j->chain_slot, j->branch_slot, j->chain_job, j->branch_job, j->list_cap, j->d_seg, j->n_jobs};
+  /* the chain nodes' k-mers and edge bytes for the chain extraction (not
+   * after a fused count, whose build placed most nodes without them) */
+  if (!j->fused) {
+    TRY(dev_alloc(j, &j->ckmer, j->list_cap * sizeof(KT)));
+    TRY(dev_alloc(j, (void**)&j->cedge, j->list_cap));
+  }
+  NodeLists nl{j->chain_slot, j->branch_slot, j->chain_job, j->branch_job, j->list_cap, j->d_seg, j->n_jobs,
+               j->fused ? nullptr : j->ckmer, j->fused ? nullptr : j->cedge};
   if (n_cand)
     hipLaunchKernelGGL((k_filter_cand<KT>), dim3(grid_for(n_cand)), dim3(BLOCK), 0, j->stream, (Slot<KT>*)j->table,
                        cand, n_cand, j->K, j->thresholding, j->th, nl, j->ctr);
@@ -2454,7 +2474,7 @@ bruno_status filter_t(bruno_job* j) {
 template <typename KT>
 bruno_status chains_t(bruno_job* j) {
   uint64_t nc = j->n_chain;
-  TRY(dev_alloc(j, &j->ckmer, std::max<uint64_t>(1, nc) * sizeof(KT)));
+  if (!j->cedge) TRY(dev_alloc(j, &j->ckmer, std::max<uint64_t>(1, nc) * sizeof(KT)));
   j->vcap = 2 * nc + 16;
   TRY(dev_alloc(j, &j->vkmer, j->vcap * sizeof(KT)));
   TRY(dev_alloc(j, &j->bkmer, std::max<uint64_t>(1, j->n_branch) * sizeof(KT)));
@@ -2462,7 +2482,7 @@ bruno_status chains_t(bruno_job* j) {
   HIP_CHECK(hipMemsetAsync(&j->ctr->lr_ill, 0, sizeof(unsigned long long), j->stream));
   hipLaunchKernelGGL((k_chain_init<KT>), dim3(grid_for(nc)), dim3(BLOCK), 0, j->stream, (const Slot<KT>*)j->table,
                      probe_span(j), j->rbits, j->K, j->chain_slot, j->chain_job, j->d_seg, j->n_jobs, nc,
-                     (KT*)j->ckmer, j->lr, (KT*)j->vkmer, j->vcap, j->ctr);
+                     (KT*)j->ckmer, j->cedge, j->lr, (KT*)j->vkmer, j->vcap, j->ctr);
   hipLaunchKernelGGL((k_termini<KT>), dim3(grid_for(j->n_branch)), dim3(BLOCK), 0, j->stream,
                      (const Slot<KT>*)j->table, probe_span(j), j->rbits, j->K, j->branch_slot, j->branch_job,
                      j->d_seg, j->n_jobs, j->n_branch, j->lr, (KT*)j->bkmer);
@@ -3163,6 +3183,7 @@ bruno_status bruno_job_reset(bruno_job* j) {
   j->peak_bytes = j->device_bytes;
   j->chain_slot = j->branch_slot = nullptr;
   j->ckmer = j->vkmer = j->bkmer = j->out_k = j->out_rep = j->bsorted = nullptr;
+  j->cedge = nullptr;
   j->lr = nullptr;
   j->list[0] = j->list[1] = nullptr;
   j->alive = nullptr;
@@ -3436,6 +3457,7 @@ static void free_after_lists(bruno_job* j) {
   j->allocs = keep;
   j->device_bytes = persistent_bytes(j) + 2 * j->list_cap * sizeof(uint64_t);
   j->ckmer = j->vkmer = j->bkmer = j->out_k = j->out_rep = j->bsorted = nullptr;
+  j->cedge = nullptr;
   j->lr = nullptr;
   j->list[0] = j->list[1] = nullptr;
   j->alive = nullptr;

@@ -752,13 +752,30 @@ struct NodeLists {
   uint64_t cap; /* capacity of each list */
   const uint64_t* seg;
   uint32_t n_jobs;
+  /* optional, by chain-list position: the chain node's k-mer (KT) and its
+   * edge byte (in-edge base + 1 | (out-edge base + 1) << 4, 0 = none), so
+   * the chain extraction reads them in order instead of the node's slot */
+  void* chain_kmer;
+  uint8_t* chain_edge;
 };
+
+/* A chain node's edge byte from its counters (at most one in and one out
+ * edge). */
+__device__ __forceinline__ uint8_t chain_edge_byte(const uint16_t (&c)[9]) {
+  uint32_t ci = 0, co = 0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    ci = c[4 + q] > 0 ? (uint32_t)q + 1 : ci;
+    co = c[q] > 0 ? (uint32_t)q + 1 : co;
+  }
+  return (uint8_t)(ci | (co << 4));
+}
 
 /* Write the record in place (sub area of the slot) with its list index,
  * mark the slot present and append it to its list at position pos. */
 template <typename KT>
 __device__ __forceinline__ void store_node(Slot<KT>* T, uint64_t s, NodeRec rec, unsigned long long pos,
-                                           const NodeLists& nl, Counters* ctr) {
+                                           const NodeLists& nl, Counters* ctr, KT k = KT(0)) {
   if (pos >= nl.cap || pos > 0x7FFFFFFFULL) {
     atomicOr(&ctr->err, ERR_LIST_FULL);
     return;
@@ -771,6 +788,10 @@ __device__ __forceinline__ void store_node(Slot<KT>* T, uint64_t s, NodeRec rec,
   const bool chain = rec.cls == CLS_CHAIN;
   (chain ? nl.chain_slot : nl.branch_slot)[pos] = s;
   if (nl.n_jobs > 1) (chain ? nl.chain_job : nl.branch_job)[pos] = job_of(nl.seg, nl.n_jobs, s);
+  if (chain && nl.chain_kmer) {
+    ((KT*)nl.chain_kmer)[pos] = k;
+    nl.chain_edge[pos] = chain_edge_byte(rec.c);
+  }
 }
 
 }  // namespace br
