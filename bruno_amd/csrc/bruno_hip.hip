@@ -1111,13 +1111,23 @@ __global__ void k_take_len(const uint32_t* hs, const uint32_t* out_len, uint64_t
   if (r < nh) len[r] = out_len[hs[r]];
 }
 
-/* every node to its chain's offset + its dist; bad counts nodes whose head
- * is missing, whose key differs from the head's, or whose dist is past the
- * chain's end (meta.len is 0 for a node that heads no chain) */
+/* A listranked node at its place in output order: its k-mer (K on the
+ * rep strand), its index among the listranked nodes and its distance. */
 template <typename KT>
-__global__ void k_scatter_ranked(const uint32_t* out_head, const uint32_t* out_dist, const KT* out_rep,
+struct SortedRec {
+  KT k;
+  uint32_t p, dist;
+};
+
+/* every node to its chain's offset + its dist, as one record (its k-mer,
+ * index and dist), so the output order's arrays and the FASTA writer read
+ * in sequence; bad counts nodes whose head is missing, whose key differs
+ * from the head's, or whose dist is past the chain's end (meta.len is 0
+ * for a node that heads no chain) */
+template <typename KT>
+__global__ void k_scatter_ranked(const uint32_t* out_head, const uint32_t* out_dist, const KT* out_rep, const KT* out_k,
                                  const uint32_t* out_job, uint64_t n, uint64_t n_chain, const HeadMeta<KT>* meta,
-                                 uint32_t* perm, unsigned long long* bad) {
+                                 SortedRec<KT>* sr, unsigned long long* bad) {
   const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const uint32_t h = out_head[p];
@@ -1127,16 +1137,24 @@ __global__ void k_scatter_ranked(const uint32_t* out_head, const uint32_t* out_d
     const uint32_t d = out_dist[p];
     const uint64_t pos = (uint64_t)m.off + d;
     ok = d < m.len && pos < n && m.rep == out_rep[p] && m.job == (out_job ? out_job[p] : 0u);
-    if (ok) perm[pos] = (uint32_t)p;
+    if (ok) sr[pos] = SortedRec<KT>{out_k[p], (uint32_t)p, d};
   }
   if (!ok) atomicAdd(bad, 1ULL);
 }
 
-/* positions no node was placed at (perm is pre-filled with ~0) */
-__global__ __launch_bounds__(BLOCK) void k_count_holes(const uint32_t* perm, uint64_t n, unsigned long long* bad) {
+/* perm and the sorted distances from the placed records; positions no node
+ * was placed at (the records are pre-filled with ones) count as bad */
+template <typename KT>
+__global__ __launch_bounds__(BLOCK) void k_sorted_split(const SortedRec<KT>* sr, uint64_t n, uint32_t* perm,
+                                                        uint32_t* dist_sorted, unsigned long long* bad) {
   const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
   uint32_t holes = 0;
-  for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += stride) holes += perm[i] == ~0u;
+  for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += stride) {
+    const SortedRec<KT> r = sr[i];
+    perm[i] = r.p;
+    dist_sorted[i] = r.dist;
+    holes += r.p == ~0u;
+  }
   if (holes) atomicAdd(bad, (unsigned long long)holes);
 }
 
@@ -1192,7 +1210,27 @@ __global__ void k_fasta_len(const uint32_t* dist_sorted, uint64_t n, int K, uint
 
 /* print_chain_as_fasta (debruijn_chain_operations.hpp:352-385): a node at
  * dist 0 starts a record "\n>" canonical(K) "\n" rep; every other node adds
- * the last base of K (its k-mer on the rep strand). */
+ * the last base of K (its k-mer on the rep strand).  From the records in
+ * output order (sr, after placement by chains) or through perm. */
+template <typename KT>
+__global__ void k_fasta_write_sorted(const SortedRec<KT>* sr, const KT* out_rep, const uint64_t* off, uint64_t n,
+                                     int K, char* buf) {
+  kops<KT> ops(K);
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const SortedRec<KT> r = sr[i];
+  char* o = buf + off[i];
+  if (r.dist == 0) {
+    o[0] = '\n';
+    o[1] = '>';
+    write_kmer<KT>(o + 2, ops.canon(r.k), K);
+    o[2 + K] = '\n';
+    write_kmer<KT>(o + 3 + K, out_rep[r.p], K);
+  } else {
+    const char A[4] = {'A', 'C', 'G', 'T'};
+    o[0] = A[(int)(r.k & 3)];
+  }
+}
 template <typename KT>
 __global__ void k_fasta_write(const uint32_t* perm, const KT* out_k, const KT* out_rep, const uint32_t* dist_sorted,
                               const uint64_t* off, uint64_t n, int K, char* buf) {
@@ -2842,8 +2880,10 @@ bruno_status job_bounds(bruno_job* j, const uint32_t* job_of_item, const uint32_
  * nodes sit in dist order.  Otherwise *placed = false and the caller
  * sorts. */
 template <typename KT>
-bruno_status place_by_chains(bruno_job* j, uint64_t n, const uint32_t* out_head, const uint32_t* out_len, bool* placed) {
+bruno_status place_by_chains(bruno_job* j, uint64_t n, const uint32_t* out_head, const uint32_t* out_len, bool* placed,
+                             SortedRec<KT>** sr_out) {
   *placed = false;
+  *sr_out = nullptr;
   const uint64_t nc = j->n_chain;
   const uint32_t nj = j->n_jobs;
   j->hctr->n_cand = 0;
@@ -2890,18 +2930,27 @@ bruno_status place_by_chains(bruno_job* j, uint64_t n, const uint32_t* out_head,
   TRY(dev_alloc(j, (void**)&meta, nc * sizeof(HeadMeta<KT>)));
   TRY(dev_alloc(j, (void**)&bad, 8));
   /* len 0 marks a node that heads no chain */
+  SortedRec<KT>* sr = nullptr;
+  TRY(dev_alloc(j, (void**)&sr, n * sizeof(SortedRec<KT>)));
   HIP_CHECK(hipMemsetAsync(meta, 0, nc * sizeof(HeadMeta<KT>), j->stream));
-  HIP_CHECK(hipMemsetAsync(j->perm, 0xFF, n * 4, j->stream));
+  HIP_CHECK(hipMemsetAsync(sr, 0xFF, n * sizeof(SortedRec<KT>), j->stream));
   HIP_CHECK(hipMemsetAsync(bad, 0, 8, j->stream));
   hipLaunchKernelGGL((k_head_meta<KT>), gh, dim3(BLOCK), 0, j->stream, hs, coff, out_head, out_len,
                      (const KT*)j->out_rep, j->out_job, nh, nc, meta, bad);
   hipLaunchKernelGGL((k_scatter_ranked<KT>), gn, dim3(BLOCK), 0, j->stream, out_head, j->out_dist,
-                     (const KT*)j->out_rep, j->out_job, n, nc, meta, j->perm, bad);
-  hipLaunchKernelGGL(k_count_holes, dim3(grid_for(n)), dim3(BLOCK), 0, j->stream, j->perm, n, bad);
+                     (const KT*)j->out_rep, (const KT*)j->out_k, j->out_job, n, nc, meta, sr, bad);
+  hipLaunchKernelGGL((k_sorted_split<KT>), dim3(grid_for(n)), dim3(BLOCK), 0, j->stream, sr, n, j->perm,
+                     j->dist_sorted, bad);
   unsigned long long hbad = 0;
   HIP_CHECK(hipMemcpyAsync(&hbad, bad, 8, hipMemcpyDeviceToHost, j->stream));
   HIP_CHECK(hipStreamSynchronize(j->stream));
   *placed = hbad == 0;
+  if (*placed) {
+    *sr_out = sr;
+  } else {
+    a = sr;
+    dev_free(j, a, n * sizeof(SortedRec<KT>));
+  }
   a = heads;
   dev_free(j, a, n * 4);
   a = hkeys;
@@ -2955,7 +3004,8 @@ bruno_status compact_t(bruno_job* j) {
    * a batch adds a last pass on the job, so each job's nodes are contiguous */
   j->fasta_off.assign(nj + 1, 0);
   bool placed = false;
-  if (n > 0 && out_head) TRY(place_by_chains<KT>(j, n, out_head, out_len, &placed));
+  SortedRec<KT>* sr = nullptr; /* the nodes in output order, when placed by chains */
+  if (n > 0 && out_head) TRY(place_by_chains<KT>(j, n, out_head, out_len, &placed, &sr));
   j->out_by_chains = placed;
   if (n > 0 && !placed) {
     uint32_t* pin = nullptr;
@@ -2989,8 +3039,9 @@ bruno_status compact_t(bruno_job* j) {
     void* t = nullptr;
     size_t tmp = 0;
     TRY(job_bounds(j, j->out_job, j->perm, n, j->chain_start));
-    hipLaunchKernelGGL(k_gather_u32, dim3(grid_for(n, BLOCK, 1u << 30)), dim3(BLOCK), 0, j->stream, j->out_dist,
-                       j->perm, j->dist_sorted, n);
+    if (!sr)
+      hipLaunchKernelGGL(k_gather_u32, dim3(grid_for(n, BLOCK, 1u << 30)), dim3(BLOCK), 0, j->stream, j->out_dist,
+                         j->perm, j->dist_sorted, n);
     /* .chain.fasta */
     uint64_t *len = nullptr, *off = nullptr;
     TRY(dev_alloc(j, (void**)&len, n * 8));
@@ -3007,8 +3058,12 @@ bruno_status compact_t(bruno_job* j) {
     HIP_CHECK(hipStreamSynchronize(j->stream));
     dev_free(j, t, tmp);
     TRY(dev_alloc(j, (void**)&j->fasta, total + 1));
-    hipLaunchKernelGGL((k_fasta_write<KT>), dim3(grid_for(n, BLOCK, 1u << 30)), dim3(BLOCK), 0, j->stream, j->perm,
-                       (const KT*)j->out_k, (const KT*)j->out_rep, j->dist_sorted, off, n, K, j->fasta);
+    if (sr)
+      hipLaunchKernelGGL((k_fasta_write_sorted<KT>), dim3(grid_for(n, BLOCK, 1u << 30)), dim3(BLOCK), 0, j->stream,
+                         sr, (const KT*)j->out_rep, off, n, K, j->fasta);
+    else
+      hipLaunchKernelGGL((k_fasta_write<KT>), dim3(grid_for(n, BLOCK, 1u << 30)), dim3(BLOCK), 0, j->stream, j->perm,
+                         (const KT*)j->out_k, (const KT*)j->out_rep, j->dist_sorted, off, n, K, j->fasta);
     j->fasta_len = total;
     /* byte range of each job's records */
     j->fasta_off[nj] = total;
@@ -3029,6 +3084,8 @@ bruno_status compact_t(bruno_job* j) {
     dev_free(j, a, n * 8);
     a = off;
     dev_free(j, a, (n + 1) * 8);
+    a = sr;
+    dev_free(j, a, n * sizeof(SortedRec<KT>));
   } else {
     j->fasta_len = 0;
     j->chain_start.assign(nj + 1, 0);
