@@ -356,7 +356,7 @@ __global__ __launch_bounds__(BS) void k_parse(ParseArgs a) {
     if (MODE == 2 && a.n_jobs > 1) j_inst += nk;
     for_each_instance<KT>(s, L, K, lane, ops, [&](uint64_t i, KT km, uint8_t e) {
       if (MODE == 3) {
-        const uint64_t h = kmer_hash(km);
+        const uint64_t h = sketch_hash(km);
         if ((h >> (64 - SKETCH_SAMPLE_BITS)) == 0) {
           const uint32_t idx = (uint32_t)(h >> (64 - SKETCH_SAMPLE_BITS - SKETCH_INDEX_BITS)) & (SKETCH_M - 1);
           const uint64_t rest = (h << (SKETCH_SAMPLE_BITS + SKETCH_INDEX_BITS)) |

@@ -38,28 +38,39 @@ BR_HD uint64_t fmix64(uint64_t x) {
   return x;
 }
 
-BR_HD uint64_t kmer_hash(uint64_t v) { return fmix64(v * 0x9e3779b97f4a7c15ULL + 0x632be59bd9b4e019ULL); }
-/* fmix64 is a bijection (odd multipliers, xor-shifts by >= 32 bits undo
- * themselves); its inverse: */
-BR_HD uint64_t fmix64_inv(uint64_t h) {
-  h ^= h >> 33;
-  h *= 0x9cb4b2f8129337dbULL; /* 0xc4ceb9fe1a85ec53^-1 mod 2^64 */
-  h ^= h >> 33;
-  h *= 0x4f74430c22a54005ULL; /* 0xff51afd7ed558ccd^-1 */
-  h ^= h >> 33;
-  return h;
+/* The sketch's hash of a one-word k-mer (two multiplies of fmix64 and one
+ * more: every output bit depends on every input bit, as a HyperLogLog's
+ * leading-zero counts need). */
+BR_HD uint64_t sketch_hash(uint64_t v) { return fmix64(v * 0x9e3779b97f4a7c15ULL + 0x632be59bd9b4e019ULL); }
+
+/* The table's hash: a bijection of 64-bit words with one multiply (a
+ * xor-shift by 32 undoes itself, the multiplier is odd), so the k-mer comes
+ * back from its hash; the product's top bits, which pick the bucket and the
+ * region, depend on every input bit, and the last xor-shift folds them
+ * into the bits below that pick the home slot.  Each instance is hashed
+ * once by the parser and unhashed once by the region build, both bound by
+ * instruction issue, where a 64-bit multiply is several quarter-rate ops. */
+constexpr uint64_t KMIX_MUL = 0xd6e8feb86659fd93ULL;
+constexpr uint64_t KMIX_INV = 0xcfee444d8b59a89bULL; /* KMIX_MUL^-1 mod 2^64 */
+BR_HD uint64_t kmix(uint64_t x) {
+  x ^= x >> 32;
+  x *= KMIX_MUL;
+  return x ^ (x >> 32);
 }
-/* so kmer_hash of a one-word k-mer is a bijection too; its inverse recovers
- * the k-mer from its hash */
-BR_HD uint64_t kmer_unhash(uint64_t h) {
-  return (fmix64_inv(h) - 0x632be59bd9b4e019ULL) * 0xf1de83e19937733dULL; /* 0x9e3779b97f4a7c15^-1 */
+BR_HD uint64_t kmix_inv(uint64_t h) {
+  h ^= h >> 32;
+  h *= KMIX_INV;
+  return h ^ (h >> 32);
 }
-BR_HD uint64_t kmer_hash(u128 v) {
-  return fmix64((uint64_t)v ^ fmix64((uint64_t)(v >> 64) + 0x9e3779b97f4a7c15ULL));
-}
+BR_HD uint64_t kmer_hash(uint64_t v) { return kmix(v); }
+BR_HD uint64_t kmer_unhash(uint64_t h) { return kmix_inv(h); }
+BR_HD uint64_t kmer_hash(u128 v) { return kmix((uint64_t)v ^ kmix((uint64_t)(v >> 64) + 0x9e3779b97f4a7c15ULL)); }
 /* a two-word k-mer from its hash and its high word */
 BR_HD u128 kmer_unhash(uint64_t h, uint64_t hi) {
-  return ((u128)hi << 64) | (fmix64_inv(h) ^ fmix64(hi + 0x9e3779b97f4a7c15ULL));
+  return ((u128)hi << 64) | (kmix_inv(h) ^ kmix(hi + 0x9e3779b97f4a7c15ULL));
+}
+BR_HD uint64_t sketch_hash(u128 v) {
+  return fmix64((uint64_t)v ^ fmix64((uint64_t)(v >> 64) + 0x9e3779b97f4a7c15ULL));
 }
 
 /* Reverse complement of a 64-bit word holding 32 bases: complement is the
