@@ -49,16 +49,17 @@ def count_bytes(n_bases, n_inst):
     return n_bases * 1 + n_inst * 2 * 64
 
 
-def measured_traffic(config, scale, kernel):
+def measured_traffic(config, scale, sized_from, kernel):
     """HBM bytes per launch of a kernel on this workload, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE runs (profiles/hbm_traffic.json), or
-    None when no counter run covers the workload."""
+    None when no counter run covers the workload (same table sizing)."""
     try:
         with open(os.path.join(ROOT, "profiles", "hbm_traffic.json")) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    if t.get("workload") != config or float(t.get("scale", -1)) != float(scale):
+    if (t.get("workload") != config or float(t.get("scale", -1)) != float(scale)
+            or t.get("table_sized_from", "model") != sized_from):
         return None
     return t["bytes_per_launch"].get(kernel)
 
@@ -211,6 +212,10 @@ def main():
     ap.add_argument("--edge-outputs", action="store_true",
                     help="after the timed steps, run one more step without table recycling and time the "
                          ".branch.edges / .chain.edges getters (non-benchmark outputs)")
+    ap.add_argument("--size-from", choices=("model", "sketch"), default="sketch",
+                    help="size the hash table from the generator's error model of the distinct k-mers, or from "
+                         "the library's HyperLogLog sketch of the reads (bruno_estimate_distinct, untimed), as a "
+                         "caller without a model would")
     ap.add_argument("--fuse-filter", action="store_true",
                     help="run the filter inside the region build's write-back (bruno_job_set_fused_filter) "
                          "instead of as a pass of its own after the count")
@@ -262,7 +267,11 @@ def main():
         tuned = tune(k, cfg["thresholds"], bases.ptr, offs.ptr, s.n_reads, lambda x: capacity_for(s, k, x),
                      log=lambda *a: print("autotune:", *a, file=sys.stderr, flush=True))
         lf = tuned["load_factor"]
-    cap = capacity_for(s, k, lf)
+    if args.size_from == "sketch":
+        from bruno_amd.pipeline import estimate_distinct
+        est, cap = estimate_distinct(k, bases.ptr, offs.ptr, s.n_reads, lf)
+    else:
+        est, cap = None, capacity_for(s, k, lf)
     job = bruno_amd.GraphJob(k, cfg["thresholds"], capacity=cap, edge_filter=args.edge_filter)
     # steps repeat one job: its table is cleared beside list ranking and
     # output of each step, so the next step's reset has nothing left to clear
@@ -334,6 +343,7 @@ def main():
             "k": k, "n_reads": s.n_reads, "read_len": s.read_len, "genome_bp": s.genome_len,
             "k_mer_instances": n_inst, "list_rank": "min_update" if args.variant else "list_rank",
             "load_factor": lf,
+            "table_sized_from": args.size_from if est is None else "sketch (%.4g distinct estimated)" % est,
             "input": "FASTQ bytes in HBM (%.1f GB), split on the device in each step" % (fq.nbytes / 1e9)
                      if fq is not None else "concatenated bases + offsets in HBM",
         },
@@ -352,7 +362,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(args.config, args.scale, "count_phase")
+            "traffic": measured_traffic(args.config, args.scale, args.size_from, "count_phase")
                        if lf == 0.5 and stats["count_by_regions"] and fq is None else None,
             "traffic_note": "FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00 (factors from a 4 GiB copy): bytes the L2 "
                             "caches fetched and wrote back, Infinity Cache (MALL) hits included, so not DRAM "
@@ -370,6 +380,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": filter_bytes(stats, k) / (ph["filter_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "traffic": measured_traffic(args.config, args.scale, args.size_from, "filter_phase")
+                       if lf == 0.5 and stats["count_by_regions"] and fq is None else None,
             "bytes_per_unit": "16 B per distinct k2-mer read (24 B for k > 31) + 32 B per kept k-mer written (48 B)",
         },
     }
