@@ -116,15 +116,15 @@ __device__ __forceinline__ void wave_exclusive_scan(const uint32_t* cnt, uint32_
  * thread per digit) fixes where digit d's records of this tile go, and
  * store(d, j, rec) writes the j-th of them, so each digit's run is written
  * to consecutive places. */
-template <typename R, typename DigitF, typename ReserveF, typename StoreF>
-__device__ __forceinline__ void tile_split(const R (&r)[SPLIT_ITEMS], const bool (&ok)[SPLIT_ITEMS], int nbins, R* buf,
+template <typename R, int ITEMS, typename DigitF, typename ReserveF, typename StoreF>
+__device__ __forceinline__ void tile_split(const R (&r)[ITEMS], const bool (&ok)[ITEMS], int nbins, R* buf,
                                            uint32_t* cnt, uint32_t* start, DigitF digit, ReserveF reserve,
                                            StoreF store) {
   for (int i = threadIdx.x; i < nbins; i += SPLIT_BS) cnt[i] = 0;
   __syncthreads();
-  uint32_t rank[SPLIT_ITEMS], d[SPLIT_ITEMS];
+  uint32_t rank[ITEMS], d[ITEMS];
 #pragma unroll
-  for (int k = 0; k < SPLIT_ITEMS; ++k) {
+  for (int k = 0; k < ITEMS; ++k) {
     d[k] = ok[k] ? digit(rec_key(r[k])) : 0;
     rank[k] = ok[k] ? atomicAdd(&cnt[d[k]], 1u) : 0;
   }
@@ -134,7 +134,7 @@ __device__ __forceinline__ void tile_split(const R (&r)[SPLIT_ITEMS], const bool
   /* per digit: its count in this tile -> its output position */
   for (int i = threadIdx.x; i < nbins; i += SPLIT_BS) reserve(i, cnt[i]);
 #pragma unroll
-  for (int k = 0; k < SPLIT_ITEMS; ++k)
+  for (int k = 0; k < ITEMS; ++k)
     if (ok[k]) buf[start[d[k]] + rank[k]] = r[k];
   __syncthreads();
   /* records in digit order; each digit's run goes to consecutive places */
@@ -196,7 +196,7 @@ __global__ __launch_bounds__(SPLIT_BS) void k_split1(Split1Args a) {
     if (t + per < t1) seg_tile_load(a, t + per, rn, okn, bn);
     unsigned long long* cur = a.cursor + ((uint64_t)bucket << b1);
     const uint64_t* limit = a.limit + ((uint64_t)bucket << b1);
-    tile_split(
+    tile_split<R, SPLIT_ITEMS>(
         r, ok, nb, buf, cnt, start, [&](uint64_t x) -> uint32_t { return b1 ? (uint32_t)(x >> (64 - b1)) : 0u; },
         [&](int i, uint32_t c) {
           pos[i] = c ? atomicAdd(&cur[i], (unsigned long long)c) : 0ULL;
@@ -247,9 +247,17 @@ struct Split2Args {
   uint64_t* region_start;         /* [n_sb << b2 (+1)]: first record of every region */
 };
 
+/* 8-byte records in tiles of 8,192 (16 per thread): each region's run of a
+ * tile is twice as long as with 4,096, so fewer, longer writes; 16-byte
+ * records keep 4,096 (the tile fills the same 64 KB of LDS) */
+template <typename R> struct Split2Shape {
+  static constexpr int ITEMS = sizeof(R) == 8 ? 2 * SPLIT_ITEMS : SPLIT_ITEMS;
+  static constexpr int TILE = SPLIT_BS * ITEMS;
+};
 template <typename R>
-__global__ __launch_bounds__(SPLIT_BS) void k_split2(Split2Args a) {
-  __shared__ R buf[SPLIT_TILE];
+__global__ __launch_bounds__(SPLIT_BS) __attribute__((amdgpu_waves_per_eu(4))) void k_split2(Split2Args a) {
+  constexpr int ITEMS = Split2Shape<R>::ITEMS, TILE = Split2Shape<R>::TILE;
+  __shared__ R buf[TILE];
   const R* in = (const R*)a.in;
   R* out = (R*)a.out;
   __shared__ uint32_t cnt[1 << SPLIT2_MAX_BITS], start[1 << SPLIT2_MAX_BITS];
@@ -286,29 +294,29 @@ __global__ __launch_bounds__(SPLIT_BS) void k_split2(Split2Args a) {
     __syncthreads();
     /* scatter pass: tile by tile, each digit's run after the previous
      * tile's; the next tile loads while one is sorted and written */
-    R rn[SPLIT_ITEMS];
-    bool okn[SPLIT_ITEMS];
+    R rn[ITEMS];
+    bool okn[ITEMS];
 #pragma unroll
-    for (int k = 0; k < SPLIT_ITEMS; ++k) {
+    for (int k = 0; k < ITEMS; ++k) {
       const uint64_t i = lo + (uint64_t)k * SPLIT_BS + threadIdx.x;
       okn[k] = i < hi;
       rn[k] = okn[k] ? in[i] : rec_zero<R>();
     }
-    for (uint64_t i0 = lo; i0 < hi; i0 += SPLIT_TILE) {
-      R r[SPLIT_ITEMS];
-      bool ok[SPLIT_ITEMS];
+    for (uint64_t i0 = lo; i0 < hi; i0 += TILE) {
+      R r[ITEMS];
+      bool ok[ITEMS];
 #pragma unroll
-      for (int k = 0; k < SPLIT_ITEMS; ++k) {
+      for (int k = 0; k < ITEMS; ++k) {
         r[k] = rn[k];
         ok[k] = okn[k];
       }
 #pragma unroll
-      for (int k = 0; k < SPLIT_ITEMS; ++k) {
-        const uint64_t i = i0 + SPLIT_TILE + (uint64_t)k * SPLIT_BS + threadIdx.x;
+      for (int k = 0; k < ITEMS; ++k) {
+        const uint64_t i = i0 + TILE + (uint64_t)k * SPLIT_BS + threadIdx.x;
         okn[k] = i < hi;
         rn[k] = okn[k] ? in[i] : rec_zero<R>();
       }
-      tile_split(
+      tile_split<R, ITEMS>(
           r, ok, (int)nb, buf, cnt, start, digit,
           [&](int i, uint32_t c) {
             pos[i] = next[i];
