@@ -2167,7 +2167,7 @@ bruno_status count_by_regions(bruno_job* j, const ParseArgs& a, bool* done) {
       for (auto& r : ranges) {
         if (r[1] <= r[0]) continue;
         s1.seg[s1.n_seg++] = SplitSeg{r[0], r[1], tiles, (uint32_t)b, 0};
-        tiles += (r[1] - r[0] + SPLIT_TILE - 1) / SPLIT_TILE;
+        tiles += (r[1] - r[0] + SplitShape<R>::TILE - 1) / SplitShape<R>::TILE;
       }
     }
   }

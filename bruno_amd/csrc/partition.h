@@ -147,18 +147,26 @@ __device__ __forceinline__ void tile_split(const R (&r)[ITEMS], const bool (&ok)
   __syncthreads();
 }
 
+/* Both passes sort 8-byte records in tiles of 8,192 (16 per thread), so a
+ * sub-bucket's or region's run of a tile is twice as long as with 4,096;
+ * 16-byte records keep 4,096 (the tile fills the same 64 KB of LDS). */
+template <typename R> struct SplitShape {
+  static constexpr int ITEMS = sizeof(R) == 8 ? 2 * SPLIT_ITEMS : SPLIT_ITEMS;
+  static constexpr int TILE = SPLIT_BS * ITEMS;
+};
+
 /* the records of tile t of the split segments (ok: a real record) */
-template <typename R>
-__device__ __forceinline__ void seg_tile_load(const Split1Args& a, uint64_t t, R (&r)[SPLIT_ITEMS],
-                                              bool (&ok)[SPLIT_ITEMS], uint32_t& bucket) {
+template <typename R, int ITEMS>
+__device__ __forceinline__ void seg_tile_load(const Split1Args& a, uint64_t t, R (&r)[ITEMS], bool (&ok)[ITEMS],
+                                              uint32_t& bucket) {
   int g = 0;
   while (g + 1 < a.n_seg && a.seg[g + 1].first_tile <= t) ++g;
   const SplitSeg& sg = a.seg[g];
-  const uint64_t lo = sg.begin + (t - sg.first_tile) * SPLIT_TILE;
+  const uint64_t lo = sg.begin + (t - sg.first_tile) * (uint64_t)(SPLIT_BS * ITEMS);
   const R* in = (const R*)a.in;
   bucket = sg.bucket;
 #pragma unroll
-  for (int k = 0; k < SPLIT_ITEMS; ++k) {
+  for (int k = 0; k < ITEMS; ++k) {
     const uint64_t i = lo + (uint64_t)k * SPLIT_BS + threadIdx.x;
     r[k] = i < sg.end ? in[i] : rec_zero<R>();
     ok[k] = i < sg.end && (uint32_t)(rec_key(r[k]) & 31) != REC_CODE_FILL;
@@ -170,8 +178,9 @@ __device__ __forceinline__ void seg_tile_load(const Split1Args& a, uint64_t t, R
  * lines at their ends) meet in one L2 instead of being written back in
  * parts by two.  The grid is a multiple of SPLIT_XCDS. */
 template <typename R>
-__global__ __launch_bounds__(SPLIT_BS) void k_split1(Split1Args a) {
-  __shared__ R buf[SPLIT_TILE];
+__global__ __launch_bounds__(SPLIT_BS) __attribute__((amdgpu_waves_per_eu(4))) void k_split1(Split1Args a) {
+  constexpr int ITEMS = SplitShape<R>::ITEMS;
+  __shared__ R buf[SplitShape<R>::TILE];
   __shared__ uint32_t cnt[1 << SPLIT1_MAX_BITS], start[1 << SPLIT1_MAX_BITS];
   __shared__ unsigned long long pos[1 << SPLIT1_MAX_BITS], lim[1 << SPLIT1_MAX_BITS];
   R* out = (R*)a.out;
@@ -179,15 +188,15 @@ __global__ __launch_bounds__(SPLIT_BS) void k_split1(Split1Args a) {
   const int nb = 1 << b1;
   const uint32_t x = blockIdx.x % SPLIT_XCDS, per = gridDim.x / SPLIT_XCDS;
   const uint64_t t0 = a.xcd_tiles[x] + blockIdx.x / SPLIT_XCDS, t1 = a.xcd_tiles[x + 1];
-  R rn[SPLIT_ITEMS];
-  bool okn[SPLIT_ITEMS];
+  R rn[ITEMS];
+  bool okn[ITEMS];
   uint32_t bn = 0;
   if (t0 < t1) seg_tile_load(a, t0, rn, okn, bn);
   for (uint64_t t = t0; t < t1; t += per) {
-    R r[SPLIT_ITEMS];
-    bool ok[SPLIT_ITEMS];
+    R r[ITEMS];
+    bool ok[ITEMS];
 #pragma unroll
-    for (int k = 0; k < SPLIT_ITEMS; ++k) {
+    for (int k = 0; k < ITEMS; ++k) {
       r[k] = rn[k];
       ok[k] = okn[k];
     }
@@ -196,7 +205,7 @@ __global__ __launch_bounds__(SPLIT_BS) void k_split1(Split1Args a) {
     if (t + per < t1) seg_tile_load(a, t + per, rn, okn, bn);
     unsigned long long* cur = a.cursor + ((uint64_t)bucket << b1);
     const uint64_t* limit = a.limit + ((uint64_t)bucket << b1);
-    tile_split<R, SPLIT_ITEMS>(
+    tile_split<R, ITEMS>(
         r, ok, nb, buf, cnt, start, [&](uint64_t x) -> uint32_t { return b1 ? (uint32_t)(x >> (64 - b1)) : 0u; },
         [&](int i, uint32_t c) {
           pos[i] = c ? atomicAdd(&cur[i], (unsigned long long)c) : 0ULL;
@@ -219,15 +228,16 @@ __global__ __launch_bounds__(SPLIT_BS) void k_split_count(Split1Args a, unsigned
   __shared__ uint32_t cnt[1 << SPLIT1_MAX_BITS];
   const int b1 = a.b1;
   const int nb = 1 << b1;
+  constexpr int ITEMS = SplitShape<R>::ITEMS;
   for (uint64_t t = blockIdx.x; t < a.n_tiles; t += gridDim.x) {
-    R r[SPLIT_ITEMS];
-    bool ok[SPLIT_ITEMS];
+    R r[ITEMS];
+    bool ok[ITEMS];
     uint32_t bucket = 0;
     seg_tile_load(a, t, r, ok, bucket);
     for (int i = threadIdx.x; i < nb; i += SPLIT_BS) cnt[i] = 0;
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < SPLIT_ITEMS; ++k)
+    for (int k = 0; k < ITEMS; ++k)
       if (ok[k]) atomicAdd(&cnt[b1 ? (uint32_t)(rec_key(r[k]) >> (64 - b1)) : 0u], 1u);
     __syncthreads();
     for (int i = threadIdx.x; i < nb; i += SPLIT_BS)
@@ -247,16 +257,9 @@ struct Split2Args {
   uint64_t* region_start;         /* [n_sb << b2 (+1)]: first record of every region */
 };
 
-/* 8-byte records in tiles of 8,192 (16 per thread): each region's run of a
- * tile is twice as long as with 4,096, so fewer, longer writes; 16-byte
- * records keep 4,096 (the tile fills the same 64 KB of LDS) */
-template <typename R> struct Split2Shape {
-  static constexpr int ITEMS = sizeof(R) == 8 ? 2 * SPLIT_ITEMS : SPLIT_ITEMS;
-  static constexpr int TILE = SPLIT_BS * ITEMS;
-};
 template <typename R>
 __global__ __launch_bounds__(SPLIT_BS) __attribute__((amdgpu_waves_per_eu(4))) void k_split2(Split2Args a) {
-  constexpr int ITEMS = Split2Shape<R>::ITEMS, TILE = Split2Shape<R>::TILE;
+  constexpr int ITEMS = SplitShape<R>::ITEMS, TILE = SplitShape<R>::TILE;
   __shared__ R buf[TILE];
   const R* in = (const R*)a.in;
   R* out = (R*)a.out;
