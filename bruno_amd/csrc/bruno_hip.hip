@@ -1589,6 +1589,27 @@ inline hipStream_t shared_side_stream() {
   return s[dev];
 }
 
+/* A second stream per device for the region builds of a count, beside the
+ * multisplit pass on the job's stream (ordered through events, as the side
+ * stream). */
+inline hipStream_t shared_count_stream() {
+  static std::mutex mu;
+  static hipStream_t s[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> g(mu);
+  if (!s[dev] && hipStreamCreateWithFlags(&s[dev], hipStreamNonBlocking) != hipSuccess) {
+    (void)hipGetLastError();
+    s[dev] = nullptr;
+  }
+  return s[dev];
+}
+/* sub-bucket groups of a pipelined region count */
+constexpr int COUNT_GROUPS = 8;
+
 enum Stage { ST_CREATED = 0, ST_COUNTED = 1, ST_FILTERED = 2, ST_CHAINS = 3, ST_RANKED = 4, ST_COMPACTED = 5 };
 
 }  // namespace
@@ -1695,6 +1716,7 @@ struct bruno_job {
   bool recycle = false;
   hipStream_t side = nullptr;
   hipEvent_t ev_chains = nullptr, ev_cleared = nullptr;
+  hipEvent_t cev[COUNT_GROUPS + 1] = {}; /* the pipelined count's group events */
   bool clear_pending = false;
   bruno_phase_ms ms{};
 
@@ -2209,9 +2231,9 @@ bruno_status count_by_regions(bruno_job* j, const ParseArgs& a, bool* done) {
   HIP_CHECK(hipMemcpyAsync(d_out, out.data(), (n_sb + 1) * 8, hipMemcpyHostToDevice, j->stream));
   HIP_CHECK(hipMemcpyAsync(d_base, base.data(), n_sb * 8, hipMemcpyHostToDevice, j->stream));
   HIP_CHECK(hipMemcpyAsync(rstart + nreg, &out[n_sb], 8, hipMemcpyHostToDevice, j->stream));
-  Split2Args s2{recs2, d_base, d_out, n_sb, 64 - b1 - b2, b2, recs, rstart};
-  hipLaunchKernelGGL(k_split2<R>, dim3((unsigned)std::min<uint64_t>(n_sb, split_grid)), dim3(SPLIT_BS), 0, j->stream,
-                     s2);
+  hipLaunchKernelGGL(k_region_bounds, dim3(grid_for(n_sb + 1, BLOCK, 1u << 30)), dim3(BLOCK), 0, j->stream, d_out, n_sb,
+                     b2, rstart);
+  Split2Args s2{recs2, d_base, d_out, 0, n_sb, 64 - b1 - b2, b2, recs, rstart};
   const int valid = j->table_valid ? 1 : 0;
   /* a first count builds every region: it also leaves the filter scan's
    * bits (stale once another count follows) */
@@ -2269,27 +2291,66 @@ bruno_status count_by_regions(bruno_job* j, const ParseArgs& a, bool* done) {
       fuse = false;
     }
   }
-  RegionArgs g{j->table, j->rspan, j->rbits, nreg, recs, rstart, valid, j->spill, j->spcap, spq, spq_n, spq_cap,
-               j->ctr, cbits, sbits, stride, j->thresholding, scan_lo(j), fuse ? 1 : 0, j->K, j->th, nl, holes};
-  auto build = [&](auto fuse_tag) {
+  RegionArgs g{j->table, j->rspan, j->rbits, nreg, 0, nreg, recs, rstart, valid, j->spill, j->spcap, spq, spq_n,
+               spq_cap, j->ctr, cbits, sbits, stride, j->thresholding, scan_lo(j), fuse ? 1 : 0, j->K, j->th, nl,
+               holes};
+  auto build = [&](auto fuse_tag, hipStream_t st, dim3 b4, dim3 b2_) {
     constexpr bool F = decltype(fuse_tag)::value;
     if (RB == 8) {
       if (j->rspan <= REGION_SMALL_SLOTS)
-        hipLaunchKernelGGL((k_region_build<uint64_t, REGION_SMALL_SLOTS, 256, F>), g4, dim3(256), 0, j->stream, g);
+        hipLaunchKernelGGL((k_region_build<uint64_t, REGION_SMALL_SLOTS, 256, F>), b4, dim3(256), 0, st, g);
       else
-        hipLaunchKernelGGL((k_region_build<uint64_t, REGION_MAX_SLOTS, 512, F>), g2, dim3(512), 0, j->stream, g);
+        hipLaunchKernelGGL((k_region_build<uint64_t, REGION_MAX_SLOTS, 512, F>), b2_, dim3(512), 0, st, g);
     } else {
       if (j->rspan <= REGION_WIDE_SMALL_SLOTS)
-        hipLaunchKernelGGL((k_region_build<u128, REGION_WIDE_SMALL_SLOTS, 256, F>), g4, dim3(256), 0, j->stream, g);
+        hipLaunchKernelGGL((k_region_build<u128, REGION_WIDE_SMALL_SLOTS, 256, F>), b4, dim3(256), 0, st, g);
       else
-        hipLaunchKernelGGL((k_region_build<u128, REGION_WIDE_SLOTS, 512, F>), g2, dim3(512), 0, j->stream, g);
+        hipLaunchKernelGGL((k_region_build<u128, REGION_WIDE_SLOTS, 512, F>), b2_, dim3(512), 0, st, g);
     }
   };
-  if (fuse) build(std::true_type{});
-  else build(std::false_type{});
-  if (spq_cap)
-    hipLaunchKernelGGL(k_spill_flush, dim3((unsigned)n_waves), dim3(BLOCK), 0, j->stream, spq, spq_n, spq_cap,
-                       j->spill, j->spcap, j->ctr);
+  /* Pass 2 and the builds, pipelined by groups of sub-buckets: the build
+   * of a group (bound by instruction issue and LDS) runs on a second stream
+   * beside the next group's multisplit (bound by HBM).  One group when the
+   * filter is fused (its list chunks and holes belong to one launch) or
+   * the sub-buckets are few. */
+  hipStream_t bst = (fuse || n_sb < 8 * COUNT_GROUPS) ? nullptr : shared_count_stream();
+  if (bst) {
+    for (hipEvent_t& e : j->cev)
+      if (!e && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
+        e = nullptr;
+        bst = nullptr;
+      }
+  }
+  const int ng = bst ? COUNT_GROUPS : 1;
+  if (!bst) bst = j->stream;
+  for (int gi = 0; gi < ng; ++gi) {
+    const uint64_t sb0 = n_sb * gi / ng, sb1 = n_sb * (gi + 1) / ng;
+    if (sb1 == sb0) continue;
+    s2.sb0 = sb0;
+    s2.n_sb = sb1 - sb0;
+    hipLaunchKernelGGL(k_split2<R>, dim3((unsigned)std::min<uint64_t>(sb1 - sb0, split_grid)), dim3(SPLIT_BS), 0,
+                       j->stream, s2);
+    if (bst != j->stream) {
+      HIP_CHECK(hipEventRecord(j->cev[gi], j->stream));
+      HIP_CHECK(hipStreamWaitEvent(bst, j->cev[gi], 0));
+    }
+    g.q0 = sb0 << b2;
+    g.q1 = sb1 << b2;
+    const uint64_t nq = g.q1 - g.q0;
+    const dim3 b4((unsigned)std::min<uint64_t>(nq, 4 * REGION_BUILD_CU_SLOTS)),
+        b2d((unsigned)std::min<uint64_t>(nq, 2 * REGION_BUILD_CU_SLOTS));
+    if (fuse) build(std::true_type{}, bst, b4, b2d);
+    else build(std::false_type{}, bst, b4, b2d);
+    /* this launch's waves' side-table queues */
+    if (spq_cap)
+      hipLaunchKernelGGL(k_spill_flush, dim3(small ? b4.x * 4 : b2d.x * 8), dim3(BLOCK), 0, bst, spq, spq_n, spq_cap,
+                         j->spill, j->spcap, j->ctr);
+  }
+  if (bst != j->stream) {
+    HIP_CHECK(hipEventRecord(j->cev[COUNT_GROUPS], bst));
+    HIP_CHECK(hipStreamWaitEvent(j->stream, j->cev[COUNT_GROUPS], 0));
+  }
   /* instances and dropped reads into the job's counters */
   hipLaunchKernelGGL(k_add_u64, dim3(1), dim3(64), 0, j->stream, (unsigned long long*)&j->ctr->n_instances, stat, 1);
   hipLaunchKernelGGL(k_add_u64, dim3(1), dim3(64), 0, j->stream, (unsigned long long*)&j->ctr->n_dropped, stat + 1,
@@ -3216,6 +3277,8 @@ bruno_status bruno_job_destroy(bruno_job* j) {
    * ev_cleared) */
   if (j->clear_pending && j->ev_cleared) (void)hipEventSynchronize(j->ev_cleared);
   if (j->ev_chains) (void)hipEventDestroy(j->ev_chains);
+  for (hipEvent_t e : j->cev)
+    if (e) (void)hipEventDestroy(e);
   if (j->ev_cleared) (void)hipEventDestroy(j->ev_cleared);
   for (void* p : j->allocs) pool_free(p);
   j->allocs.clear();

@@ -246,10 +246,19 @@ __global__ __launch_bounds__(SPLIT_BS) void k_split_count(Split1Args a, unsigned
   }
 }
 
+/* region_start of every sub-bucket's first region (its output start), so a
+ * region build of the sub-buckets before sb finds its last region's end
+ * before the pass over sb has run */
+__global__ void k_region_bounds(const uint64_t* out_start, uint64_t n_sb, int b2, uint64_t* region_start) {
+  const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s <= n_sb) region_start[s << b2] = out_start[s];
+}
+
 struct Split2Args {
   const void* in;                 /* records (RecOf<KT>::T) */
   const uint64_t* in_start;       /* [n_sb]: sub-bucket s holds in[in_start[s], + its count) */
   const uint64_t* out_start;      /* [n_sb + 1]: and goes to out[out_start[s], out_start[s + 1]) */
+  uint64_t sb0;                   /* this launch: sub-buckets [sb0, sb0 + n_sb) */
   uint64_t n_sb;
   int shift;                      /* digit = (rec >> shift) & (2^b2 - 1) */
   int b2;
@@ -268,7 +277,7 @@ __global__ __launch_bounds__(SPLIT_BS) __attribute__((amdgpu_waves_per_eu(4))) v
   const int b2 = a.b2, shift = a.shift;
   const uint32_t nb = 1u << b2, m = nb - 1;
   auto digit = [&](uint64_t x) -> uint32_t { return b2 ? (uint32_t)(x >> shift) & m : 0u; };
-  for (uint64_t s = blockIdx.x; s < a.n_sb; s += gridDim.x) {
+  for (uint64_t s = a.sb0 + blockIdx.x; s < a.sb0 + a.n_sb; s += gridDim.x) {
     const uint64_t o0 = a.out_start[s], n = a.out_start[s + 1] - o0;
     const uint64_t lo = a.in_start[s], hi = lo + n;
     /* count pass: every digit's records in the sub-bucket (a region's

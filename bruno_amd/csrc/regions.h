@@ -281,6 +281,7 @@ struct RegionArgs {
   uint64_t span;             /* slots per region */
   uint32_t rbits;            /* region bits (the bucket's 5 and rb below them) */
   uint64_t n_regions;        /* 2^rbits */
+  uint64_t q0, q1;           /* this launch: regions [q0, q1) */
   const void* recs;          /* records (RecOf<KT>::T) grouped by region */
   const uint64_t* region_start; /* [n_regions + 1]: region q holds recs[region_start[q], region_start[q + 1]) */
   int valid;                 /* the table holds earlier counts (else start from EMPTY) */
@@ -655,7 +656,7 @@ __global__ __launch_bounds__(BLOCK) void k_spill_flush(const uint64_t* spq, cons
   block_add(mine, &ctr->n_k2);
 }
 
-/* Build regions [qa, qb), workgroup i taking an even share of the regions.
+/* Build regions [q0, q1), workgroup i taking an even share [qa, qb) of them.
  * Per region: set up the slots in LDS (EMPTY, or the table's slots when an
  * earlier count wrote them; a region without records is then left as it
  * is), add the region's records (REGION_ITEMS per thread at a time), write
@@ -673,7 +674,9 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) void k_
   __shared__ LdsRegion<KT, SLOTS> L;
   const uint32_t span = (uint32_t)g.span;
   const uint64_t nreg = g.n_regions;
-  const uint64_t qa = (uint64_t)blockIdx.x * nreg / gridDim.x, qb = ((uint64_t)blockIdx.x + 1) * nreg / gridDim.x;
+  const uint64_t nlaunch = g.q1 - g.q0;
+  const uint64_t qa = g.q0 + (uint64_t)blockIdx.x * nlaunch / gridDim.x,
+                 qb = g.q0 + ((uint64_t)blockIdx.x + 1) * nlaunch / gridDim.x;
   const R* recs = (const R*)g.recs;
   unsigned long long my_new = 0, my_k2 = 0;
   constexpr uint64_t TILE = (uint64_t)REGION_ITEMS * BS;

@@ -197,3 +197,34 @@ def test_spilled_entries_in_regions(k):
         g, _ = compare_all(k, th, b, off, capacity=REGION_CAP)
         assert g["stats"]["count_by_regions"] == 1
         assert g["stats"]["n_overflow"] >= 40
+
+
+def spill_reads(k, seed=5):
+    rng = np.random.default_rng(seed)
+    reads = []
+    for _ in range(40):
+        core = "".join(rng.choice(list("ACGT"), k))
+        for a in "ACGT":
+            for c in "ACGT":
+                r = "".join(rng.choice(list("ACGT"), 10)) + a + core + c + "".join(rng.choice(list("ACGT"), 10))
+                reads += [r] * 5 + [r.translate(str.maketrans("ACGT", "TGCA"))[::-1]] * 2
+    return [reads[i] for i in rng.permutation(len(reads))]
+
+
+@pytest.mark.parametrize("case", [CASES[3], CASES[5], CASES[7], CASES[8]])
+def test_pipelined_region_builds(case):
+    """2^14 regions and more (2^24 slots in regions of at most 1,280):
+    the second pass splits the sub-buckets in groups, and each group's
+    region builds run on a second stream while the next group is split.
+    Same results, spilled slots included."""
+    old = get_tuning()
+    set_tuning(region_slots=1280)
+    try:
+        k, reads = _reads(case)
+        for rs in (reads, reads + spill_reads(k)):
+            b, off = pack(rs)
+            for th in (None, thresholds_L(2)):
+                g, _ = compare_all(k, th, b, off, capacity=1 << 24)
+                assert g["stats"]["count_by_regions"] == 1
+    finally:
+        set_tuning(**old)
