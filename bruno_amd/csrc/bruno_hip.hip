@@ -1608,7 +1608,7 @@ inline hipStream_t shared_count_stream() {
   return s[dev];
 }
 /* sub-bucket groups of a pipelined region count */
-constexpr int COUNT_GROUPS = 8;
+constexpr int COUNT_GROUPS = 16;
 
 enum Stage { ST_CREATED = 0, ST_COUNTED = 1, ST_FILTERED = 2, ST_CHAINS = 3, ST_RANKED = 4, ST_COMPACTED = 5 };
 
@@ -2312,8 +2312,9 @@ bruno_status count_by_regions(bruno_job* j, const ParseArgs& a, bool* done) {
    * of a group (bound by instruction issue and LDS) runs on a second stream
    * beside the next group's multisplit (bound by HBM).  One group when the
    * filter is fused (its list chunks and holes belong to one launch) or
-   * the sub-buckets are few. */
-  hipStream_t bst = (fuse || n_sb < 8 * COUNT_GROUPS) ? nullptr : shared_count_stream();
+   * the sub-buckets are few.  A pipelined pass 2 runs one workgroup per CU,
+   * which leaves every CU's LDS room for a region build beside it. */
+  hipStream_t bst = (fuse || n_sb < 4 * COUNT_GROUPS) ? nullptr : shared_count_stream();
   if (bst) {
     for (hipEvent_t& e : j->cev)
       if (!e && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
@@ -2329,8 +2330,8 @@ bruno_status count_by_regions(bruno_job* j, const ParseArgs& a, bool* done) {
     if (sb1 == sb0) continue;
     s2.sb0 = sb0;
     s2.n_sb = sb1 - sb0;
-    hipLaunchKernelGGL(k_split2<R>, dim3((unsigned)std::min<uint64_t>(sb1 - sb0, split_grid)), dim3(SPLIT_BS), 0,
-                       j->stream, s2);
+    hipLaunchKernelGGL(k_split2<R>, dim3((unsigned)std::min<uint64_t>(sb1 - sb0, ng > 1 ? device_cus() : split_grid)),
+                       dim3(SPLIT_BS), 0, j->stream, s2);
     if (bst != j->stream) {
       HIP_CHECK(hipEventRecord(j->cev[gi], j->stream));
       HIP_CHECK(hipStreamWaitEvent(bst, j->cev[gi], 0));
