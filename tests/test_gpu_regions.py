@@ -212,19 +212,20 @@ def spill_reads(k, seed=5):
 
 
 @pytest.mark.parametrize("case", [CASES[3], CASES[5], CASES[7], CASES[8]])
-def test_pipelined_region_builds(case):
-    """2^14 regions and more (2^24 slots in regions of at most 1,280):
-    the second pass splits the sub-buckets in groups, and each group's
-    region builds run on a second stream while the next group is split.
-    Same results, spilled slots included."""
+@pytest.mark.parametrize("region_slots,capacity", [(1280, 1 << 24), (2560, 1 << 25)])
+def test_pipelined_region_builds(case, region_slots, capacity):
+    """2^14 regions and more: the second pass splits the sub-buckets in
+    groups, and each group's region builds run on a second stream while
+    the next group is split (small regions: the 256-thread build; large
+    ones: the 512-thread build).  Same results, spilled slots included."""
     old = get_tuning()
-    set_tuning(region_slots=1280)
+    set_tuning(region_slots=region_slots)
     try:
         k, reads = _reads(case)
         for rs in (reads, reads + spill_reads(k)):
             b, off = pack(rs)
             for th in (None, thresholds_L(2)):
-                g, _ = compare_all(k, th, b, off, capacity=1 << 24)
+                g, _ = compare_all(k, th, b, off, capacity=capacity)
                 assert g["stats"]["count_by_regions"] == 1
     finally:
         set_tuning(**old)
