@@ -558,7 +558,7 @@ def run_batched(args, free):
         "per_job_results_equal": bool(same and inst1 == inst),
         "peak_device_gb": peak / 1e9,
         "roofline": {
-            "kernel": "k_parse (parse + k2 count), all groups",
+            "kernel": "count phase, all groups (a batch: k_parse, the atomic count; a large job alone: the region count)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,

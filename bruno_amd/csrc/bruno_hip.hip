@@ -3267,7 +3267,15 @@ bruno_status bruno_batch_create(const bruno_job_desc* d, uint32_t n_jobs, const 
   if (!capacities || n_jobs == 0 || n_jobs > BRUNO_MAX_BATCH_JOBS) return BRUNO_BAD_INPUT;
   for (uint32_t i = 0; i < n_jobs; ++i)
     if (capacities[i] == 0) return BRUNO_BAD_INPUT;
-  return create_common(d, n_jobs, capacities, out);
+  /* one job: a plain job (the region layout when its table is large) */
+  return create_common(d, n_jobs, capacities, out, n_jobs == 1);
+}
+
+/* Slots from which a one-job table counts by regions (0: never, the
+ * region count switched off); for the request server, not exported. */
+__attribute__((visibility("hidden"))) uint64_t br_region_count_min_slots() {
+  const uint32_t cr = tuning().count_regions;
+  return cr == 0 ? 0 : cr >= 2 ? REGION_MIN_CAP_ANY : REGION_MIN_CAP;
 }
 
 bruno_status bruno_job_destroy(bruno_job* j) {

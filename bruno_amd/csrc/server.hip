@@ -36,6 +36,9 @@
 #include "../../include/bruno_hip.h"
 #include "pool.h"
 
+/* bruno_hip.hip: slots from which a one-job table counts by regions */
+extern "C" uint64_t br_region_count_min_slots();
+
 namespace {
 
 __global__ void k_add_offset(uint64_t* o, uint64_t n, uint64_t add) {
@@ -393,6 +396,10 @@ bruno_status bruno_server_submit(bruno_server* S, const bruno_request* q, uint64
   r->capacity = q->job.capacity;
   if (r->capacity == 0) r->capacity = std::max<uint64_t>(4096, 2 * r->n_bases);
   r->footprint = footprint(r->job, r->capacity, r->n_reads, r->n_bases);
+  /* a table large enough to count by regions runs as a batch of its own:
+   * the region count beats the batched atomic count there */
+  const uint64_t solo = br_region_count_min_slots();
+  r->alone = solo && r->capacity >= solo;
   std::lock_guard<std::mutex> lk(S->mu);
   if (S->stop) return BRUNO_STATE;
   r->ticket = S->next_ticket++;

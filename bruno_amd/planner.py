@@ -39,11 +39,24 @@ def footprint(s, k, load_factor=0.5, cap=None):
     return int(table + present + reads + chains)
 
 
-def plan_groups(cfgs, k, budget_bytes, load_factor=0.5):
-    """Consecutive jobs packed into launch groups whose estimated footprint
-    stays within the budget (a job larger than the budget runs alone)."""
+# A one-job table of at least this many slots counts by regions (the
+# library's default count_regions 1); the batched count is the atomic one.
+REGION_COUNT_SLOTS = 1 << 27
+
+
+def plan_groups(cfgs, k, budget_bytes, load_factor=0.5, solo_slots=REGION_COUNT_SLOTS):
+    """Jobs packed in arrival order into launch groups whose estimated
+    footprint stays within the budget (a job larger than the budget runs
+    alone).  A job whose table reaches solo_slots (0: none) runs alone too,
+    in its place: counted by regions it goes faster than in a batch, and a
+    batch gains nothing on a job that fills the device by itself.  The
+    other jobs keep filling the open group past it, which then runs when
+    it is full (or at the end)."""
     groups, cur, used = [], [], 0
     for i, s in enumerate(cfgs):
+        if solo_slots and capacity(s, k, load_factor) >= solo_slots:
+            groups.append([i])
+            continue
         f = footprint(s, k, load_factor)
         if cur and used + f > budget_bytes:
             groups.append(cur)

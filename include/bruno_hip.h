@@ -308,9 +308,11 @@ bruno_status bruno_get_chain_edges(bruno_job* job, bruno_chain_edge* buf, uint64
  * (bruno_filter, bruno_build_chains, bruno_list_rank, bruno_compact,
  * bruno_run_pipeline, bruno_job_reset, bruno_job_destroy); the whole-job
  * getters return BRUNO_BAD_INPUT on a batch of more than one job, and the
- * per-job getters below take the job index.  The reference runs one
- * assembly per MPI job (compact_dbg_freq_construct_clean.cpp:431-840); it
- * has no batched counterpart. */
+ * per-job getters below take the job index.  A batch of one job is a
+ * plain job (its table may take the region layout and count by regions,
+ * as bruno_job_create's).  The reference runs one assembly per MPI job
+ * (compact_dbg_freq_construct_clean.cpp:431-840); it has no batched
+ * counterpart. */
 #define BRUNO_MAX_BATCH_JOBS 65536u
 
 bruno_status bruno_batch_create(const bruno_job_desc* desc, uint32_t n_jobs, const uint64_t* capacities,

@@ -229,3 +229,25 @@ def test_pipelined_region_builds(case, region_slots, capacity):
                 assert g["stats"]["count_by_regions"] == 1
     finally:
         set_tuning(**old)
+
+
+@pytest.mark.parametrize("case", [CASES[5], CASES[7]])
+def test_one_job_batch_counts_by_regions(case):
+    """A batch of one job is a plain job: its large table takes the region
+    layout and counts by regions, with the results of the job run alone."""
+    k, reads = _reads(case)
+    b, off = pack(reads)
+    th = thresholds_L(2)
+    bt = bruno_amd.GraphBatch(k, th, [REGION_CAP])
+    bt.parse_count(b, off, [0, len(reads)])
+    bt.run(1)
+    v = bt.job(0)
+    st, u, br = v.stats(), v.unitigs(), v.branch_fasta()
+    bt.close()
+    assert st["count_by_regions"] == 1
+    job = bruno_amd.GraphJob(k, th, capacity=REGION_CAP)
+    job.parse_count(b, off)
+    job.run(1)
+    assert (u, br) == (job.unitigs(), job.branch_fasta())
+    assert st["n_unitigs"] == job.stats()["n_unitigs"]
+    job.close()

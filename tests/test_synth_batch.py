@@ -24,6 +24,30 @@ def test_plan_groups_respects_budget_and_order():
     from bruno_amd.planner import footprint
     budget = 40e9
     groups = plan_groups(jobs, 31, budget)
-    assert [i for g in groups for i in g] == list(range(64))
+    assert sorted(i for g in groups for i in g) == list(range(64))
     for g in groups:
+        assert g == sorted(g)
         assert len(g) == 1 or sum(footprint(jobs[i], 31) for i in g) <= budget
+
+
+def test_plan_groups_runs_region_sized_jobs_alone():
+    """Jobs whose table reaches the region-count size get a launch group of
+    their own, in arrival order; the others fill groups within the budget
+    past them."""
+    from bruno_amd.planner import REGION_COUNT_SLOTS, capacity, footprint
+    jobs = config4_jobs(seed=4, n_jobs=64)
+    big = [capacity(j, 31) >= REGION_COUNT_SLOTS for j in jobs]
+    assert 0 < sum(big) < len(jobs)
+    budget = 40e9
+    groups = plan_groups(jobs, 31, budget)
+    assert sorted(i for g in groups for i in g) == list(range(64))
+    for g in groups:
+        assert g == sorted(g)
+        assert len(g) == 1 or (not any(big[i] for i in g) and sum(footprint(jobs[i], 31) for i in g) <= budget)
+    assert sum(len(g) == 1 and big[g[0]] for g in groups) == sum(big)
+    # the small jobs need as many groups as their footprint asks, not one per gap between large jobs
+    small = [footprint(j, 31) for j, b in zip(jobs, big) if not b]
+    assert sum(len(g) > 1 or not big[g[0]] for g in groups) <= 2 * sum(small) / budget + 2
+    # solo_slots=0: budget packing only, consecutive
+    flat = plan_groups(jobs, 31, budget, solo_slots=0)
+    assert [i for g in flat for i in g] == list(range(64))
