@@ -54,7 +54,7 @@ class Tuning(ctypes.Structure):
     _fields_ = [("parse_block", ctypes.c_uint32), ("parse_grid", ctypes.c_uint32),
                 ("scan_grid", ctypes.c_uint32), ("ruler_sample", ctypes.c_uint32),
                 ("count_regions", ctypes.c_uint32), ("region_slots", ctypes.c_uint32),
-                ("fuse_list_cap", ctypes.c_uint32)]
+                ("fuse_list_cap", ctypes.c_uint32), ("count_groups", ctypes.c_uint32)]
 
 
 class ServerDesc(ctypes.Structure):

@@ -4,6 +4,8 @@ The knobs change speed only, never results:
   load factor   hash slots per distinct k-mer (count probes vs. table size,
                 which the filter scan and the table reset stream);
   region_slots  region size of a region-layout table (k <= 31 plain jobs);
+  count_groups  sub-bucket groups of the region count's split / build
+                pipeline (0: automatic);
   parse_block   workgroup size of the atomic parse + count kernel;
   parse_grid    waves of the atomic parse + count launch;
   scan_grid     workgroups of the filter's table scan (atomic-count tables);
@@ -28,6 +30,7 @@ PARSE_BLOCKS = (128, 256, 512)
 PARSE_GRIDS = (16384, 32768, 65536, 131072, 262144)
 SCAN_GRIDS = (4096, 8192, 16384, 32768)
 RULERS = (16, 32, 64, 128, 256)
+COUNT_GROUPS = (0, 1, 4, 8, 16)
 REGION_SLOTS = (2560, 1280)
 
 
@@ -97,6 +100,13 @@ def tune(k, thresholds, bases_ptr, offsets_ptr, n_reads, capacity_for_lf, reps=2
             trials.append({"knob": "region_slots", "value": rs, "ms": res[rs]})
             say("region slots %d: %.1f ms" % (rs, res[rs]))
         set_tuning(region_slots=min(res, key=res.get))
+        res = {}
+        for cg in COUNT_GROUPS:
+            set_tuning(count_groups=cg)
+            res[cg] = run(lf, ("parse_count_ms",))
+            trials.append({"knob": "count_groups", "value": cg, "ms": res[cg]})
+            say("count groups %d: %.1f ms" % (cg, res[cg]))
+        set_tuning(count_groups=min(res, key=res.get))
     else:
         # 2b. launch shape of the atomic count
         res = {}

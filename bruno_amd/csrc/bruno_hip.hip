@@ -1607,8 +1607,9 @@ inline hipStream_t shared_count_stream() {
   }
   return s[dev];
 }
-/* sub-bucket groups of a pipelined region count */
-constexpr int COUNT_GROUPS = 16;
+/* sub-bucket groups of a pipelined region count: at most this many chosen
+ * automatically, at most MAX_COUNT_GROUPS by the tuning knob */
+constexpr uint64_t COUNT_GROUPS = 16, MAX_COUNT_GROUPS = 64;
 
 enum Stage { ST_CREATED = 0, ST_COUNTED = 1, ST_FILTERED = 2, ST_CHAINS = 3, ST_RANKED = 4, ST_COMPACTED = 5 };
 
@@ -1716,7 +1717,10 @@ struct bruno_job {
   bool recycle = false;
   hipStream_t side = nullptr;
   hipEvent_t ev_chains = nullptr, ev_cleared = nullptr;
-  hipEvent_t cev[COUNT_GROUPS + 1] = {}; /* the pipelined count's group events */
+  /* the pipelined count's events: a group split (job stream -> build
+   * stream; each wait takes the record made just before it), all builds
+   * done (back) */
+  hipEvent_t cev[2] = {};
   bool clear_pending = false;
   bruno_phase_ms ms{};
 
@@ -1738,7 +1742,7 @@ namespace {
 /* Process-wide tuning knobs (bruno_set_tuning). */
 std::mutex g_tuning_mu;
 /* defaults: the autotuner's picks on the 100 Mbp / 33 M-read workload */
-bruno_tuning g_tuning = {256, 65536, 32768, 64, 1, 2560, 0};
+bruno_tuning g_tuning = {256, 65536, 32768, 64, 1, 2560, 0, 0};
 bruno_tuning tuning() {
   std::lock_guard<std::mutex> g(g_tuning_mu);
   return g_tuning;
@@ -2310,11 +2314,19 @@ bruno_status count_by_regions(bruno_job* j, const ParseArgs& a, bool* done) {
   };
   /* Pass 2 and the builds, pipelined by groups of sub-buckets: the build
    * of a group (bound by instruction issue and LDS) runs on a second stream
-   * beside the next group's multisplit (bound by HBM).  One group when the
-   * filter is fused (its list chunks and holes belong to one launch) or
-   * the sub-buckets are few.  A pipelined pass 2 runs one workgroup per CU,
-   * which leaves every CU's LDS room for a region build beside it. */
-  hipStream_t bst = (fuse || n_sb < 4 * COUNT_GROUPS) ? nullptr : shared_count_stream();
+   * beside the next group's multisplit (bound by HBM).  A pipelined pass 2
+   * runs one workgroup per CU, which leaves every CU's LDS room for a
+   * region build beside it.  By default only large counts are pipelined,
+   * from 8 sub-buckets per CU, in groups of at least half a sub-bucket per
+   * CU: on fewer sub-buckets the split's narrower launches and the builds'
+   * shared CUs cost more than the overlap gives (measured: 512 sub-buckets
+   * in two groups, 4.5 -> 5.6 ms).  One group when the filter is fused (its
+   * list chunks and holes belong to one launch). */
+  const uint64_t cus = device_cus();
+  uint64_t want = tuning().count_groups;
+  if (want == 0) want = n_sb >= 8 * cus ? std::min<uint64_t>(COUNT_GROUPS, 2 * n_sb / cus) : 1;
+  want = std::min<uint64_t>(std::min<uint64_t>(want, MAX_COUNT_GROUPS), n_sb);
+  hipStream_t bst = (fuse || want < 2) ? nullptr : shared_count_stream();
   if (bst) {
     for (hipEvent_t& e : j->cev)
       if (!e && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
@@ -2323,9 +2335,9 @@ bruno_status count_by_regions(bruno_job* j, const ParseArgs& a, bool* done) {
         bst = nullptr;
       }
   }
-  const int ng = bst ? COUNT_GROUPS : 1;
+  const uint64_t ng = bst ? want : 1;
   if (!bst) bst = j->stream;
-  for (int gi = 0; gi < ng; ++gi) {
+  for (uint64_t gi = 0; gi < ng; ++gi) {
     const uint64_t sb0 = n_sb * gi / ng, sb1 = n_sb * (gi + 1) / ng;
     if (sb1 == sb0) continue;
     s2.sb0 = sb0;
@@ -2333,8 +2345,8 @@ bruno_status count_by_regions(bruno_job* j, const ParseArgs& a, bool* done) {
     hipLaunchKernelGGL(k_split2<R>, dim3((unsigned)std::min<uint64_t>(sb1 - sb0, ng > 1 ? device_cus() : split_grid)),
                        dim3(SPLIT_BS), 0, j->stream, s2);
     if (bst != j->stream) {
-      HIP_CHECK(hipEventRecord(j->cev[gi], j->stream));
-      HIP_CHECK(hipStreamWaitEvent(bst, j->cev[gi], 0));
+      HIP_CHECK(hipEventRecord(j->cev[0], j->stream));
+      HIP_CHECK(hipStreamWaitEvent(bst, j->cev[0], 0));
     }
     g.q0 = sb0 << b2;
     g.q1 = sb1 << b2;
@@ -2349,8 +2361,8 @@ bruno_status count_by_regions(bruno_job* j, const ParseArgs& a, bool* done) {
                          j->spill, j->spcap, j->ctr);
   }
   if (bst != j->stream) {
-    HIP_CHECK(hipEventRecord(j->cev[COUNT_GROUPS], bst));
-    HIP_CHECK(hipStreamWaitEvent(j->stream, j->cev[COUNT_GROUPS], 0));
+    HIP_CHECK(hipEventRecord(j->cev[1], bst));
+    HIP_CHECK(hipStreamWaitEvent(j->stream, j->cev[1], 0));
   }
   /* instances and dropped reads into the job's counters */
   hipLaunchKernelGGL(k_add_u64, dim3(1), dim3(64), 0, j->stream, (unsigned long long*)&j->ctr->n_instances, stat, 1);
@@ -4487,7 +4499,7 @@ bruno_status bruno_set_tuning(const bruno_tuning* t) {
   if (!t) return BRUNO_BAD_INPUT;
   const bool pow2 = t->ruler_sample >= 8 && t->ruler_sample <= 4096 && !(t->ruler_sample & (t->ruler_sample - 1));
   if ((t->parse_block != 128 && t->parse_block != 256 && t->parse_block != 512) || t->parse_grid < 64 ||
-      t->scan_grid < 1 || !pow2 || t->count_regions > 2 ||
+      t->scan_grid < 1 || !pow2 || t->count_regions > 2 || t->count_groups > MAX_COUNT_GROUPS ||
       (t->region_slots != REGION_MAX_SLOTS && t->region_slots != REGION_SMALL_SLOTS))
     return BRUNO_BAD_INPUT;
   std::lock_guard<std::mutex> g(g_tuning_mu);

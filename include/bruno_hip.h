@@ -385,6 +385,11 @@ typedef struct {
   uint32_t fuse_list_cap;  /* a count with the filter fused gives each node list room for at
                               most this many nodes (0: every node the table can hold); the
                               nodes beyond it are left to bruno_filter                      */
+  uint32_t count_groups;   /* a region count's second split pass and region builds run in
+                              this many groups of sub-buckets, a group's builds beside the
+                              next group's split (1: one after the other; at most 64);
+                              0: 16 groups from 8 sub-buckets per CU (the largest tables),
+                              else 1                                                         */
 } bruno_tuning;
 bruno_status bruno_set_tuning(const bruno_tuning* t);
 bruno_status bruno_get_tuning(bruno_tuning* t);

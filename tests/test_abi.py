@@ -62,11 +62,13 @@ def test_tuning_knobs_validated_on_the_host():
     import pytest
     before = get_tuning()
     for bad in ({"parse_block": 192}, {"ruler_sample": 100}, {"ruler_sample": 4}, {"parse_grid": 1},
-                {"count_regions": 3}, {"region_slots": 2048}):
+                {"count_regions": 3}, {"region_slots": 2048}, {"count_groups": 65}):
         with pytest.raises(bruno_amd.BrunoError):
             set_tuning(**bad)
     assert get_tuning() == before
-    set_tuning(parse_block=512, ruler_sample=128, count_regions=2, region_slots=1280)
+    assert before["count_groups"] == 0  # automatic
+    set_tuning(parse_block=512, ruler_sample=128, count_regions=2, region_slots=1280, count_groups=64)
     t = get_tuning()
-    assert (t["parse_block"], t["ruler_sample"], t["count_regions"], t["region_slots"]) == (512, 128, 2, 1280)
+    assert (t["parse_block"], t["ruler_sample"], t["count_regions"], t["region_slots"], t["count_groups"]) == \
+        (512, 128, 2, 1280, 64)
     set_tuning(**before)

@@ -214,12 +214,13 @@ def spill_reads(k, seed=5):
 @pytest.mark.parametrize("case", [CASES[3], CASES[5], CASES[7], CASES[8]])
 @pytest.mark.parametrize("region_slots,capacity", [(1280, 1 << 24), (2560, 1 << 25)])
 def test_pipelined_region_builds(case, region_slots, capacity):
-    """2^14 regions and more: the second pass splits the sub-buckets in
-    groups, and each group's region builds run on a second stream while
-    the next group is split (small regions: the 256-thread build; large
+    """2^14 regions and more, 16 pipeline groups (count_groups; the
+    automatic choice pipelines only the largest tables): the
+    second pass splits the sub-buckets in groups, and each group's region
+    builds run on a second stream while the next group is split (small regions: the 256-thread build; large
     ones: the 512-thread build).  Same results, spilled slots included."""
     old = get_tuning()
-    set_tuning(region_slots=region_slots)
+    set_tuning(region_slots=region_slots, count_groups=16)
     try:
         k, reads = _reads(case)
         for rs in (reads, reads + spill_reads(k)):
@@ -251,3 +252,18 @@ def test_one_job_batch_counts_by_regions(case):
     assert (u, br) == (job.unitigs(), job.branch_fasta())
     assert st["n_unitigs"] == job.stats()["n_unitigs"]
     job.close()
+
+
+@pytest.mark.parametrize("groups", [1, 3, 64])
+def test_pipeline_group_counts(groups):
+    """Any number of pipeline groups (one, uneven groups, one sub-bucket
+    per group) gives the same results."""
+    old = get_tuning()
+    set_tuning(region_slots=1280, count_groups=groups)
+    try:
+        k, reads = _reads(CASES[5])
+        b, off = pack(reads + spill_reads(k))
+        g, _ = compare_all(k, thresholds_L(2), b, off, capacity=1 << 24)
+        assert g["stats"]["count_by_regions"] == 1
+    finally:
+        set_tuning(**old)
