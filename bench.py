@@ -238,6 +238,9 @@ def main():
     # calls inside the timed steps
     free0, _ = mem_info()
     pool_reserve(int(POOL_FRACTION * free0))
+    if args.knob:
+        from bruno_amd.autotune import set_tuning
+        set_tuning(**{kv.split("=")[0]: int(kv.split("=")[1]) for kv in args.knob})
     if args.config == "config4":
         return run_batched(args, free0)
     if args.config == "config5":
@@ -255,9 +258,6 @@ def main():
 
     lf = args.load_factor
     tuned = None
-    if args.knob:
-        from bruno_amd.autotune import set_tuning
-        set_tuning(**{kv.split("=")[0]: int(kv.split("=")[1]) for kv in args.knob})
     if args.tuning:
         from bruno_amd.autotune import apply
         tuned = apply(args.tuning)

@@ -97,6 +97,9 @@ struct bruno_server {
 
 namespace {
 
+/* a request with a table of at least this many slots runs alone */
+constexpr uint64_t SOLO_MIN_SLOTS = 1ULL << 26;
+
 /* Estimated peak HBM of one job: table and side table, the present list
  * (8 B per distinct k-mer, at most one per slot), the chain, list-ranking
  * and output arrays (a quarter of the table for reads of one genome), the
@@ -396,10 +399,10 @@ bruno_status bruno_server_submit(bruno_server* S, const bruno_request* q, uint64
   r->capacity = q->job.capacity;
   if (r->capacity == 0) r->capacity = std::max<uint64_t>(4096, 2 * r->n_bases);
   r->footprint = footprint(r->job, r->capacity, r->n_reads, r->n_bases);
-  /* a table large enough to count by regions runs as a batch of its own:
-   * the region count beats the batched atomic count there */
-  const uint64_t solo = br_region_count_min_slots();
-  r->alone = solo && r->capacity >= solo;
+  /* a large table runs as a batch of its own, counted by regions: from
+   * about 2^26 slots that beats the job's share of a batch (atomic count) */
+  const uint64_t region = br_region_count_min_slots();
+  r->alone = region && r->capacity >= std::max<uint64_t>(region, SOLO_MIN_SLOTS);
   std::lock_guard<std::mutex> lk(S->mu);
   if (S->stop) return BRUNO_STATE;
   r->ticket = S->next_ticket++;

@@ -39,12 +39,15 @@ def footprint(s, k, load_factor=0.5, cap=None):
     return int(table + present + reads + chains)
 
 
-# A one-job table of at least this many slots counts by regions (the
-# library's default count_regions 1); the batched count is the atomic one.
-REGION_COUNT_SLOTS = 1 << 27
+# A job with a table of at least this many slots runs alone.  Alone, a job
+# of 2^22 slots or more counts by regions (the batched count is the atomic
+# one); measured on config-4 jobs, from about 5e7 k-mer instances (tables of
+# about 2^26 slots) its region count and later phases alone cost less than
+# its share of a batch.
+SOLO_SLOTS = 1 << 26
 
 
-def plan_groups(cfgs, k, budget_bytes, load_factor=0.5, solo_slots=REGION_COUNT_SLOTS):
+def plan_groups(cfgs, k, budget_bytes, load_factor=0.5, solo_slots=SOLO_SLOTS):
     """Jobs packed in arrival order into launch groups whose estimated
     footprint stays within the budget (a job larger than the budget runs
     alone).  A job whose table reaches solo_slots (0: none) runs alone too,

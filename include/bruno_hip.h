@@ -374,12 +374,11 @@ typedef struct {
   uint32_t ruler_sample;   /* list ranking: one sampled ruler per this many chain nodes on
                               average (power of two, 8..4096): the walk length between
                               rulers and the size of the contracted list                    */
-  uint32_t count_regions;  /* 1: plain jobs (k <= 31) with tables of >= 2^27 slots lay the
-                              table out in LDS-sized regions and count by writing instance
-                              records grouped by region, then building each region on chip;
-                              2: the same from 2^22 slots; 0: one probe window and atomic
-                              counting in HBM.  Read when a table is allocated and at each
-                              count.                                                         */
+  uint32_t count_regions;  /* 1 or 2: plain jobs (k <= 63) with tables of >= 2^22 slots lay
+                              the table out in LDS-sized regions and count by writing
+                              instance records grouped by region, then building each region
+                              on chip; 0: one probe window and atomic counting in HBM.  Read
+                              when a table is allocated and at each count.                   */
   uint32_t region_slots;   /* most slots of a region when a region table is allocated:
                               2560 (80 KB of LDS per build workgroup) or 1280 (40 KB)       */
   uint32_t fuse_list_cap;  /* a count with the filter fused gives each node list room for at

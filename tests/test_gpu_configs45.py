@@ -23,7 +23,7 @@ import bruno_amd
 from bruno_amd.configs import config4_jobs, config5_jobs
 from bruno_amd.device import DeviceBuffer, mem_info, pool_trim, synchronize
 from bruno_amd.pipeline import synth_batch_device, synth_reads_device, thresholds_L
-from bruno_amd.planner import REGION_COUNT_SLOTS, capacity, footprint, plan_groups
+from bruno_amd.planner import SOLO_SLOTS, capacity, footprint, plan_groups
 from bruno_amd.server import Server
 from gpu_common import run_oracle
 
@@ -130,7 +130,7 @@ def test_config4_full_size_batched_invariants_and_alone():
     inst = 0
     for g in groups:
         res = run_group(k, jobs, g)
-        if len(g) == 1 and capacity(jobs[g[0]], k) >= REGION_COUNT_SLOTS:
+        if len(g) == 1 and capacity(jobs[g[0]], k) >= SOLO_SLOTS:
             assert res[g[0]]["stats"]["count_by_regions"] == 1  # a large job alone counts by regions
         for i in g:
             r = res[i]

@@ -2,7 +2,7 @@
 LDS) against the oracle, and against the atomic count on the same inputs,
 for one-word (k <= 31) and two-word (k <= 63) keys.
 count_regions 2 gives tables of >= 2^22 slots of plain k <= 31 jobs the
-region layout (the default bar, 2^27 slots, is for speed only)."""
+region layout (the default bar is the same, 2^22 slots)."""
 import numpy as np
 import pytest
 

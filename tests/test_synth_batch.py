@@ -31,12 +31,12 @@ def test_plan_groups_respects_budget_and_order():
 
 
 def test_plan_groups_runs_region_sized_jobs_alone():
-    """Jobs whose table reaches the region-count size get a launch group of
+    """Jobs whose table reaches the solo size get a launch group of
     their own, in arrival order; the others fill groups within the budget
     past them."""
-    from bruno_amd.planner import REGION_COUNT_SLOTS, capacity, footprint
+    from bruno_amd.planner import SOLO_SLOTS, capacity, footprint
     jobs = config4_jobs(seed=4, n_jobs=64)
-    big = [capacity(j, 31) >= REGION_COUNT_SLOTS for j in jobs]
+    big = [capacity(j, 31) >= SOLO_SLOTS for j in jobs]
     assert 0 < sum(big) < len(jobs)
     budget = 40e9
     groups = plan_groups(jobs, 31, budget)
