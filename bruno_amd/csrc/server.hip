@@ -10,7 +10,9 @@
  * one batch (bruno_batch_create ... bruno_run_pipeline).  Requests that
  * arrive while batches are on the device wait for the next free worker and
  * the memory the running batches give back, so several batches overlap on
- * the device.  A request whose table proves too small is retried on its own
+ * the device.  A request with a large table (2^26 slots or more) runs on
+ * its own, counted by regions, which beats its share of a batch's atomic
+ * count.  A request whose table proves too small is retried on its own
  * with more slots.  All device memory comes from the library's pool,
  * so a running server makes no hipMalloc / hipFree calls.
  *

@@ -436,7 +436,8 @@ bruno_status bruno_synth_batch_host(const bruno_synth_desc* descs, uint32_t n_jo
  * later waiting one with the same k and thresholds whose estimated HBM
  * footprint fits the free budget form the next batch, so requests that
  * arrive while others are on the device join the next batch and batches
- * overlap.  A job whose table proves too small runs again on its own with
+ * overlap.  A job with a table of 2^26 slots or more runs on its own
+ * (counted by regions).  A job whose table proves too small runs again on its own with
  * four times the slots, and then with a table that cannot fill (two slots
  * per base).  Each job's result equals what
  * bruno_run_pipeline gives for it alone.  No counterpart in the reference
