@@ -69,3 +69,16 @@ def plan_groups(cfgs, k, budget_bytes, load_factor=0.5, solo_slots=SOLO_SLOTS):
     if cur:
         groups.append(cur)
     return groups
+
+
+def instances(s, k):
+    """k-mer instances of a synthetic job's reads."""
+    return s.n_reads * max(0, s.read_len - k + 1)
+
+
+def short_first(groups, cfgs, k):
+    """Launch groups ordered by k-mer instances per job, fewest first
+    (Smith's rule: for jobs submitted together this minimises the sum of
+    their completion times, taking a group's time as proportional to its
+    instances; the total time is the same in any order)."""
+    return sorted(groups, key=lambda g: (sum(instances(cfgs[i], k) for i in g) / len(g), g[0]))

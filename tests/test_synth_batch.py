@@ -51,3 +51,13 @@ def test_plan_groups_runs_region_sized_jobs_alone():
     # solo_slots=0: budget packing only, consecutive
     flat = plan_groups(jobs, 31, budget, solo_slots=0)
     assert [i for g in flat for i in g] == list(range(64))
+
+
+def test_short_first_orders_groups_by_instances_per_job():
+    from bruno_amd.planner import instances, short_first
+    jobs = config4_jobs(seed=4, n_jobs=64)
+    groups = plan_groups(jobs, 31, 40e9)
+    order = short_first(groups, jobs, 31)
+    assert sorted(map(tuple, order)) == sorted(map(tuple, groups))
+    w = [sum(instances(jobs[i], 31) for i in g) / len(g) for g in order]
+    assert w == sorted(w)
